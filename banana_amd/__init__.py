@@ -1,0 +1,7 @@
+"""banana_amd: MI355X (gfx950) FLIP substep of Banana's FLIP_Solver<3,float>, behind a C ABI.
+
+The compute path is libbnn_flip.so (hand-written HIP kernels); this package holds the build step,
+the ctypes view of the ABI and scene descriptions. There is no CPU fallback.
+"""
+from ._lib import BnnError, load  # noqa: F401
+from .solver import FLIPSolver, generate_scene_host, make_params, make_spec  # noqa: F401

@@ -1,0 +1,986 @@
+// flip_kernels.hpp -- gfx950 kernels of the FLIP substep (included once, by flip_solver.hip).
+//
+// Layout in HBM (one scene):
+//   particles   SoA float px..vz, grid coordinates gx..gz, uint32 id; kept cell-sorted after binning
+//   MAC faces   u (nx+1,ny,nz), v (nx,ny+1,nz), w (nx,ny,nz+1), x-fastest (Array.h:276)
+//   cells       phi (nx,ny,nz) float; cell_start (nx*ny*nz + 1) uint32
+//   pressure    8^3-cell bricks holding >= 1 fluid cell, 512 cells contiguous per brick
+//               (brick-major, x-fastest inside a brick): vectors, matrix diagonal and the three
+//               "upper" couplings (+x,+y,+z); a brick's 6 neighbours come from a slot table.
+#pragma once
+
+#include "flip_device.hpp"
+
+namespace bnn {
+
+constexpr int kBlock = 256;
+
+//==================================================================================================
+// moveParticles: RK2 through the grid + domain-box projection (PIC_Solver.cpp:353-373, 1137-1144;
+// BoundaryObject.cpp:72-90)
+//==================================================================================================
+struct VelGrid {
+    const float *u, *v, *w;
+};
+
+__device__ __forceinline__ void vel_from_grid(const VelGrid& G, const GridGeom& g, float gpx, float gpy, float gpz, float& ux,
+                                              float& uy, float& uz) {
+    ux = interp3(G.u, g.nx + 1, g.ny, g.nz, gpx, gpy - 0.5f, gpz - 0.5f);
+    uy = interp3(G.v, g.nx, g.ny + 1, g.nz, gpx - 0.5f, gpy, gpz - 0.5f);
+    uz = interp3(G.w, g.nx, g.ny, g.nz + 1, gpx - 0.5f, gpy - 0.5f, gpz);
+}
+
+__global__ void __launch_bounds__(kBlock) k_move(Particles P, uint32_t n, VelGrid G, GridGeom g, BoxDev dom, float dt,
+                                                 uint32_t advSteps, int reflect, float reflMult) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    float x = P.px[p], y = P.py[p], z = P.pz[p];
+    float vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
+    bool  changed = false;
+    const float sub = dt / (float)advSteps;
+    for(uint32_t s = 0; s < advSteps; ++s) {
+        // trace_rk2
+        float ux, uy, uz;
+        vel_from_grid(G, g, (x - g.bminx) / g.dx, (y - g.bminy) / g.dx, (z - g.bminz) / g.dx, ux, uy, uz);
+        const float h  = 0.5f * sub;
+        const float mx = x + h * ux, my = y + h * uy, mz = z + h * uz;
+        vel_from_grid(G, g, (mx - g.bminx) / g.dx, (my - g.bminy) / g.dx, (mz - g.bminz) / g.dx, ux, uy, uz);
+        x = x + sub * ux;
+        y = y + sub * uy;
+        z = z + sub * uz;
+        // constrainToBoundary(ppos, pvel)
+        const float phi = box_sd_out(dom, x, y, z);
+        if(phi < 0) {
+            float gx, gy, gz;
+            box_grad_out(dom, x, y, z, gx, gy, gz);
+            const float m2 = gx * gx + gy * gy + gz * gz;
+            if(m2 > kFltMin) {
+                const float s = sqrtf(m2);
+                gx /= s;
+                gy /= s;
+                gz /= s;
+                x -= phi * gx;
+                y -= phi * gy;
+                z -= phi * gz;
+                if(reflect) {
+                    const float dn = gx * vx + gy * vy + gz * vz;
+                    vx             = (vx - 2.f * dn * gx) * reflMult;
+                    vy             = (vy - 2.f * dn * gy) * reflMult;
+                    vz             = (vz - 2.f * dn * gz) * reflMult;
+                    changed        = true;
+                }
+            }
+        }
+    }
+    P.px[p] = x;
+    P.py[p] = y;
+    P.pz[p] = z;
+    if(changed) {
+        P.vx[p] = vx;
+        P.vy[p] = vy;
+        P.vz[p] = vz;
+    }
+}
+
+//==================================================================================================
+// Binning: stable counting sort by linear cell index (Grid.cpp:92-114, 283-309)
+//==================================================================================================
+__device__ __forceinline__ int cell_axis(float p, float bmin, float dx, int n) {
+    int c = (int)((p - bmin) / dx);  // Grid.h:156-162 (truncation toward zero)
+    return c < 0 ? 0 : (c > n - 1 ? n - 1 : c);  // Grid::getValidCellIdx clamp
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_key(const float* __restrict__ px, const float* __restrict__ py,
+                                                     const float* __restrict__ pz, uint32_t n, GridGeom g,
+                                                     uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    const int      i = cell_axis(px[p], g.bminx, g.dx, g.nx);
+    const int      j = cell_axis(py[p], g.bminy, g.dx, g.ny);
+    const int      k = cell_axis(pz[p], g.bminz, g.dx, g.nz);
+    const uint32_t c = (uint32_t)lin3(i, j, k, g.nx, g.ny);
+    key[p]           = c;
+    atomicAdd(&count[c], 1u);
+}
+
+// counts return to zero here (each particle takes one slot), so the count array needs no clearing
+__global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t* __restrict__ key, uint32_t n,
+                                                    const uint32_t* __restrict__ cell_start, uint32_t* __restrict__ count,
+                                                    uint32_t* __restrict__ perm) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    const uint32_t c    = key[p];
+    const uint32_t slot = cell_start[c] + atomicSub(&count[c], 1u) - 1u;
+    perm[slot]          = p;
+}
+
+// make each cell's segment ascending: the counting sort becomes stable (ties by particle index)
+__global__ void __launch_bounds__(kBlock) k_segsort(const uint32_t* __restrict__ cell_start, uint32_t ncells,
+                                                    uint32_t* __restrict__ perm) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= ncells) return;
+    const uint32_t b = cell_start[c], e = cell_start[c + 1];
+    if(e - b < 2) return;
+    for(uint32_t i = b + 1; i < e; ++i) {
+        uint32_t v = perm[i];
+        uint32_t j = i;
+        while(j > b && perm[j - 1] > v) {
+            perm[j] = perm[j - 1];
+            --j;
+        }
+        perm[j] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_permute(Particles in, Particles out, const uint32_t* __restrict__ perm, uint32_t n,
+                                                    GridGeom g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const uint32_t s = perm[i];
+    const float    x = in.px[s], y = in.py[s], z = in.pz[s];
+    out.px[i]        = x;
+    out.py[i]        = y;
+    out.pz[i]        = z;
+    out.vx[i]        = in.vx[s];
+    out.vy[i]        = in.vy[s];
+    out.vz[i]        = in.vz[s];
+    out.id[i]        = in.id[s];
+    out.gx[i]        = (x - g.bminx) / g.dx;
+    out.gy[i]        = (y - g.bminy) / g.dx;
+    out.gz[i]        = (z - g.bminz) / g.dx;
+}
+
+//--------------------------------------------------------------------------------------------------
+// device-wide exclusive scan of uint32 (3 phases); tile = 256 threads x 8 items
+//--------------------------------------------------------------------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile  = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t  x    = v;
+#pragma unroll
+    for(int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if(lane >= o) x += y;
+    }
+    if(lane == 63) sh[wid] = x;
+    __syncthreads();
+    uint32_t off = 0;
+    total        = 0;
+    for(int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        if(w < wid) off += sh[w];
+        total += sh[w];
+    }
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restrict__ in, size_t n, uint32_t* __restrict__ partial) {
+    __shared__ uint32_t sh[kBlock / 64];
+    const size_t        base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t            s    = 0;
+#pragma unroll
+    for(int i = 0; i < kScanItems; ++i)
+        if(base + i < n) s += in[base + i];
+    uint32_t total;
+    block_exclusive_scan(s, sh, total);
+    if(threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of the per-tile partials, in chunks of 256
+__global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t* partial, uint32_t np, uint32_t* total_out) {
+    __shared__ uint32_t sh[kBlock / 64];
+    uint32_t            carry = 0;
+    for(uint32_t b = 0; b < np; b += kBlock) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < np ? partial[i] : 0u;
+        uint32_t       tot;
+        const uint32_t ex = block_exclusive_scan(v, sh, tot);
+        if(i < np) partial[i] = carry + ex;
+        carry += tot;
+    }
+    if(threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* __restrict__ in, size_t n, const uint32_t* __restrict__ partial,
+                                                      uint32_t* __restrict__ out) {
+    __shared__ uint32_t sh[kBlock / 64];
+    const size_t        base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t            v[kScanItems];
+    uint32_t            s = 0;
+#pragma unroll
+    for(int i = 0; i < kScanItems; ++i) {
+        v[i] = base + i < n ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, sh, total) + partial[blockIdx.x];
+#pragma unroll
+    for(int i = 0; i < kScanItems; ++i) {
+        if(base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+}
+
+//==================================================================================================
+// mapParticles2Grid as a gather over cell-sorted particles (FLIP_Solver.cpp:57-95,
+// ArrayHelpers.cpp:50-76). Thread (i,j,k) owns u(i,j,k), v(i,j,k), w(i,j,k); it visits the particles
+// of cells [i-1,i+1]x[j-1,j+1]x[k-1,k+1] in sorted order and keeps each particle's exact trilinear
+// weight for the face when the face is one of the particle's 8 stencil corners.
+//==================================================================================================
+struct FaceOut {
+    float *u, *v, *w;
+    uint8_t *vu, *vv, *vw;
+};
+
+__device__ __forceinline__ bool stencil_weight(float q, int i_high, int target, float& wgt) {
+    int   i;
+    float f;
+    bary(q, i_high, i, f);
+    if(target == i) {
+        wgt = 1.0f - f;
+        return true;
+    }
+    if(target == i + 1) {
+        wgt = f;
+        return true;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
+    const uint32_t t  = blockIdx.x * blockDim.x + threadIdx.x;
+    const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
+    if(t >= (uint32_t)(ex * ey * ez)) return;
+    const int i = t % ex, j = (t / ex) % ey, k = t / (ex * ey);
+    const bool doU = j < g.ny && k < g.nz;
+    const bool doV = i < g.nx && k < g.nz;
+    const bool doW = i < g.nx && j < g.ny;
+    float mu = 0.f, wu = 0.f, mv = 0.f, wv = 0.f, mw = 0.f, ww = 0.f;
+    const int x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1);
+    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
+        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y) {
+            const size_t   r  = lin3(0, y, z, g.nx, g.ny);
+            const uint32_t pb = cell_start[r + x0], pe = cell_start[r + x1 + 1];
+            for(uint32_t p = pb; p < pe; ++p) {
+                const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+                if(doU) {
+                    float a, b, c;
+                    if(stencil_weight(gx, g.nx + 1, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) &&
+                       stencil_weight(gz - 0.5f, g.nz, k, c)) {
+                        const float wt = a * b * c;
+                        mu += wt * P.vx[p];
+                        wu += wt;
+                    }
+                }
+                if(doV) {
+                    float a, b, c;
+                    if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy, g.ny + 1, j, b) &&
+                       stencil_weight(gz - 0.5f, g.nz, k, c)) {
+                        const float wt = a * b * c;
+                        mv += wt * P.vy[p];
+                        wv += wt;
+                    }
+                }
+                if(doW) {
+                    float a, b, c;
+                    if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) &&
+                       stencil_weight(gz, g.nz + 1, k, c)) {
+                        const float wt = a * b * c;
+                        mw += wt * P.vz[p];
+                        ww += wt;
+                    }
+                }
+            }
+        }
+    if(doU) {
+        const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
+        const bool   ok = wu > kFltMin;
+        out.u[id]       = ok ? mu / wu : 0.f;
+        out.vu[id]      = ok ? 1 : 0;
+    }
+    if(doV) {
+        const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
+        const bool   ok = wv > kFltMin;
+        out.v[id]       = ok ? mv / wv : 0.f;
+        out.vv[id]      = ok ? 1 : 0;
+    }
+    if(doW) {
+        const size_t id = lin3(i, j, k, g.nx, g.ny);
+        const bool   ok = ww > kFltMin;
+        out.w[id]       = ok ? mw / ww : 0.f;
+        out.vw[id]      = ok ? 1 : 0;
+    }
+}
+
+//==================================================================================================
+// extrapolateVelocity: one Jacobi layer of one component (PIC_Solver.cpp:992-1090)
+//==================================================================================================
+__global__ void __launch_bounds__(kBlock) k_extrap_layer(const float* __restrict__ in, float* __restrict__ out,
+                                                         const uint8_t* __restrict__ vin, uint8_t* __restrict__ vout,
+                                                         uint8_t* __restrict__ ext, int sx, int sy, int sz, int firstLayer) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)sx * sy * sz;
+    if(t >= N) return;
+    const int  i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    const bool interior = i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2;
+    const uint8_t v0   = vin[t];
+    if(interior && !v0) {
+        const size_t sxy   = (size_t)sx * sy;
+        float        sum   = 0.f;
+        uint32_t     count = 0;
+        if(vin[t + 1]) { sum += in[t + 1]; ++count; }
+        if(vin[t - 1]) { sum += in[t - 1]; ++count; }
+        if(vin[t + sx]) { sum += in[t + sx]; ++count; }
+        if(vin[t - sx]) { sum += in[t - sx]; ++count; }
+        if(vin[t + sxy]) { sum += in[t + sxy]; ++count; }
+        if(vin[t - sxy]) { sum += in[t - sxy]; ++count; }
+        if(count > 0) {
+            out[t]  = sum / (float)count;
+            vout[t] = 1;
+            ext[t]  = 1;
+            return;
+        }
+    }
+    out[t]  = in[t];
+    vout[t] = v0;
+    if(firstLayer) ext[t] = 0;
+}
+
+//==================================================================================================
+// constrainGridVelocity, component d (PIC_Solver.cpp:1094-1115, ArrayHelpers.cpp:214-251)
+//==================================================================================================
+__device__ __forceinline__ void interp_gradient(const float* __restrict__ b, int sx, int sy, int sz, float x, float y, float z,
+                                                float& gx, float& gy, float& gz) {
+    int   i, j, k;
+    float fi, fj, fk;
+    bary(x, sx, i, fi);
+    bary(y, sy, j, fj);
+    bary(z, sz, k, fk);
+    auto  at   = [&](int a, int c, int e) { return b[lin3(a, c, e, sx, sy)]; };
+    float v000 = at(i, j, k), v001 = at(i, j, k + 1), v010 = at(i, j + 1, k), v011 = at(i, j + 1, k + 1);
+    float v100 = at(i + 1, j, k), v101 = at(i + 1, j, k + 1), v110 = at(i + 1, j + 1, k), v111 = at(i + 1, j + 1, k + 1);
+    gx = bilerpf(v100 - v000, v110 - v010, v101 - v001, v111 - v011, fj, fk);
+    gy = bilerpf(v010 - v000, v110 - v100, v011 - v001, v111 - v101, fi, fk);
+    gz = bilerpf(v001 - v000, v101 - v100, v011 - v010, v111 - v110, fi, fj);
+}
+
+__global__ void __launch_bounds__(kBlock) k_constrain(VelGrid G, GridGeom g, int d, const uint8_t* __restrict__ ext,
+                                                      const float* __restrict__ weight, const float* __restrict__ bsdf,
+                                                      float* __restrict__ out) {
+    const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
+    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N  = (size_t)sx * sy * sz;
+    if(t >= N) return;
+    const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
+    float        val = cur[t];
+    if(ext[t] == 1 && weight[t] < kFltMin) {
+        const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+        float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+        gp[d]           = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+        float vx, vy, vz, nx, ny, nz;
+        vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
+        interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
+        const float m2 = nx * nx + ny * ny + nz * nz;
+        if(m2 > kFltMin) {
+            const float s = sqrtf(m2);
+            nx /= s;
+            ny /= s;
+            nz /= s;
+            const float dt = vx * nx + vy * ny + vz * nz;
+            vx -= dt * nx;
+            vy -= dt * ny;
+            vz -= dt * nz;
+            val = d == 0 ? vx : (d == 1 ? vy : vz);
+        }
+    }
+    out[t] = val;
+}
+
+//==================================================================================================
+// addGravity (PIC_Solver.cpp:505-529): Earth / Directional add a constant per component
+//==================================================================================================
+__global__ void __launch_bounds__(kBlock) k_add_const(float* __restrict__ a, size_t n, float add, int subtract) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= n) return;
+    if(subtract) a[t] -= add;
+    else a[t] += add;
+}
+
+// ToCenter / FromCenter: the reference walks velocities[0]'s index range for every component
+// (PIC_Solver.cpp:521); indices outside component d's array are skipped here.
+__global__ void __launch_bounds__(kBlock) k_gravity_center(float* __restrict__ a, GridGeom g, int d, float cx, float cy, float cz,
+                                                           int toCenter, float dt) {
+    const int    sx = g.nx + 1, sy = g.ny, sz = g.nz;
+    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= (size_t)sx * sy * sz) return;
+    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    const int ax = g.nx + (d == 0), ay = g.ny + (d == 1), az = g.nz + (d == 2);
+    if(i >= ax || j >= ay || k >= az) return;
+    float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+    gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+    const float wx = gp[0] * g.dx + g.bminx, wy = gp[1] * g.dx + g.bminy, wz = gp[2] * g.dx + g.bminz;
+    float       dv[3];
+    dv[0]          = toCenter ? cx - wx : wx - cx;
+    dv[1]          = toCenter ? cy - wy : wy - cy;
+    dv[2]          = toCenter ? cz - wz : wz - cz;
+    const float is = 1.0f / sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    a[lin3(i, j, k, ax, ay)] += (9.81f * (dv[d] * is)) * dt;
+}
+
+//==================================================================================================
+// computeCellWeights (PIC_Solver.cpp:543-597); counts faces with weight < FLT_MIN (solid faces)
+//==================================================================================================
+__global__ void __launch_bounds__(kBlock) k_weights(const float* __restrict__ b, GridGeom g, float* __restrict__ wu,
+                                                    float* __restrict__ wv, float* __restrict__ ww, uint32_t* solidFaces) {
+    const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
+    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= (size_t)sx * sy * sz) return;
+    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    auto      B = [&](int a, int c, int e) { return b[lin3(a, c, e, sx, sy)]; };
+    uint32_t  solid = 0;
+    if(j < g.ny && k < g.nz) {
+        float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j + 1, k), B(i, j, k + 1), B(i, j + 1, k + 1)));
+        wu[lin3(i, j, k, g.nx + 1, g.ny)] = w;
+        solid += w < kFltMin;
+    }
+    if(i < g.nx && k < g.nz) {
+        float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j, k + 1), B(i + 1, j, k), B(i + 1, j, k + 1)));
+        wv[lin3(i, j, k, g.nx, g.ny + 1)] = w;
+        solid += w < kFltMin;
+    }
+    if(i < g.nx && j < g.ny) {
+        float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j + 1, k), B(i + 1, j, k), B(i + 1, j + 1, k)));
+        ww[lin3(i, j, k, g.nx, g.ny)] = w;
+        solid += w < kFltMin;
+    }
+    if(solid) atomicAdd(solidFaces, solid);
+}
+
+// computeBoundarySDF with the domain box as the boundary (PIC_Solver.cpp:1158-1168)
+__global__ void __launch_bounds__(kBlock) k_boundary_sdf(float* __restrict__ b, GridGeom g, BoxDev dom) {
+    const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
+    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= (size_t)sx * sy * sz) return;
+    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    b[t]        = box_sd_out(dom, (float)i * g.dx + g.bminx, (float)j * g.dx + g.bminy, (float)k * g.dx + g.bminz);
+}
+
+//==================================================================================================
+// computeFluidSDF as a gather (PIC_Solver.cpp:601-689): phi(c) = min over particles whose clamped
+// 3x3x3 window holds c of |centre(c) - p| - sdfRadius; min is exact, so the result is bit-identical
+// to the reference's locked scatter. Then the solid extension.
+//==================================================================================================
+__global__ void __launch_bounds__(kBlock) k_fluid_sdf(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g,
+                                                      const float* __restrict__ bsdf, float* __restrict__ phi) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= (uint32_t)(g.nx * g.ny * g.nz)) return;
+    const int   i = t % g.nx, j = (t / g.nx) % g.ny, k = t / (g.nx * g.ny);
+    const float sx = ((float)i + 0.5f) * g.dx + g.bminx;
+    const float sy = ((float)j + 0.5f) * g.dx + g.bminy;
+    const float sz = ((float)k + 0.5f) * g.dx + g.bminz;
+    float       best = g.dx * 3.0f;
+    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
+        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y)
+            for(int x = max(i - 1, 0); x <= min(i + 1, g.nx - 1); ++x) {
+                const size_t   c        = lin3(x, y, z, g.nx, g.ny);
+                const uint32_t pb       = cell_start[c], pe = cell_start[c + 1];
+                const bool     boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
+                for(uint32_t p = pb; p < pe; ++p) {
+                    const float ppx = P.px[p], ppy = P.py[p], ppz = P.pz[p];
+                    if(boundary) {
+                        // particles clamped into a boundary cell: use their true (unclamped) window
+                        const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
+                                  ck = (int)((ppz - g.bminz) / g.dx);
+                        if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) continue;
+                    }
+                    const float v = len3f(sx - ppx, sy - ppy, sz - ppz) - g.sdfRadius;
+                    if(v < best) best = v;
+                }
+            }
+    if(best < g.halfdx) {
+        const int sxn = g.nx + 1, syn = g.ny + 1;
+        auto      B   = [&](int a, int c, int e) { return bsdf[lin3(a, c, e, sxn, syn)]; };
+        const float s = 0.125f * (B(i, j, k) + B(i + 1, j, k) + B(i, j + 1, k) + B(i + 1, j + 1, k) + B(i, j, k + 1) +
+                                  B(i + 1, j, k + 1) + B(i, j + 1, k + 1) + B(i + 1, j + 1, k + 1));
+        if(s < 0) best = -g.halfdx;
+    }
+    phi[t] = best;
+}
+
+//==================================================================================================
+// Pressure bricks
+//==================================================================================================
+// one block (512 threads) per brick: flag bricks holding a fluid cell, count fluid cells
+__global__ void __launch_bounds__(512) k_brick_flags(const float* __restrict__ phi, GridGeom g, uint32_t* __restrict__ flag,
+                                                     uint32_t* __restrict__ nFluid) {
+    const int b  = blockIdx.x;
+    const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+    const int l  = threadIdx.x;
+    const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+    int fluid = 0;
+    if(i < g.nx && j < g.ny && k < g.nz) fluid = phi[lin3(i, j, k, g.nx, g.ny)] < 0.f;
+    const int cnt = __syncthreads_count(fluid);
+    if(l == 0) {
+        flag[b] = cnt > 0 ? 1u : 0u;
+        if(cnt) atomicAdd(nFluid, (uint32_t)cnt);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_brick_slots(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
+                                                        uint32_t nb, int32_t* __restrict__ brickSlot, int32_t* __restrict__ slotBrick) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if(b >= nb) return;
+    if(flag[b]) {
+        brickSlot[b]        = (int32_t)scan[b];
+        slotBrick[scan[b]] = (int32_t)b;
+    } else {
+        brickSlot[b] = -1;
+    }
+}
+
+// neighbour slots per active brick: -x, +x, -y, +y, -z, +z
+__global__ void __launch_bounds__(kBlock) k_brick_nbrs(const int32_t* __restrict__ slotBrick, uint32_t nSlots,
+                                                       const int32_t* __restrict__ brickSlot, GridGeom g, int32_t* __restrict__ nbr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if(s >= nSlots) return;
+    const int b  = slotBrick[s];
+    const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+    auto      at = [&](int x, int y, int z) -> int32_t {
+        if(x < 0 || y < 0 || z < 0 || x >= g.nbx || y >= g.nby || z >= g.nbz) return -1;
+        return brickSlot[(z * g.nby + y) * g.nbx + x];
+    };
+    nbr[6 * s + 0] = at(bx - 1, by, bz);
+    nbr[6 * s + 1] = at(bx + 1, by, bz);
+    nbr[6 * s + 2] = at(bx, by - 1, bz);
+    nbr[6 * s + 3] = at(bx, by + 1, bz);
+    nbr[6 * s + 4] = at(bx, by, bz - 1);
+    nbr[6 * s + 5] = at(bx, by, bz + 1);
+}
+
+// vector index of the neighbour of local cell l in direction dir (-1 when absent)
+__device__ __forceinline__ int32_t nbr_index(const int32_t* __restrict__ nbr, int32_t s, int l, int dir) {
+    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    switch(dir) {
+        case 0: return lx > 0 ? s * 512 + l - 1 : (nbr[6 * s + 0] >= 0 ? nbr[6 * s + 0] * 512 + l + 7 : -1);
+        case 1: return lx < 7 ? s * 512 + l + 1 : (nbr[6 * s + 1] >= 0 ? nbr[6 * s + 1] * 512 + l - 7 : -1);
+        case 2: return ly > 0 ? s * 512 + l - 8 : (nbr[6 * s + 2] >= 0 ? nbr[6 * s + 2] * 512 + l + 56 : -1);
+        case 3: return ly < 7 ? s * 512 + l + 8 : (nbr[6 * s + 3] >= 0 ? nbr[6 * s + 3] * 512 + l - 56 : -1);
+        case 4: return lz > 0 ? s * 512 + l - 64 : (nbr[6 * s + 4] >= 0 ? nbr[6 * s + 4] * 512 + l + 448 : -1);
+        default: return lz < 7 ? s * 512 + l + 64 : (nbr[6 * s + 5] >= 0 ? nbr[6 * s + 5] * 512 + l - 448 : -1);
+    }
+}
+
+//==================================================================================================
+// computeSystem (PIC_Solver.cpp:693-893), matrix-free form. Per fluid interior cell: rhs (FP64,
+// negative divergence), diagonal and the +x/+y/+z couplings (-w*dt when both cells are fluid), in
+// FP64 and rounded to FP32. Cells outside [1, n-2] keep an empty row, as in the reference.
+//==================================================================================================
+struct SystemOut {
+    double *rhs64, *diag64, *cx64, *cy64, *cz64;  // cx64..: may be null (FP32 mode)
+    float * rhs32, *diag32, *cx32, *cy32, *cz32, *invdiag32;
+};
+
+__global__ void __launch_bounds__(512) k_assemble(const int32_t* __restrict__ slotBrick, GridGeom g, const float* __restrict__ phi,
+                                                  VelGrid G, const float* __restrict__ wu, const float* __restrict__ wv,
+                                                  const float* __restrict__ ww, float dt, SystemOut S) {
+    const int s = blockIdx.x, l = threadIdx.x;
+    const int b = slotBrick[s];
+    const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+    const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+    const size_t o = (size_t)s * 512 + l;
+    double rhs = 0, diag = 0, cx = 0, cy = 0, cz = 0;
+    const bool interior = i >= 1 && j >= 1 && k >= 1 && i <= g.nx - 2 && j <= g.ny - 2 && k <= g.nz - 2;
+    if(interior) {
+        auto P = [&](int a, int c, int e) { return phi[lin3(a, c, e, g.nx, g.ny)]; };
+        const float pc = P(i, j, k);
+        if(pc < 0) {
+            const int    ux = g.nx + 1, vy = g.ny + 1;
+            const double wr = wu[lin3(i + 1, j, k, ux, g.ny)], wl = wu[lin3(i, j, k, ux, g.ny)];
+            const double wt = wv[lin3(i, j + 1, k, g.nx, vy)], wb = wv[lin3(i, j, k, g.nx, vy)];
+            const double wf = ww[lin3(i, j, k + 1, g.nx, g.ny)], wn = ww[lin3(i, j, k, g.nx, g.ny)];
+            rhs -= wr * (double)G.u[lin3(i + 1, j, k, ux, g.ny)];
+            rhs += wl * (double)G.u[lin3(i, j, k, ux, g.ny)];
+            rhs -= wt * (double)G.v[lin3(i, j + 1, k, g.nx, vy)];
+            rhs += wb * (double)G.v[lin3(i, j, k, g.nx, vy)];
+            rhs -= wf * (double)G.w[lin3(i, j, k + 1, g.nx, g.ny)];
+            rhs += wn * (double)G.w[lin3(i, j, k, g.nx, g.ny)];
+            const double ddt = (double)dt;
+            const double pcd = (double)pc;
+            auto side = [&](float pn, double w, double& coupling, bool upper) {
+                const double term = w * ddt;
+                if(pn < 0) {
+                    diag += term;
+                    if(upper) coupling = -term;
+                } else {
+                    const double th = fraction_inside2(pcd, (double)pn);
+                    diag += term / (th > 0.01 ? th : 0.01);
+                }
+            };
+            double dummy = 0;
+            side(P(i + 1, j, k), wr, cx, true);
+            side(P(i - 1, j, k), wl, dummy, false);
+            side(P(i, j + 1, k), wt, cy, true);
+            side(P(i, j - 1, k), wb, dummy, false);
+            side(P(i, j, k + 1), wf, cz, true);
+            side(P(i, j, k - 1), wn, dummy, false);
+            // couplings into boundary-layer cells (empty rows) are dropped
+            if(i + 1 > g.nx - 2) cx = 0;
+            if(j + 1 > g.ny - 2) cy = 0;
+            if(k + 1 > g.nz - 2) cz = 0;
+        }
+    }
+    if(S.rhs64) S.rhs64[o] = rhs;
+    if(S.diag64) {
+        S.diag64[o] = diag;
+        S.cx64[o]   = cx;
+        S.cy64[o]   = cy;
+        S.cz64[o]   = cz;
+    }
+    S.rhs32[o]     = (float)rhs;
+    S.diag32[o]    = (float)diag;
+    S.cx32[o]      = (float)cx;
+    S.cy32[o]      = (float)cy;
+    S.cz32[o]      = (float)cz;
+    S.invdiag32[o] = diag > 0 ? (float)(1.0 / diag) : 0.f;
+}
+
+//==================================================================================================
+// PCG (PCGSolver.cpp:143-199) with Jacobi preconditioning, FP32 vectors and FP64 scalars.
+// Scalars and the stop flag live in device memory; every kernel returns at once after the stop
+// flag is set, so the host can queue iterations in chunks without reading back per iteration.
+// Reductions: per-block partials, then the last block to finish folds them in fixed order.
+//==================================================================================================
+struct PcgState {
+    double   rho, alpha, beta, tmp;
+    float    tol, res, bmax, pad0;
+    uint32_t iter, maxIter, done, success;
+};
+
+// Reductions are two-level and deterministic: each block of a main kernel reduces the bricks it
+// owns (bricks b, b + gridDim.x, ...) and writes one partial with a plain store; a one-block fold
+// kernel then combines the partials in index order. Kernel boundaries order the two, so no
+// device-scope fences are needed inside the main kernels.
+constexpr int kMaxRedBlocks = 2048;
+
+template<int NT> __device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if(threadIdx.x == 0)
+        for(int i = 0; i < NT / 64; ++i) t += sh[i];
+    __syncthreads();
+    return t;  // valid in thread 0
+}
+template<int NT> __device__ __forceinline__ double block_max(double v, double* sh) {
+    v = wave_maxd(v);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if(threadIdx.x == 0)
+        for(int i = 0; i < NT / 64; ++i) t = fmax(t, sh[i]);
+    __syncthreads();
+    return t;
+}
+
+// fold n partials in fixed order (1024-thread block); sum (or max) valid in thread 0
+__device__ __forceinline__ double fold(const double* __restrict__ p, int n, bool isMax, double* sh) {
+    double a = 0;
+    for(int i = threadIdx.x; i < n; i += 1024) a = isMax ? fmax(a, p[i]) : a + p[i];
+    return isMax ? block_max<1024>(a, sh) : block_sum<1024>(a, sh);
+}
+
+// init: x = 0, r = b, s = 0, z = M^-1 r; partials of z.r and max|b|
+__global__ void __launch_bounds__(512) k_pcg_init(const float* __restrict__ b, const float* __restrict__ invdiag,
+                                                  float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
+                                                  float* __restrict__ s, uint32_t nSlots, double* __restrict__ pa,
+                                                  double* __restrict__ pb) {
+    __shared__ double sh[8];
+    double            rz = 0, m = 0;
+    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+        const size_t o  = (size_t)slot * 512 + threadIdx.x;
+        const float  bv = b[o];
+        const float  zv = invdiag[o] * bv;
+        x[o]            = 0.f;
+        r[o]            = bv;
+        z[o]            = zv;
+        s[o]            = 0.f;
+        rz += (double)zv * (double)bv;
+        m = fmax(m, (double)fabsf(bv));
+    }
+    rz = block_sum<512>(rz, sh);
+    m  = block_max<512>(m, sh);
+    if(threadIdx.x == 0) {
+        pa[blockIdx.x] = rz;
+        pb[blockIdx.x] = m;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_pcg_init_fin(PcgState* st, const double* __restrict__ pa, const double* __restrict__ pb,
+                                                       int n, float relTol, uint32_t maxIter) {
+    __shared__ double sh[16];
+    const double      A = fold(pa, n, false, sh);
+    const double      B = fold(pb, n, true, sh);
+    if(threadIdx.x != 0) return;
+    st->bmax    = (float)B;
+    st->res     = (float)B;
+    st->iter    = 0;
+    st->maxIter = maxIter;
+    st->beta    = 0;
+    st->rho     = A;
+    st->done    = 0;
+    st->success = 0;
+    st->tol     = relTol * (float)B;  // PCGSolver.cpp:160
+    if((float)B < kFltMin) {          // PCGSolver.cpp:155-158
+        st->done    = 1;
+        st->success = 1;
+    } else if(A < (double)kFltMin || isnan(A)) {  // PCGSolver.cpp:164-167
+        st->done = 1;
+    } else if(maxIter == 0) {
+        st->done = 1;
+    }
+}
+
+// s_out = z + beta s_in ; q = A s_out ; partial s_out . q
+__global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nbr, const float* __restrict__ z,
+                                                  const float* __restrict__ sin, float* __restrict__ sout, float* __restrict__ q,
+                                                  const float* __restrict__ diag, const float* __restrict__ cx,
+                                                  const float* __restrict__ cy, const float* __restrict__ cz, uint32_t nSlots,
+                                                  const PcgState* __restrict__ st, double* __restrict__ pa) {
+    __shared__ double sh[8];
+    if(st->done) return;
+    const float beta = (float)st->beta;
+    const int   l    = threadIdx.x;
+    double      d    = 0;
+    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+        const int     s   = (int)slot;
+        const size_t  o   = (size_t)s * 512 + l;
+        auto          S   = [&](int32_t idx) { return z[idx] + beta * sin[idx]; };
+        const float   sc  = z[o] + beta * sin[o];
+        const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
+        const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
+        const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
+        float         acc = 0.f;
+        if(izm >= 0) acc += cz[izm] * S(izm);
+        if(iym >= 0) acc += cy[iym] * S(iym);
+        if(ixm >= 0) acc += cx[ixm] * S(ixm);
+        acc += diag[o] * sc;
+        if(ixp >= 0) acc += cx[o] * S(ixp);
+        if(iyp >= 0) acc += cy[o] * S(iyp);
+        if(izp >= 0) acc += cz[o] * S(izp);
+        sout[o] = sc;
+        q[o]    = acc;
+        d += (double)sc * (double)acc;
+    }
+    d = block_sum<512>(d, sh);
+    if(threadIdx.x == 0) pa[blockIdx.x] = d;
+}
+
+// tmp = s.q ; alpha = rho / tmp (PCGSolver.cpp:174-180)
+__global__ void __launch_bounds__(1024) k_pcg_alpha(PcgState* st, const double* __restrict__ pa, int n) {
+    __shared__ double sh[16];
+    if(st->done) return;
+    const double A = fold(pa, n, false, sh);
+    if(threadIdx.x != 0) return;
+    st->tmp = A;
+    if(A < (double)kFltMin) {
+        st->done    = 1;
+        st->success = 1;
+        st->iter += 1;
+    } else {
+        st->alpha = st->rho / A;
+    }
+}
+
+// x += alpha s ; r -= alpha q ; z = M^-1 r ; partials z.r and max|r|
+__global__ void __launch_bounds__(512) k_pcg_update(const float* __restrict__ s, const float* __restrict__ q,
+                                                    const float* __restrict__ invdiag, float* __restrict__ x, float* __restrict__ r,
+                                                    float* __restrict__ z, uint32_t nSlots, const PcgState* __restrict__ st,
+                                                    double* __restrict__ pa, double* __restrict__ pb) {
+    __shared__ double sh[8];
+    if(st->done) return;
+    const float alpha = (float)st->alpha;
+    double      rz = 0, m = 0;
+    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+        const size_t o  = (size_t)slot * 512 + threadIdx.x;
+        const float  xv = x[o] + alpha * s[o];
+        const float  rv = r[o] + (-alpha) * q[o];
+        const float  zv = invdiag[o] * rv;
+        x[o]            = xv;
+        r[o]            = rv;
+        z[o]            = zv;
+        rz += (double)zv * (double)rv;
+        m = fmax(m, (double)fabsf(rv));
+    }
+    rz = block_sum<512>(rz, sh);
+    m  = block_max<512>(m, sh);
+    if(threadIdx.x == 0) {
+        pa[blockIdx.x] = rz;
+        pb[blockIdx.x] = m;
+    }
+}
+
+// res = max|r| ; stop test (PCGSolver.cpp:182-187) ; beta = rz / rho (:190-194)
+__global__ void __launch_bounds__(1024) k_pcg_beta(PcgState* st, const double* __restrict__ pa, const double* __restrict__ pb, int n) {
+    __shared__ double sh[16];
+    if(st->done) return;
+    const double A = fold(pa, n, false, sh);
+    const double B = fold(pb, n, true, sh);
+    if(threadIdx.x != 0) return;
+    st->iter += 1;
+    st->res = (float)B;
+    if((float)B < st->tol) {
+        st->done    = 1;
+        st->success = 1;
+    } else if(st->iter >= st->maxIter) {
+        st->done = 1;
+    } else {
+        st->beta = A / st->rho;
+        st->rho  = A;
+    }
+}
+
+//--------------------------------------------------------------------------------------------------
+// FP64 defect correction: r64 = b64 - A64 x64, max|r64|, max|b64|, r32 = (float) r64 (inner rhs)
+//--------------------------------------------------------------------------------------------------
+struct OuterState {
+    double res, bmax;
+};
+
+__global__ void __launch_bounds__(512) k_residual64(const int32_t* __restrict__ nbr, const double* __restrict__ x,
+                                                    const double* __restrict__ b, const double* __restrict__ diag,
+                                                    const double* __restrict__ cx, const double* __restrict__ cy,
+                                                    const double* __restrict__ cz, float* __restrict__ r32, uint32_t nSlots,
+                                                    double* __restrict__ pa, double* __restrict__ pb, int first) {
+    __shared__ double sh[8];
+    const int         l = threadIdx.x;
+    double            m = 0, bm = 0;
+    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+        const int    s  = (int)slot;
+        const size_t o  = (size_t)s * 512 + l;
+        double       ax = 0.0;
+        if(!first) {
+            const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
+            const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
+            const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
+            if(izm >= 0) ax += cz[izm] * x[izm];
+            if(iym >= 0) ax += cy[iym] * x[iym];
+            if(ixm >= 0) ax += cx[ixm] * x[ixm];
+            ax += diag[o] * x[o];
+            if(ixp >= 0) ax += cx[o] * x[ixp];
+            if(iyp >= 0) ax += cy[o] * x[iyp];
+            if(izp >= 0) ax += cz[o] * x[izp];
+        }
+        const double bv = b[o];
+        const double rv = bv - ax;
+        r32[o]          = (float)rv;
+        m               = fmax(m, fabs(rv));
+        bm              = fmax(bm, fabs(bv));
+    }
+    m  = block_max<512>(m, sh);
+    bm = block_max<512>(bm, sh);
+    if(threadIdx.x == 0) {
+        pa[blockIdx.x] = m;
+        pb[blockIdx.x] = bm;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_residual64_fin(OuterState* ost, const double* __restrict__ pa, const double* __restrict__ pb,
+                                                         int n) {
+    __shared__ double sh[16];
+    const double      M  = fold(pa, n, true, sh);
+    const double      BM = fold(pb, n, true, sh);
+    if(threadIdx.x == 0) {
+        ost->res  = M;
+        ost->bmax = BM;
+    }
+}
+
+__global__ void __launch_bounds__(512) k_accumulate64(double* __restrict__ x64, const float* __restrict__ d, uint32_t nSlots, int first) {
+    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+        const size_t o = (size_t)slot * 512 + threadIdx.x;
+        x64[o]         = first ? (double)d[o] : x64[o] + (double)d[o];
+    }
+}
+
+//==================================================================================================
+// updateProjectedVelocity (PIC_Solver.cpp:911-981), one thread per face; faces the reference does
+// not update are zeroed with valid = 0 (its serial pass, :973-980)
+//==================================================================================================
+template<class PT> struct PressView {
+    const PT*      x;
+    const int32_t* brickSlot;
+};
+
+template<class PT> __device__ __forceinline__ PT pressure_at(const PressView<PT>& pv, const GridGeom& g, int i, int j, int k) {
+    const int32_t s = pv.brickSlot[((k >> 3) * g.nby + (j >> 3)) * g.nbx + (i >> 3)];
+    return pv.x[(size_t)s * 512 + ((k & 7) * 8 + (j & 7)) * 8 + (i & 7)];
+}
+
+template<class PT>
+__global__ void __launch_bounds__(kBlock) k_update_velocity(float* __restrict__ vel, uint8_t* __restrict__ valid,
+                                                            const float* __restrict__ weight, const float* __restrict__ phi,
+                                                            PressView<PT> pv, GridGeom g, int d, float dt) {
+    const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
+    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= (size_t)sx * sy * sz) return;
+    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    const int ai = d == 0 ? i : (d == 1 ? j : k);       // index along the face normal
+    const int an = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+    bool      upd = false;
+    if(ai > 0 && ai < an && weight[t] > 0.f) {
+        const int   li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
+        const float pc = phi[lin3(i, j, k, g.nx, g.ny)];
+        const float pl = phi[lin3(li, lj, lk, g.nx, g.ny)];
+        if(pc < 0 || pl < 0) {
+            const PT prc = pc < 0 ? pressure_at(pv, g, i, j, k) : PT(0);
+            const PT prl = pl < 0 ? pressure_at(pv, g, li, lj, lk) : PT(0);
+            PT       th  = fraction_inside2((PT)pl, (PT)pc);
+            th           = th > PT(0.01) ? th : PT(0.01);  // std::max(0.01, th)
+            vel[t]       = (float)((PT)vel[t] - (PT)dt * (prc - prl) / th);
+            upd          = true;
+        }
+    }
+    if(!upd) vel[t] = 0.f;
+    valid[t] = upd ? 1 : 0;
+}
+
+//==================================================================================================
+// mapGrid2Particles with the FLIP blend (FLIP_Solver.cpp:99-133, MathHelpers.h:549-551)
+//==================================================================================================
+struct OldGrid {
+    const float *u, *v, *w;
+};
+
+__global__ void __launch_bounds__(kBlock) k_g2p(Particles P, uint32_t n, VelGrid G, OldGrid O, GridGeom g, float ratio) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+    float       ux, uy, uz;
+    vel_from_grid(G, g, gx, gy, gz, ux, uy, uz);
+    const float dx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
+    const float dy = interp3_diff(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f);
+    const float dz = interp3_diff(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz);
+    const float a  = 1.0f - ratio;
+    P.vx[p]        = a * ux + ratio * (P.vx[p] + dx);
+    P.vy[p]        = a * uy + ratio * (P.vy[p] + dy);
+    P.vz[p]        = a * uz + ratio * (P.vz[p] + dz);
+}
+
+// max |a| over an array (timestepCFL, PIC_Solver.cpp:342-349); result via atomicMax on the bits
+__global__ void __launch_bounds__(kBlock) k_maxabs(const float* __restrict__ a, size_t n, uint32_t* out) {
+    __shared__ float sh[kBlock / 64];
+    float            m = 0.f;
+    for(size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) m = fmaxf(m, fabsf(a[t]));
+    m = wave_max(m);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        for(int i = 1; i < kBlock / 64; ++i) m = fmaxf(m, sh[i]);
+        atomicMax(out, __float_as_uint(m));
+    }
+}
+
+}  // namespace bnn

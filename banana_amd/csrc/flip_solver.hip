@@ -1,0 +1,1288 @@
+// flip_solver.hip -- host side of the MI355X FLIP substep and its C ABI (include/bnn_flip.h).
+//
+// bnn::FlipSolver owns one scene's device state and queues the stage kernels of
+// FLIP_Solver<3,float>::advanceVelocity (FLIP_Solver.cpp:42-53) on its own HIP stream.
+#include "flip_kernels.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bnn_flip.h"
+
+namespace bnn {
+
+struct Error : std::runtime_error {
+    bnn_status code;
+    Error(bnn_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_OK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if(e_ != hipSuccess)                                                                             \
+            throw ::bnn::Error(e_ == hipErrorOutOfMemory ? BNN_ERR_OUT_OF_MEMORY : BNN_ERR_DEVICE,      \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+    } while(0)
+
+static inline unsigned blocks(size_t n, unsigned bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
+
+// device buffer, grows on demand (contents are not preserved across growth)
+struct DevBuf {
+    void*  p     = nullptr;
+    size_t bytes = 0;
+    DevBuf()     = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if(p) (void)hipFree(p);
+        p     = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t b) {
+        if(b <= bytes && p) return;
+        release();
+        if(b == 0) b = 256;
+        HIP_OK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template<class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct ParticleBufs {
+    DevBuf px, py, pz, vx, vy, vz, gx, gy, gz, id;
+    void   reserve(size_t n) {
+        for(DevBuf* b : {&px, &py, &pz, &vx, &vy, &vz, &gx, &gy, &gz}) b->reserve(n * 4);
+        id.reserve(n * 4);
+    }
+    Particles view() const {
+        return Particles{px.as<float>(), py.as<float>(), pz.as<float>(), vx.as<float>(), vy.as<float>(),
+                         vz.as<float>(), gx.as<float>(), gy.as<float>(), gz.as<float>(), id.as<uint32_t>()};
+    }
+};
+
+//--------------------------------------------------------------------------------------------------
+// Seeded scene generator (host): one particle per sub-cell of the grid's sub-cell lattice whose
+// centre lies inside an object, jittered inside the sub-cell; see bnn_scene_spec.
+//--------------------------------------------------------------------------------------------------
+static inline uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static inline double unit01(uint64_t seed, uint64_t id, uint32_t stream) {
+    uint64_t h = mix64(seed ^ mix64(id * 0x2545F4914F6CDD1Dull + stream));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct GridDims {
+    float bmin[3], dx;
+    int   n[3];
+};
+// ParticleSolverData.cpp:247-253 (domain grown by NExpandCells*dx) + Grid.cpp:38-55 (n = ceil)
+static GridDims grid_dims(const bnn_flip_params& p) {
+    GridDims g;
+    g.dx = p.cell_size;
+    for(int d = 0; d < 3; ++d) {
+        float ex  = g.dx * float(p.n_expand_cells);
+        g.bmin[d] = p.box_min[d] - ex;
+        float hi  = p.box_max[d] + ex;
+        g.n[d]    = (int)std::ceil((hi - g.bmin[d]) / g.dx);
+    }
+    return g;
+}
+
+static uint64_t generate_scene(const bnn_flip_params& prm, const bnn_scene_spec& sp, uint64_t seed, float* pos, float* vel,
+                               uint64_t cap) {
+    const GridDims G   = grid_dims(prm);
+    const int      ppa = sp.ppa > 0 ? sp.ppa : 2;
+    const double   h   = (double)G.dx / ppa;
+    const int64_t  N[3] = {(int64_t)G.n[0] * ppa, (int64_t)G.n[1] * ppa, (int64_t)G.n[2] * ppa};
+    const int      nb = std::min(std::max(sp.n_boxes, 0), 8), ns = std::min(std::max(sp.n_spheres, 0), 8);
+    auto in_obj = [&](int o, const double c[3]) {
+        if(o < nb) {
+            for(int d = 0; d < 3; ++d)
+                if(c[d] < (double)sp.box_min[o][d] || c[d] > (double)sp.box_max[o][d]) return false;
+            return true;
+        }
+        const int s  = o - nb;
+        double    r2 = 0;
+        for(int d = 0; d < 3; ++d) {
+            double e = c[d] - (double)sp.sphere_center[s][d];
+            r2 += e * e;
+        }
+        return r2 <= (double)sp.sphere_radius[s] * (double)sp.sphere_radius[s];
+    };
+    uint64_t count = 0;
+    for(int o = 0; o < nb + ns; ++o) {
+        int64_t lo[3], hi[3];
+        for(int d = 0; d < 3; ++d) {
+            double a, b;
+            if(o < nb) {
+                a = sp.box_min[o][d];
+                b = sp.box_max[o][d];
+            } else {
+                a = (double)sp.sphere_center[o - nb][d] - sp.sphere_radius[o - nb];
+                b = (double)sp.sphere_center[o - nb][d] + sp.sphere_radius[o - nb];
+            }
+            lo[d] = std::max<int64_t>(0, (int64_t)std::floor((a - G.bmin[d]) / h - 0.5) - 1);
+            hi[d] = std::min<int64_t>(N[d] - 1, (int64_t)std::ceil((b - G.bmin[d]) / h - 0.5) + 1);
+        }
+        for(int64_t mz = lo[2]; mz <= hi[2]; ++mz)
+            for(int64_t my = lo[1]; my <= hi[1]; ++my)
+                for(int64_t mx = lo[0]; mx <= hi[0]; ++mx) {
+                    const double c[3] = {(double)G.bmin[0] + ((double)mx + 0.5) * h, (double)G.bmin[1] + ((double)my + 0.5) * h,
+                                         (double)G.bmin[2] + ((double)mz + 0.5) * h};
+                    if(!in_obj(o, c)) continue;
+                    bool taken = false;
+                    for(int q = 0; q < o && !taken; ++q) taken = in_obj(q, c);
+                    if(taken) continue;
+                    if(pos && count < cap) {
+                        const uint64_t gid = ((uint64_t)mz * (uint64_t)N[1] + (uint64_t)my) * (uint64_t)N[0] + (uint64_t)mx;
+                        for(int d = 0; d < 3; ++d) {
+                            const double u = unit01(seed, gid, (uint32_t)d);
+                            pos[count * 3 + d] = (float)(c[d] + (u - 0.5) * (double)sp.jitter * h);
+                            double v0 = o < nb ? sp.box_vel[o][d] : sp.sphere_vel[o - nb][d];
+                            if(sp.vel_sigma > 0) {
+                                const double u1 = unit01(seed, gid, 3u + 2u * d), u2 = unit01(seed, gid, 4u + 2u * d);
+                                v0 += (double)sp.vel_sigma * (std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(6.283185307179586 * u2));
+                            }
+                            vel[count * 3 + d] = (float)v0;
+                        }
+                    }
+                    ++count;
+                }
+    }
+    return count;
+}
+
+//--------------------------------------------------------------------------------------------------
+class FlipSolver {
+public:
+    bnn_flip_params prm;
+    GridGeom        g{};
+    BoxDev          dom{};
+    float           grav[3] = {0, -9.81f, 0};
+    std::string     lastError;
+    hipStream_t     stream = nullptr;
+
+    // particles
+    uint64_t     np = 0, cap = 0;
+    ParticleBufs bufA, bufB;
+    ParticleBufs *cur = &bufA, *alt = &bufB;
+    DevBuf       key, perm;
+    bool         binned = false;  // cell_start / g* valid for the current particle positions
+
+    // grid
+    size_t nU = 0, nV = 0, nW = 0, nC = 0, nN = 0;
+    DevBuf u, v, w, ut, vt, wt, uo, vo, wo, wu, wv, ww;
+    DevBuf vu, vv, vw, vtmp, eu, ew;
+    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters;
+    bool     weightsDirty    = true;   // boundary SDF changed since the weights were computed
+    bool     weightsComputed = false;  // computeCellWeights has run at least once
+    bool     bsdfNonzero     = false;  // the boundary SDF has been computed or set
+    uint32_t solidFaces      = 0;      // faces with weight < FLT_MIN in the current weights
+
+    // bricks + pressure system
+    uint32_t nb = 0, nSlots = 0, slotCap = 0, nFluid = 0;
+    DevBuf   bflag, bscan, brickSlot, slotBrick, nbrTab;
+    DevBuf   x, r, z, s0, s1, q, rhs32, diag32, cx32, cy32, cz32, invd;
+    DevBuf   rhs64, diag64, cx64, cy64, cz64, x64;
+    DevBuf   pa, pb, pst, ost;
+    PcgState*   hostSt  = nullptr;  // pinned
+    OuterState* hostOst = nullptr;
+    uint32_t*   hostU32 = nullptr;
+    bool        systemValid = false;
+    bool        lastRefined = false;
+
+    // stats
+    bnn_flip_stats st{};
+    double         stageAcc[BNN_NUM_STAGES] = {};
+    // frame bookkeeping (ParticleSolverData.h:43-50)
+    float    frameTime     = 0;
+    uint32_t finishedFrame = 0;
+
+    explicit FlipSolver(const bnn_flip_params& p) : prm(p) {
+        validate();
+        const GridDims G = grid_dims(prm);
+        g.nx = G.n[0];
+        g.ny = G.n[1];
+        g.nz = G.n[2];
+        g.bminx = G.bmin[0];
+        g.bminy = G.bmin[1];
+        g.bminz = G.bmin[2];
+        g.dx        = G.dx;
+        g.halfdx    = 0.5f * G.dx;
+        g.sdfRadius = G.dx * (float)(1.01 * std::sqrt(3) / 2.0);  // PIC_Data.cpp:43
+        g.nbx = (g.nx + 7) / 8;
+        g.nby = (g.ny + 7) / 8;
+        g.nbz = (g.nz + 7) / 8;
+        for(int d = 0; d < 3; ++d) {
+            dom.lo[d] = prm.box_min[d];
+            dom.hi[d] = prm.box_max[d];
+        }
+        // gravity (ParticleSolverData.cpp:256-271)
+        if(prm.gravity_type == 1) {
+            const float* gd = prm.gravity_direction;
+            float        l2 = gd[0] * gd[0] + gd[1] * gd[1] + gd[2] * gd[2];
+            if(l2 < 1.1920929e-07f) {
+                prm.gravity_type = 0;
+            } else {
+                float is = 1.0f / std::sqrt(l2);
+                for(int d = 0; d < 3; ++d) grav[d] = 9.81f * (gd[d] * is);
+            }
+        }
+        if(prm.gravity_type == 0) {
+            grav[0] = 0;
+            grav[1] = -9.81f;
+            grav[2] = 0;
+        }
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        allocGrid();
+        HIP_OK(hipHostMalloc((void**)&hostSt, sizeof(PcgState), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc((void**)&hostOst, sizeof(OuterState), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc((void**)&hostU32, 64, hipHostMallocDefault));
+        if(prm.boundary_sdf_at_init) computeBoundarySDF();
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    ~FlipSolver() {
+        if(stream) (void)hipStreamSynchronize(stream);
+        for(hipEvent_t e : evPool) (void)hipEventDestroy(e);
+        if(hostSt) (void)hipHostFree(hostSt);
+        if(hostOst) (void)hipHostFree(hostOst);
+        if(hostU32) (void)hipHostFree(hostU32);
+        if(stream) (void)hipStreamDestroy(stream);
+    }
+
+    void validate() {
+        if(!(prm.cell_size > 0)) throw Error(BNN_ERR_INVALID_ARG, "CellSize must be > 0");
+        for(int d = 0; d < 3; ++d)
+            if(!(prm.box_max[d] > prm.box_min[d])) throw Error(BNN_ERR_INVALID_ARG, "SimulationDomainBox: BoxMax must exceed BoxMin");
+        if(prm.advection_steps < 1) throw Error(BNN_ERR_INVALID_ARG, "AdvectionSteps must be >= 1");
+        if(prm.correct_position) throw Error(BNN_ERR_UNSUPPORTED, "CorrectPosition (PIC_Solver.cpp:377-485) is not implemented");
+        if(prm.gravity_type < 0 || prm.gravity_type > 3) throw Error(BNN_ERR_INVALID_ARG, "GravityType must be 0..3");
+        if(prm.preconditioner != BNN_PRECOND_JACOBI) throw Error(BNN_ERR_UNSUPPORTED, "only the Jacobi preconditioner is built");
+        const GridDims G = grid_dims(prm);
+        const double   C = (double)G.n[0] * G.n[1] * G.n[2];
+        if(C > 1.0e9 || G.n[0] < 3 || G.n[1] < 3 || G.n[2] < 3) throw Error(BNN_ERR_INVALID_ARG, "grid size out of range");
+    }
+
+    void allocGrid() {
+        nU = (size_t)(g.nx + 1) * g.ny * g.nz;
+        nV = (size_t)g.nx * (g.ny + 1) * g.nz;
+        nW = (size_t)g.nx * g.ny * (g.nz + 1);
+        nC = (size_t)g.nx * g.ny * g.nz;
+        nN = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+        nb = (uint32_t)(g.nbx * g.nby * g.nbz);
+        for(auto* b : {&u, &ut, &uo, &wu}) b->reserve(nU * 4);
+        for(auto* b : {&v, &vt, &vo, &wv}) b->reserve(nV * 4);
+        for(auto* b : {&w, &wt, &wo, &ww}) b->reserve(nW * 4);
+        vu.reserve(nU);
+        vv.reserve(nV);
+        vw.reserve(nW);
+        eu.reserve(nU);
+        ev_.reserve(std::max(nV, (size_t)1));
+        ew.reserve(nW);
+        vtmp.reserve(std::max(nU, std::max(nV, nW)));
+        phi.reserve(nC * 4);
+        bsdf.reserve(nN * 4);
+        cellStart.reserve((nC + 1) * 4);
+        cellCount.reserve(nC * 4);
+        scanPart.reserve((blocks(std::max(nC, (size_t)nb), kScanTile) + 1) * 4);
+        counters.reserve(256);
+        bflag.reserve(nb * 4);
+        bscan.reserve(nb * 4);
+        brickSlot.reserve(nb * 4);
+        pst.reserve(sizeof(PcgState));
+        ost.reserve(sizeof(OuterState));
+        // zero everything the reference zero-fills at resize (PIC_Data.cpp:101-117)
+        for(auto* b : {&u, &v, &w, &ut, &vt, &wt, &uo, &vo, &wo, &wu, &wv, &ww, &vu, &vv, &vw, &eu, &ev_, &ew, &vtmp, &phi, &bsdf,
+                       &cellStart, &cellCount, &counters, &pst, &ost})
+            HIP_OK(hipMemsetAsync(b->p, 0, b->bytes, stream));
+        weightsDirty = true;
+    }
+    DevBuf ev_;  // extrapolation flags of v (named apart from the event table)
+
+    void ensureParticles(uint64_t n) {
+        if(n <= cap) return;
+        const uint64_t c = n + n / 8 + 1024;
+        bufA.reserve(c);
+        bufB.reserve(c);
+        key.reserve(c * 4);
+        perm.reserve(c * 4);
+        cap = c;
+    }
+
+    void ensureSlots(uint32_t n) {
+        if(n <= slotCap && slotCap) return;
+        const uint32_t c  = std::max<uint32_t>(n + n / 4 + 16, 64);
+        const size_t   nv = (size_t)c * 512;
+        for(auto* b : {&x, &r, &z, &s0, &s1, &q, &rhs32, &diag32, &cx32, &cy32, &cz32, &invd}) b->reserve(nv * 4);
+        slotBrick.reserve(c * 4);
+        nbrTab.reserve((size_t)c * 6 * 4);
+        pa.reserve((size_t)kMaxRedBlocks * 8);
+        pb.reserve((size_t)kMaxRedBlocks * 8);
+        if(prm.pressure_mode == BNN_PRESSURE_REFINED)
+            for(auto* b : {&rhs64, &diag64, &cx64, &cy64, &cz64, &x64}) b->reserve(nv * 8);
+        else
+            rhs64.reserve(nv * 8);
+        slotCap = c;
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    void setParticles(const float* pos, const float* vel, uint64_t n) {
+        if(n > 0xFFFFFFF0ull) throw Error(BNN_ERR_INVALID_ARG, "too many particles for 32-bit particle ids");
+        if(n && (!pos || !vel)) throw Error(BNN_ERR_INVALID_ARG, "null particle array");
+        ensureParticles(std::max<uint64_t>(n, 1));
+        std::vector<float> tmp(n);
+        std::vector<uint32_t> ids(n);
+        ParticleBufs& P = *cur;
+        DevBuf* dst[6] = {&P.px, &P.py, &P.pz, &P.vx, &P.vy, &P.vz};
+        for(int a = 0; a < 6; ++a) {
+            const float* src = a < 3 ? pos : vel;
+            for(uint64_t i = 0; i < n; ++i) tmp[i] = src[3 * i + (a % 3)];
+            if(n) HIP_OK(hipMemcpyAsync(dst[a]->p, tmp.data(), n * 4, hipMemcpyHostToDevice, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+        }
+        for(uint64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+        if(n) HIP_OK(hipMemcpyAsync(P.id.p, ids.data(), n * 4, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemsetAsync(cellCount.p, 0, cellCount.bytes, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        np     = n;
+        binned = false;
+    }
+
+    void getParticles(float* pos, float* vel) {
+        if(!np) return;
+        std::vector<float>    a(np);
+        std::vector<uint32_t> ids(np);
+        HIP_OK(hipMemcpyAsync(ids.data(), cur->id.p, np * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        const DevBuf* src[6] = {&cur->px, &cur->py, &cur->pz, &cur->vx, &cur->vy, &cur->vz};
+        for(int k = 0; k < 6; ++k) {
+            float* out = k < 3 ? pos : vel;
+            if(!out) continue;
+            HIP_OK(hipMemcpyAsync(a.data(), src[k]->p, np * 4, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            for(uint64_t i = 0; i < np; ++i) out[3 * (uint64_t)ids[i] + (k % 3)] = a[i];
+        }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // timing: HIP event pairs recorded on the solver stream, read once at the end of a step.
+    // tag < BNN_NUM_STAGES: a stage; tag >= 100: kernel timer (tag - 100)
+    struct TimerRec {
+        int    tag;
+        size_t b, e;
+    };
+    std::vector<hipEvent_t> evPool;
+    size_t                  evUsed = 0;
+    std::vector<TimerRec>   timerRecs;
+    double                  ktAcc[BNN_NUM_KERNEL_TIMERS] = {};
+    uint64_t                ktCnt[BNN_NUM_KERNEL_TIMERS] = {};
+    size_t recordEvent() {
+        if(evUsed == evPool.size()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            evPool.push_back(e);
+        }
+        HIP_OK(hipEventRecord(evPool[evUsed], stream));
+        return evUsed++;
+    }
+    void openTimer(int tag) { timerRecs.push_back(TimerRec{tag, recordEvent(), (size_t)-1}); }
+    void closeTimer(int tag) {
+        for(size_t i = timerRecs.size(); i-- > 0;)
+            if(timerRecs[i].tag == tag && timerRecs[i].e == (size_t)-1) {
+                timerRecs[i].e = recordEvent();
+                return;
+            }
+    }
+    void stageBegin(int s) {
+        if(prm.timing) openTimer(s);
+    }
+    void stageEnd(int s) {
+        if(prm.timing) closeTimer(s);
+    }
+    void ktBegin(int k) {
+        if(prm.timing >= 2) openTimer(100 + k);
+    }
+    void ktEnd(int k) {
+        if(prm.timing >= 2) closeTimer(100 + k);
+    }
+    void collectTimers() {
+        if(timerRecs.empty()) return;
+        HIP_OK(hipStreamSynchronize(stream));
+        for(const auto& t : timerRecs) {
+            if(t.e == (size_t)-1) continue;
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, evPool[t.b], evPool[t.e]));
+            if(t.tag >= 100) {
+                ktAcc[t.tag - 100] += ms;
+                ktCnt[t.tag - 100] += 1;
+            } else {
+                stageAcc[t.tag] += ms;
+            }
+        }
+        timerRecs.clear();
+        evUsed = 0;
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    void scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total) {
+        const unsigned nt = blocks(n, kScanTile);
+        uint32_t*      part = scanPart.as<uint32_t>();
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, part);
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, stream, part, nt, total);
+        hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, stream, in, n, (const uint32_t*)part, out);
+    }
+
+    void computeBoundarySDF() {
+        hipLaunchKernelGGL(k_boundary_sdf, dim3(blocks(nN)), dim3(kBlock), 0, stream, bsdf.as<float>(), g, dom);
+        weightsDirty = true;
+        bsdfNonzero  = true;
+    }
+
+    // PIC_Solver.cpp:353-373
+    void moveParticles(float dt) {
+        if(!np) return;
+        VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
+        hipLaunchKernelGGL(k_move, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, g, dom, dt,
+                           prm.advection_steps, prm.reflect_velocity_at_boundary, prm.boundary_reflection_multiplier);
+        binned = false;
+    }
+
+    // cell-sorted binning: key + count, scan, scatter, per-cell ordering, permute
+    void binParticles() {
+        if(!np) {
+            HIP_OK(hipMemsetAsync(cellStart.p, 0, cellStart.bytes, stream));
+            binned = true;
+            return;
+        }
+        const ParticleBufs& P = *cur;
+        hipLaunchKernelGGL(k_cell_key, dim3(blocks(np)), dim3(kBlock), 0, stream, P.px.as<float>(), P.py.as<float>(),
+                           P.pz.as<float>(), (uint32_t)np, g, key.as<uint32_t>(), cellCount.as<uint32_t>());
+        scan(cellCount.as<uint32_t>(), cellStart.as<uint32_t>(), nC, cellStart.as<uint32_t>() + nC);
+        hipLaunchKernelGGL(k_scatter, dim3(blocks(np)), dim3(kBlock), 0, stream, key.as<uint32_t>(), (uint32_t)np,
+                           cellStart.as<uint32_t>(), cellCount.as<uint32_t>(), perm.as<uint32_t>());
+        hipLaunchKernelGGL(k_segsort, dim3(blocks(nC)), dim3(kBlock), 0, stream, cellStart.as<uint32_t>(), (uint32_t)nC,
+                           perm.as<uint32_t>());
+        hipLaunchKernelGGL(k_permute, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), alt->view(), perm.as<uint32_t>(),
+                           (uint32_t)np, g);
+        std::swap(cur, alt);
+        binned = true;
+    }
+
+    void requireBinned() {
+        if(!binned) binParticles();
+    }
+
+    // FLIP_Solver.cpp:57-95
+    void mapParticles2Grid() {
+        requireBinned();
+        FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
+        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+        ktBegin(BNN_KT_P2G);
+        hipLaunchKernelGGL(k_p2g, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(), g,
+                           out);
+        ktEnd(BNN_KT_P2G);
+    }
+
+    // PIC_Solver.cpp:985-1090: 4 layers per component, ping-pong with the temp arrays
+    void extrapolateVelocity() {
+        struct C {
+            DevBuf *a, *t, *val, *ext;
+            int     sx, sy, sz;
+        } comps[3] = {{&u, &ut, &vu, &eu, g.nx + 1, g.ny, g.nz}, {&v, &vt, &vv, &ev_, g.nx, g.ny + 1, g.nz}, {&w, &wt, &vw, &ew, g.nx, g.ny, g.nz + 1}};
+        for(auto& c : comps) {
+            const size_t n = (size_t)c.sx * c.sy * c.sz;
+            for(int layer = 0; layer < 4; ++layer) {
+                const bool even = (layer % 2) == 0;
+                float*     in   = even ? c.a->as<float>() : c.t->as<float>();
+                float*     out  = even ? c.t->as<float>() : c.a->as<float>();
+                uint8_t*   vin  = even ? c.val->as<uint8_t>() : vtmp.as<uint8_t>();
+                uint8_t*   vout = even ? vtmp.as<uint8_t>() : c.val->as<uint8_t>();
+                hipLaunchKernelGGL(k_extrap_layer, dim3(blocks(n)), dim3(kBlock), 0, stream, (const float*)in, out,
+                                   (const uint8_t*)vin, vout, c.ext->as<uint8_t>(), c.sx, c.sy, c.sz, layer == 0 ? 1 : 0);
+            }
+        }
+    }
+
+    void ensureWeights() {
+        if(!weightsDirty) return;
+        HIP_OK(hipMemsetAsync(counters.p, 0, 4, stream));
+        hipLaunchKernelGGL(k_weights, dim3(blocks(nN)), dim3(kBlock), 0, stream, (const float*)bsdf.as<float>(), g, wu.as<float>(),
+                           wv.as<float>(), ww.as<float>(), counters.as<uint32_t>());
+        HIP_OK(hipMemcpyAsync(hostU32, counters.p, 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        solidFaces      = hostU32[0];
+        weightsDirty    = false;
+        weightsComputed = true;
+    }
+
+    // PIC_Solver.cpp:1094-1115. Uses the weights as they stand: before the first pressure projection
+    // they are the zero fill of PIC_Data.cpp:109 (FLIP_Solver.cpp:45 constrains before
+    // computeCellWeights runs), so every extrapolated face qualifies then. A no-op while the boundary
+    // SDF is identically zero (zero gradient) or no face has weight < FLT_MIN.
+    void constrainGridVelocity() {
+        if(!bsdfNonzero) return;
+        if(weightsComputed && solidFaces == 0) return;
+        DevBuf* vel[3] = {&u, &v, &w};
+        DevBuf* tmp[3] = {&ut, &vt, &wt};
+        DevBuf* ext[3] = {&eu, &ev_, &ew};
+        DevBuf* wgt[3] = {&wu, &wv, &ww};
+        size_t  cnt[3] = {nU, nV, nW};
+        for(int d = 0; d < 3; ++d) {
+            VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
+            hipLaunchKernelGGL(k_constrain, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, G, g, d, (const uint8_t*)ext[d]->as<uint8_t>(),
+                               (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(), tmp[d]->as<float>());
+            HIP_OK(hipMemcpyAsync(vel[d]->p, tmp[d]->p, cnt[d] * 4, hipMemcpyDeviceToDevice, stream));
+        }
+    }
+
+    void backupGridVelocity() {  // FLIP_Data.h:48-52
+        HIP_OK(hipMemcpyAsync(uo.p, u.p, nU * 4, hipMemcpyDeviceToDevice, stream));
+        HIP_OK(hipMemcpyAsync(vo.p, v.p, nV * 4, hipMemcpyDeviceToDevice, stream));
+        HIP_OK(hipMemcpyAsync(wo.p, w.p, nW * 4, hipMemcpyDeviceToDevice, stream));
+    }
+
+    void addGravity(float dt) {  // PIC_Solver.cpp:505-529
+        if(!prm.apply_gravity) return;
+        if(prm.gravity_type == 0) {
+            const float gdt = 9.81f * dt;
+            hipLaunchKernelGGL(k_add_const, dim3(blocks(nV)), dim3(kBlock), 0, stream, v.as<float>(), nV, gdt, 1);
+        } else if(prm.gravity_type == 1) {
+            DevBuf* vel[3] = {&u, &v, &w};
+            size_t  cnt[3] = {nU, nV, nW};
+            for(int d = 0; d < 3; ++d)
+                hipLaunchKernelGGL(k_add_const, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, vel[d]->as<float>(), cnt[d], grav[d] * dt, 0);
+        } else {
+            DevBuf* vel[3] = {&u, &v, &w};
+            for(int d = 0; d < 3; ++d)
+                hipLaunchKernelGGL(k_gravity_center, dim3(blocks(nU)), dim3(kBlock), 0, stream, vel[d]->as<float>(), g, d,
+                                   prm.gravity_center[0], prm.gravity_center[1], prm.gravity_center[2], prm.gravity_type == 2 ? 1 : 0, dt);
+        }
+    }
+
+    void computeFluidSDF() {  // PIC_Solver.cpp:601-689
+        requireBinned();
+        ktBegin(BNN_KT_FLUID_SDF);
+        hipLaunchKernelGGL(k_fluid_sdf, dim3(blocks(nC)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
+                           g, (const float*)bsdf.as<float>(), phi.as<float>());
+        ktEnd(BNN_KT_FLUID_SDF);
+    }
+
+    // active bricks + matrix-free system (PIC_Solver.cpp:693-893)
+    void computeSystem(float dt) {
+        HIP_OK(hipMemsetAsync(counters.p, 0, 16, stream));
+        uint32_t* cnt = counters.as<uint32_t>();
+        hipLaunchKernelGGL(k_brick_flags, dim3(nb), dim3(512), 0, stream, (const float*)phi.as<float>(), g, bflag.as<uint32_t>(), cnt + 1);
+        scan(bflag.as<uint32_t>(), bscan.as<uint32_t>(), nb, cnt + 2);
+        HIP_OK(hipMemcpyAsync(hostU32, counters.p, 16, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        nFluid = hostU32[1];
+        nSlots = hostU32[2];
+        if(nSlots > nb) throw Error(BNN_ERR_DEVICE, "active brick count exceeds the brick grid");
+        ensureSlots(std::max<uint32_t>(nSlots, 1));
+        hipLaunchKernelGGL(k_brick_slots, dim3(blocks(nb)), dim3(kBlock), 0, stream, (const uint32_t*)bflag.as<uint32_t>(),
+                           (const uint32_t*)bscan.as<uint32_t>(), nb, brickSlot.as<int32_t>(), slotBrick.as<int32_t>());
+        if(nSlots == 0) {
+            systemValid = true;
+            return;
+        }
+        hipLaunchKernelGGL(k_brick_nbrs, dim3(blocks(nSlots)), dim3(kBlock), 0, stream, (const int32_t*)slotBrick.as<int32_t>(), nSlots,
+                           (const int32_t*)brickSlot.as<int32_t>(), g, nbrTab.as<int32_t>());
+        const bool refined = prm.pressure_mode == BNN_PRESSURE_REFINED;
+        SystemOut  S{rhs64.as<double>(), refined ? diag64.as<double>() : nullptr, refined ? cx64.as<double>() : nullptr,
+                    refined ? cy64.as<double>() : nullptr, refined ? cz64.as<double>() : nullptr, rhs32.as<float>(), diag32.as<float>(),
+                    cx32.as<float>(), cy32.as<float>(), cz32.as<float>(), invd.as<float>()};
+        VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
+        hipLaunchKernelGGL(k_assemble, dim3(nSlots), dim3(512), 0, stream, (const int32_t*)slotBrick.as<int32_t>(), g,
+                           (const float*)phi.as<float>(), G, (const float*)wu.as<float>(), (const float*)wv.as<float>(),
+                           (const float*)ww.as<float>(), dt, S);
+        systemValid = true;
+    }
+
+    unsigned redBlocks() const { return std::min<unsigned>(std::max<uint32_t>(nSlots, 1u), (unsigned)kMaxRedBlocks); }
+
+    // FP32 PCG on b -> x (PCGSolver.cpp:143-199); returns success
+    bool pcg32(const float* b, float relTol, uint32_t maxIter, uint32_t& iters, float& res, float& bmax) {
+        PcgState*      dst = pst.as<PcgState>();
+        double*        dpa = pa.as<double>();
+        double*        dpb = pb.as<double>();
+        float *        sA = s0.as<float>(), *sB = s1.as<float>();
+        const unsigned G   = redBlocks();
+        hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(512), 0, stream, b, (const float*)invd.as<float>(), x.as<float>(), r.as<float>(),
+                           z.as<float>(), sA, nSlots, dpa, dpb);
+        hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (const double*)dpb, (int)G, relTol,
+                           maxIter);
+        uint32_t chunk = 4, launched = 0;
+        for(;;) {
+            HIP_OK(hipMemcpyAsync(hostSt, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            if(hostSt->done) break;
+            // never queue more iterations than can remain
+            const uint32_t left = hostSt->maxIter > hostSt->iter ? hostSt->maxIter - hostSt->iter : 1u;
+            const uint32_t n    = std::min(chunk, left);
+            for(uint32_t c = 0; c < n; ++c) {
+                ktBegin(BNN_KT_PCG_SPMV);
+                hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(512), 0, stream, (const int32_t*)nbrTab.as<int32_t>(), (const float*)z.as<float>(),
+                                   (const float*)sA, sB, q.as<float>(), (const float*)diag32.as<float>(), (const float*)cx32.as<float>(),
+                                   (const float*)cy32.as<float>(), (const float*)cz32.as<float>(), nSlots, (const PcgState*)dst, dpa);
+                ktEnd(BNN_KT_PCG_SPMV);
+                hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (int)G);
+                ktBegin(BNN_KT_PCG_UPDATE);
+                hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(512), 0, stream, (const float*)sB, (const float*)q.as<float>(),
+                                   (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst,
+                                   dpa, dpb);
+                ktEnd(BNN_KT_PCG_UPDATE);
+                hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (const double*)dpb, (int)G);
+                std::swap(sA, sB);
+            }
+            launched += n;
+            chunk = std::min<uint32_t>(chunk * 2, 64);
+            if(launched > maxIter + 256) throw Error(BNN_ERR_DEVICE, "PCG device state stopped advancing");
+        }
+        iters = hostSt->iter;
+        res   = hostSt->res;
+        bmax  = hostSt->bmax;
+        return hostSt->success != 0;
+    }
+
+    bool solveSystem() {  // PIC_Solver.cpp:897-907
+        lastRefined = prm.pressure_mode == BNN_PRESSURE_REFINED;
+        if(nSlots == 0) {
+            st.pcg_iterations   = 0;
+            st.outer_iterations = 0;
+            st.pcg_residual     = 0;
+            st.rhs_max          = 0;
+            st.pcg_success      = 1;
+            return true;
+        }
+        if(!lastRefined) {
+            uint32_t it;
+            float    res, bmax;
+            bool     ok         = pcg32(rhs32.as<float>(), prm.cg_relative_tolerance, prm.max_cg_iteration, it, res, bmax);
+            st.pcg_iterations   = it;
+            st.outer_iterations = 1;
+            st.pcg_residual     = res;
+            st.rhs_max          = bmax;
+            st.pcg_success      = ok;
+            return ok;
+        }
+        // FP64 defect correction around FP32 PCG: r = b - A x in FP64, solve A d = r in FP32, x += d
+        const double   tol      = prm.refine_tolerance > 0 ? prm.refine_tolerance : 1e-10;
+        const float    innerTol = 1e-4f;
+        const uint32_t maxOuter = 30;
+        const unsigned G        = redBlocks();
+        uint32_t       total = 0, outer = 0;
+        bool           ok = false;
+        double         bmax = 0, res = 0;
+        for(int first = 1;; first = 0) {
+            hipLaunchKernelGGL(k_residual64, dim3(G), dim3(512), 0, stream, (const int32_t*)nbrTab.as<int32_t>(),
+                               (const double*)x64.as<double>(), (const double*)rhs64.as<double>(), (const double*)diag64.as<double>(),
+                               (const double*)cx64.as<double>(), (const double*)cy64.as<double>(), (const double*)cz64.as<double>(),
+                               rhs32.as<float>(), nSlots, pa.as<double>(), pb.as<double>(), first);
+            hipLaunchKernelGGL(k_residual64_fin, dim3(1), dim3(1024), 0, stream, ost.as<OuterState>(), (const double*)pa.as<double>(),
+                               (const double*)pb.as<double>(), (int)G);
+            HIP_OK(hipMemcpyAsync(hostOst, ost.p, sizeof(OuterState), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            if(first) {
+                bmax = hostOst->bmax;
+                if(bmax < 1.17549435e-38) {
+                    HIP_OK(hipMemsetAsync(x64.p, 0, (size_t)nSlots * 512 * 8, stream));
+                    ok  = true;
+                    res = 0;
+                    break;
+                }
+            }
+            res = hostOst->res;
+            if(res <= tol * bmax) {
+                if(first) HIP_OK(hipMemsetAsync(x64.p, 0, (size_t)nSlots * 512 * 8, stream));
+                ok = true;
+                break;
+            }
+            if(outer >= maxOuter) break;
+            uint32_t it;
+            float    r32, b32;
+            pcg32(rhs32.as<float>(), innerTol, prm.max_cg_iteration, it, r32, b32);
+            total += it;
+            hipLaunchKernelGGL(k_accumulate64, dim3(G), dim3(512), 0, stream, x64.as<double>(), (const float*)x.as<float>(), nSlots, first);
+            ++outer;
+            if(it == 0) break;  // inner solve made no progress
+        }
+        st.pcg_iterations   = total;
+        st.outer_iterations = outer;
+        st.pcg_residual     = res;
+        st.rhs_max          = bmax;
+        st.pcg_success      = ok;
+        return ok;
+    }
+
+    void updateProjectedVelocity(float dt) {  // PIC_Solver.cpp:911-981
+        DevBuf* vel[3] = {&u, &v, &w};
+        DevBuf* val[3] = {&vu, &vv, &vw};
+        DevBuf* wgt[3] = {&wu, &wv, &ww};
+        size_t  cnt[3] = {nU, nV, nW};
+        for(int d = 0; d < 3; ++d) {
+            if(lastRefined && nSlots) {
+                PressView<double> pv{x64.as<double>(), brickSlot.as<int32_t>()};
+                hipLaunchKernelGGL(k_update_velocity<double>, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, vel[d]->as<float>(),
+                                   val[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)phi.as<float>(), pv, g, d, dt);
+            } else {
+                PressView<float> pv{x.as<float>(), brickSlot.as<int32_t>()};
+                hipLaunchKernelGGL(k_update_velocity<float>, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, vel[d]->as<float>(),
+                                   val[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)phi.as<float>(), pv, g, d, dt);
+            }
+        }
+    }
+
+    bool pressureProjection(float dt) {  // PIC_Solver.cpp:533-539
+        stageBegin(BNN_STAGE_WEIGHTS);
+        ensureWeights();
+        stageEnd(BNN_STAGE_WEIGHTS);
+        stageBegin(BNN_STAGE_PHI);
+        computeFluidSDF();
+        stageEnd(BNN_STAGE_PHI);
+        stageBegin(BNN_STAGE_ASSEMBLY);
+        computeSystem(dt);
+        stageEnd(BNN_STAGE_ASSEMBLY);
+        stageBegin(BNN_STAGE_PCG);
+        bool ok = solveSystem();
+        stageEnd(BNN_STAGE_PCG);
+        stageBegin(BNN_STAGE_UPDATE);
+        updateProjectedVelocity(dt);
+        stageEnd(BNN_STAGE_UPDATE);
+        return ok;
+    }
+
+    void mapGrid2Particles() {  // FLIP_Solver.cpp:99-133
+        if(!np) return;
+        requireBinned();
+        VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
+        OldGrid O{uo.as<float>(), vo.as<float>(), wo.as<float>()};
+        hipLaunchKernelGGL(k_g2p, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, O, g, prm.pic_flip_ratio);
+    }
+
+    bool advanceVelocity(float dt) {  // FLIP_Solver.cpp:42-53
+        stageBegin(BNN_STAGE_BIN);
+        requireBinned();
+        stageEnd(BNN_STAGE_BIN);
+        stageBegin(BNN_STAGE_P2G);
+        mapParticles2Grid();
+        stageEnd(BNN_STAGE_P2G);
+        stageBegin(BNN_STAGE_EXTRAPOLATE);
+        extrapolateVelocity();
+        stageEnd(BNN_STAGE_EXTRAPOLATE);
+        stageBegin(BNN_STAGE_CONSTRAIN);
+        constrainGridVelocity();
+        backupGridVelocity();
+        addGravity(dt);
+        stageEnd(BNN_STAGE_CONSTRAIN);
+        bool ok = pressureProjection(dt);
+        stageBegin(BNN_STAGE_EXTRAPOLATE);
+        extrapolateVelocity();
+        stageEnd(BNN_STAGE_EXTRAPOLATE);
+        stageBegin(BNN_STAGE_CONSTRAIN);
+        constrainGridVelocity();
+        stageEnd(BNN_STAGE_CONSTRAIN);
+        stageBegin(BNN_STAGE_G2P);
+        mapGrid2Particles();
+        stageEnd(BNN_STAGE_G2P);
+        return ok;
+    }
+
+    void beginStats() {
+        for(double& a : stageAcc) a = 0;
+        for(double& a : ktAcc) a = 0;
+        for(uint64_t& c : ktCnt) c = 0;
+    }
+    void endStats(float dt) {
+        HIP_OK(hipStreamSynchronize(stream));
+        collectTimers();
+        for(int s = 0; s < BNN_NUM_STAGES; ++s) st.stage_ms[s] = prm.timing ? stageAcc[s] : 0.0;
+        for(int k = 0; k < BNN_NUM_KERNEL_TIMERS; ++k) {
+            st.kernel_ms[k]       = ktAcc[k];
+            st.kernel_launches[k] = ktCnt[k];
+        }
+        // algorithmic (compulsory) bytes per stage
+        const double Np = (double)np, C = (double)nC, F = (double)(nU + nV + nW), V = (double)nN, A = (double)nFluid;
+        st.stage_bytes[BNN_STAGE_MOVE]        = 24 * Np + 4 * F;
+        st.stage_bytes[BNN_STAGE_BIN]         = 72 * Np + 12 * C;
+        st.stage_bytes[BNN_STAGE_P2G]         = 24 * Np + 5 * F;
+        st.stage_bytes[BNN_STAGE_EXTRAPOLATE] = 2 * 11 * F;
+        st.stage_bytes[BNN_STAGE_CONSTRAIN]   = 8 * F + 8 * F / 3;
+        st.stage_bytes[BNN_STAGE_WEIGHTS]     = 0;
+        st.stage_bytes[BNN_STAGE_PHI]         = 12 * Np + 13 * C + 4 * V;
+        st.stage_bytes[BNN_STAGE_ASSEMBLY]    = 9 * C + 4 * C + 8 * F + 24 * A;
+        st.stage_bytes[BNN_STAGE_PCG]         = 64 * A * st.pcg_iterations;
+        st.stage_bytes[BNN_STAGE_UPDATE]      = 4 * C + 4 * A + 13 * F;
+        st.stage_bytes[BNN_STAGE_G2P]         = 36 * Np + 8 * F + 12 * F;
+        st.stage_bytes[BNN_STAGE_OTHER]       = 0;
+        st.num_particles     = np;
+        st.num_cells         = nC;
+        st.num_active_cells  = nFluid;
+        st.num_active_bricks = nSlots;
+        st.last_substep      = dt;
+    }
+
+    bool step(float dt) {  // PIC_Solver.cpp:292-294 (moveParticles, advanceVelocity)
+        if(!(dt > 0)) throw Error(BNN_ERR_INVALID_ARG, "dt must be > 0");
+        beginStats();
+        stageBegin(BNN_STAGE_MOVE);
+        moveParticles(dt);
+        stageEnd(BNN_STAGE_MOVE);
+        stageBegin(BNN_STAGE_BIN);
+        binParticles();
+        stageEnd(BNN_STAGE_BIN);
+        bool ok = advanceVelocity(dt);
+        endStats(dt);
+        ++st.steps;
+        return ok;
+    }
+
+    float timestepCFL() {  // PIC_Solver.cpp:342-349
+        HIP_OK(hipMemsetAsync(counters.p, 0, 4, stream));
+        DevBuf* vel[3] = {&u, &v, &w};
+        size_t  cnt[3] = {nU, nV, nW};
+        for(int d = 0; d < 3; ++d)
+            hipLaunchKernelGGL(k_maxabs, dim3(std::min<unsigned>(blocks(cnt[d]), 2048)), dim3(kBlock), 0, stream, (const float*)vel[d]->as<float>(),
+                               cnt[d], counters.as<uint32_t>());
+        HIP_OK(hipMemcpyAsync(hostU32, counters.p, 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        float m;
+        std::memcpy(&m, hostU32, 4);
+        float maxVel = std::max(kFltMin, m);
+        float ts     = maxVel > kFltMin ? prm.cfl_factor * (g.dx / maxVel) : 3.40282347e+38f;
+        return std::min(std::max(ts, prm.min_timestep), prm.max_timestep);
+    }
+
+    bool advanceFrame() {  // PIC_Solver.cpp:267-311
+        const float fd = prm.frame_duration;
+        frameTime      = 0;
+        uint32_t count = 0;
+        bool     ok    = true;
+        float    last  = 0;
+        while(frameTime < fd) {
+            float sub = timestepCFL();
+            float rem = fd - frameTime;
+            if(frameTime + sub >= fd) sub = rem;
+            else if(frameTime + 1.5f * sub >= fd) sub = rem * 0.5f;
+            ok &= step(sub);
+            last = sub;
+            frameTime += sub;
+            ++count;
+        }
+        ++finishedFrame;
+        st.substep_count = count;
+        st.last_substep  = last;
+        return ok;
+    }
+
+    // readback of a per-cell pressure-system vector in the reference's activeCellIdx order
+    template<class T> void gatherActive(const DevBuf& vec, double* dst, uint64_t* nActive) {
+        std::vector<float>   ph(nC);
+        std::vector<int32_t> bs(nb);
+        HIP_OK(hipMemcpyAsync(ph.data(), phi.p, nC * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(bs.data(), brickSlot.p, nb * 4, hipMemcpyDeviceToHost, stream));
+        std::vector<T> xv((size_t)nSlots * 512);
+        if(nSlots) HIP_OK(hipMemcpyAsync(xv.data(), vec.p, xv.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        uint64_t cnt = 0;
+        for(int k = 0; k < g.nz; ++k)
+            for(int j = 0; j < g.ny; ++j)
+                for(int i = 0; i < g.nx; ++i) {
+                    if(!(ph[((size_t)k * g.ny + j) * g.nx + i] < 0)) continue;
+                    if(dst) {
+                        const int32_t s = bs[((k >> 3) * g.nby + (j >> 3)) * g.nbx + (i >> 3)];
+                        dst[cnt]        = s < 0 ? 0.0 : (double)xv[(size_t)s * 512 + ((k & 7) * 8 + (j & 7)) * 8 + (i & 7)];
+                    }
+                    ++cnt;
+                }
+        if(nActive) *nActive = cnt;
+    }
+
+    void getGrid(int f, void* dst) {
+        if(!dst) throw Error(BNN_ERR_INVALID_ARG, "null destination");
+        const DevBuf* src = nullptr;
+        size_t        bytes = 0;
+        switch(f) {
+            case BNN_FIELD_U: src = &u; bytes = nU * 4; break;
+            case BNN_FIELD_V: src = &v; bytes = nV * 4; break;
+            case BNN_FIELD_W: src = &w; bytes = nW * 4; break;
+            case BNN_FIELD_WEIGHT_U: src = &wu; bytes = nU * 4; break;
+            case BNN_FIELD_WEIGHT_V: src = &wv; bytes = nV * 4; break;
+            case BNN_FIELD_WEIGHT_W: src = &ww; bytes = nW * 4; break;
+            case BNN_FIELD_VALID_U: src = &vu; bytes = nU; break;
+            case BNN_FIELD_VALID_V: src = &vv; bytes = nV; break;
+            case BNN_FIELD_VALID_W: src = &vw; bytes = nW; break;
+            case BNN_FIELD_EXTRAP_U: src = &eu; bytes = nU; break;
+            case BNN_FIELD_EXTRAP_V: src = &ev_; bytes = nV; break;
+            case BNN_FIELD_EXTRAP_W: src = &ew; bytes = nW; break;
+            case BNN_FIELD_OLD_U: src = &uo; bytes = nU * 4; break;
+            case BNN_FIELD_OLD_V: src = &vo; bytes = nV * 4; break;
+            case BNN_FIELD_OLD_W: src = &wo; bytes = nW * 4; break;
+            case BNN_FIELD_PHI: src = &phi; bytes = nC * 4; break;
+            case BNN_FIELD_BOUNDARY_SDF: src = &bsdf; bytes = nN * 4; break;
+            default: throw Error(BNN_ERR_INVALID_ARG, "unknown grid field");
+        }
+        HIP_OK(hipMemcpyAsync(dst, src->p, bytes, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    void setGrid(int f, const void* src) {
+        if(!src) throw Error(BNN_ERR_INVALID_ARG, "null source");
+        DevBuf* dst   = nullptr;
+        size_t  bytes = 0;
+        switch(f) {
+            case BNN_FIELD_U: dst = &u; bytes = nU * 4; break;
+            case BNN_FIELD_V: dst = &v; bytes = nV * 4; break;
+            case BNN_FIELD_W: dst = &w; bytes = nW * 4; break;
+            case BNN_FIELD_VALID_U: dst = &vu; bytes = nU; break;
+            case BNN_FIELD_VALID_V: dst = &vv; bytes = nV; break;
+            case BNN_FIELD_VALID_W: dst = &vw; bytes = nW; break;
+            case BNN_FIELD_EXTRAP_U: dst = &eu; bytes = nU; break;
+            case BNN_FIELD_EXTRAP_V: dst = &ev_; bytes = nV; break;
+            case BNN_FIELD_EXTRAP_W: dst = &ew; bytes = nW; break;
+            case BNN_FIELD_OLD_U: dst = &uo; bytes = nU * 4; break;
+            case BNN_FIELD_OLD_V: dst = &vo; bytes = nV * 4; break;
+            case BNN_FIELD_OLD_W: dst = &wo; bytes = nW * 4; break;
+            case BNN_FIELD_PHI: dst = &phi; bytes = nC * 4; break;
+            case BNN_FIELD_BOUNDARY_SDF: dst = &bsdf; bytes = nN * 4; weightsDirty = true; bsdfNonzero = true; break;
+            default: throw Error(BNN_ERR_INVALID_ARG, "field cannot be set");
+        }
+        HIP_OK(hipMemcpyAsync(dst->p, src, bytes, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    void getBins(uint32_t* permOut, uint32_t* startOut) {
+        if(!binned) throw Error(BNN_ERR_STATE, "no binning for the current particle positions");
+        if(permOut && np) HIP_OK(hipMemcpyAsync(permOut, perm.p, np * 4, hipMemcpyDeviceToHost, stream));
+        if(startOut) HIP_OK(hipMemcpyAsync(startOut, cellStart.p, (nC + 1) * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+};
+
+}  // namespace bnn
+
+//==================================================================================================
+// C ABI
+//==================================================================================================
+struct bnn_flip {
+    std::unique_ptr<bnn::FlipSolver> s;
+    std::string                      err;
+};
+
+namespace {
+template<class F> bnn_status guard(bnn_flip* h, F&& f) {
+    if(!h || !h->s) return BNN_ERR_INVALID_ARG;
+    try {
+        h->err.clear();
+        return f(*h->s);
+    } catch(const bnn::Error& e) {
+        h->err = e.what();
+        return e.code;
+    } catch(const std::bad_alloc&) {
+        h->err = "host out of memory";
+        return BNN_ERR_OUT_OF_MEMORY;
+    } catch(const std::exception& e) {
+        h->err = e.what();
+        return BNN_ERR_DEVICE;
+    }
+}
+thread_local std::string g_createError;
+}  // namespace
+
+extern "C" {
+
+void bnn_flip_default_params(bnn_flip_params* p) {
+    if(!p) return;
+    std::memset(p, 0, sizeof(*p));
+    for(int d = 0; d < 3; ++d) {
+        p->box_min[d] = -1.f;  // SimulationParameters::domainBMin default (ParticleSolverData.h:103-104)
+        p->box_max[d] = 1.f;
+    }
+    p->cell_size                      = 1.0f / 64.0f;
+    p->ratio_cell_size_pradius        = 2.0f;
+    p->n_expand_cells                 = 2;
+    p->min_timestep                   = 1.0e-6f;
+    p->max_timestep                   = 1.0f / 30.0f;
+    p->cfl_factor                     = 1.0f;
+    p->frame_duration                 = (float)(1.0 / 30);
+    p->cg_relative_tolerance          = 1e-15f;
+    p->max_cg_iteration               = 5000;
+    p->pic_flip_ratio                 = 0.97f;
+    p->advection_steps                = 1;
+    p->gravity_type                   = 0;
+    p->apply_gravity                  = 1;
+    p->boundary_reflection_multiplier = 1.0f;
+    p->sort_frequency                 = 10;
+    p->pressure_mode                  = BNN_PRESSURE_FP32;
+    p->refine_tolerance               = 1e-10;
+    p->preconditioner                 = BNN_PRECOND_JACOBI;
+    p->timing                         = 0;
+}
+
+bnn_status bnn_flip_create(const bnn_flip_params* p, bnn_flip** out) {
+    if(!p || !out) return BNN_ERR_INVALID_ARG;
+    *out = nullptr;
+    try {
+        auto* h = new bnn_flip();
+        h->s    = std::make_unique<bnn::FlipSolver>(*p);
+        *out    = h;
+        return BNN_OK;
+    } catch(const bnn::Error& e) {
+        g_createError = e.what();
+        return e.code;
+    } catch(const std::exception& e) {
+        g_createError = e.what();
+        return BNN_ERR_DEVICE;
+    }
+}
+
+bnn_status bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_flip** out) {
+    (void)json_text;
+    (void)seed;
+    if(out) *out = nullptr;
+    g_createError = "scene JSON loading is not built yet";
+    return BNN_ERR_UNSUPPORTED;
+}
+
+bnn_status bnn_flip_destroy(bnn_flip* h) {
+    if(!h) return BNN_ERR_INVALID_ARG;
+    try {
+        delete h;
+    } catch(...) {
+        return BNN_ERR_DEVICE;
+    }
+    return BNN_OK;
+}
+
+const char* bnn_flip_last_error(const bnn_flip* h) { return h ? h->err.c_str() : g_createError.c_str(); }
+
+bnn_status bnn_flip_get_params(const bnn_flip* h, bnn_flip_params* p) {
+    if(!h || !h->s || !p) return BNN_ERR_INVALID_ARG;
+    *p = h->s->prm;
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_grid_dims(const bnn_flip* h, int32_t n[3]) {
+    if(!h || !h->s || !n) return BNN_ERR_INVALID_ARG;
+    n[0] = h->s->g.nx;
+    n[1] = h->s->g.ny;
+    n[2] = h->s->g.nz;
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_grid_origin(const bnn_flip* h, float bmin[3], float* cell_size) {
+    if(!h || !h->s || !bmin || !cell_size) return BNN_ERR_INVALID_ARG;
+    bmin[0]    = h->s->g.bminx;
+    bmin[1]    = h->s->g.bminy;
+    bmin[2]    = h->s->g.bminz;
+    *cell_size = h->s->g.dx;
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_set_particles(bnn_flip* h, const float* pos, const float* vel, uint64_t n) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.setParticles(pos, vel, n);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_generate_scene_host(const bnn_flip_params* p, const bnn_scene_spec* spec, uint64_t seed, float* pos, float* vel,
+                                   uint64_t cap, uint64_t* n) {
+    if(!p || !spec || !n) return BNN_ERR_INVALID_ARG;
+    if((pos == nullptr) != (vel == nullptr)) return BNN_ERR_INVALID_ARG;
+    *n = bnn::generate_scene(*p, *spec, seed, pos, vel, pos ? cap : 0);
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_generate_scene(bnn_flip* h, const bnn_scene_spec* spec, uint64_t seed) {
+    if(!spec) return BNN_ERR_INVALID_ARG;
+    return guard(h, [&](bnn::FlipSolver& s) {
+        const uint64_t     n = bnn::generate_scene(s.prm, *spec, seed, nullptr, nullptr, 0);
+        std::vector<float> pos(n * 3), vel(n * 3);
+        bnn::generate_scene(s.prm, *spec, seed, pos.data(), vel.data(), n);
+        s.setParticles(pos.data(), vel.data(), n);
+        return BNN_OK;
+    });
+}
+
+uint64_t bnn_flip_num_particles(const bnn_flip* h) { return (h && h->s) ? h->s->np : 0; }
+
+bnn_status bnn_flip_get_particles(bnn_flip* h, float* pos, float* vel) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.getParticles(pos, vel);
+        return BNN_OK;
+    });
+}
+
+static bnn_status solve_status(const bnn::FlipSolver& s, bool ok) {
+    return (!ok && s.prm.exit_if_pressure_projection_failed) ? BNN_ERR_NOT_CONVERGED : BNN_OK;
+}
+
+bnn_status bnn_flip_step(bnn_flip* h, float dt) {
+    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, s.step(dt)); });
+}
+
+bnn_status bnn_flip_advance_frame(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, s.advanceFrame()); });
+}
+
+bnn_status bnn_flip_do_frame_simulation(bnn_flip* h, uint32_t frame) {
+    (void)frame;
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(s.prm.enable_sort_particle) throw bnn::Error(BNN_ERR_UNSUPPORTED, "EnableSortParticle (z-sort) is not built yet");
+        return solve_status(s, s.advanceFrame());
+    });
+}
+
+bnn_status bnn_flip_timestep_cfl(bnn_flip* h, float* dt) {
+    if(!dt) return BNN_ERR_INVALID_ARG;
+    return guard(h, [&](bnn::FlipSolver& s) {
+        *dt = s.timestepCFL();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_move_particles(bnn_flip* h, float dt) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.moveParticles(dt);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_sort_particles(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!s.prm.enable_sort_particle) return BNN_OK;  // PIC_Solver.cpp:316-318
+        throw bnn::Error(BNN_ERR_UNSUPPORTED, "EnableSortParticle (z-sort) is not built yet");
+    });
+}
+
+bnn_status bnn_flip_advance_velocity(bnn_flip* h, float dt) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.beginStats();
+        bool ok = s.advanceVelocity(dt);
+        s.endStats(dt);
+        return solve_status(s, ok);
+    });
+}
+
+bnn_status bnn_flip_map_particles_to_grid(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.mapParticles2Grid();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_map_grid_to_particles(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.mapGrid2Particles();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_extrapolate_velocity(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.extrapolateVelocity();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_constrain_grid_velocity(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.constrainGridVelocity();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_pressure_projection(bnn_flip* h, float dt) {
+    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, s.pressureProjection(dt)); });
+}
+
+bnn_status bnn_flip_compute_fluid_sdf(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.computeFluidSDF();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_compute_cell_weights(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.weightsDirty = true;
+        s.ensureWeights();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_bin_particles(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.binParticles();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_grid(bnn_flip* h, int32_t field, void* dst) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.getGrid(field, dst);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_pressure(bnn_flip* h, double* dst, uint64_t* n_active) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!s.systemValid) throw bnn::Error(BNN_ERR_STATE, "no pressure solve yet");
+        if(s.lastRefined) s.gatherActive<double>(s.x64, dst, n_active);
+        else s.gatherActive<float>(s.x, dst, n_active);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_rhs(bnn_flip* h, double* dst, uint64_t* n_active) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!s.systemValid) throw bnn::Error(BNN_ERR_STATE, "no pressure system yet");
+        s.gatherActive<double>(s.rhs64, dst, n_active);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_bins(bnn_flip* h, uint32_t* perm, uint32_t* cell_start) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.getBins(perm, cell_start);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_stats(bnn_flip* h, bnn_flip_stats* out) {
+    if(!out) return BNN_ERR_INVALID_ARG;
+    return guard(h, [&](bnn::FlipSolver& s) {
+        *out = s.st;
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_set_grid(bnn_flip* h, int32_t field, const void* src) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.setGrid(field, src);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_synchronize(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        HIP_OK(hipStreamSynchronize(s.stream));
+        return BNN_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,239 @@
+"""FLIP_Solver-shaped Python view of one device-resident scene (a thin wrapper on include/bnn_flip.h).
+
+Method names follow the reference's override points (PIC_Solver.h:64-70, FLIP_Solver.h:49-52,
+ParticleSolver.h:51-64); every call goes through libbnn_flip.so to the gfx950 kernels.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD, BnnError, Params, SceneSpec, Stats
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_params(d=None, **kw):
+    """bnn_flip_params from the library defaults (Constants.h:31-45) overridden by d / kw."""
+    L = _lib.load()
+    p = Params()
+    L.bnn_flip_default_params(C.byref(p))
+    for k, v in dict(d or {}, **kw).items():
+        if isinstance(v, (list, tuple, np.ndarray)):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def make_spec(d):
+    s = SceneSpec()
+    s.ppa = d.get("ppa", 2)
+    s.jitter = d.get("jitter", 1.0)
+    s.vel_sigma = d.get("vel_sigma", 0.0)
+    boxes = d.get("boxes", [])
+    s.n_boxes = len(boxes)
+    for i, b in enumerate(boxes):
+        for a in range(3):
+            s.box_min[i][a] = b["min"][a]
+            s.box_max[i][a] = b["max"][a]
+            s.box_vel[i][a] = b.get("vel", (0, 0, 0))[a]
+    sph = d.get("spheres", [])
+    s.n_spheres = len(sph)
+    for i, b in enumerate(sph):
+        s.sphere_radius[i] = b["radius"]
+        for a in range(3):
+            s.sphere_center[i][a] = b["center"][a]
+            s.sphere_vel[i][a] = b.get("vel", (0, 0, 0))[a]
+    return s
+
+
+def generate_scene_host(params, spec, seed):
+    """Host-side seeded generator (bnn_generate_scene_host); returns (pos, vel) as (n,3) float32."""
+    L = _lib.load()
+    p = params if isinstance(params, Params) else make_params(params)
+    s = spec if isinstance(spec, SceneSpec) else make_spec(spec)
+    n = C.c_uint64()
+    _check(L.bnn_generate_scene_host(C.byref(p), C.byref(s), seed, None, None, 0, C.byref(n)), None)
+    pos = np.zeros((n.value, 3), np.float32)
+    vel = np.zeros((n.value, 3), np.float32)
+    _check(L.bnn_generate_scene_host(C.byref(p), C.byref(s), seed, _ptr(pos), _ptr(vel), n.value, C.byref(n)), None)
+    return pos, vel
+
+
+def _check(status, h):
+    if status != _lib.BNN_OK:
+        L = _lib.load()
+        msg = L.bnn_flip_last_error(h)
+        raise BnnError(status, msg.decode() if msg else "")
+
+
+class FLIPSolver:
+    """One scene resident in HBM. `params` is a dict of bnn_flip_params fields (reference JSON key
+    meanings) or a Params struct."""
+
+    def __init__(self, params, **kw):
+        self.L = _lib.load()
+        self._p = params if isinstance(params, Params) else make_params(params, **kw)
+        h = C.c_void_p()
+        st = self.L.bnn_flip_create(C.byref(self._p), C.byref(h))
+        if st != _lib.BNN_OK:
+            raise BnnError(st, (self.L.bnn_flip_last_error(None) or b"").decode())
+        self.h = h
+        n = np.zeros(3, np.int32)
+        self.L.bnn_flip_grid_dims(self.h, _ptr(n))
+        self.n = tuple(int(x) for x in n)
+        bmin = np.zeros(3, np.float32)
+        dx = C.c_float()
+        self.L.bnn_flip_grid_origin(self.h, _ptr(bmin), C.byref(dx))
+        self.bmin, self.dx = bmin, np.float32(dx.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bnn_flip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, status):
+        _check(status, self.h)
+        return status
+
+    # --- scene ------------------------------------------------------------------------------
+    def set_particles(self, pos, vel):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        assert pos.shape == vel.shape and pos.ndim == 2 and pos.shape[1] == 3
+        self._c(self.L.bnn_flip_set_particles(self.h, _ptr(pos), _ptr(vel), pos.shape[0]))
+
+    def generate_scene(self, spec, seed):
+        s = spec if isinstance(spec, SceneSpec) else make_spec(spec)
+        self._c(self.L.bnn_flip_generate_scene(self.h, C.byref(s), seed))
+
+    def num_particles(self):
+        return int(self.L.bnn_flip_num_particles(self.h))
+
+    def particles(self):
+        n = self.num_particles()
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        self._c(self.L.bnn_flip_get_particles(self.h, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    # --- stepping (PIC_Solver::advanceFrame and its override points) ---------------------------
+    def step(self, dt):
+        return self._c(self.L.bnn_flip_step(self.h, dt))
+
+    def advance_frame(self):
+        return self._c(self.L.bnn_flip_advance_frame(self.h))
+
+    def do_frame_simulation(self, frame):
+        return self._c(self.L.bnn_flip_do_frame_simulation(self.h, frame))
+
+    def timestep_cfl(self):
+        dt = C.c_float()
+        self._c(self.L.bnn_flip_timestep_cfl(self.h, C.byref(dt)))
+        return float(dt.value)
+
+    def move_particles(self, dt):
+        self._c(self.L.bnn_flip_move_particles(self.h, dt))
+
+    def sort_particles(self):
+        self._c(self.L.bnn_flip_sort_particles(self.h))
+
+    def advance_velocity(self, dt):
+        return self._c(self.L.bnn_flip_advance_velocity(self.h, dt))
+
+    def map_particles_to_grid(self):
+        self._c(self.L.bnn_flip_map_particles_to_grid(self.h))
+
+    def map_grid_to_particles(self):
+        self._c(self.L.bnn_flip_map_grid_to_particles(self.h))
+
+    def extrapolate_velocity(self):
+        self._c(self.L.bnn_flip_extrapolate_velocity(self.h))
+
+    def constrain_grid_velocity(self):
+        self._c(self.L.bnn_flip_constrain_grid_velocity(self.h))
+
+    def pressure_projection(self, dt):
+        return self._c(self.L.bnn_flip_pressure_projection(self.h, dt))
+
+    def compute_fluid_sdf(self):
+        self._c(self.L.bnn_flip_compute_fluid_sdf(self.h))
+
+    def compute_cell_weights(self):
+        self._c(self.L.bnn_flip_compute_cell_weights(self.h))
+
+    def bin_particles(self):
+        self._c(self.L.bnn_flip_bin_particles(self.h))
+
+    def synchronize(self):
+        self._c(self.L.bnn_flip_synchronize(self.h))
+
+    # --- readback ---------------------------------------------------------------------------
+    def field_shape(self, name):
+        nx, ny, nz = self.n
+        if name.endswith("u"):
+            return (nz, ny, nx + 1)
+        if name.endswith("v"):
+            return (nz, ny + 1, nx)
+        if name.endswith("w"):
+            return (nz + 1, ny, nx)
+        if name == "boundary_sdf":
+            return (nz + 1, ny + 1, nx + 1)
+        return (nz, ny, nx)
+
+    def grid(self, name):
+        dtype = np.int8 if name.startswith(("valid", "extrap")) else np.float32
+        out = np.zeros(self.field_shape(name), dtype)
+        self._c(self.L.bnn_flip_get_grid(self.h, FIELD[name], _ptr(out)))
+        return out
+
+    def set_grid(self, name, arr):
+        dtype = np.int8 if name.startswith(("valid", "extrap")) else np.float32
+        a = np.ascontiguousarray(arr, dtype)
+        assert a.shape == self.field_shape(name), (a.shape, self.field_shape(name))
+        self._c(self.L.bnn_flip_set_grid(self.h, FIELD[name], _ptr(a)))
+
+    def pressure(self):
+        n = C.c_uint64()
+        self._c(self.L.bnn_flip_get_pressure(self.h, None, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        self._c(self.L.bnn_flip_get_pressure(self.h, _ptr(out), C.byref(n)))
+        return out
+
+    def rhs(self):
+        n = C.c_uint64()
+        self._c(self.L.bnn_flip_get_rhs(self.h, None, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        self._c(self.L.bnn_flip_get_rhs(self.h, _ptr(out), C.byref(n)))
+        return out
+
+    def bins(self):
+        n = self.num_particles()
+        perm = np.zeros(n, np.uint32)
+        start = np.zeros(self.n[0] * self.n[1] * self.n[2] + 1, np.uint32)
+        self._c(self.L.bnn_flip_get_bins(self.h, _ptr(perm), _ptr(start)))
+        return perm, start
+
+    def stats(self):
+        s = Stats()
+        self._c(self.L.bnn_flip_get_stats(self.h, C.byref(s)))
+        d = {k: getattr(s, k) for k, _ in Stats._fields_ if k not in ("stage_ms", "stage_bytes", "kernel_ms", "kernel_launches")}
+        d["kernel_ms"] = {n: s.kernel_ms[i] for i, n in enumerate(_lib.KERNEL_TIMER_NAMES)}
+        d["kernel_launches"] = {n: s.kernel_launches[i] for i, n in enumerate(_lib.KERNEL_TIMER_NAMES)}
+        d["stage_ms"] = {n: s.stage_ms[i] for i, n in enumerate(_lib.STAGE_NAMES)}
+        d["stage_bytes"] = {n: s.stage_bytes[i] for i, n in enumerate(_lib.STAGE_NAMES)}
+        return d

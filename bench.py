@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""FLIP particle-steps/s on one MI355X.
+
+Workload (N=1): the large single scene -- 256^3 dam break [0,0.4]x[0,0.6]x[0,1] plus a falling
+sphere (centre (0.7,0.75,0.5), radius 0.12, v0 = (0,-2,0)), 8 particles per cell, solid walls on
+all faces, dt = 1/600, FP32 particle state, pressure by FP32 PCG under an FP64 defect-correction
+loop to 1e-10 relative residual. A step = one full FLIP substep (move, bin, P2G, extrapolate,
+constrain, gravity, fluid SDF, assembly, PCG, velocity update, extrapolate, constrain, G2P) with the
+scene resident in HBM.
+
+Prints one JSON line. `roofline` is for the dominant kernel (the PCG SpMV), from HIP events around
+every launch over the timed steps; `cpu_baseline` times the oracle (FP32, the reference's IC(0)
+PCG) on the host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FLIP particle-steps/s at 256³; served scenes·steps/s + p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes_per_cell():
+    # k_pcg_spmv per active cell: read z, s (8 B) + diag, cx, cy, cz (16 B); write s, q (8 B)
+    return 32.0
+
+
+def cpu_baseline(n, threads, steps=3):
+    """The oracle in FP32 (reference algorithm incl. serial IC(0)-PCG, threaded particle/grid loops)
+    on the n^3 version of the workload, one substep; returns (particle-steps/s, seconds, particles)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import banana_amd
+    from banana_amd import scenes
+    import oracle_lib
+
+    if not os.path.exists(oracle_lib.ORACLE_SO):
+        oracle_lib.build()
+    params, spec = scenes.dam_break(n, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
+    pos, vel = banana_amd.generate_scene_host(params, spec, 2024)
+    o = oracle_lib.Oracle(params, precision=32, threads=threads)
+    o.set_particles(pos, vel)
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(steps):
+        o.step(np.float32(1.0 / 600.0))
+        iters += o.pcg_stats()[0]
+    dt = time.perf_counter() - t0
+    return pos.shape[0] * steps / dt, dt, pos.shape[0], iters / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="grid edge in cells (256 = the headline workload)")
+    ap.add_argument("--pressure", choices=["refined", "fp32"], default="refined")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=128, help="grid edge of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="substeps of the CPU baseline sample")
+    ap.add_argument("--timing", type=int, default=2, help="0: none, 1: stage events, 2: + per-kernel events")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        log(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+            "(scenes are independent and would shard one per GPU; see the design notes)")
+        sys.exit(2)
+
+    import numpy as np
+
+    import banana_amd
+    from banana_amd import scenes
+
+    params, spec = scenes.dam_break(args.n, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
+    params.update(pressure_mode=1 if args.pressure == "refined" else 0, refine_tolerance=1e-10, timing=args.timing)
+    dt = np.float32(1.0 / 600.0)
+    t0 = time.perf_counter()
+    pos, vel = banana_amd.generate_scene_host(params, spec, 2024)
+    log(f"generated {pos.shape[0]} particles in {time.perf_counter() - t0:.1f}s")
+    sim = banana_amd.FLIPSolver(params)
+    sim.set_particles(pos, vel)
+    del pos, vel
+    log(f"grid {sim.n}, particles {sim.num_particles()}")
+
+    for i in range(args.warmup):
+        t1 = time.perf_counter()
+        sim.step(dt)
+        st = sim.stats()
+        log(f"warmup {i + 1}/{args.warmup}: {1e3 * (time.perf_counter() - t1):.1f} ms, pcg {st['pcg_iterations']} "
+            f"(outer {st['outer_iterations']}), active cells {st['num_active_cells']}")
+    sim.synchronize()
+
+    spmv_ms = 0.0
+    spmv_launches = 0
+    spmv_bytes = 0.0
+    pcg_iters = 0
+    per_step = []
+    stage_ms = {}
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        sim.step(dt)
+        st = sim.stats()  # the step has synchronized (end of step collects its events)
+        per_step.append(time.perf_counter() - ts)
+        it = st["pcg_iterations"]
+        pcg_iters += it
+        spmv_ms += st["kernel_ms"]["pcg_spmv"]
+        spmv_launches += st["kernel_launches"]["pcg_spmv"]
+        spmv_bytes += spmv_bytes_per_cell() * st["num_active_cells"] * it
+        for k, v in st["stage_ms"].items():
+            stage_ms[k] = stage_ms.get(k, 0.0) + v
+        log(f"step {i + 1}/{args.steps}: {1e3 * per_step[-1]:.1f} ms, pcg {it} (outer {st['outer_iterations']}), "
+            f"residual {st['pcg_residual']:.2e}/{st['rhs_max']:.2e}, active cells {st['num_active_cells']}")
+    sim.synchronize()
+    elapsed = time.perf_counter() - t_start
+    np_ = sim.num_particles()
+    value = np_ * args.steps / elapsed
+
+    roofline = None
+    if spmv_launches and spmv_ms > 0:
+        avg_ms = spmv_ms / spmv_launches
+        achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+        roofline = {"kernel": "k_pcg_spmv", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_us": round(avg_ms * 1e3, 2), "launches": spmv_launches,
+                    "bytes_per_launch": spmv_bytes / max(pcg_iters, 1)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = os.cpu_count() or 1
+        try:
+            aff = len(os.sched_getaffinity(0))
+            threads = min(threads, aff)
+        except Exception:
+            pass
+        threads = min(threads, 16)
+        log(f"cpu baseline: oracle FP32 on {threads} threads, {args.cpu_n}^3 ...")
+        v, secs, npc, iters = cpu_baseline(args.cpu_n, threads, args.cpu_steps)
+        cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"first {args.cpu_steps} substeps of the {args.cpu_n}^3 version of this scene ({npc} particles), "
+                         f"FP32 oracle with the reference's IC(0)-PCG at rel-tol 1e-6 ({iters:.0f} iterations/step), {secs:.1f} s"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"FLIP dam break {args.n}^3 + falling sphere, 8 particles/cell, solid walls, dt=1/600, "
+                               f"pressure {'FP32 PCG under FP64 defect correction to 1e-10' if args.pressure == 'refined' else 'FP32 PCG rel-tol 1e-6'}",
+                   "grid": list(sim.n), "particles": np_, "pcg_iterations_per_step": pcg_iters / args.steps,
+                   "preconditioner": "jacobi", "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

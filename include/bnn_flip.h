@@ -1,0 +1,231 @@
+/*
+ * bnn_flip.h -- C ABI of the MI355X (gfx950) FLIP substep: a drop-in for the hot path of
+ * Banana's FLIP_Solver<3,float> (Simulation/ParticleSolvers/PICFluid/FLIP_Solver.h:30-60,
+ * PIC_Solver.h:40-100). Plain C, opaque handles, plain pointers and sizes.
+ *
+ * Conventions
+ *  - Every call returns a bnn_status; BNN_OK == 0. On error the handle keeps a message readable
+ *    with bnn_flip_last_error(). Nothing here exits the process (the reference calls exit() from
+ *    __BNN_REQUIRE, Macros.h:225-236, and on a failed solve when ExitIfPressureProjectionFailed,
+ *    PIC_Solver.cpp:901-906; this ABI reports BNN_ERR_NOT_CONVERGED instead).
+ *  - Host arrays are AoS float triples (12 B per particle), like ParticleSimulationData::positions
+ *    (ParticleSolverData.h:197). Particle readback is always in the order of the last
+ *    bnn_flip_set_particles / bnn_flip_generate_scene call ("user order"), whatever order the
+ *    device keeps internally, unless bnn_flip_sort_particles reordered it (reference z-sort).
+ *  - A handle is used by one host thread at a time; different handles may be used concurrently.
+ *  - Grid arrays are x-fastest (Array.h:276): index ((k*ny + j)*nx + i) with the array's own dims:
+ *    u (nx+1, ny, nz), v (nx, ny+1, nz), w (nx, ny, nz+1), cell fields (nx, ny, nz),
+ *    boundary SDF nodes (nx+1, ny+1, nz+1).
+ */
+#ifndef BNN_FLIP_H
+#define BNN_FLIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bnn_status {
+    BNN_OK                = 0,
+    BNN_ERR_INVALID_ARG   = 1,
+    BNN_ERR_NOT_CONVERGED = 2, /* PCG hit MaxCGIteration or broke down (PCGSolver.cpp:164-199) */
+    BNN_ERR_DEVICE        = 3, /* HIP runtime error */
+    BNN_ERR_OUT_OF_MEMORY = 4,
+    BNN_ERR_UNSUPPORTED   = 5, /* a reference option this build does not implement */
+    BNN_ERR_STATE         = 6, /* call out of order (e.g. step before particles exist) */
+    BNN_ERR_REJECTED      = 7  /* serving: admission refused (HBM budget) */
+} bnn_status;
+
+/* Pressure solve precision (extension; the reference is FP32 throughout, Setup.h:58). */
+typedef enum bnn_pressure_mode {
+    BNN_PRESSURE_FP32    = 0, /* FP32 vectors, FP64 dot products and alpha/beta, stop on
+                                 max|r| < CGRelativeTolerance * max|rhs| (PCGSolver.cpp:154-186) */
+    BNN_PRESSURE_REFINED = 1  /* FP32 PCG inner solves under an FP64 defect-correction loop, stop
+                                 on max|b - A x| <= refine_tolerance * max|b| (FP64 residual) */
+} bnn_pressure_mode;
+
+typedef enum bnn_preconditioner {
+    BNN_PRECOND_JACOBI = 0, /* PCGSolver::JACOBI (PCGSolver.cpp:301-313) */
+    BNN_PRECOND_MG     = 1  /* geometric multigrid V-cycle (extension, see the design notes) */
+} bnn_preconditioner;
+
+/* Parameters: field names follow the reference JSON keys (ParticleSolverData.cpp:33-78, 148-211,
+ * PIC_Data.cpp:33-38); defaults from bnn_flip_default_params() equal Constants.h:31-45 and the
+ * member initialisers of ParticleSolverData.h:38-175. */
+typedef struct bnn_flip_params {
+    float    box_min[3], box_max[3];        /* SimulationDomainBox.BoxMin/BoxMax (required) */
+    float    cell_size;                     /* CellSize (1/64) */
+    float    ratio_cell_size_pradius;       /* RatioCellSizePRadius (2) */
+    uint32_t n_expand_cells;                /* NExpandCells (2) */
+    float    min_timestep, max_timestep;    /* MinTimestep (1e-6), MaxTimestep (1/30) */
+    float    cfl_factor;                    /* CFLFactor (1) */
+    float    frame_duration;                /* FrameDuration (1/30) */
+    float    cg_relative_tolerance;         /* CGRelativeTolerance (1e-15) */
+    uint32_t max_cg_iteration;              /* MaxCGIteration (5000) */
+    float    pic_flip_ratio;                /* PIC_FLIP_ratio (0.97) */
+    uint32_t advection_steps;               /* AdvectionSteps (1) */
+    int32_t  gravity_type;                  /* GravityType: 0 Earth, 1 Directional, 2 ToCenter, 3 FromCenter */
+    float    gravity_direction[3];          /* GravityDirection */
+    float    gravity_center[3];             /* GravityCenter */
+    int32_t  apply_gravity;                 /* ApplyGravity (true) */
+    int32_t  reflect_velocity_at_boundary;  /* ReflectVelocityAtBoundary (false) */
+    float    boundary_reflection_multiplier;/* BoundaryReflectionMultiplier (1) */
+    int32_t  boundary_sdf_at_init;          /* extension: compute the boundary SDF once at scene
+                                               creation (false = reference behaviour, walls act
+                                               only through particle projection) */
+    int32_t  correct_position;              /* CorrectPosition (false; true -> BNN_ERR_UNSUPPORTED) */
+    int32_t  exit_if_pressure_projection_failed; /* ExitIfPressureProjectionFailed: when set, a
+                                               failed solve makes step() return
+                                               BNN_ERR_NOT_CONVERGED instead of BNN_OK */
+    int32_t  enable_sort_particle;          /* EnableSortParticle (false) */
+    uint32_t sort_frequency;                /* SortFrequency (10) */
+    /* extensions */
+    int32_t  pressure_mode;                 /* bnn_pressure_mode (FP32) */
+    double   refine_tolerance;              /* REFINED outer tolerance (1e-10) */
+    int32_t  preconditioner;                /* bnn_preconditioner (JACOBI) */
+    int32_t  timing;                        /* 1: record per-stage HIP events (stats.stage_ms) */
+} bnn_flip_params;
+
+/* Seeded scene generator input (replaces ParticleGenerator, ParticleGenerator.hpp:87-136): one
+ * particle per sub-cell of the grid's sub-cell lattice (ppa^3 per cell) whose centre lies inside an
+ * object, jittered uniformly (fraction `jitter`) inside its sub-cell; objects in order, a sub-cell
+ * claimed by an earlier object is skipped; optional N(0, vel_sigma^2) velocity per component. */
+typedef struct bnn_scene_spec {
+    int32_t n_boxes;
+    float   box_min[8][3], box_max[8][3], box_vel[8][3];
+    int32_t n_spheres;
+    float   sphere_center[8][3], sphere_radius[8], sphere_vel[8][3];
+    float   vel_sigma;
+    int32_t ppa;
+    float   jitter;
+} bnn_scene_spec;
+
+/* Grid fields for bnn_flip_get_grid (element type in brackets). */
+typedef enum bnn_field {
+    BNN_FIELD_U = 0, BNN_FIELD_V = 1, BNN_FIELD_W = 2,                     /* [float] velocities */
+    BNN_FIELD_WEIGHT_U = 3, BNN_FIELD_WEIGHT_V = 4, BNN_FIELD_WEIGHT_W = 5, /* [float] open fraction */
+    BNN_FIELD_VALID_U = 6, BNN_FIELD_VALID_V = 7, BNN_FIELD_VALID_W = 8,   /* [int8] */
+    BNN_FIELD_EXTRAP_U = 9, BNN_FIELD_EXTRAP_V = 10, BNN_FIELD_EXTRAP_W = 11, /* [int8] */
+    BNN_FIELD_OLD_U = 12, BNN_FIELD_OLD_V = 13, BNN_FIELD_OLD_W = 14,     /* [float] backup */
+    BNN_FIELD_PHI = 15,          /* [float] fluid SDF; label fluid <=> phi < 0 */
+    BNN_FIELD_BOUNDARY_SDF = 16  /* [float] node SDF of the solid boundary */
+} bnn_field;
+
+#define BNN_NUM_STAGES 12
+/* stage order in bnn_flip_stats.stage_ms / stage_bytes */
+typedef enum bnn_stage {
+    BNN_STAGE_MOVE = 0,        /* moveParticles (RK2 + boundary projection) */
+    BNN_STAGE_BIN = 1,         /* cell keys, counting sort, payload permute */
+    BNN_STAGE_P2G = 2,         /* mapParticles2Grid */
+    BNN_STAGE_EXTRAPOLATE = 3, /* both extrapolateVelocity calls */
+    BNN_STAGE_CONSTRAIN = 4,   /* both constrainGridVelocity calls + backup + gravity */
+    BNN_STAGE_WEIGHTS = 5,     /* computeCellWeights (only when the boundary changed) */
+    BNN_STAGE_PHI = 6,         /* computeFluidSDF + labels */
+    BNN_STAGE_ASSEMBLY = 7,    /* active bricks + computeSystem */
+    BNN_STAGE_PCG = 8,         /* solveSystem */
+    BNN_STAGE_UPDATE = 9,      /* updateProjectedVelocity */
+    BNN_STAGE_G2P = 10,        /* computeChangesGridVelocity + mapGrid2Particles */
+    BNN_STAGE_OTHER = 11
+} bnn_stage;
+
+/* per-kernel timers (timing >= 2): a HIP event pair on the solver stream around every launch */
+#define BNN_NUM_KERNEL_TIMERS 4
+typedef enum bnn_kernel_timer {
+    BNN_KT_PCG_SPMV   = 0, /* s = z + beta s; q = A s; s.q */
+    BNN_KT_PCG_UPDATE = 1, /* x, r updates; max|r|; z = M^-1 r; z.r */
+    BNN_KT_P2G        = 2,
+    BNN_KT_FLUID_SDF  = 3
+} bnn_kernel_timer;
+
+typedef struct bnn_flip_stats {
+    uint64_t num_particles;
+    uint64_t num_cells;            /* nx*ny*nz */
+    uint64_t num_active_cells;     /* fluid cells (rows of the pressure system) */
+    uint64_t num_active_bricks;    /* 8^3 bricks holding at least one fluid cell */
+    uint32_t pcg_iterations;       /* total inner iterations of the last solve */
+    uint32_t outer_iterations;     /* REFINED: defect-correction passes of the last solve */
+    double   pcg_residual;         /* last max|r| (FP32 mode) or max|b - A x| (REFINED) */
+    double   rhs_max;              /* max|rhs| of the last solve */
+    int32_t  pcg_success;
+    float    last_substep;         /* substep of the last step / frame */
+    uint32_t substep_count;        /* substeps of the last advance_frame */
+    double   stage_ms[BNN_NUM_STAGES];    /* last step, HIP events on the solver stream (timing=1) */
+    double   stage_bytes[BNN_NUM_STAGES]; /* algorithmic bytes of the last step per stage */
+    uint64_t steps;                /* steps since creation */
+    double   kernel_ms[BNN_NUM_KERNEL_TIMERS];       /* summed launch durations, last step */
+    uint64_t kernel_launches[BNN_NUM_KERNEL_TIMERS]; /* launches timed, last step */
+} bnn_flip_stats;
+
+typedef struct bnn_flip bnn_flip;
+
+/* --- lifecycle (ParticleSolver::loadScene, ParticleSolver.cpp:30-168) ------------------------ */
+void        bnn_flip_default_params(bnn_flip_params* p);
+bnn_status  bnn_flip_create(const bnn_flip_params* p, bnn_flip** out);
+/* scene JSON in the reference format (GlobalParameters / SimulationParameters /
+ * ParticleGenerators with Box or Sphere geometry), ParticleSolver.cpp:30-168 */
+bnn_status  bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_flip** out);
+bnn_status  bnn_flip_destroy(bnn_flip* h);
+const char* bnn_flip_last_error(const bnn_flip* h);
+bnn_status  bnn_flip_get_params(const bnn_flip* h, bnn_flip_params* p);
+bnn_status  bnn_flip_grid_dims(const bnn_flip* h, int32_t n[3]);
+/* grid origin bMin (domain grown by NExpandCells) and cell size */
+bnn_status  bnn_flip_grid_origin(const bnn_flip* h, float bmin[3], float* cell_size);
+
+/* --- particles (PIC_Data::addParticles, PIC_Data.cpp:60-72) ------------------------------------ */
+bnn_status  bnn_flip_set_particles(bnn_flip* h, const float* pos_xyz, const float* vel_xyz, uint64_t n);
+bnn_status  bnn_flip_generate_scene(bnn_flip* h, const bnn_scene_spec* spec, uint64_t seed);
+/* host-only generator: writes up to `cap` particles, returns the count in *n (pos/vel may be NULL) */
+bnn_status  bnn_generate_scene_host(const bnn_flip_params* p, const bnn_scene_spec* spec, uint64_t seed,
+                                    float* pos_xyz, float* vel_xyz, uint64_t cap, uint64_t* n);
+uint64_t    bnn_flip_num_particles(const bnn_flip* h);
+bnn_status  bnn_flip_get_particles(bnn_flip* h, float* pos_xyz, float* vel_xyz);
+
+/* --- stepping (PIC_Solver::advanceFrame, PIC_Solver.cpp:267-311) ------------------------------ */
+/* one body of the substep loop with substep = dt: moveParticles -> advanceVelocity */
+bnn_status  bnn_flip_step(bnn_flip* h, float dt);
+/* the full advanceFrame: CFL timestep, fit-to-frame, substeps, finishedFrame += 1 */
+bnn_status  bnn_flip_advance_frame(bnn_flip* h);
+/* ParticleSolver::doFrameSimulation (ParticleSolver.cpp:200-219): sortParticles + advanceFrame */
+bnn_status  bnn_flip_do_frame_simulation(bnn_flip* h, uint32_t frame);
+/* PIC_Solver::timestepCFL (PIC_Solver.cpp:342-349) */
+bnn_status  bnn_flip_timestep_cfl(bnn_flip* h, float* dt);
+
+/* --- override points of the reference (PIC_Solver.h:64-70, FLIP_Solver.h:49-52) ------------- */
+bnn_status  bnn_flip_move_particles(bnn_flip* h, float dt);          /* PIC_Solver.cpp:353-373 */
+bnn_status  bnn_flip_sort_particles(bnn_flip* h);                     /* PIC_Solver.cpp:315-328 */
+bnn_status  bnn_flip_advance_velocity(bnn_flip* h, float dt);        /* FLIP_Solver.cpp:42-53 */
+bnn_status  bnn_flip_map_particles_to_grid(bnn_flip* h);              /* FLIP_Solver.cpp:57-95 */
+bnn_status  bnn_flip_map_grid_to_particles(bnn_flip* h);              /* FLIP_Solver.cpp:99-111 */
+bnn_status  bnn_flip_extrapolate_velocity(bnn_flip* h);               /* PIC_Solver.cpp:985-1090 */
+bnn_status  bnn_flip_constrain_grid_velocity(bnn_flip* h);            /* PIC_Solver.cpp:1094-1115 */
+bnn_status  bnn_flip_pressure_projection(bnn_flip* h, float dt);     /* PIC_Solver.cpp:533-539 */
+bnn_status  bnn_flip_compute_fluid_sdf(bnn_flip* h);                  /* PIC_Solver.cpp:601-689 */
+bnn_status  bnn_flip_compute_cell_weights(bnn_flip* h);               /* PIC_Solver.cpp:543-597 */
+/* cell-sorted binning of the current particles (Grid::collectIndexToCells +
+ * getParticleIdxSortedByCell, Grid.cpp:92-114, 283-309); permutes the device particle arrays */
+bnn_status  bnn_flip_bin_particles(bnn_flip* h);
+
+/* --- readback ------------------------------------------------------------------------------- */
+bnn_status  bnn_flip_get_grid(bnn_flip* h, int32_t field, void* dst);
+/* pressure per fluid cell in the reference's activeCellIdx order (linear x-fastest numbering of
+ * phi < 0 cells, PIC_Solver.cpp:706-712); *n_active receives the count; dst may be NULL */
+bnn_status  bnn_flip_get_pressure(bnn_flip* h, double* dst, uint64_t* n_active);
+/* right-hand side of the last assembled system, same order */
+bnn_status  bnn_flip_get_rhs(bnn_flip* h, double* dst, uint64_t* n_active);
+/* last binning: perm[i] = index (in the array order before the sort) of sorted slot i,
+ * cell_start[c] = first sorted slot of cell c (ncells+1 entries). Either may be NULL. */
+bnn_status  bnn_flip_get_bins(bnn_flip* h, uint32_t* perm, uint32_t* cell_start);
+bnn_status  bnn_flip_get_stats(bnn_flip* h, bnn_flip_stats* s);
+/* overwrite a grid field (same element types as bnn_flip_get_grid) -- the device-side analogue of
+ * restoring grid velocities from a memory state (PIC_Solver.cpp:157-201); setting
+ * BNN_FIELD_BOUNDARY_SDF recomputes the face weights before their next use */
+bnn_status  bnn_flip_set_grid(bnn_flip* h, int32_t field, const void* src);
+/* wait for all queued device work of this handle */
+bnn_status  bnn_flip_synchronize(bnn_flip* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BNN_FLIP_H */

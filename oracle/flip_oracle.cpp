@@ -1,0 +1,1395 @@
+// flip_oracle.cpp -- CPU restatement of Banana's 3-D FLIP substep (test infrastructure).
+//
+// THIS FILE IS THE ORACLE. It is test infrastructure only: the test suite, the smoke check and
+// the CPU-baseline leg of bench.py load it as a checker. Nothing in the shipped library links,
+// includes or calls it.
+//
+// It restates, in the same floating-point operation order, the reference solver
+//   Simulation/ParticleSolvers/PICFluid/FLIP_Solver<3,float> + PIC_Solver<3,float>
+// and the BananaCore helpers it calls. Every function cites the reference file:line it follows
+// (paths relative to the reference checkout). The code is templated on the real type:
+//   R = float  -> "parity mode": the reference's FP32 arithmetic and operation order;
+//   R = double -> "FP64 oracle": the same algorithm in double precision.
+//
+// Deterministic by default (threads = 1): particle loops run in index order and every reduction
+// is a serial left-to-right sum. With threads > 1 the particle scatter uses CAS float adds and
+// the PCG reductions use fixed chunks, as the reference does with TBB (Scheduler.h:127-141,
+// AtomicOperations.h:31-49); that mode exists to time the reference algorithm on host cores.
+//
+// Parity status of each function is recorded next to it ("pinned by" / "parity unpinned").
+// Build: see oracle/Makefile (g++ -O2 -ffp-contract=off; x86-64 baseline ISA, so no FMA).
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+//------------------------------------------------------------------------------------------------
+// small thread pool: parallel_for over [0,n) split into `nthreads` contiguous chunks
+//------------------------------------------------------------------------------------------------
+class Pool {
+public:
+    explicit Pool(int n) : m_n(n < 1 ? 1 : n) {
+        for(int t = 1; t < m_n; ++t) {
+            m_workers.emplace_back([this, t]() { worker(t); });
+        }
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(m_mu);
+            m_quit = true;
+            ++m_gen;
+        }
+        m_cv.notify_all();
+        for(auto& w : m_workers) w.join();
+    }
+    int size() const { return m_n; }
+    // fn(begin, end, chunk_id)
+    void run(size_t n, const std::function<void(size_t, size_t, int)>& fn) {
+        if(m_n == 1 || n < 2) {
+            fn(0, n, 0);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_mu);
+            m_fn   = &fn;
+            m_size = n;
+            m_done = 0;
+            ++m_gen;
+        }
+        m_cv.notify_all();
+        chunk(0);
+        std::unique_lock<std::mutex> lk(m_mu);
+        m_cvDone.wait(lk, [this]() { return m_done == m_n - 1; });
+        m_fn = nullptr;
+    }
+    template<class F> void parallel_for(size_t n, F f) {
+        run(n, [&](size_t b, size_t e, int) { for(size_t i = b; i < e; ++i) f(i); });
+    }
+
+private:
+    void chunk(int t) {
+        size_t b = m_size * (size_t)t / (size_t)m_n;
+        size_t e = m_size * (size_t)(t + 1) / (size_t)m_n;
+        (*m_fn)(b, e, t);
+    }
+    void worker(int t) {
+        uint64_t seen = 0;
+        for(;;) {
+            std::unique_lock<std::mutex> lk(m_mu);
+            m_cv.wait(lk, [&]() { return m_gen != seen; });
+            seen = m_gen;
+            if(m_quit) return;
+            lk.unlock();
+            chunk(t);
+            lk.lock();
+            ++m_done;
+            if(m_done == m_n - 1) m_cvDone.notify_one();
+        }
+    }
+    int                                            m_n;
+    std::vector<std::thread>                       m_workers;
+    std::mutex                                     m_mu;
+    std::condition_variable                        m_cv, m_cvDone;
+    uint64_t                                       m_gen  = 0;
+    bool                                           m_quit = false;
+    const std::function<void(size_t, size_t, int)>* m_fn  = nullptr;
+    size_t                                         m_size = 0;
+    int                                            m_done = 0;
+};
+
+//------------------------------------------------------------------------------------------------
+template<class R> struct V3 {
+    R x, y, z;
+    R&       operator[](int d) { return d == 0 ? x : (d == 1 ? y : z); }
+    const R& operator[](int d) const { return d == 0 ? x : (d == 1 ? y : z); }
+};
+
+template<class T> struct Arr3 {
+    int            nx = 0, ny = 0, nz = 0;
+    std::vector<T> d;
+    void resize(int a, int b, int c, T v) {
+        nx = a; ny = b; nz = c;
+        d.assign((size_t)a * b * c, v);
+    }
+    void   assign(T v) { std::fill(d.begin(), d.end(), v); }
+    size_t idx(int i, int j, int k) const { return ((size_t)k * ny + j) * nx + i; }  // Array.h:276
+    T&       operator()(int i, int j, int k) { return d[idx(i, j, k)]; }
+    const T& operator()(int i, int j, int k) const { return d[idx(i, j, k)]; }
+    size_t   size() const { return d.size(); }
+};
+
+template<class R> constexpr R Tiny() { return std::numeric_limits<R>::min(); }  // Setup.h:335
+template<class R> constexpr R Huge() { return std::numeric_limits<R>::max(); }  // Setup.h:336
+
+// MathHelpers.h:515-529 -- get_barycentric with clamping (f forced to 0 / 1 at the clamps)
+template<class R> inline void get_barycentric(R x, int& i, R& f, int i_low, int i_high) {
+    R s = std::floor(x);
+    i   = static_cast<int>(s);
+    if(i < i_low) {
+        i = i_low;
+        f = 0;
+    } else if(i > i_high - 2) {
+        i = i_high - 2;
+        f = 1;
+    } else {
+        f = (R)(x - s);
+    }
+}
+// MathHelpers.h:548-573
+template<class R> inline R lerp(R v0, R v1, R f) { return (R(1.0) - f) * v0 + f * v1; }
+template<class R> inline R bilerp(R v00, R v10, R v01, R v11, R fx, R fy) {
+    return lerp(lerp(v00, v10, fx), lerp(v01, v11, fx), fy);
+}
+template<class R> inline R trilerp(R v000, R v100, R v010, R v110, R v001, R v101, R v011, R v111, R fx, R fy, R fz) {
+    return lerp(bilerp(v000, v100, v010, v110, fx, fy), bilerp(v001, v101, v011, v111, fx, fy), fz);
+}
+// MathHelpers.h:804-821 -- 2-point fraction_inside (value < 0 is inside)
+template<class R> inline R fraction_inside(R l, R r) {
+    if(l < 0 && r < 0) return R(1.0);
+    if(l < 0 && r >= 0) return l / (l - r);
+    if(l >= 0 && r < 0) return r / (r - l);
+    return R(0);
+}
+// MathHelpers.h:790-797
+template<class R> inline void cycle_array(R* arr, int size) {
+    R t = arr[0];
+    for(int i = 0; i < size - 1; ++i) arr[i] = arr[i + 1];
+    arr[size - 1] = t;
+}
+// MathHelpers.h:825-895 -- 4-corner fraction_inside (square cut-cell area)
+template<class R> R fraction_inside4(R phi_bl, R phi_br, R phi_tl, R phi_tr) {
+    int inside_count = (phi_bl < 0 ? 1 : 0) + (phi_tl < 0 ? 1 : 0) + (phi_br < 0 ? 1 : 0) + (phi_tr < 0 ? 1 : 0);
+    R   list[]       = {phi_bl, phi_br, phi_tr, phi_tl};
+    if(inside_count == 4) {
+        return R(1.0);
+    } else if(inside_count == 3) {
+        while(list[0] < 0) cycle_array(list, 4);
+        R side0 = R(1.0) - fraction_inside(list[0], list[3]);
+        R side1 = R(1.0) - fraction_inside(list[0], list[1]);
+        return R(1.0) - R(0.5) * side0 * side1;
+    } else if(inside_count == 2) {
+        while(list[0] >= 0 || !(list[1] < 0 || list[2] < 0)) cycle_array(list, 4);
+        if(list[1] < 0) {
+            R side_left  = fraction_inside(list[0], list[3]);
+            R side_right = fraction_inside(list[1], list[2]);
+            return R(0.5) * (side_left + side_right);
+        } else {
+            R middle_point = R(0.25) * (list[0] + list[1] + list[2] + list[3]);
+            if(middle_point < 0) {
+                R area  = R(0);
+                R side1 = R(1.0) - fraction_inside(list[0], list[3]);
+                R side3 = R(1.0) - fraction_inside(list[2], list[3]);
+                area += R(0.5) * side1 * side3;
+                R side2 = R(1.0) - fraction_inside(list[2], list[1]);
+                R side0 = R(1.0) - fraction_inside(list[0], list[1]);
+                area += R(0.5) * side0 * side2;
+                return R(1.0) - area;
+            } else {
+                R area  = R(0);
+                R side0 = fraction_inside(list[0], list[1]);
+                R side1 = fraction_inside(list[0], list[3]);
+                area += R(0.5) * side0 * side1;
+                R side2 = fraction_inside(list[2], list[1]);
+                R side3 = fraction_inside(list[2], list[3]);
+                area += R(0.5) * side2 * side3;
+                return area;
+            }
+        }
+    } else if(inside_count == 1) {
+        while(list[0] >= 0) cycle_array(list, 4);
+        R side0 = fraction_inside(list[0], list[3]);
+        R side1 = fraction_inside(list[0], list[1]);
+        return R(0.5) * side0 * side1;
+    }
+    return R(0);
+}
+template<class R> inline R clamp01(R a) { return a < R(0) ? R(0) : (a > R(1.0) ? R(1.0) : a); }  // MathHelpers.h:368-377
+
+// glm::length of a 3-vector: sqrt((x*x + y*y) + z*z) -- glm compute_dot order (parity unpinned: glm absent)
+template<class R> inline R len3(R x, R y, R z) { return std::sqrt(x * x + y * y + z * z); }
+
+//------------------------------------------------------------------------------------------------
+// Box boundary SDF, negative OUTSIDE the box (BoundaryObject.h:47 passes bNegativeInside=false)
+// GeometryObjects.cpp:246-268 (BoxObject::signedDistance), NumberHelpers.h:338-345 (isInside)
+//------------------------------------------------------------------------------------------------
+template<class R> struct BoxSDF {
+    R lo[3], hi[3];
+    R signedDistance(const V3<R>& p, bool negInside) const {
+        R    mind   = Huge<R>();
+        bool inside = true;
+        for(int d = 0; d < 3; ++d)
+            if(p[d] < lo[d] || p[d] > hi[d]) inside = false;
+        if(inside) {
+            for(int d = 0; d < 3; ++d) mind = std::min(mind, std::min(p[d] - lo[d], hi[d] - p[d]));
+            mind = -mind;
+        } else {
+            R cp[3];
+            for(int d = 0; d < 3; ++d) cp[d] = std::max(std::min(p[d], hi[d]), lo[d]);
+            mind = len3(p.x - cp[0], p.y - cp[1], p.z - cp[2]);
+        }
+        if(!negInside) mind = -mind;
+        return mind;  // uniform scale 1
+    }
+    // GeometryObjects.cpp:37-85 (gradSignedDistance, 3-D branch, dx = 1e-4)
+    V3<R> grad(const V3<R>& p, bool negInside) const {
+        const R dx   = R(1e-4);
+        auto    sd   = [&](R a, R b, R c) { return signedDistance(V3<R>{a, b, c}, negInside); };
+        R       v000 = sd(p.x - dx, p.y - dx, p.z - dx);
+        R       v001 = sd(p.x - dx, p.y - dx, p.z + dx);
+        R       v010 = sd(p.x - dx, p.y + dx, p.z - dx);
+        R       v011 = sd(p.x - dx, p.y + dx, p.z + dx);
+        R       v100 = sd(p.x + dx, p.y - dx, p.z - dx);
+        R       v101 = sd(p.x + dx, p.y - dx, p.z + dx);
+        R       v110 = sd(p.x + dx, p.y + dx, p.z - dx);
+        R       v111 = sd(p.x + dx, p.y + dx, p.z + dx);
+        R ddx00 = v100 - v000, ddx10 = v110 - v010, ddx01 = v101 - v001, ddx11 = v111 - v011;
+        R ddy00 = v010 - v000, ddy10 = v110 - v100, ddy01 = v011 - v001, ddy11 = v111 - v101;
+        R ddz00 = v001 - v000, ddz10 = v101 - v100, ddz01 = v011 - v010, ddz11 = v111 - v110;
+        V3<R> g;
+        g.x = (((ddx00 + ddx01) + ddx10) + ddx11) * R(0.25);
+        g.y = (((ddy00 + ddy01) + ddy10) + ddy11) * R(0.25);
+        g.z = (((ddz00 + ddz01) + ddz10) + ddz11) * R(0.25);
+        return g;
+    }
+};
+
+//------------------------------------------------------------------------------------------------
+// Parameters (ParticleSolverData.h:38-175, PIC_Data.h:38-44, Constants.h:31-45)
+//------------------------------------------------------------------------------------------------
+}  // namespace orc
+
+extern "C" {
+struct orc_params {
+    float    box_min[3], box_max[3];          // SimulationDomainBox.BoxMin / BoxMax
+    float    cell_size;                        // CellSize
+    float    ratio_cell_size_pradius;          // RatioCellSizePRadius
+    uint32_t n_expand_cells;                   // NExpandCells
+    float    min_timestep, max_timestep;       // MinTimestep, MaxTimestep
+    float    cfl_factor;                       // CFLFactor
+    float    frame_duration;                   // FrameDuration
+    float    cg_relative_tolerance;            // CGRelativeTolerance
+    uint32_t max_cg_iteration;                 // MaxCGIteration
+    float    pic_flip_ratio;                   // PIC_FLIP_ratio
+    uint32_t advection_steps;                  // AdvectionSteps
+    int32_t  gravity_type;                     // 0 Earth, 1 Directional, 2 ToCenter, 3 FromCenter
+    float    gravity_direction[3];             // GravityDirection
+    float    gravity_center[3];                // GravityCenter
+    int32_t  apply_gravity;                    // ApplyGravity
+    int32_t  reflect_velocity_at_boundary;     // ReflectVelocityAtBoundary
+    float    boundary_reflection_multiplier;   // BoundaryReflectionMultiplier
+    int32_t  boundary_sdf_at_init;             // (extension) compute boundary SDF once at scene load
+};
+}
+
+namespace orc {
+
+enum Field {
+    F_U = 0, F_V, F_W,
+    F_WEIGHT_U, F_WEIGHT_V, F_WEIGHT_W,
+    F_VALID_U, F_VALID_V, F_VALID_W,
+    F_EXTRAP_U, F_EXTRAP_V, F_EXTRAP_W,
+    F_OLD_U, F_OLD_V, F_OLD_W,
+    F_PHI, F_BOUNDARY_SDF, F_ACTIVE_IDX,
+    F_TMP_U, F_TMP_V, F_TMP_W,
+    F_COUNT
+};
+
+struct Base {
+    virtual ~Base() {}
+};
+
+template<class R> class Solver : public Base {
+public:
+    using Vec = V3<R>;
+    orc_params prm;
+    Pool       pool;
+    // derived parameters
+    R     bmin[3], bmax[3], dx, invdx, halfdx, sdfRadius, particleRadius;
+    int   n[3];
+    R     gravity[3];
+    BoxSDF<R> domain;
+    // particles
+    std::vector<Vec> pos, vel;
+    // grid (PIC_Data.cpp:101-117, FLIP_Data.h:37-46)
+    Arr3<R>    vels[3], weights[3], tmpVels[3], oldVels[3], dVels[3];
+    Arr3<char> valids[3], extrapolates[3], tmpValids[3];
+    Arr3<uint32_t> activeCellIdx;
+    Arr3<R>    fluidSDF, boundarySDF;
+    // system
+    uint32_t                                    nActive = 0;
+    std::vector<std::vector<std::pair<uint32_t, R>>> rows;  // sorted (col, val) per row
+    std::vector<R>                              rhs, pressure;
+    // pcg settings/outputs (PCGSolver.h:93-108)
+    R        tolFactor;
+    uint32_t maxIters;
+    R        minDiagonalRatio = R(0.25);
+    uint32_t outIters         = 0;
+    R        outResidual      = 0;
+    bool     outSuccess       = true;
+    int      precond          = 2;  // 0 none, 1 Jacobi, 2 IC(0) "MICCL0_SYMMETRIC" (the reference's choice)
+    // frame bookkeeping (ParticleSolverData.h:43-50)
+    R        frameTime = 0;
+    uint32_t finishedFrame = 0;
+    uint32_t substepCount = 0;
+    R        lastSubstep = 0;
+
+    Solver(const orc_params& p, int threads) : prm(p), pool(threads) { makeReady(); }
+
+    // ParticleSolverData.cpp:229-272 + PIC_Data.cpp:41-44, 128-144 + Grid.cpp:38-55
+    void makeReady() {
+        dx             = R(prm.cell_size);
+        particleRadius = dx / R(prm.ratio_cell_size_pradius);
+        for(int d = 0; d < 3; ++d) {
+            R ex    = dx * R(prm.n_expand_cells);
+            bmin[d] = R(prm.box_min[d]) - ex;
+            bmax[d] = R(prm.box_max[d]) + ex;
+            domain.lo[d] = R(prm.box_min[d]);
+            domain.hi[d] = R(prm.box_max[d]);
+        }
+        sdfRadius = dx * R(1.01 * std::sqrt(3) / 2.0);
+        invdx     = R(1.0) / dx;
+        halfdx    = R(0.5) * dx;
+        for(int d = 0; d < 3; ++d) n[d] = static_cast<int>(std::ceil((bmax[d] - bmin[d]) / dx));
+        // gravity (ParticleSolverData.cpp:256-271)
+        if(prm.gravity_type == 1) {
+            R gx = R(prm.gravity_direction[0]), gy = R(prm.gravity_direction[1]), gz = R(prm.gravity_direction[2]);
+            R l2 = gx * gx + gy * gy + gz * gz;
+            if(l2 < std::numeric_limits<R>::epsilon()) {
+                prm.gravity_type = 0;
+            } else {
+                R is = R(1) / std::sqrt(l2);  // glm::normalize = x * inversesqrt(dot(x, x))
+                gravity[0] = R(9.81) * (gx * is);
+                gravity[1] = R(9.81) * (gy * is);
+                gravity[2] = R(9.81) * (gz * is);
+            }
+        }
+        if(prm.gravity_type == 0) {
+            gravity[0] = 0; gravity[1] = R(-9.81); gravity[2] = 0;
+        }
+        for(int d = 0; d < 3; ++d) {
+            int e[3] = {0, 0, 0};
+            e[d]     = 1;
+            vels[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], R(0));
+            weights[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], R(0));
+            tmpVels[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], R(0));
+            oldVels[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], R(0));
+            dVels[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], R(0));
+            valids[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], 0);
+            extrapolates[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], 0);
+            tmpValids[d].resize(n[0] + e[0], n[1] + e[1], n[2] + e[2], 0);
+        }
+        activeCellIdx.resize(n[0], n[1], n[2], 0u);
+        fluidSDF.resize(n[0], n[1], n[2], R(0));
+        boundarySDF.resize(n[0] + 1, n[1] + 1, n[2] + 1, R(0));  // zero-filled: PIC_Data.cpp:116
+        tolFactor = std::max(R(prm.cg_relative_tolerance), R(1e-30));  // PCGSolver.cpp:59-64
+        maxIters  = prm.max_cg_iteration;
+        if(prm.boundary_sdf_at_init) computeBoundarySDF();
+    }
+
+    // Grid.h:197
+    Vec gridCoord(const Vec& p) const { return Vec{(p.x - bmin[0]) / dx, (p.y - bmin[1]) / dx, (p.z - bmin[2]) / dx}; }
+
+    // ArrayHelpers.cpp:91-102 (interpolateValueLinear, 3-D)
+    R interp(const Arr3<R>& g, R px, R py, R pz) const {
+        int i, j, k;
+        R   fi, fj, fk;
+        get_barycentric(px, i, fi, 0, g.nx);
+        get_barycentric(py, j, fj, 0, g.ny);
+        get_barycentric(pz, k, fk, 0, g.nz);
+        return trilerp(g(i, j, k), g(i + 1, j, k), g(i, j + 1, k), g(i + 1, j + 1, k),
+                       g(i, j, k + 1), g(i + 1, j, k + 1), g(i, j + 1, k + 1), g(i + 1, j + 1, k + 1), fi, fj, fk);
+    }
+    // PIC_Solver.cpp:1119-1133 (getVelocityFromGrid): gridPos - extra, extra = 0.5 except on the axis
+    Vec velFromGrid(const Arr3<R>* g, const Vec& gp) const {
+        Vec v;
+        v.x = interp(g[0], gp.x, gp.y - R(0.5), gp.z - R(0.5));
+        v.y = interp(g[1], gp.x - R(0.5), gp.y, gp.z - R(0.5));
+        v.z = interp(g[2], gp.x - R(0.5), gp.y - R(0.5), gp.z);
+        return v;
+    }
+
+    // PIC_Solver.cpp:1137-1144
+    Vec trace_rk2(const Vec& p, R dt) const {
+        Vec gp   = gridCoord(p);
+        Vec pvel = velFromGrid(vels, gp);
+        R   h    = R(0.5) * dt;
+        Vec mid{p.x + h * pvel.x, p.y + h * pvel.y, p.z + h * pvel.z};
+        gp   = gridCoord(mid);
+        pvel = velFromGrid(vels, gp);
+        return Vec{p.x + dt * pvel.x, p.y + dt * pvel.y, p.z + dt * pvel.z};
+    }
+
+    // BoundaryObject.cpp:72-90 (constrainToBoundary with velocity)
+    bool constrainToBoundary(Vec& p, Vec& v) const {
+        R phi = domain.signedDistance(p, false);
+        if(phi < 0) {
+            Vec g  = domain.grad(p, false);
+            R   m2 = g.x * g.x + g.y * g.y + g.z * g.z;
+            if(m2 > Tiny<R>()) {
+                R s = std::sqrt(m2);
+                g.x /= s; g.y /= s; g.z /= s;
+                p.x -= phi * g.x; p.y -= phi * g.y; p.z -= phi * g.z;
+                if(prm.reflect_velocity_at_boundary) {
+                    // glm::reflect(I, N) = I - 2 dot(N, I) N
+                    R dn = g.x * v.x + g.y * v.y + g.z * v.z;
+                    R m  = R(prm.boundary_reflection_multiplier);
+                    v    = Vec{(v.x - R(2) * dn * g.x) * m, (v.y - R(2) * dn * g.y) * m, (v.z - R(2) * dn * g.z) * m};
+                    return true;
+                }
+            }
+        }
+        return false;
+    }
+
+    // PIC_Solver.cpp:353-373
+    void moveParticles(R dt) {
+        const R sub = dt / R(prm.advection_steps);
+        pool.parallel_for(pos.size(), [&](size_t p) {
+            Vec  pp = pos[p], pv = vel[p];
+            bool changed = false;
+            for(uint32_t i = 0; i < prm.advection_steps; ++i) {
+                pp = trace_rk2(pp, sub);
+                if(constrainToBoundary(pp, pv)) changed = true;
+            }
+            pos[p] = pp;
+            if(changed) vel[p] = pv;
+        });
+    }
+
+    // float/double CAS add (AtomicOperations.h:31-49), used only when threads > 1
+    static void atomicAddR(R& dst, R v) {
+        std::atomic_ref<R> a(dst);
+        R cur = a.load(std::memory_order_relaxed);
+        while(!a.compare_exchange_weak(cur, cur + v, std::memory_order_relaxed)) {}
+    }
+
+    // FLIP_Solver.cpp:57-95 (+ ArrayHelpers.cpp:50-76 getCoordinatesAndWeights)
+    void mapParticles2Grid() {
+        for(int d = 0; d < 3; ++d) {
+            vels[d].assign(0);
+            tmpVels[d].assign(0);
+            valids[d].assign(0);
+        }
+        const bool par = pool.size() > 1;
+        pool.parallel_for(pos.size(), [&](size_t p) {
+            Vec gp = gridCoord(pos[p]);
+            for(int d = 0; d < 3; ++d) {
+                R q[3] = {gp.x - (d == 0 ? R(0) : R(0.5)), gp.y - (d == 1 ? R(0) : R(0.5)), gp.z - (d == 2 ? R(0) : R(0.5))};
+                int ii[3];
+                R   ff[3];
+                get_barycentric(q[0], ii[0], ff[0], 0, vels[d].nx);
+                get_barycentric(q[1], ii[1], ff[1], 0, vels[d].ny);
+                get_barycentric(q[2], ii[2], ff[2], 0, vels[d].nz);
+                R fi = ff[0], fj = ff[1], fk = ff[2];
+                R w[8] = {(R(1.0) - fi) * (R(1.0) - fj) * (R(1.0) - fk), fi * (R(1.0) - fj) * (R(1.0) - fk),
+                          (R(1.0) - fi) * fj * (R(1.0) - fk),            fi * fj * (R(1.0) - fk),
+                          (R(1.0) - fi) * (R(1.0) - fj) * fk,            fi * (R(1.0) - fj) * fk,
+                          (R(1.0) - fi) * fj * fk,                       fi * fj * fk};
+                for(int c = 0; c < 8; ++c) {
+                    int    i = ii[0] + (c & 1), j = ii[1] + ((c >> 1) & 1), k = ii[2] + ((c >> 2) & 1);
+                    R      mom = w[c] * vel[p][d];
+                    size_t id  = vels[d].idx(i, j, k);
+                    if(par) {
+                        atomicAddR(vels[d].d[id], mom);
+                        atomicAddR(tmpVels[d].d[id], w[c]);
+                    } else {
+                        vels[d].d[id] += mom;
+                        tmpVels[d].d[id] += w[c];
+                    }
+                }
+            }
+        });
+        for(int d = 0; d < 3; ++d) {
+            pool.parallel_for(vels[d].size(), [&](size_t i) {
+                R wsum = tmpVels[d].d[i];
+                if(wsum > Tiny<R>()) {
+                    vels[d].d[i] /= wsum;
+                    valids[d].d[i] = 1;
+                }
+            });
+        }
+    }
+
+    // PIC_Solver.cpp:992-1090 (4 Jacobi layers over the interior index range)
+    void extrapolate(Arr3<R>& grid, Arr3<R>& temp, Arr3<char>& valid, Arr3<char>& oldValid, Arr3<char>& extrap) {
+        extrap.assign(0);
+        temp.d = grid.d;
+        for(int layers = 0; layers < 4; ++layers) {
+            oldValid.d = valid.d;
+            pool.parallel_for((size_t)(grid.nz - 2), [&](size_t kk) {
+                int k = (int)kk + 1;
+                for(int j = 1; j < grid.ny - 1; ++j)
+                    for(int i = 1; i < grid.nx - 1; ++i) {
+                        if(oldValid(i, j, k)) continue;
+                        R        sum   = R(0);
+                        uint32_t count = 0;
+                        if(oldValid(i + 1, j, k)) { sum += grid(i + 1, j, k); ++count; }
+                        if(oldValid(i - 1, j, k)) { sum += grid(i - 1, j, k); ++count; }
+                        if(oldValid(i, j + 1, k)) { sum += grid(i, j + 1, k); ++count; }
+                        if(oldValid(i, j - 1, k)) { sum += grid(i, j - 1, k); ++count; }
+                        if(oldValid(i, j, k + 1)) { sum += grid(i, j, k + 1); ++count; }
+                        if(oldValid(i, j, k - 1)) { sum += grid(i, j, k - 1); ++count; }
+                        if(count > 0) {
+                            temp(i, j, k)   = sum / static_cast<R>(count);
+                            valid(i, j, k)  = 1;
+                            extrap(i, j, k) = 1;
+                        }
+                    }
+            });
+            grid.d = temp.d;
+        }
+    }
+    void extrapolateVelocity() {  // PIC_Solver.cpp:985-989
+        for(int d = 0; d < 3; ++d) extrapolate(vels[d], tmpVels[d], valids[d], tmpValids[d], extrapolates[d]);
+    }
+
+    // ArrayHelpers.cpp:214-251 (interpolateGradient, 3-D)
+    Vec interpolateGradient(R px, R py, R pz, const Arr3<R>& g) const {
+        int i, j, k;
+        R   fi, fj, fk;
+        get_barycentric(px, i, fi, 0, g.nx);
+        get_barycentric(py, j, fj, 0, g.ny);
+        get_barycentric(pz, k, fk, 0, g.nz);
+        R v000 = g(i, j, k), v001 = g(i, j, k + 1), v010 = g(i, j + 1, k), v011 = g(i, j + 1, k + 1);
+        R v100 = g(i + 1, j, k), v101 = g(i + 1, j, k + 1), v110 = g(i + 1, j + 1, k), v111 = g(i + 1, j + 1, k + 1);
+        R ddx00 = v100 - v000, ddx10 = v110 - v010, ddx01 = v101 - v001, ddx11 = v111 - v011;
+        R ddy00 = v010 - v000, ddy10 = v110 - v100, ddy01 = v011 - v001, ddy11 = v111 - v101;
+        R ddz00 = v001 - v000, ddz10 = v101 - v100, ddz01 = v011 - v010, ddz11 = v111 - v110;
+        return Vec{bilerp(ddx00, ddx10, ddx01, ddx11, fj, fk), bilerp(ddy00, ddy10, ddy01, ddy11, fi, fk),
+                   bilerp(ddz00, ddz10, ddz01, ddz11, fi, fj)};
+    }
+
+    // PIC_Solver.cpp:1094-1115 (component d reads the already-updated components < d)
+    void constrainGridVelocity() {
+        for(int d = 0; d < 3; ++d) {
+            tmpVels[d].d = vels[d].d;
+            Arr3<R>& a = vels[d];
+            pool.parallel_for((size_t)a.nz, [&](size_t kk) {
+                int k = (int)kk;
+                for(int j = 0; j < a.ny; ++j)
+                    for(int i = 0; i < a.nx; ++i) {
+                        if(extrapolates[d](i, j, k) == 1 && weights[d](i, j, k) < Tiny<R>()) {
+                            R gp[3] = {R(i) + R(0.5), R(j) + R(0.5), R(k) + R(0.5)};
+                            gp[d]   = R(i * (d == 0) + j * (d == 1) + k * (d == 2));
+                            Vec gpos{gp[0], gp[1], gp[2]};
+                            Vec v   = velFromGrid(vels, gpos);
+                            Vec nrm = interpolateGradient(gp[0], gp[1], gp[2], boundarySDF);
+                            R   m2  = nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z;
+                            if(m2 > Tiny<R>()) {
+                                R s = std::sqrt(m2);
+                                nrm.x /= s; nrm.y /= s; nrm.z /= s;
+                                R dt = v.x * nrm.x + v.y * nrm.y + v.z * nrm.z;
+                                v.x -= dt * nrm.x; v.y -= dt * nrm.y; v.z -= dt * nrm.z;
+                                tmpVels[d](i, j, k) = v[d];
+                            }
+                        }
+                    }
+            });
+            vels[d].d = tmpVels[d].d;
+        }
+    }
+
+    // FLIP_Data.h:48-52
+    void backupGridVelocity() {
+        for(int d = 0; d < 3; ++d) oldVels[d].d = vels[d].d;
+    }
+
+    // PIC_Solver.cpp:505-529
+    void addGravity(R dt) {
+        if(!prm.apply_gravity) return;
+        if(prm.gravity_type == 0) {
+            R g = R(9.81) * dt;
+            for(auto& x : vels[1].d) x -= g;
+        } else if(prm.gravity_type == 1) {
+            for(int d = 0; d < 3; ++d) {
+                R g = gravity[d] * dt;
+                for(auto& x : vels[d].d) x += g;
+            }
+        } else {
+            // ToCenter / FromCenter: the reference loops over velocities[0].vsize() for every
+            // component (PIC_Solver.cpp:521), i.e. it indexes each component by the u-face range.
+            for(int d = 0; d < 3; ++d) {
+                Arr3<R>& a = vels[0];
+                for(int k = 0; k < a.nz; ++k)
+                    for(int j = 0; j < a.ny; ++j)
+                        for(int i = 0; i < a.nx; ++i) {
+                            R gp[3] = {R(i) + R(0.5), R(j) + R(0.5), R(k) + R(0.5)};
+                            gp[d]   = R(d == 0 ? i : (d == 1 ? j : k));
+                            R w[3]  = {gp[0] * dx + bmin[0], gp[1] * dx + bmin[1], gp[2] * dx + bmin[2]};
+                            R c[3]  = {R(prm.gravity_center[0]), R(prm.gravity_center[1]), R(prm.gravity_center[2])};
+                            R dv[3];
+                            for(int e = 0; e < 3; ++e) dv[e] = (prm.gravity_type == 2) ? c[e] - w[e] : w[e] - c[e];
+                            R is = R(1) / std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+                            R gd = R(9.81) * (dv[d] * is);
+                            if(vels[d].nx > i && vels[d].ny > j && vels[d].nz > k) vels[d](i, j, k) += gd * dt;
+                        }
+            }
+        }
+    }
+
+    // PIC_Solver.cpp:543-597
+    void computeCellWeights() {
+        const auto& b = boundarySDF;
+        for(int k = 0; k <= n[2]; ++k)
+            for(int j = 0; j <= n[1]; ++j)
+                for(int i = 0; i <= n[0]; ++i) {
+                    if(i < weights[0].nx && j < weights[0].ny && k < weights[0].nz)
+                        weights[0](i, j, k) = clamp01(R(1.0) - fraction_inside4(b(i, j, k), b(i, j + 1, k), b(i, j, k + 1), b(i, j + 1, k + 1)));
+                    if(i < weights[1].nx && j < weights[1].ny && k < weights[1].nz)
+                        weights[1](i, j, k) = clamp01(R(1.0) - fraction_inside4(b(i, j, k), b(i, j, k + 1), b(i + 1, j, k), b(i + 1, j, k + 1)));
+                    if(i < weights[2].nx && j < weights[2].ny && k < weights[2].nz)
+                        weights[2](i, j, k) = clamp01(R(1.0) - fraction_inside4(b(i, j, k), b(i, j + 1, k), b(i + 1, j, k), b(i + 1, j + 1, k)));
+                }
+    }
+
+    static void atomicMinR(R& dst, R v) {
+        std::atomic_ref<R> a(dst);
+        R cur = a.load(std::memory_order_relaxed);
+        while(v < cur && !a.compare_exchange_weak(cur, v, std::memory_order_relaxed)) {}
+    }
+
+    // PIC_Solver.cpp:601-689 (min is order independent, so the spin lock is replaced by a CAS min)
+    void computeFluidSDF() {
+        fluidSDF.assign(dx * R(3.0));
+        const bool par = pool.size() > 1;
+        pool.parallel_for(pos.size(), [&](size_t p) {
+            const Vec& pp = pos[p];
+            int c[3], lo[3], hi[3];
+            for(int d = 0; d < 3; ++d) {
+                c[d]  = static_cast<int>((pp[d] - bmin[d]) / dx);   // Grid.h:156-162
+                lo[d] = std::max(0, c[d] - 1);
+                hi[d] = std::min(c[d] + 1, n[d] - 1);
+            }
+            for(int k = lo[2]; k <= hi[2]; ++k)
+                for(int j = lo[1]; j <= hi[1]; ++j)
+                    for(int i = lo[0]; i <= hi[0]; ++i) {
+                        // Grid.h:184-188: Vec3<R>(i+0.5, j+0.5, k+0.5) * dx + bMin
+                        R sx = R(i + 0.5) * dx + bmin[0];
+                        R sy = R(j + 0.5) * dx + bmin[1];
+                        R sz = R(k + 0.5) * dx + bmin[2];
+                        R phi = len3(sx - pp.x, sy - pp.y, sz - pp.z) - sdfRadius;
+                        if(par) {
+                            atomicMinR(fluidSDF(i, j, k), phi);
+                        } else if(phi < fluidSDF(i, j, k)) {
+                            fluidSDF(i, j, k) = phi;
+                        }
+                    }
+        });
+        // solid extension (PIC_Solver.cpp:671-687)
+        const auto& b = boundarySDF;
+        for(int k = 0; k < n[2]; ++k)
+            for(int j = 0; j < n[1]; ++j)
+                for(int i = 0; i < n[0]; ++i) {
+                    if(fluidSDF(i, j, k) < halfdx) {
+                        R s = R(0.125) * (b(i, j, k) + b(i + 1, j, k) + b(i, j + 1, k) + b(i + 1, j + 1, k) + b(i, j, k + 1) +
+                                          b(i + 1, j, k + 1) + b(i, j + 1, k + 1) + b(i + 1, j + 1, k + 1));
+                        if(s < 0) fluidSDF(i, j, k) = -halfdx;
+                    }
+                }
+    }
+
+    void addElement(uint32_t r, uint32_t c, R v) {  // SparseMatrix.cpp:87-96
+        auto& row = rows[r];
+        auto  it  = std::lower_bound(row.begin(), row.end(), c, [](const std::pair<uint32_t, R>& a, uint32_t b) { return a.first < b; });
+        if(it != row.end() && it->first == c) it->second += v;
+        else row.insert(it, {c, v});
+    }
+
+    // PIC_Solver.cpp:693-893. Cells with phi == 0 exactly are skipped (the reference would write
+    // row 0 for them through activeCellIdx's zero fill; see the design notes).
+    void computeSystem(R dt) {
+        activeCellIdx.assign(0u);
+        nActive = 0;
+        for(size_t i = 0; i < fluidSDF.size(); ++i)
+            if(fluidSDF.d[i] < 0) activeCellIdx.d[i] = nActive++;
+        rows.assign(nActive, {});
+        rhs.assign(nActive, R(0));
+        pressure.assign(nActive, R(0));
+        const auto &u = vels[0], &v = vels[1], &w = vels[2];
+        const auto &wu = weights[0], &wv = weights[1], &ww = weights[2];
+        for(int k = 1; k < n[2] - 1; ++k)
+            for(int j = 1; j < n[1] - 1; ++j)
+                for(int i = 1; i < n[0] - 1; ++i) {
+                    const R pc = fluidSDF(i, j, k);
+                    if(!(pc < 0)) continue;
+                    const R pr = fluidSDF(i + 1, j, k), pl = fluidSDF(i - 1, j, k);
+                    const R pt = fluidSDF(i, j + 1, k), pb = fluidSDF(i, j - 1, k);
+                    const R pf = fluidSDF(i, j, k + 1), pn = fluidSDF(i, j, k - 1);
+                    R rv = R(0);
+                    const R tr = wu(i + 1, j, k) * dt; rv -= wu(i + 1, j, k) * u(i + 1, j, k);
+                    const R tl = wu(i, j, k) * dt;     rv += wu(i, j, k) * u(i, j, k);
+                    const R tt = wv(i, j + 1, k) * dt; rv -= wv(i, j + 1, k) * v(i, j + 1, k);
+                    const R tb = wv(i, j, k) * dt;     rv += wv(i, j, k) * v(i, j, k);
+                    const R tf = ww(i, j, k + 1) * dt; rv -= ww(i, j, k + 1) * w(i, j, k + 1);
+                    const R tn = ww(i, j, k) * dt;     rv += ww(i, j, k) * w(i, j, k);
+                    const uint32_t ic = activeCellIdx(i, j, k);
+                    rhs[ic]           = rv;
+                    R ct              = R(0);
+                    auto side = [&](R pnb, R term, int a, int b2, int c2) {
+                        if(pnb < 0) {
+                            ct += term;
+                            addElement(ic, activeCellIdx(a, b2, c2), -term);
+                        } else {
+                            R theta = std::max(R(0.01), fraction_inside(pc, pnb));
+                            ct += term / theta;
+                        }
+                    };
+                    side(pr, tr, i + 1, j, k);
+                    side(pl, tl, i - 1, j, k);
+                    side(pt, tt, i, j + 1, k);
+                    side(pb, tb, i, j - 1, k);
+                    side(pf, tf, i, j, k + 1);
+                    side(pn, tn, i, j, k - 1);
+                    addElement(ic, ic, ct);
+                }
+    }
+
+    //--------------------------------------------------------------------------------------------
+    // PCG (PCGSolver.cpp:143-199, 221-269, 398-466)
+    //--------------------------------------------------------------------------------------------
+    std::vector<R>        icInvDiag, icColValue, icADiag;
+    std::vector<uint32_t> icColIndex, icColStart;
+    std::vector<R>        jacobi;
+
+    R dotv(const std::vector<R>& a, const std::vector<R>& b) {
+        if(pool.size() == 1) {
+            R s = 0;
+            for(size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+            return s;
+        }
+        std::vector<R> part(pool.size(), R(0));
+        pool.run(a.size(), [&](size_t b0, size_t e0, int t) {
+            R s = 0;
+            for(size_t i = b0; i < e0; ++i) s += a[i] * b[i];
+            part[t] = s;
+        });
+        R s = 0;
+        for(R x : part) s += x;
+        return s;
+    }
+    R maxAbs(const std::vector<R>& a) {
+        R m = 0;
+        for(R x : a) m = std::max(m, std::abs(x));
+        return m;
+    }
+    void spmv(const std::vector<R>& x, std::vector<R>& y) {  // SparseMatrix.cpp:220-235
+        y.resize(nActive);
+        pool.parallel_for(nActive, [&](size_t i) {
+            R t = 0;
+            for(const auto& e : rows[i]) t += e.second * x[e.first];
+            y[i] = t;
+        });
+    }
+    void formIC0() {  // PCGSolver.cpp:398-466
+        const uint32_t N = nActive;
+        icInvDiag.assign(N, 0); icADiag.assign(N, 0); icColStart.assign(N + 1, 0);
+        icColValue.clear(); icColIndex.clear();
+        for(uint32_t i = 0; i < N; ++i) {
+            icColStart[i] = (uint32_t)icColIndex.size();
+            for(const auto& e : rows[i]) {
+                if(e.first > i) {
+                    icColIndex.push_back(e.first);
+                    icColValue.push_back(e.second);
+                } else if(e.first == i) {
+                    icInvDiag[i] = icADiag[i] = e.second;
+                }
+            }
+        }
+        icColStart[N] = (uint32_t)icColIndex.size();
+        for(uint32_t k = 0; k < N; ++k) {
+            if(icADiag[k] < Tiny<R>()) continue;
+            if(icInvDiag[k] < minDiagonalRatio * icADiag[k]) icInvDiag[k] = R(1.0) / std::sqrt(icADiag[k]);
+            else icInvDiag[k] = R(1.0) / std::sqrt(icInvDiag[k]);
+            for(uint32_t p = icColStart[k]; p < icColStart[k + 1]; ++p) icColValue[p] *= icInvDiag[k];
+            for(uint32_t p = icColStart[k]; p < icColStart[k + 1]; ++p) {
+                uint32_t j    = icColIndex[p];
+                R        mult = icColValue[p];
+                icInvDiag[j] -= mult * icColValue[p];
+                uint32_t a = p + 1, b = icColStart[j], as = icColStart[k + 1], bs = icColStart[j + 1];
+                while(a < as && b < bs) {
+                    if(icColIndex[b] == icColIndex[a]) {
+                        icColValue[b] -= mult * icColValue[a];
+                        ++a; ++b;
+                    } else if(icColIndex[b] < icColIndex[a]) {
+                        ++b;
+                    } else {
+                        ++a;
+                    }
+                }
+            }
+        }
+    }
+    // PCGSolver.cpp:301-313. Rows with a zero diagonal exist (solid-extension cells in the wall band
+    // whose six faces all have weight 0); the reference's 1/a_ii would be inf there, the IC(0) it
+    // actually uses skips them (PCGSolver.cpp:419-421). Their preconditioner entry is 0 here.
+    void formJacobi() {
+        jacobi.assign(nActive, R(0));
+        for(uint32_t i = 0; i < nActive; ++i)
+            for(const auto& e : rows[i])
+                if(e.first == i) jacobi[i] = e.second > R(0) ? R(1.0) / e.second : R(0);  // zero rows: see below
+    }
+    void applyPrecond(const std::vector<R>& x, std::vector<R>& out) {  // PCGSolver.cpp:221-269
+        if(precond == 0) { out = x; return; }
+        if(precond == 1) {
+            out.resize(x.size());
+            for(size_t i = 0; i < x.size(); ++i) out[i] = jacobi[i] * x[i];
+            return;
+        }
+        out = x;
+        const uint32_t N = nActive;
+        for(uint32_t i = 0; i < N; ++i) {
+            out[i] *= icInvDiag[i];
+            for(uint32_t j = icColStart[i]; j < icColStart[i + 1]; ++j) out[icColIndex[j]] -= icColValue[j] * out[i];
+        }
+        if(N == 0) return;
+        uint32_t i = N;
+        do {
+            --i;
+            for(uint32_t j = icColStart[i]; j < icColStart[i + 1]; ++j) out[i] -= icColValue[j] * out[icColIndex[j]];
+            out[i] *= icInvDiag[i];
+        } while(i > 0);
+    }
+    bool solve() {  // PCGSolver.cpp:143-199 (solve_precond, zero initial guess)
+        std::vector<R> z, s, r;
+        pressure.assign(nActive, R(0));
+        r           = rhs;
+        outResidual = maxAbs(r);
+        if(outResidual < std::numeric_limits<R>::min()) {
+            outIters   = 0;
+            outSuccess = true;
+            return true;
+        }
+        R tol = tolFactor * outResidual;
+        if(precond == 2) formIC0();
+        if(precond == 1) formJacobi();
+        applyPrecond(r, z);
+        R rho = dotv(z, r);
+        if(rho < std::numeric_limits<R>::min() || std::isnan(rho)) {
+            outIters   = 0;
+            outSuccess = false;
+            return false;
+        }
+        s = z;
+        for(uint32_t it = 0; it < maxIters; ++it) {
+            spmv(s, z);
+            R tmp = dotv(s, z);
+            if(tmp < std::numeric_limits<R>::min()) {
+                outIters   = it + 1;
+                outSuccess = true;
+                return true;
+            }
+            R alpha = rho / tmp;
+            for(size_t i = 0; i < s.size(); ++i) pressure[i] += alpha * s[i];
+            for(size_t i = 0; i < z.size(); ++i) r[i] += -alpha * z[i];
+            outResidual = maxAbs(r);
+            if(outResidual < tol) {
+                outIters   = it + 1;
+                outSuccess = true;
+                return true;
+            }
+            applyPrecond(r, z);
+            R rho_new = dotv(z, r);
+            R beta    = rho_new / rho;
+            for(size_t i = 0; i < z.size(); ++i) z[i] += beta * s[i];
+            s.swap(z);
+            rho = rho_new;
+        }
+        outIters   = maxIters;
+        outSuccess = false;
+        return false;
+    }
+
+    // PIC_Solver.cpp:911-981
+    void updateProjectedVelocity(R dt) {
+        for(int d = 0; d < 3; ++d) valids[d].assign(0);
+        auto P = [&](int i, int j, int k, R phi) { return phi < 0 ? pressure[activeCellIdx(i, j, k)] : R(0); };
+        for(int k = 0; k < n[2]; ++k)
+            for(int j = 0; j < n[1]; ++j)
+                for(int i = 0; i < n[0]; ++i) {
+                    const R pc = fluidSDF(i, j, k);
+                    const R pl = i > 0 ? fluidSDF(i - 1, j, k) : R(0);
+                    const R pb = j > 0 ? fluidSDF(i, j - 1, k) : R(0);
+                    const R pn = k > 0 ? fluidSDF(i, j, k - 1) : R(0);
+                    const R prc = P(i, j, k, pc);
+                    const R prl = i > 0 ? P(i - 1, j, k, pl) : R(0);
+                    const R prb = j > 0 ? P(i, j - 1, k, pb) : R(0);
+                    const R prn = k > 0 ? P(i, j, k - 1, pn) : R(0);
+                    if(i > 0 && (pc < 0 || pl < 0) && weights[0](i, j, k) > 0) {
+                        R theta = std::max(R(0.01), fraction_inside(pl, pc));
+                        vels[0](i, j, k) -= dt * (prc - prl) / theta;
+                        valids[0](i, j, k) = 1;
+                    }
+                    if(j > 0 && (pc < 0 || pb < 0) && weights[1](i, j, k) > 0) {
+                        R theta = std::max(R(0.01), fraction_inside(pb, pc));
+                        vels[1](i, j, k) -= dt * (prc - prb) / theta;
+                        valids[1](i, j, k) = 1;
+                    }
+                    if(k > 0 && weights[2](i, j, k) > 0 && (pc < 0 || pn < 0)) {
+                        R theta = std::max(R(0.01), fraction_inside(pn, pc));
+                        vels[2](i, j, k) -= dt * (prc - prn) / theta;
+                        valids[2](i, j, k) = 1;
+                    }
+                }
+        for(int d = 0; d < 3; ++d)
+            for(size_t i = 0; i < vels[d].size(); ++i)
+                if(valids[d].d[i] == 0) vels[d].d[i] = 0;
+    }
+
+    // PIC_Solver.cpp:533-539
+    bool pressureProjection(R dt) {
+        computeCellWeights();
+        computeFluidSDF();
+        computeSystem(dt);
+        bool ok = solve();
+        updateProjectedVelocity(dt);
+        return ok;
+    }
+
+    // FLIP_Solver.cpp:115-121
+    void computeChangesGridVelocity() {
+        for(int d = 0; d < 3; ++d)
+            for(size_t i = 0; i < vels[d].size(); ++i) dVels[d].d[i] = vels[d].d[i] - oldVels[d].d[i];
+    }
+
+    // FLIP_Solver.cpp:99-111, 125-133
+    void mapGrid2Particles() {
+        const R r = R(prm.pic_flip_ratio);
+        pool.parallel_for(pos.size(), [&](size_t p) {
+            Vec gp  = gridCoord(pos[p]);
+            Vec gv  = velFromGrid(vels, gp);
+            Vec dgv = velFromGrid(dVels, gp);
+            Vec pv  = vel[p];
+            for(int d = 0; d < 3; ++d) vel[p][d] = (R(1.0) - r) * gv[d] + r * (pv[d] + dgv[d]);
+        });
+    }
+
+    // FLIP_Solver.cpp:42-53
+    bool advanceVelocity(R dt) {
+        mapParticles2Grid();
+        extrapolateVelocity();
+        constrainGridVelocity();
+        backupGridVelocity();
+        addGravity(dt);
+        bool ok = pressureProjection(dt);
+        extrapolateVelocity();
+        constrainGridVelocity();
+        computeChangesGridVelocity();
+        mapGrid2Particles();
+        return ok;
+    }
+
+    // one body of the advanceFrame substep loop (PIC_Solver.cpp:292-294), CorrectPosition = false
+    bool step(R dt) {
+        moveParticles(dt);
+        return advanceVelocity(dt);
+    }
+
+    // PIC_Solver.cpp:342-349
+    R timestepCFL() {
+        R maxVel = Tiny<R>();
+        for(int d = 0; d < 3; ++d) maxVel = std::max(maxVel, maxAbs(vels[d].d));
+        R ts = maxVel > Tiny<R>() ? R(prm.cfl_factor) * (dx / maxVel) : Huge<R>();
+        return std::min(std::max(ts, R(prm.min_timestep)), R(prm.max_timestep));
+    }
+
+    // PIC_Solver.cpp:267-311 (no dynamic objects, no frame IO)
+    bool advanceFrame() {
+        const R fd = R(prm.frame_duration);
+        frameTime    = 0;
+        substepCount = 0;
+        bool ok      = true;
+        while(frameTime < fd) {
+            R sub = timestepCFL();
+            R rem = fd - frameTime;
+            if(frameTime + sub >= fd) sub = rem;
+            else if(frameTime + R(1.5) * sub >= fd) sub = rem * R(0.5);
+            ok &= step(sub);
+            lastSubstep = sub;
+            frameTime += sub;
+            ++substepCount;
+        }
+        ++finishedFrame;
+        return ok;
+    }
+
+    // PIC_Solver.cpp:1158-1168 with the domain box as the only boundary object
+    void computeBoundarySDF() {
+        for(int k = 0; k <= n[2]; ++k)
+            for(int j = 0; j <= n[1]; ++j)
+                for(int i = 0; i <= n[0]; ++i) {
+                    Vec w{R(i) * dx + bmin[0], R(j) * dx + bmin[1], R(k) * dx + bmin[2]};
+                    boundarySDF(i, j, k) = std::min(Huge<R>(), domain.signedDistance(w, false));
+                }
+    }
+
+    //--------------------------------------------------------------------------------------------
+    template<class T> void* fieldPtr(int f, size_t& count, int& kind) {
+        // kind: 0 = R, 1 = char, 2 = uint32
+        kind = 0;
+        switch(f) {
+            case F_U: case F_V: case F_W: count = vels[f].size(); return vels[f].d.data();
+            case F_WEIGHT_U: case F_WEIGHT_V: case F_WEIGHT_W: count = weights[f - F_WEIGHT_U].size(); return weights[f - F_WEIGHT_U].d.data();
+            case F_VALID_U: case F_VALID_V: case F_VALID_W: kind = 1; count = valids[f - F_VALID_U].size(); return valids[f - F_VALID_U].d.data();
+            case F_EXTRAP_U: case F_EXTRAP_V: case F_EXTRAP_W: kind = 1; count = extrapolates[f - F_EXTRAP_U].size(); return extrapolates[f - F_EXTRAP_U].d.data();
+            case F_OLD_U: case F_OLD_V: case F_OLD_W: count = oldVels[f - F_OLD_U].size(); return oldVels[f - F_OLD_U].d.data();
+            case F_PHI: count = fluidSDF.size(); return fluidSDF.d.data();
+            case F_BOUNDARY_SDF: count = boundarySDF.size(); return boundarySDF.d.data();
+            case F_ACTIVE_IDX: kind = 2; count = activeCellIdx.size(); return activeCellIdx.d.data();
+            case F_TMP_U: case F_TMP_V: case F_TMP_W: count = tmpVels[f - F_TMP_U].size(); return tmpVels[f - F_TMP_U].d.data();
+        }
+        count = 0;
+        return nullptr;
+    }
+};
+
+//------------------------------------------------------------------------------------------------
+// Cell binning: Grid::collectIndexToCells + getParticleIdxSortedByCell (Grid.cpp:92-114, 283-309).
+// The reference's order inside a cell is its spin-lock push_back order (nondeterministic); the
+// oracle defines it as ascending particle index (a stable counting sort by linear cell index).
+// Pinned by: the Grid known-answer test (Grid/_Grid.Test.hpp:35-44), brute-force counts and the
+// permutation / sortedness invariants in tests/test_oracle_binning.py.
+//------------------------------------------------------------------------------------------------
+static inline int cellIndexAxis(float p, float bmin, float dx) { return static_cast<int>((p - bmin) / dx); }
+
+//------------------------------------------------------------------------------------------------
+// Morton z-order key of NeighborSearch::z_sort (NeighborSearch.cpp:44-86, DataStructures.h:32,
+// Morton.h: morton3D_64_encode = m3D_e_sLUT<uint_fast64_t, uint_fast32_t>, Morton3D.h:54-66).
+// On LP64 Linux uint_fast32_t is 64 bits wide, so the table loop runs over 8 bytes and the 64-bit
+// result keeps the low 21.33 bits of each coordinate interleave (x at bit 3b, y 3b+1, z 3b+2).
+// Restated bit by bit from the interleave definition (a for-loop, not the lookup tables).
+//------------------------------------------------------------------------------------------------
+static inline uint64_t spread3(uint64_t v, int nbits) {
+    uint64_t r = 0;
+    for(int b = 0; b < nbits; ++b) r |= ((v >> b) & 1ull) << (3 * b);
+    return r;
+}
+static inline uint64_t mortonKeyOracle(const float* x, float invCellSize) {
+    uint64_t c[3];
+    for(int d = 0; d < 3; ++d) {
+        float   tmp = x[d] - float(1.2345);
+        int     k   = tmp >= 0 ? static_cast<int>(invCellSize * tmp) : static_cast<int>(invCellSize * tmp) - 1;
+        int64_t v   = static_cast<int64_t>(k) - (static_cast<int64_t>(std::numeric_limits<int>::lowest()) + 1);
+        c[d]        = static_cast<uint64_t>(v);
+    }
+    // m3D_e_sLUT with an 8-byte coordinate: answer = answer << 24 | chunk(byte i), i = 7..0.
+    // Only the chunks of bytes 0, 1 and the low 16 bits of byte 2's chunk survive in 64 bits.
+    uint64_t answer = 0;
+    for(int i = 8; i > 0; --i) {
+        int      shift = (i - 1) * 8;
+        uint64_t chunk = spread3((c[2] >> shift) & 0xFF, 8) << 2 | spread3((c[1] >> shift) & 0xFF, 8) << 1 |
+                         spread3((c[0] >> shift) & 0xFF, 8);
+        answer = (answer << 24) | chunk;
+    }
+    return answer;
+}
+
+//------------------------------------------------------------------------------------------------
+// Seeded scene generator (replaces the reference's unseeded ParticleGenerator,
+// ParticleGenerator.hpp:87-136): one particle per sub-cell of the grid's sub-cell lattice
+// (ppa^3 per cell) whose centre lies inside an object, jittered uniformly inside its sub-cell.
+// Objects are visited in order; a sub-cell already claimed by an earlier object is skipped.
+//------------------------------------------------------------------------------------------------
+struct GridDesc {
+    float bmin[3], dx;
+    int   n[3];
+    explicit GridDesc(const orc_params& p) {
+        dx = p.cell_size;
+        for(int d = 0; d < 3; ++d) {
+            float ex = dx * float(p.n_expand_cells);
+            bmin[d]  = p.box_min[d] - ex;
+            float bx = p.box_max[d] + ex;
+            n[d]     = static_cast<int>(std::ceil((bx - bmin[d]) / dx));
+        }
+    }
+};
+static inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static inline double rand01(uint64_t seed, uint64_t id, uint32_t stream) {
+    uint64_t h = splitmix64(seed ^ splitmix64(id * 0x2545F4914F6CDD1Dull + stream));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace orc
+
+extern "C" {
+
+struct orc_scene_spec {
+    int32_t  n_boxes;
+    float    box_min[8][3], box_max[8][3], box_vel[8][3];
+    int32_t  n_spheres;
+    float    sphere_center[8][3], sphere_radius[8], sphere_vel[8][3];
+    float    vel_sigma;        // > 0: add N(0, sigma^2) per component
+    int32_t  ppa;              // particles per axis per cell (2 -> 8 per cell)
+    float    jitter;           // 1.0 = uniform over the whole sub-cell
+};
+
+// returns the particle count; pos/vel may be null (count only)
+uint64_t orc_generate_scene(const orc_params* prm, const orc_scene_spec* sp, uint64_t seed, float* pos, float* vel) {
+    using namespace orc;
+    GridDesc      tmp(*prm);  // the solver's grid (ParticleSolverData.cpp:247-253, Grid.cpp:38-55)
+    const int     ppa = sp->ppa > 0 ? sp->ppa : 2;
+    const double  h   = (double)tmp.dx / ppa;
+    const int64_t NX = (int64_t)tmp.n[0] * ppa, NY = (int64_t)tmp.n[1] * ppa, NZ = (int64_t)tmp.n[2] * ppa;
+    uint64_t      count = 0;
+    auto center = [&](int64_t m, int d) { return (double)tmp.bmin[d] + ((double)m + 0.5) * h; };
+    auto inBox  = [&](int b, const double c[3]) {
+        for(int d = 0; d < 3; ++d)
+            if(c[d] < (double)sp->box_min[b][d] || c[d] > (double)sp->box_max[b][d]) return false;
+        return true;
+    };
+    auto inSph = [&](int s, const double c[3]) {
+        double r2 = 0;
+        for(int d = 0; d < 3; ++d) {
+            double e = c[d] - (double)sp->sphere_center[s][d];
+            r2 += e * e;
+        }
+        return r2 <= (double)sp->sphere_radius[s] * (double)sp->sphere_radius[s];
+    };
+    const int nobj = sp->n_boxes + sp->n_spheres;
+    for(int o = 0; o < nobj; ++o) {
+        double lo[3], hi[3];
+        for(int d = 0; d < 3; ++d) {
+            if(o < sp->n_boxes) {
+                lo[d] = sp->box_min[o][d];
+                hi[d] = sp->box_max[o][d];
+            } else {
+                int s = o - sp->n_boxes;
+                lo[d] = (double)sp->sphere_center[s][d] - sp->sphere_radius[s];
+                hi[d] = (double)sp->sphere_center[s][d] + sp->sphere_radius[s];
+            }
+        }
+        int64_t mlo[3], mhi[3], MN[3] = {NX, NY, NZ};
+        for(int d = 0; d < 3; ++d) {
+            mlo[d] = std::max<int64_t>(0, (int64_t)std::floor((lo[d] - tmp.bmin[d]) / h - 0.5) - 1);
+            mhi[d] = std::min<int64_t>(MN[d] - 1, (int64_t)std::ceil((hi[d] - tmp.bmin[d]) / h - 0.5) + 1);
+        }
+        for(int64_t mz = mlo[2]; mz <= mhi[2]; ++mz)
+            for(int64_t my = mlo[1]; my <= mhi[1]; ++my)
+                for(int64_t mx = mlo[0]; mx <= mhi[0]; ++mx) {
+                    double c[3] = {center(mx, 0), center(my, 1), center(mz, 2)};
+                    bool   in   = o < sp->n_boxes ? inBox(o, c) : inSph(o - sp->n_boxes, c);
+                    if(!in) continue;
+                    bool claimed = false;
+                    for(int q = 0; q < o && !claimed; ++q) claimed = q < sp->n_boxes ? inBox(q, c) : inSph(q - sp->n_boxes, c);
+                    if(claimed) continue;
+                    if(pos) {
+                        uint64_t gid = ((uint64_t)mz * (uint64_t)NY + (uint64_t)my) * (uint64_t)NX + (uint64_t)mx;
+                        for(int d = 0; d < 3; ++d) {
+                            double u = rand01(seed, gid, (uint32_t)d);
+                            pos[count * 3 + d] = (float)(c[d] + (u - 0.5) * (double)sp->jitter * h);
+                            double v0 = o < sp->n_boxes ? sp->box_vel[o][d] : sp->sphere_vel[o - sp->n_boxes][d];
+                            if(sp->vel_sigma > 0) {
+                                double u1 = rand01(seed, gid, 3u + 2u * d), u2 = rand01(seed, gid, 4u + 2u * d);
+                                double g  = std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(6.283185307179586 * u2);
+                                v0 += (double)sp->vel_sigma * g;
+                            }
+                            vel[count * 3 + d] = (float)v0;
+                        }
+                    }
+                    ++count;
+                }
+    }
+    return count;
+}
+
+void* orc_create(const orc_params* p, int precision, int threads) {
+    if(precision == 64) return static_cast<orc::Base*>(new orc::Solver<double>(*p, threads));
+    return static_cast<orc::Base*>(new orc::Solver<float>(*p, threads));
+}
+void orc_destroy(void* h) { delete static_cast<orc::Base*>(h); }
+
+}  // extern "C"
+
+namespace {
+template<class F> auto dispatch(void* h, F&& f) {
+    auto* b = static_cast<orc::Base*>(h);
+    if(auto* s = dynamic_cast<orc::Solver<double>*>(b)) return f(*s);
+    return f(*dynamic_cast<orc::Solver<float>*>(b));
+}
+}  // namespace
+
+extern "C" {
+
+void orc_grid_dims(void* h, int32_t* n) {
+    dispatch(h, [&](auto& s) { for(int d = 0; d < 3; ++d) n[d] = s.n[d]; return 0; });
+}
+void orc_set_precond(void* h, int32_t kind) {
+    dispatch(h, [&](auto& s) { s.precond = kind; return 0; });
+}
+void orc_set_cg(void* h, double tol, uint32_t maxIters) {
+    dispatch(h, [&](auto& s) {
+        using R = std::decay_t<decltype(s.dx)>;
+        s.tolFactor = std::max(R(tol), R(1e-30));
+        s.maxIters  = maxIters;
+        return 0;
+    });
+}
+void orc_set_particles(void* h, const float* pos, const float* vel, uint64_t n) {
+    dispatch(h, [&](auto& s) {
+        s.pos.resize(n);
+        s.vel.resize(n);
+        for(uint64_t i = 0; i < n; ++i)
+            for(int d = 0; d < 3; ++d) {
+                s.pos[i][d] = pos[3 * i + d];
+                s.vel[i][d] = vel[3 * i + d];
+            }
+        return 0;
+    });
+}
+uint64_t orc_num_particles(void* h) {
+    return dispatch(h, [&](auto& s) { return (uint64_t)s.pos.size(); });
+}
+void orc_get_particles(void* h, float* pos, float* vel) {
+    dispatch(h, [&](auto& s) {
+        for(size_t i = 0; i < s.pos.size(); ++i)
+            for(int d = 0; d < 3; ++d) {
+                if(pos) pos[3 * i + d] = (float)s.pos[i][d];
+                if(vel) vel[3 * i + d] = (float)s.vel[i][d];
+            }
+        return 0;
+    });
+}
+// copies a grid field as double (R fields), int8 (flags) or uint32 (active index); returns count
+uint64_t orc_get_field(void* h, int32_t f, void* dst) {
+    return dispatch(h, [&](auto& s) -> uint64_t {
+        using R = std::decay_t<decltype(s.dx)>;
+        size_t count;
+        int    kind;
+        void*  p = s.template fieldPtr<R>(f, count, kind);
+        if(!p || !dst) return count;
+        if(kind == 0) for(size_t i = 0; i < count; ++i) ((double*)dst)[i] = (double)((R*)p)[i];
+        if(kind == 1) std::memcpy(dst, p, count);
+        if(kind == 2) std::memcpy(dst, p, count * 4);
+        return count;
+    });
+}
+uint64_t orc_set_field(void* h, int32_t f, const void* src) {
+    return dispatch(h, [&](auto& s) -> uint64_t {
+        using R = std::decay_t<decltype(s.dx)>;
+        size_t count;
+        int    kind;
+        void*  p = s.template fieldPtr<R>(f, count, kind);
+        if(!p) return 0;
+        if(kind == 0) for(size_t i = 0; i < count; ++i) ((R*)p)[i] = (R)((const double*)src)[i];
+        if(kind == 1) std::memcpy(p, src, count);
+        if(kind == 2) std::memcpy(p, src, count * 4);
+        return count;
+    });
+}
+uint32_t orc_num_active(void* h) {
+    return dispatch(h, [&](auto& s) { return s.nActive; });
+}
+void orc_get_system(void* h, double* rhs, double* pressure) {
+    dispatch(h, [&](auto& s) {
+        for(uint32_t i = 0; i < s.nActive; ++i) {
+            if(rhs) rhs[i] = (double)s.rhs[i];
+            if(pressure) pressure[i] = (double)s.pressure[i];
+        }
+        return 0;
+    });
+}
+// row entries of the assembled matrix, 7 slots per row (col = -1 when unused)
+void orc_get_matrix(void* h, int32_t* cols, double* vals) {
+    dispatch(h, [&](auto& s) {
+        for(uint32_t i = 0; i < s.nActive; ++i) {
+            for(int c = 0; c < 7; ++c) { cols[7 * i + c] = -1; vals[7 * i + c] = 0; }
+            int c = 0;
+            for(const auto& e : s.rows[i]) {
+                if(c < 7) { cols[7 * i + c] = (int32_t)e.first; vals[7 * i + c] = (double)e.second; }
+                ++c;
+            }
+        }
+        return 0;
+    });
+}
+void orc_pcg_stats(void* h, uint32_t* iters, double* residual, int32_t* success) {
+    dispatch(h, [&](auto& s) {
+        *iters    = s.outIters;
+        *residual = (double)s.outResidual;
+        *success  = s.outSuccess ? 1 : 0;
+        return 0;
+    });
+}
+int32_t orc_step(void* h, float dt) {
+    return dispatch(h, [&](auto& s) { return s.step(dt) ? 0 : 1; });
+}
+int32_t orc_advance_frame(void* h, float* last_substep, uint32_t* substeps) {
+    return dispatch(h, [&](auto& s) {
+        int r = s.advanceFrame() ? 0 : 1;
+        if(last_substep) *last_substep = (float)s.lastSubstep;
+        if(substeps) *substeps = s.substepCount;
+        return r;
+    });
+}
+// individual stages, named after the reference methods
+void orc_move_particles(void* h, float dt) { dispatch(h, [&](auto& s) { s.moveParticles(dt); return 0; }); }
+void orc_map_particles_to_grid(void* h) { dispatch(h, [&](auto& s) { s.mapParticles2Grid(); return 0; }); }
+void orc_extrapolate_velocity(void* h) { dispatch(h, [&](auto& s) { s.extrapolateVelocity(); return 0; }); }
+void orc_constrain_grid_velocity(void* h) { dispatch(h, [&](auto& s) { s.constrainGridVelocity(); return 0; }); }
+void orc_backup_grid_velocity(void* h) { dispatch(h, [&](auto& s) { s.backupGridVelocity(); return 0; }); }
+void orc_add_gravity(void* h, float dt) { dispatch(h, [&](auto& s) { s.addGravity(dt); return 0; }); }
+void orc_compute_cell_weights(void* h) { dispatch(h, [&](auto& s) { s.computeCellWeights(); return 0; }); }
+void orc_compute_fluid_sdf(void* h) { dispatch(h, [&](auto& s) { s.computeFluidSDF(); return 0; }); }
+void orc_compute_system(void* h, float dt) { dispatch(h, [&](auto& s) { s.computeSystem(dt); return 0; }); }
+int32_t orc_solve_system(void* h) { return dispatch(h, [&](auto& s) { return s.solve() ? 0 : 1; }); }
+void orc_update_projected_velocity(void* h, float dt) { dispatch(h, [&](auto& s) { s.updateProjectedVelocity(dt); return 0; }); }
+void orc_compute_changes_grid_velocity(void* h) { dispatch(h, [&](auto& s) { s.computeChangesGridVelocity(); return 0; }); }
+void orc_map_grid_to_particles(void* h) { dispatch(h, [&](auto& s) { s.mapGrid2Particles(); return 0; }); }
+void orc_compute_boundary_sdf(void* h) { dispatch(h, [&](auto& s) { s.computeBoundarySDF(); return 0; }); }
+float orc_timestep_cfl(void* h) { return dispatch(h, [&](auto& s) { return (float)s.timestepCFL(); }); }
+
+// stable counting sort by linear cell index; out_perm[i] = source index of sorted slot i,
+// cell_start has ncells+1 entries (exclusive scan of the per-cell counts)
+void orc_bin_particles(const float* pos, uint64_t np, const float* bmin, float dx, const int32_t* n, uint32_t* cell_key,
+                       uint32_t* out_perm, uint32_t* cell_start) {
+    const uint64_t C = (uint64_t)n[0] * n[1] * n[2];
+    std::vector<uint32_t> count(C + 1, 0);
+    for(uint64_t p = 0; p < np; ++p) {
+        int c[3];
+        for(int d = 0; d < 3; ++d) {
+            c[d] = orc::cellIndexAxis(pos[3 * p + d], bmin[d], dx);
+            c[d] = std::min(std::max(c[d], 0), n[d] - 1);  // Grid::getValidCellIdx clamp (Grid.h:164-176)
+        }
+        uint32_t key = (uint32_t)(((uint64_t)c[2] * n[1] + c[1]) * n[0] + c[0]);
+        cell_key[p]  = key;
+        ++count[key];
+    }
+    uint32_t acc = 0;
+    for(uint64_t c = 0; c < C; ++c) {
+        cell_start[c] = acc;
+        acc += count[c];
+    }
+    cell_start[C] = acc;
+    std::vector<uint32_t> cur(cell_start, cell_start + C);
+    for(uint64_t p = 0; p < np; ++p) out_perm[cur[cell_key[p]]++] = (uint32_t)p;
+}
+void orc_cell_index(const float* pos, uint64_t np, const float* bmin, float dx, int32_t* out_ijk) {
+    for(uint64_t p = 0; p < np; ++p)
+        for(int d = 0; d < 3; ++d) out_ijk[3 * p + d] = orc::cellIndexAxis(pos[3 * p + d], bmin[d], dx);
+}
+void orc_morton_keys(const float* pos, uint64_t np, float inv_cell_size, uint64_t* keys) {
+    for(uint64_t p = 0; p < np; ++p) keys[p] = orc::mortonKeyOracle(pos + 3 * p, inv_cell_size);
+}
+// NeighborSearch::z_sort permutation made stable (ties by ascending index)
+void orc_morton_sort(const float* pos, uint64_t np, float inv_cell_size, uint32_t* perm) {
+    std::vector<uint64_t> k(np);
+    for(uint64_t p = 0; p < np; ++p) k[p] = orc::mortonKeyOracle(pos + 3 * p, inv_cell_size);
+    for(uint64_t p = 0; p < np; ++p) perm[p] = (uint32_t)p;
+    std::stable_sort(perm, perm + np, [&](uint32_t a, uint32_t b) { return k[a] < k[b]; });
+}
+
+}  // extern "C"

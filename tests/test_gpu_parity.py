@@ -1,0 +1,301 @@
+"""GPU parity: every stage of the gfx950 path against the oracle on identical seeded inputs.
+
+Bars: binning, cell indices and fluid/air labels bit-exact; particle
+velocities after one step <= 1e-5 relative L2 vs the FP32 oracle; converged pressure <= 1e-5 (FP32
+mode) / <= 1e-9 (refined) relative L2 vs the FP64 oracle on the same grid state.
+"""
+import numpy as np
+import pytest
+
+from banana_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DT = np.float32(1.0 / 600.0)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def make_pair(oracle, bnn, n=32, sphere=False, walls=False, seed=11, shuffle=False, vel_sigma=0.0, **extra):
+    params, spec = scenes.dam_break(n, sphere=sphere, boundary_sdf_at_init=walls)
+    spec["vel_sigma"] = vel_sigma
+    params.update(extra)
+    pos, vel = bnn.generate_scene_host(params, spec, seed)
+    if shuffle:
+        rng = np.random.default_rng(seed)
+        idx = rng.permutation(pos.shape[0])
+        pos, vel = pos[idx], vel[idx]
+    g = bnn.FLIPSolver(params)
+    g.set_particles(pos, vel)
+    o = oracle.Oracle(params)
+    o.set_particles(pos, vel)
+    return params, pos, vel, g, o
+
+
+def test_grid_dims_match(oracle, bnn):
+    for n in (24, 37, 64, 96):
+        params, _ = scenes.dam_break(n)
+        g = bnn.FLIPSolver(params)
+        o = oracle.Oracle(params)
+        assert g.n == o.n
+        dims, bmin, dx = scenes.grid_dims(params)
+        assert g.n == dims and np.array_equal(g.bmin, bmin) and g.dx == dx
+        g.close()
+
+
+def test_binning_bit_exact(oracle, bnn):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, shuffle=True)
+    g.bin_particles()
+    perm, start = g.bins()
+    key, perm_o, start_o = oracle.bin_particles(pos, g.bmin, g.dx, g.n)
+    assert np.array_equal(start, start_o)
+    assert np.array_equal(perm, perm_o)
+    # offsets are the exclusive scan of per-cell counts and the permutation is sorted by key
+    assert start[-1] == pos.shape[0]
+    assert np.all(np.diff(key[perm]) >= 0)
+
+
+def test_cell_index_kat(oracle, bnn):
+    """Grid/_Grid.Test.hpp:35-44 through the device binning: Grid(0,1,0.1) -> 10^3 cells,
+    (0.15,0.25,-0.5) binned to the clamped cell (1,2,0)."""
+    params = scenes.base_params(10)
+    params.update(box_min=(0.2, 0.2, 0.2), box_max=(0.8, 0.8, 0.8))  # bMin = 0, bMax = 1 after the band
+    params["cell_size"] = 0.1
+    params["n_expand_cells"] = 2
+    g = bnn.FLIPSolver(params)
+    assert g.n == (10, 10, 10)
+    pos = np.array([[0.15, 0.25, -0.5], [0.15, 0.25, 0.5], [0.25, 0.25, 0.5]], np.float32)
+    g.set_particles(pos, np.zeros_like(pos))
+    g.bin_particles()
+    perm, start = g.bins()
+    counts = np.diff(start.astype(np.int64)).reshape(10, 10, 10)
+    assert counts[0, 2, 1] == 1 and counts[5, 2, 1] == 1 and counts[5, 2, 2] == 1
+    assert counts.sum() == 3
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_fluid_sdf_labels_bit_exact(oracle, bnn, walls):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, walls=walls, shuffle=True)
+    g.compute_fluid_sdf()
+    o.compute_fluid_sdf()
+    phi_g = g.grid("phi")
+    phi_o = o.field(oracle.F_PHI).astype(np.float32)
+    assert np.array_equal(phi_g.view(np.uint32), phi_o.view(np.uint32)), np.abs(phi_g - phi_o).max()
+    assert np.array_equal(phi_g < 0, phi_o < 0)
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_weights_and_boundary_sdf_bit_exact(oracle, bnn, walls):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=24, walls=walls)
+    # before the first pressure projection the weights are the zero fill (PIC_Data.cpp:109)
+    assert not np.any(g.grid("weight_u")) and not np.any(g.grid("weight_w"))
+    o.compute_cell_weights()
+    g.compute_cell_weights()
+    b_g = g.grid("boundary_sdf")
+    assert np.array_equal(b_g, o.field(oracle.F_BOUNDARY_SDF).astype(np.float32))
+    for name, f in (("weight_u", oracle.F_WEIGHT_U), ("weight_v", oracle.F_WEIGHT_V), ("weight_w", oracle.F_WEIGHT_W)):
+        assert np.array_equal(g.grid(name), o.field(f).astype(np.float32)), name
+
+
+def test_p2g(oracle, bnn):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, vel_sigma=0.5, shuffle=True)
+    g.map_particles_to_grid()
+    o.map_particles_to_grid()
+    for c, f, fv in (("u", oracle.F_U, oracle.F_VALID_U), ("v", oracle.F_V, oracle.F_VALID_V), ("w", oracle.F_W, oracle.F_VALID_W)):
+        a = g.grid(c)
+        b = o.field(f)
+        assert np.array_equal(g.grid("valid_" + c), o.field(fv)), c
+        # same weights, different summation order: a few FP32 ulps of the face velocity scale
+        assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, np.abs(b).max()), c
+
+
+def test_extrapolate_bit_exact(oracle, bnn):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, vel_sigma=0.5)
+    o.map_particles_to_grid()
+    for c, f, fv in (("u", oracle.F_U, oracle.F_VALID_U), ("v", oracle.F_V, oracle.F_VALID_V), ("w", oracle.F_W, oracle.F_VALID_W)):
+        g.set_grid(c, o.field(f))
+        g.set_grid("valid_" + c, o.field(fv))
+    g.extrapolate_velocity()
+    o.extrapolate_velocity()
+    for c, f, fv, fe in (("u", oracle.F_U, oracle.F_VALID_U, oracle.F_EXTRAP_U), ("v", oracle.F_V, oracle.F_VALID_V, oracle.F_EXTRAP_V),
+                         ("w", oracle.F_W, oracle.F_VALID_W, oracle.F_EXTRAP_W)):
+        assert np.array_equal(g.grid(c), o.field(f).astype(np.float32)), c
+        assert np.array_equal(g.grid("valid_" + c), o.field(fv)), c
+        assert np.array_equal(g.grid("extrap_" + c), o.field(fe)), c
+
+
+@pytest.mark.parametrize("weights_computed", [False, True])
+def test_constrain_with_walls_bit_exact(oracle, bnn, weights_computed):
+    """With the boundary SDF set, constrainGridVelocity before the first computeCellWeights sees
+    the zero-filled weights (FLIP_Solver.cpp:45 runs before PIC_Solver.cpp:534), so every
+    extrapolated face is projected; afterwards only solid faces are."""
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=24, walls=True, vel_sigma=0.5)
+    if weights_computed:
+        o.compute_cell_weights()
+        g.compute_cell_weights()
+    o.map_particles_to_grid()
+    o.extrapolate_velocity()
+    for c, f, fv, fe in (("u", oracle.F_U, oracle.F_VALID_U, oracle.F_EXTRAP_U), ("v", oracle.F_V, oracle.F_VALID_V, oracle.F_EXTRAP_V),
+                         ("w", oracle.F_W, oracle.F_VALID_W, oracle.F_EXTRAP_W)):
+        g.set_grid(c, o.field(f))
+        g.set_grid("extrap_" + c, o.field(fe))
+    g.constrain_grid_velocity()
+    o.constrain_grid_velocity()
+    changed = 0
+    for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
+        assert np.array_equal(g.grid(c), o.field(f).astype(np.float32)), c
+
+
+def test_move_particles_bit_exact(oracle, bnn):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, walls=True, vel_sigma=1.0)
+    # a non-trivial grid velocity: P2G + extrapolation of the random particle velocities
+    o.map_particles_to_grid()
+    o.extrapolate_velocity()
+    for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
+        g.set_grid(c, o.field(f))
+    big = np.float32(0.05)  # large step: particles leave the box and get projected back
+    g.move_particles(big)
+    o.move_particles(big)
+    pg, _ = g.particles()
+    po, _ = o.particles()
+    assert np.array_equal(pg, po), np.abs(pg - po).max()
+
+
+def test_g2p_bit_exact(oracle, bnn):
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, vel_sigma=0.5, shuffle=True)
+    o.map_particles_to_grid()
+    o.extrapolate_velocity()
+    o.backup_grid_velocity()
+    o.add_gravity(DT)
+    for c, f, fo in (("u", oracle.F_U, oracle.F_OLD_U), ("v", oracle.F_V, oracle.F_OLD_V), ("w", oracle.F_W, oracle.F_OLD_W)):
+        g.set_grid(c, o.field(f))
+        g.set_grid("old_" + c, o.field(fo))
+    g.map_grid_to_particles()
+    o.compute_changes_grid_velocity()
+    o.map_grid_to_particles()
+    _, vg = g.particles()
+    _, vo = o.particles()
+    assert np.array_equal(vg, vo), np.abs(vg - vo).max()
+
+
+def test_dam_break_step1_known_answer(oracle, bnn):
+    """From rest with the reference's default boundary handling the RHS is exactly zero: PCG runs
+    0 iterations, particles do not move and every velocity is (0, -9.81 dt, 0)."""
+    params, spec = scenes.dam_break(64)
+    g = bnn.FLIPSolver(params)
+    g.generate_scene(spec, 1234)
+    pos0, _ = g.particles()
+    assert pos0.shape[0] == 502656
+    g.step(DT)
+    st = g.stats()
+    assert st["pcg_iterations"] == 0 and st["pcg_success"] == 1
+    pos, vel = g.particles()
+    assert np.array_equal(pos, pos0)
+    assert np.all(vel[:, 0] == 0) and np.all(vel[:, 2] == 0)
+    assert np.all(vel[:, 1] == np.float32(9.81) * DT * -1)
+
+
+def _oracle_state_before_projection(oracle, o):
+    o.map_particles_to_grid()
+    o.extrapolate_velocity()
+    o.constrain_grid_velocity()
+    o.backup_grid_velocity()
+    o.add_gravity(DT)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "refined"])
+def test_pressure_vs_fp64_oracle(oracle, bnn, mode):
+    params, spec = scenes.dam_break(32, sphere=True, boundary_sdf_at_init=True)
+    spec["vel_sigma"] = 0.5
+    pos, vel = bnn.generate_scene_host(params, spec, 5)
+    o32 = oracle.Oracle(params)
+    o32.set_particles(pos, vel)
+    o32.compute_cell_weights()
+    _oracle_state_before_projection(oracle, o32)
+    o32.compute_fluid_sdf()
+    state = {f: o32.field(f) for f in (oracle.F_U, oracle.F_V, oracle.F_W, oracle.F_WEIGHT_U, oracle.F_WEIGHT_V,
+                                       oracle.F_WEIGHT_W, oracle.F_PHI, oracle.F_BOUNDARY_SDF)}
+    # FP64 oracle on exactly this grid state
+    o64 = oracle.Oracle(params, precision=64)
+    for f, a in state.items():
+        o64.set_field(f, a)
+    o64.compute_system(DT)
+    o64.set_cg(1e-15, 20000)
+    o64.solve_system()
+    _, p64 = o64.system()
+    it64, res64, ok64 = o64.pcg_stats()
+    assert ok64
+    o64.update_projected_velocity(DT)
+    # device: same particles (phi is recomputed on the device, bit-exact) and grid velocities
+    extra = dict(pressure_mode=1 if mode == "refined" else 0, refine_tolerance=1e-11)
+    g = bnn.FLIPSolver(params, **extra)
+    g.set_particles(pos, vel)
+    for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
+        g.set_grid(c, state[f])
+    g.pressure_projection(DT)
+    assert np.array_equal(g.grid("phi"), state[oracle.F_PHI].astype(np.float32))
+    pg = g.pressure()
+    assert pg.shape == p64.shape
+    err = rel_l2(pg, p64)
+    st = g.stats()
+    print(f"{mode}: pressure rel L2 vs FP64 oracle = {err:.3e}, iters={st['pcg_iterations']} outer={st['outer_iterations']}")
+    assert err <= (1e-9 if mode == "refined" else 1e-5)
+    for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
+        ev = rel_l2(g.grid(c), o64.field(f))
+        assert ev <= (1e-6 if mode == "refined" else 1e-5), (c, ev)
+
+
+CASES = [dict(sphere=False, walls=True), dict(sphere=True, walls=True), dict(sphere=True, walls=False)]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_step_parity(oracle, bnn, case):
+    """Whole substeps with the pressure solved accurately on both sides: the device in REFINED mode
+    (FP64 residual <= 1e-10), the FP32 oracle with the reference's IC(0)-PCG run to its FP32 floor
+    (CGRelativeTolerance 1e-7), and the FP64 oracle to 1e-14."""
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=case["sphere"], walls=case["walls"],
+                                       cg_relative_tolerance=1e-7, pressure_mode=1, refine_tolerance=1e-10)
+    o64 = oracle.Oracle(params, precision=64)
+    o64.set_particles(pos, vel)
+    o64.set_cg(1e-14, 20000)
+    for s in range(3):
+        g.step(DT)
+        o.step(DT)
+        o64.step(DT)
+        pg, vg = g.particles()
+        po, vo = o.particles()
+        p64, v64 = o64.particles()
+        e32, e64 = rel_l2(vg, vo), rel_l2(vg, v64)
+        print(f"step {s + 1}: vel rel L2 vs FP32 oracle {e32:.3e}, vs FP64 oracle {e64:.3e}, "
+              f"max |dpos| {np.abs(pg - po).max():.3e}, iters gpu {g.stats()['pcg_iterations']} oracle {o.pcg_stats()[0]}")
+        if s == 0:
+            assert e32 <= 1e-5 and e64 <= 1e-5
+        assert e32 <= 1e-4 and e64 <= 1e-4
+        assert np.abs(pg - po).max() <= 1e-5
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_step_plain_fp32_mode(oracle, bnn, case):
+    """The plain FP32 mode at the reference's stopping rule (max|r| < 1e-6 max|rhs|): device
+    Jacobi-PCG and oracle IC(0)-PCG each carry their own truncation error of the pressure solve, so
+    their difference is bounded by the solver error, not by FP32 rounding."""
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=case["sphere"], walls=case["walls"],
+                                       cg_relative_tolerance=1e-6)
+    o64 = oracle.Oracle(params, precision=64)
+    o64.set_particles(pos, vel)
+    o64.set_cg(1e-14, 20000)
+    g.step(DT)
+    o.step(DT)
+    o64.step(DT)
+    _, vg = g.particles()
+    _, vo = o.particles()
+    _, v64 = o64.particles()
+    e_dev, e_ref = rel_l2(vg, v64), rel_l2(vo, v64)
+    print(f"plain FP32 step: device vs FP64 {e_dev:.3e}, FP32 oracle vs FP64 {e_ref:.3e}, device vs FP32 oracle {rel_l2(vg, vo):.3e}")
+    assert g.stats()["pcg_success"] == 1
+    assert e_dev <= 1e-2 and e_ref <= 1e-2
