@@ -35,7 +35,8 @@ class Params(C.Structure):
         ("boundary_sdf_at_init", C.c_int32), ("correct_position", C.c_int32),
         ("exit_if_pressure_projection_failed", C.c_int32), ("enable_sort_particle", C.c_int32),
         ("sort_frequency", C.c_uint32), ("pressure_mode", C.c_int32), ("refine_tolerance", C.c_double),
-        ("preconditioner", C.c_int32), ("timing", C.c_int32),
+        ("preconditioner", C.c_int32), ("timing", C.c_int32), ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float),
+        ("mg_coarsest_sweeps", C.c_int32),
     ]
 
 

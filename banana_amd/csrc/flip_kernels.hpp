@@ -696,24 +696,26 @@ __device__ __forceinline__ double fold(const double* __restrict__ p, int n, bool
 __global__ void __launch_bounds__(512) k_pcg_init(const float* __restrict__ b, const float* __restrict__ invdiag,
                                                   float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                                                   float* __restrict__ s, uint32_t nSlots, double* __restrict__ pa,
-                                                  double* __restrict__ pb) {
+                                                  double* __restrict__ pb, int jacobi) {
     __shared__ double sh[8];
     double            rz = 0, m = 0;
     for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
         const size_t o  = (size_t)slot * 512 + threadIdx.x;
         const float  bv = b[o];
-        const float  zv = invdiag[o] * bv;
         x[o]            = 0.f;
         r[o]            = bv;
-        z[o]            = zv;
         s[o]            = 0.f;
-        rz += (double)zv * (double)bv;
+        if(jacobi) {
+            const float zv = invdiag[o] * bv;
+            z[o]           = zv;
+            rz += (double)zv * (double)bv;
+        }
         m = fmax(m, (double)fabsf(bv));
     }
-    rz = block_sum<512>(rz, sh);
-    m  = block_max<512>(m, sh);
+    if(jacobi) rz = block_sum<512>(rz, sh);
+    m = block_max<512>(m, sh);
     if(threadIdx.x == 0) {
-        pa[blockIdx.x] = rz;
+        if(jacobi) pa[blockIdx.x] = rz;
         pb[blockIdx.x] = m;
     }
 }
@@ -750,7 +752,7 @@ __global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nb
                                                   const float* __restrict__ cy, const float* __restrict__ cz, uint32_t nSlots,
                                                   const PcgState* __restrict__ st, double* __restrict__ pa) {
     __shared__ double sh[8];
-    if(st->done) return;
+    if(st && st->done) return;
     const float beta = (float)st->beta;
     const int   l    = threadIdx.x;
     double      d    = 0;
@@ -781,7 +783,7 @@ __global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nb
 // tmp = s.q ; alpha = rho / tmp (PCGSolver.cpp:174-180)
 __global__ void __launch_bounds__(1024) k_pcg_alpha(PcgState* st, const double* __restrict__ pa, int n) {
     __shared__ double sh[16];
-    if(st->done) return;
+    if(st && st->done) return;
     const double A = fold(pa, n, false, sh);
     if(threadIdx.x != 0) return;
     st->tmp = A;
@@ -798,26 +800,28 @@ __global__ void __launch_bounds__(1024) k_pcg_alpha(PcgState* st, const double* 
 __global__ void __launch_bounds__(512) k_pcg_update(const float* __restrict__ s, const float* __restrict__ q,
                                                     const float* __restrict__ invdiag, float* __restrict__ x, float* __restrict__ r,
                                                     float* __restrict__ z, uint32_t nSlots, const PcgState* __restrict__ st,
-                                                    double* __restrict__ pa, double* __restrict__ pb) {
+                                                    double* __restrict__ pa, double* __restrict__ pb, int jacobi) {
     __shared__ double sh[8];
-    if(st->done) return;
+    if(st && st->done) return;
     const float alpha = (float)st->alpha;
     double      rz = 0, m = 0;
     for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
         const size_t o  = (size_t)slot * 512 + threadIdx.x;
         const float  xv = x[o] + alpha * s[o];
         const float  rv = r[o] + (-alpha) * q[o];
-        const float  zv = invdiag[o] * rv;
         x[o]            = xv;
         r[o]            = rv;
-        z[o]            = zv;
-        rz += (double)zv * (double)rv;
+        if(jacobi) {  // z = M^-1 r fused here; the multigrid preconditioner runs after this kernel
+            const float zv = invdiag[o] * rv;
+            z[o]           = zv;
+            rz += (double)zv * (double)rv;
+        }
         m = fmax(m, (double)fabsf(rv));
     }
-    rz = block_sum<512>(rz, sh);
-    m  = block_max<512>(m, sh);
+    if(jacobi) rz = block_sum<512>(rz, sh);
+    m = block_max<512>(m, sh);
     if(threadIdx.x == 0) {
-        pa[blockIdx.x] = rz;
+        if(jacobi) pa[blockIdx.x] = rz;
         pb[blockIdx.x] = m;
     }
 }
@@ -825,7 +829,7 @@ __global__ void __launch_bounds__(512) k_pcg_update(const float* __restrict__ s,
 // res = max|r| ; stop test (PCGSolver.cpp:182-187) ; beta = rz / rho (:190-194)
 __global__ void __launch_bounds__(1024) k_pcg_beta(PcgState* st, const double* __restrict__ pa, const double* __restrict__ pb, int n) {
     __shared__ double sh[16];
-    if(st->done) return;
+    if(st && st->done) return;
     const double A = fold(pa, n, false, sh);
     const double B = fold(pb, n, true, sh);
     if(threadIdx.x != 0) return;
@@ -981,6 +985,224 @@ __global__ void __launch_bounds__(kBlock) k_maxabs(const float* __restrict__ a, 
         for(int i = 1; i < kBlock / 64; ++i) m = fmaxf(m, sh[i]);
         atomicMax(out, __float_as_uint(m));
     }
+}
+
+}  // namespace bnn
+
+namespace bnn {
+
+//==================================================================================================
+// Multigrid preconditioner (extension; the reference's IC(0) is sequential). Level l+1 aggregates
+// 2x2x2 cells of level l (piecewise-constant prolongation P); its operator is the Galerkin product
+// P^T A P, which for a 7-point operator is again 7-point:
+//   diag_C = sum_c diag_c + 2 * (couplings between children of C)
+//   cxp_C  = sum of the 4 +x couplings leaving C's children with odd local x, etc.
+// Every level stores 8^3-cell bricks (slot tables as on the fine level). The V-cycle is symmetric
+// (red-black Gauss-Seidel: red then black before the coarse correction, black then red after), so
+// M^-1 is symmetric positive definite and usable inside CG.
+//==================================================================================================
+struct LevelDev {
+    int            nx, ny, nz, nbx, nby, nbz;
+    const int32_t* brickSlot;
+    const int32_t* slotBrick;
+    const int32_t* nbr;
+    const float *  diag, *cx, *cy, *cz, *invdiag;
+    float *        x, *b, *r;
+    uint32_t       nSlots;
+};
+
+__device__ __forceinline__ int32_t level_cell_index(const LevelDev& L, int i, int j, int k) {
+    if(i < 0 || j < 0 || k < 0 || i >= L.nx || j >= L.ny || k >= L.nz) return -1;
+    const int32_t s = L.brickSlot[((k >> 3) * L.nby + (j >> 3)) * L.nbx + (i >> 3)];
+    return s < 0 ? -1 : s * 512 + ((k & 7) * 8 + (j & 7)) * 8 + (i & 7);
+}
+
+// flag the coarse bricks that hold children of active fine bricks
+__global__ void __launch_bounds__(kBlock) k_mg_flag_parents(const int32_t* __restrict__ fineSlotBrick, uint32_t nFine, int fnbx,
+                                                            int fnby, int cnbx, int cnby, uint32_t* __restrict__ cflag) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if(s >= nFine) return;
+    const int b = fineSlotBrick[s];
+    const int bx = b % fnbx, by = (b / fnbx) % fnby, bz = b / (fnbx * fnby);
+    cflag[((bz >> 1) * cnby + (by >> 1)) * cnbx + (bx >> 1)] = 1u;
+}
+
+// Galerkin coefficients of the coarse level (one block per coarse brick, thread = coarse cell)
+__global__ void __launch_bounds__(512) k_mg_coarsen(LevelDev F, LevelDev C, float* __restrict__ cdiag, float* __restrict__ ccx,
+                                                    float* __restrict__ ccy, float* __restrict__ ccz, float* __restrict__ cinv) {
+    const int s = blockIdx.x, l = threadIdx.x;
+    const int b = C.slotBrick[s];
+    const int bx = b % C.nbx, by = (b / C.nbx) % C.nby, bz = b / (C.nbx * C.nby);
+    const int I = bx * 8 + (l & 7), J = by * 8 + ((l >> 3) & 7), K = bz * 8 + (l >> 6);
+    float     d = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, inner = 0.f;
+    for(int c = 0; c < 8; ++c) {
+        const int     a0 = c & 1, a1 = (c >> 1) & 1, a2 = (c >> 2) & 1;
+        const int32_t f  = level_cell_index(F, 2 * I + a0, 2 * J + a1, 2 * K + a2);
+        if(f < 0) continue;
+        d += F.diag[f];
+        if(a0) cx += F.cx[f];
+        else inner += F.cx[f];
+        if(a1) cy += F.cy[f];
+        else inner += F.cy[f];
+        if(a2) cz += F.cz[f];
+        else inner += F.cz[f];
+    }
+    d += 2.f * inner;
+    const size_t o = (size_t)s * 512 + l;
+    cdiag[o]       = d;
+    ccx[o]         = cx;
+    ccy[o]         = cy;
+    ccz[o]         = cz;
+    cinv[o]        = d > 0.f ? 1.0f / d : 0.f;
+}
+
+__device__ __forceinline__ float level_offdiag_sum(const LevelDev& L, int s, int l, const float* __restrict__ x) {
+    const int32_t ixm = nbr_index(L.nbr, s, l, 0), ixp = nbr_index(L.nbr, s, l, 1);
+    const int32_t iym = nbr_index(L.nbr, s, l, 2), iyp = nbr_index(L.nbr, s, l, 3);
+    const int32_t izm = nbr_index(L.nbr, s, l, 4), izp = nbr_index(L.nbr, s, l, 5);
+    const size_t  o   = (size_t)s * 512 + l;
+    float         acc = 0.f;
+    if(izm >= 0) acc += L.cz[izm] * x[izm];
+    if(iym >= 0) acc += L.cy[iym] * x[iym];
+    if(ixm >= 0) acc += L.cx[ixm] * x[ixm];
+    if(ixp >= 0) acc += L.cx[o] * x[ixp];
+    if(iyp >= 0) acc += L.cy[o] * x[iyp];
+    if(izp >= 0) acc += L.cz[o] * x[izp];
+    return acc;
+}
+
+__device__ __forceinline__ int cell_color(const LevelDev& L, int s, int l) {
+    const int b = L.slotBrick[s];
+    const int bx = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
+    return ((bx * 8 + (l & 7)) + (by * 8 + ((l >> 3) & 7)) + (bz * 8 + (l >> 6))) & 1;
+}
+
+// one red-black half sweep: cells of `color` get x = (b - sum a x_nbr) / diag. With zeroInit the
+// other colour is set to 0 first (x starts at 0 in every V-cycle), which needs no neighbour reads.
+__global__ void __launch_bounds__(512) k_mg_smooth(LevelDev L, int color, int zeroInit, const PcgState* __restrict__ st) {
+    if(st && st->done) return;
+    for(uint32_t slot = blockIdx.x; slot < L.nSlots; slot += gridDim.x) {
+        const int    s = (int)slot, l = threadIdx.x;
+        const size_t o = (size_t)s * 512 + l;
+        if(cell_color(L, s, l) != color) {
+            if(zeroInit) L.x[o] = 0.f;
+            continue;
+        }
+        const float off = zeroInit ? 0.f : level_offdiag_sum(L, s, l, L.x);
+        L.x[o]          = (L.b[o] - off) * L.invdiag[o];
+    }
+}
+
+// r = b - A x
+__global__ void __launch_bounds__(512) k_mg_residual(LevelDev L, const PcgState* __restrict__ st) {
+    if(st && st->done) return;
+    for(uint32_t slot = blockIdx.x; slot < L.nSlots; slot += gridDim.x) {
+        const int    s = (int)slot, l = threadIdx.x;
+        const size_t o = (size_t)s * 512 + l;
+        L.r[o]         = L.b[o] - (L.diag[o] * L.x[o] + level_offdiag_sum(L, s, l, L.x));
+    }
+}
+
+// b_coarse(C) = sum of the fine residuals of C's 8 children (P^T r)
+__global__ void __launch_bounds__(512) k_mg_restrict(LevelDev F, LevelDev C, const PcgState* __restrict__ st) {
+    if(st && st->done) return;
+    for(uint32_t slot = blockIdx.x; slot < C.nSlots; slot += gridDim.x) {
+        const int s = (int)slot, l = threadIdx.x;
+        const int b = C.slotBrick[s];
+        const int bx = b % C.nbx, by = (b / C.nbx) % C.nby, bz = b / (C.nbx * C.nby);
+        const int I = bx * 8 + (l & 7), J = by * 8 + ((l >> 3) & 7), K = bz * 8 + (l >> 6);
+        float     sum = 0.f;
+        for(int c = 0; c < 8; ++c) {
+            const int32_t f = level_cell_index(F, 2 * I + (c & 1), 2 * J + ((c >> 1) & 1), 2 * K + ((c >> 2) & 1));
+            if(f >= 0) sum += F.r[f];
+        }
+        C.b[(size_t)s * 512 + l] = sum;
+    }
+}
+
+// x_fine += omega * x_coarse(parent) (P x_c, scaled)
+__global__ void __launch_bounds__(512) k_mg_prolong(LevelDev F, LevelDev C, float omega, const PcgState* __restrict__ st) {
+    if(st && st->done) return;
+    for(uint32_t slot = blockIdx.x; slot < F.nSlots; slot += gridDim.x) {
+        const int    s = (int)slot, l = threadIdx.x;
+        const int    b = F.slotBrick[s];
+        const int    bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+        const int    i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+        const size_t o = (size_t)s * 512 + l;
+        if(F.invdiag[o] == 0.f) continue;  // not an unknown
+        const int32_t c = level_cell_index(C, i >> 1, j >> 1, k >> 1);
+        if(c >= 0) F.x[o] += omega * C.x[c];
+    }
+}
+
+// coarsest level: x = 0, then `sweeps` red-black sweeps followed by `sweeps` black-red sweeps
+// (symmetric), all inside one block
+__global__ void __launch_bounds__(1024) k_mg_coarsest(LevelDev L, int sweeps, const PcgState* __restrict__ st) {
+    if(st && st->done) return;
+    const uint32_t n = L.nSlots * 512;
+    for(uint32_t t = threadIdx.x; t < n; t += blockDim.x) L.x[t] = 0.f;
+    __syncthreads();
+    for(int it = 0; it < 2 * sweeps; ++it) {
+        for(int half = 0; half < 2; ++half) {
+            const int color = (it < sweeps) ? half : 1 - half;
+            for(uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+                const int s = (int)(t >> 9), l = (int)(t & 511);
+                if(cell_color(L, s, l) != color) continue;
+                L.x[t] = (L.b[t] - level_offdiag_sum(L, s, l, L.x)) * L.invdiag[t];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// partial z.r for the PCG (after the V-cycle wrote z)
+__global__ void __launch_bounds__(512) k_pcg_dot_zr(const float* __restrict__ z, const float* __restrict__ r, uint32_t nSlots,
+                                                    const PcgState* __restrict__ st, double* __restrict__ pa) {
+    __shared__ double sh[8];
+    if(st && st->done) return;
+    double d = 0;
+    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+        const size_t o = (size_t)slot * 512 + threadIdx.x;
+        d += (double)z[o] * (double)r[o];
+    }
+    d = block_sum<512>(d, sh);
+    if(threadIdx.x == 0) pa[blockIdx.x] = d;
+}
+
+}  // namespace bnn
+
+namespace bnn {
+
+// coarsest level when it is a single brick (no neighbour bricks): x = 0, then `sweeps` red-black
+// sweeps and `sweeps` black-red sweeps with x held in LDS, one thread per cell
+__global__ void __launch_bounds__(512) k_mg_coarsest1(LevelDev L, int sweeps, const PcgState* __restrict__ st) {
+    __shared__ float xs[512];
+    if(st && st->done) return;
+    const int   l = threadIdx.x;
+    const int   lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const float b = L.b[l], inv = L.invdiag[l];
+    const float cxp = L.cx[l], cyp = L.cy[l], czp = L.cz[l];
+    const float cxm = lx > 0 ? L.cx[l - 1] : 0.f, cym = ly > 0 ? L.cy[l - 8] : 0.f, czm = lz > 0 ? L.cz[l - 64] : 0.f;
+    const int   color = cell_color(L, 0, l);
+    xs[l]             = 0.f;
+    __syncthreads();
+    for(int it = 0; it < 2 * sweeps; ++it) {
+        for(int half = 0; half < 2; ++half) {
+            const int c = (it < sweeps) ? half : 1 - half;
+            if(color == c) {
+                float off = 0.f;
+                if(lz > 0) off += czm * xs[l - 64];
+                if(ly > 0) off += cym * xs[l - 8];
+                if(lx > 0) off += cxm * xs[l - 1];
+                if(lx < 7) off += cxp * xs[l + 1];
+                if(ly < 7) off += cyp * xs[l + 8];
+                if(lz < 7) off += czp * xs[l + 64];
+                xs[l] = (b - off) * inv;
+            }
+            __syncthreads();
+        }
+    }
+    L.x[l] = xs[l];
 }
 
 }  // namespace bnn

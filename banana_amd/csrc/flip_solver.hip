@@ -271,7 +271,11 @@ public:
         if(prm.advection_steps < 1) throw Error(BNN_ERR_INVALID_ARG, "AdvectionSteps must be >= 1");
         if(prm.correct_position) throw Error(BNN_ERR_UNSUPPORTED, "CorrectPosition (PIC_Solver.cpp:377-485) is not implemented");
         if(prm.gravity_type < 0 || prm.gravity_type > 3) throw Error(BNN_ERR_INVALID_ARG, "GravityType must be 0..3");
-        if(prm.preconditioner != BNN_PRECOND_JACOBI) throw Error(BNN_ERR_UNSUPPORTED, "only the Jacobi preconditioner is built");
+        if(prm.preconditioner != BNN_PRECOND_JACOBI && prm.preconditioner != BNN_PRECOND_MG)
+            throw Error(BNN_ERR_INVALID_ARG, "unknown preconditioner");
+        if(prm.preconditioner == BNN_PRECOND_MG && (prm.mg_sweeps < 1 || prm.mg_sweeps > 8 || prm.mg_coarsest_sweeps < 1 ||
+                                                    prm.mg_coarsest_sweeps > 512 || !(prm.mg_omega > 0.f) || prm.mg_omega >= 2.f))
+            throw Error(BNN_ERR_INVALID_ARG, "multigrid settings out of range");
         const GridDims G = grid_dims(prm);
         const double   C = (double)G.n[0] * G.n[1] * G.n[2];
         if(C > 1.0e9 || G.n[0] < 3 || G.n[1] < 3 || G.n[2] < 3) throw Error(BNN_ERR_INVALID_ARG, "grid size out of range");
@@ -609,7 +613,116 @@ public:
         hipLaunchKernelGGL(k_assemble, dim3(nSlots), dim3(512), 0, stream, (const int32_t*)slotBrick.as<int32_t>(), g,
                            (const float*)phi.as<float>(), G, (const float*)wu.as<float>(), (const float*)wv.as<float>(),
                            (const float*)ww.as<float>(), dt, S);
+        if(prm.preconditioner == BNN_PRECOND_MG) buildHierarchy();
         systemValid = true;
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // multigrid hierarchy (rebuilt after every assembly when the MG preconditioner is selected)
+    struct MgLevel {
+        int      nx = 0, ny = 0, nz = 0, nbx = 0, nby = 0, nbz = 0;
+        uint32_t nb = 0, nSlots = 0, slotCap = 0;
+        DevBuf   flag, scan, brickSlot, slotBrick, nbr, diag, cx, cy, cz, inv, x, b, r;
+    };
+    std::vector<std::unique_ptr<MgLevel>> mg;  // mg[0] unused (the fine level lives in the PCG buffers)
+    int                                   mgLevels = 0;
+    static constexpr uint32_t             kCoarsestSlots = 1;  // coarsen until one brick remains
+
+    LevelDev fineLevel() {
+        return LevelDev{g.nx, g.ny, g.nz, g.nbx, g.nby, g.nbz, brickSlot.as<int32_t>(), slotBrick.as<int32_t>(), nbrTab.as<int32_t>(),
+                        diag32.as<float>(), cx32.as<float>(), cy32.as<float>(), cz32.as<float>(), invd.as<float>(),
+                        z.as<float>(), r.as<float>(), q.as<float>(), nSlots};
+    }
+    LevelDev levelDev(int l) {
+        if(l == 0) return fineLevel();
+        MgLevel& L = *mg[l];
+        return LevelDev{L.nx, L.ny, L.nz, L.nbx, L.nby, L.nbz, L.brickSlot.as<int32_t>(), L.slotBrick.as<int32_t>(), L.nbr.as<int32_t>(),
+                        L.diag.as<float>(), L.cx.as<float>(), L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>(),
+                        L.x.as<float>(), L.b.as<float>(), L.r.as<float>(), L.nSlots};
+    }
+
+    void buildHierarchy() {
+        mgLevels = 1;
+        if(mg.empty()) mg.emplace_back(nullptr);
+        int      nx = g.nx, ny = g.ny, nz = g.nz;
+        uint32_t prevSlots = nSlots;
+        while(prevSlots > kCoarsestSlots && (nx > 8 || ny > 8 || nz > 8)) {
+            const int l = mgLevels;
+            if((int)mg.size() <= l) mg.emplace_back(new MgLevel());
+            MgLevel&       L    = *mg[l];
+            const LevelDev prev = levelDev(l - 1);
+            nx = (nx + 1) / 2;
+            ny = (ny + 1) / 2;
+            nz = (nz + 1) / 2;
+            L.nx = nx; L.ny = ny; L.nz = nz;
+            L.nbx = (nx + 7) / 8; L.nby = (ny + 7) / 8; L.nbz = (nz + 7) / 8;
+            L.nb  = (uint32_t)(L.nbx * L.nby * L.nbz);
+            L.flag.reserve(L.nb * 4);
+            L.scan.reserve(L.nb * 4);
+            L.brickSlot.reserve(L.nb * 4);
+            HIP_OK(hipMemsetAsync(L.flag.p, 0, L.nb * 4, stream));
+            HIP_OK(hipMemsetAsync(counters.p, 0, 16, stream));
+            hipLaunchKernelGGL(k_mg_flag_parents, dim3(blocks(prev.nSlots)), dim3(kBlock), 0, stream, prev.slotBrick, prev.nSlots, prev.nbx,
+                               prev.nby, L.nbx, L.nby, L.flag.as<uint32_t>());
+            scan(L.flag.as<uint32_t>(), L.scan.as<uint32_t>(), L.nb, counters.as<uint32_t>() + 2);
+            HIP_OK(hipMemcpyAsync(hostU32, counters.p, 16, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            L.nSlots = hostU32[2];
+            if(L.nSlots == 0 || L.nSlots > L.nb) throw Error(BNN_ERR_DEVICE, "multigrid level has an invalid brick count");
+            if(L.nSlots > L.slotCap) {
+                const uint32_t c  = L.nSlots + L.nSlots / 4 + 4;
+                const size_t   nv = (size_t)c * 512 * 4;
+                for(DevBuf* b : {&L.diag, &L.cx, &L.cy, &L.cz, &L.inv, &L.x, &L.b, &L.r}) b->reserve(nv);
+                L.slotBrick.reserve(c * 4);
+                L.nbr.reserve((size_t)c * 24);
+                L.slotCap = c;
+            }
+            GridGeom lg = g;
+            lg.nx = nx; lg.ny = ny; lg.nz = nz;
+            lg.nbx = L.nbx; lg.nby = L.nby; lg.nbz = L.nbz;
+            hipLaunchKernelGGL(k_brick_slots, dim3(blocks(L.nb)), dim3(kBlock), 0, stream, (const uint32_t*)L.flag.as<uint32_t>(),
+                               (const uint32_t*)L.scan.as<uint32_t>(), L.nb, L.brickSlot.as<int32_t>(), L.slotBrick.as<int32_t>());
+            hipLaunchKernelGGL(k_brick_nbrs, dim3(blocks(L.nSlots)), dim3(kBlock), 0, stream, (const int32_t*)L.slotBrick.as<int32_t>(),
+                               L.nSlots, (const int32_t*)L.brickSlot.as<int32_t>(), lg, L.nbr.as<int32_t>());
+            const LevelDev cur = levelDev(l);
+            hipLaunchKernelGGL(k_mg_coarsen, dim3(L.nSlots), dim3(512), 0, stream, prev, cur, L.diag.as<float>(), L.cx.as<float>(),
+                               L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>());
+            prevSlots = L.nSlots;
+            ++mgLevels;
+        }
+    }
+
+    static unsigned gridFor(uint32_t slots) { return std::min<unsigned>(std::max<uint32_t>(slots, 1u), (unsigned)kMaxRedBlocks); }
+
+    // z = M^-1 r on the fine level (b = r, x = z); symmetric V-cycle
+    void vcycle(const PcgState* stp) {
+        const int   nu    = prm.mg_sweeps;
+        const float omega = prm.mg_omega;
+        for(int l = 0; l + 1 < mgLevels; ++l) {
+            const LevelDev L = levelDev(l), C = levelDev(l + 1);
+            const unsigned G = gridFor(L.nSlots);
+            for(int sw = 0; sw < nu; ++sw) {
+                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 0, sw == 0 ? 1 : 0, stp);
+                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 1, 0, stp);
+            }
+            hipLaunchKernelGGL(k_mg_residual, dim3(G), dim3(512), 0, stream, L, stp);
+            hipLaunchKernelGGL(k_mg_restrict, dim3(gridFor(C.nSlots)), dim3(512), 0, stream, L, C, stp);
+        }
+        const LevelDev Lc = levelDev(mgLevels - 1);
+        if(Lc.nSlots == 1) {
+            hipLaunchKernelGGL(k_mg_coarsest1, dim3(1), dim3(512), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
+        } else {
+            hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(1024), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
+        }
+        for(int l = mgLevels - 2; l >= 0; --l) {
+            const LevelDev L = levelDev(l), C = levelDev(l + 1);
+            const unsigned G = gridFor(L.nSlots);
+            hipLaunchKernelGGL(k_mg_prolong, dim3(G), dim3(512), 0, stream, L, C, omega, stp);
+            for(int sw = 0; sw < nu; ++sw) {
+                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 1, 0, stp);
+                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 0, 0, stp);
+            }
+        }
     }
 
     unsigned redBlocks() const { return std::min<unsigned>(std::max<uint32_t>(nSlots, 1u), (unsigned)kMaxRedBlocks); }
@@ -621,8 +734,14 @@ public:
         double*        dpb = pb.as<double>();
         float *        sA = s0.as<float>(), *sB = s1.as<float>();
         const unsigned G   = redBlocks();
+        const int      jac = prm.preconditioner == BNN_PRECOND_JACOBI ? 1 : 0;
         hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(512), 0, stream, b, (const float*)invd.as<float>(), x.as<float>(), r.as<float>(),
-                           z.as<float>(), sA, nSlots, dpa, dpb);
+                           z.as<float>(), sA, nSlots, dpa, dpb, jac);
+        if(!jac) {
+            vcycle(nullptr);
+            hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(), nSlots,
+                               (const PcgState*)nullptr, dpa);
+        }
         hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (const double*)dpb, (int)G, relTol,
                            maxIter);
         uint32_t chunk = 4, launched = 0;
@@ -643,8 +762,13 @@ public:
                 ktBegin(BNN_KT_PCG_UPDATE);
                 hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(512), 0, stream, (const float*)sB, (const float*)q.as<float>(),
                                    (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst,
-                                   dpa, dpb);
+                                   dpa, dpb, jac);
                 ktEnd(BNN_KT_PCG_UPDATE);
+                if(!jac) {
+                    vcycle(dst);
+                    hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(),
+                                       nSlots, (const PcgState*)dst, dpa);
+                }
                 hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (const double*)dpb, (int)G);
                 std::swap(sA, sB);
             }
@@ -1028,8 +1152,11 @@ void bnn_flip_default_params(bnn_flip_params* p) {
     p->sort_frequency                 = 10;
     p->pressure_mode                  = BNN_PRESSURE_FP32;
     p->refine_tolerance               = 1e-10;
-    p->preconditioner                 = BNN_PRECOND_JACOBI;
+    p->preconditioner                 = BNN_PRECOND_MG;
     p->timing                         = 0;
+    p->mg_sweeps                      = 2;
+    p->mg_omega                       = 1.8f;
+    p->mg_coarsest_sweeps             = 24;
 }
 
 bnn_status bnn_flip_create(const bnn_flip_params* p, bnn_flip** out) {

@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="grid edge in cells (256 = the headline workload)")
     ap.add_argument("--pressure", choices=["refined", "fp32"], default="refined")
+    ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="grid edge of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3, help="substeps of the CPU baseline sample")
@@ -83,7 +84,8 @@ def main():
     from banana_amd import scenes
 
     params, spec = scenes.dam_break(args.n, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
-    params.update(pressure_mode=1 if args.pressure == "refined" else 0, refine_tolerance=1e-10, timing=args.timing)
+    params.update(pressure_mode=1 if args.pressure == "refined" else 0, refine_tolerance=1e-10, timing=args.timing,
+                  preconditioner=1 if args.precond == "mg" else 0)
     dt = np.float32(1.0 / 600.0)
     t0 = time.perf_counter()
     pos, vel = banana_amd.generate_scene_host(params, spec, 2024)
@@ -167,7 +169,7 @@ def main():
         "config": {"workload": f"FLIP dam break {args.n}^3 + falling sphere, 8 particles/cell, solid walls, dt=1/600, "
                                f"pressure {'FP32 PCG under FP64 defect correction to 1e-10' if args.pressure == 'refined' else 'FP32 PCG rel-tol 1e-6'}",
                    "grid": list(sim.n), "particles": np_, "pcg_iterations_per_step": pcg_iters / args.steps,
-                   "preconditioner": "jacobi", "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()}},
+                   "preconditioner": args.precond, "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -84,8 +84,14 @@ typedef struct bnn_flip_params {
     /* extensions */
     int32_t  pressure_mode;                 /* bnn_pressure_mode (FP32) */
     double   refine_tolerance;              /* REFINED outer tolerance (1e-10) */
-    int32_t  preconditioner;                /* bnn_preconditioner (JACOBI) */
-    int32_t  timing;                        /* 1: record per-stage HIP events (stats.stage_ms) */
+    int32_t  preconditioner;                /* bnn_preconditioner (MG) */
+    int32_t  timing;                        /* 1: record per-stage HIP events (stats.stage_ms),
+                                               2: also per-launch events of the timed kernels */
+    int32_t  mg_sweeps;                     /* MG: red-black sweeps before and after the coarse
+                                               correction on each level (2) */
+    float    mg_omega;                      /* MG: scale of the coarse-grid correction (1.8; the
+                                               piecewise-constant coarse space under-corrects) */
+    int32_t  mg_coarsest_sweeps;            /* MG: symmetric sweeps on the coarsest level (24) */
 } bnn_flip_params;
 
 /* Seeded scene generator input (replaces ParticleGenerator, ParticleGenerator.hpp:87-136): one

@@ -208,8 +208,9 @@ def _oracle_state_before_projection(oracle, o):
     o.add_gravity(DT)
 
 
+@pytest.mark.parametrize("precond", [0, 1], ids=["jacobi", "mg"])
 @pytest.mark.parametrize("mode", ["fp32", "refined"])
-def test_pressure_vs_fp64_oracle(oracle, bnn, mode):
+def test_pressure_vs_fp64_oracle(oracle, bnn, mode, precond):
     params, spec = scenes.dam_break(32, sphere=True, boundary_sdf_at_init=True)
     spec["vel_sigma"] = 0.5
     pos, vel = bnn.generate_scene_host(params, spec, 5)
@@ -232,7 +233,7 @@ def test_pressure_vs_fp64_oracle(oracle, bnn, mode):
     assert ok64
     o64.update_projected_velocity(DT)
     # device: same particles (phi is recomputed on the device, bit-exact) and grid velocities
-    extra = dict(pressure_mode=1 if mode == "refined" else 0, refine_tolerance=1e-11)
+    extra = dict(pressure_mode=1 if mode == "refined" else 0, refine_tolerance=1e-11, preconditioner=precond)
     g = bnn.FLIPSolver(params, **extra)
     g.set_particles(pos, vel)
     for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
@@ -243,7 +244,8 @@ def test_pressure_vs_fp64_oracle(oracle, bnn, mode):
     assert pg.shape == p64.shape
     err = rel_l2(pg, p64)
     st = g.stats()
-    print(f"{mode}: pressure rel L2 vs FP64 oracle = {err:.3e}, iters={st['pcg_iterations']} outer={st['outer_iterations']}")
+    print(f"{mode}/{'mg' if precond else 'jacobi'}: pressure rel L2 vs FP64 oracle = {err:.3e}, "
+          f"iters={st['pcg_iterations']} outer={st['outer_iterations']}")
     assert err <= (1e-9 if mode == "refined" else 1e-5)
     for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
         ev = rel_l2(g.grid(c), o64.field(f))
@@ -253,13 +255,14 @@ def test_pressure_vs_fp64_oracle(oracle, bnn, mode):
 CASES = [dict(sphere=False, walls=True), dict(sphere=True, walls=True), dict(sphere=True, walls=False)]
 
 
+@pytest.mark.parametrize("precond", [0, 1], ids=["jacobi", "mg"])
 @pytest.mark.parametrize("case", CASES)
-def test_step_parity(oracle, bnn, case):
+def test_step_parity(oracle, bnn, case, precond):
     """Whole substeps with the pressure solved accurately on both sides: the device in REFINED mode
     (FP64 residual <= 1e-10), the FP32 oracle with the reference's IC(0)-PCG run to its FP32 floor
     (CGRelativeTolerance 1e-7), and the FP64 oracle to 1e-14."""
     params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=case["sphere"], walls=case["walls"],
-                                       cg_relative_tolerance=1e-7, pressure_mode=1, refine_tolerance=1e-10)
+                                       cg_relative_tolerance=1e-7, pressure_mode=1, refine_tolerance=1e-10, preconditioner=precond)
     o64 = oracle.Oracle(params, precision=64)
     o64.set_particles(pos, vel)
     o64.set_cg(1e-14, 20000)
@@ -299,3 +302,25 @@ def test_step_plain_fp32_mode(oracle, bnn, case):
     print(f"plain FP32 step: device vs FP64 {e_dev:.3e}, FP32 oracle vs FP64 {e_ref:.3e}, device vs FP32 oracle {rel_l2(vg, vo):.3e}")
     assert g.stats()["pcg_success"] == 1
     assert e_dev <= 1e-2 and e_ref <= 1e-2
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_mg_preconditioner_cuts_iterations(oracle, bnn, n):
+    """Same system, same tolerance: the V-cycle preconditioner needs far fewer CG iterations than
+    Jacobi and converges to the same pressure."""
+    params, spec = scenes.dam_break(n, sphere=True, boundary_sdf_at_init=True)
+    spec["vel_sigma"] = 0.5
+    pos, vel = bnn.generate_scene_host(params, spec, 6)
+    res = {}
+    for precond in (0, 1):
+        g = bnn.FLIPSolver(params, pressure_mode=1, refine_tolerance=1e-10, preconditioner=precond)
+        g.set_particles(pos, vel)
+        g.step(DT)
+        st = g.stats()
+        assert st["pcg_success"] == 1
+        res[precond] = (st["pcg_iterations"], g.pressure(), g.particles()[1])
+        g.close()
+    print(f"n={n}: iterations jacobi {res[0][0]}, mg {res[1][0]}")
+    assert res[1][0] * 3 < res[0][0]
+    assert rel_l2(res[1][1], res[0][1]) <= 1e-8
+    assert rel_l2(res[1][2], res[0][2]) <= 1e-6
