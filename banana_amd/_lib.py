@@ -36,7 +36,7 @@ class Params(C.Structure):
         ("exit_if_pressure_projection_failed", C.c_int32), ("enable_sort_particle", C.c_int32),
         ("sort_frequency", C.c_uint32), ("pressure_mode", C.c_int32), ("refine_tolerance", C.c_double),
         ("preconditioner", C.c_int32), ("timing", C.c_int32), ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float),
-        ("mg_coarsest_sweeps", C.c_int32),
+        ("mg_coarsest_sweeps", C.c_int32), ("start_frame", C.c_uint32), ("final_frame", C.c_uint32),
     ]
 
 
@@ -56,6 +56,7 @@ class Stats(C.Structure):
         ("pcg_residual", C.c_double), ("rhs_max", C.c_double), ("pcg_success", C.c_int32),
         ("last_substep", C.c_float), ("substep_count", C.c_uint32),
         ("stage_ms", C.c_double * NUM_STAGES), ("stage_bytes", C.c_double * NUM_STAGES), ("steps", C.c_uint64),
+        ("finished_frame", C.c_uint32),
         ("kernel_ms", C.c_double * NUM_KERNEL_TIMERS), ("kernel_launches", C.c_uint64 * NUM_KERNEL_TIMERS),
     ]
 
@@ -66,6 +67,7 @@ SIGNATURES = {
     "bnn_flip_default_params": (None, [_P(Params)]),
     "bnn_flip_create": (_i32, [_P(Params), _P(_vp)]),
     "bnn_flip_create_from_json": (_i32, [C.c_char_p, _u64, _P(_vp)]),
+    "bnn_scene_from_json": (_i32, [C.c_char_p, _P(Params), _P(SceneSpec), C.c_char_p, C.c_size_t]),
     "bnn_flip_destroy": (_i32, [_vp]),
     "bnn_flip_last_error": (C.c_char_p, [_vp]),
     "bnn_flip_get_params": (_i32, [_vp, _P(Params)]),

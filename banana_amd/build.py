@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "flip_solver.hip")
 OUT = os.path.join(HERE, "libbnn_flip.so")
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("flip_solver.hip", "flip_kernels.hpp", "flip_device.hpp")] + [
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("flip_solver.hip", "flip_kernels.hpp", "flip_device.hpp", "scene_json.hpp")] + [
     os.path.join(os.path.dirname(HERE), "include", "bnn_flip.h")]
 
 HIPCC_FLAGS = [
@@ -25,6 +25,10 @@ def hipcc():
             return c
 
 
+CLI_SRC = os.path.join(HERE, "csrc", "bnn_flip_cli.cpp")
+CLI_OUT = os.path.join(HERE, "bnn_flip_cli")
+
+
 def build(force=True, verbose=False):
     if not force and os.path.exists(OUT) and all(os.path.getmtime(OUT) >= os.path.getmtime(d) for d in DEPS):
         return OUT
@@ -34,6 +38,13 @@ def build(force=True, verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    # the command-line driver links the library (rpath = its own directory)
+    cli = ["g++", "-std=c++17", "-O2", "-o", CLI_OUT + ".tmp", CLI_SRC, "-L" + HERE, "-lbnn_flip",
+           "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cli), flush=True)
+    subprocess.check_call(cli)
+    os.replace(CLI_OUT + ".tmp", CLI_OUT)
     return OUT
 
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/bnn_flip.h"
+#include "scene_json.hpp"
 
 namespace bnn {
 
@@ -1007,6 +1008,7 @@ public:
             ++count;
         }
         ++finishedFrame;
+        st.finished_frame = finishedFrame;
         st.substep_count = count;
         st.last_substep  = last;
         return ok;
@@ -1157,6 +1159,8 @@ void bnn_flip_default_params(bnn_flip_params* p) {
     p->mg_sweeps                      = 2;
     p->mg_omega                       = 1.8f;
     p->mg_coarsest_sweeps             = 24;
+    p->start_frame                    = 1;
+    p->final_frame                    = 1;
 }
 
 bnn_status bnn_flip_create(const bnn_flip_params* p, bnn_flip** out) {
@@ -1176,12 +1180,42 @@ bnn_status bnn_flip_create(const bnn_flip_params* p, bnn_flip** out) {
     }
 }
 
+bnn_status bnn_scene_from_json(const char* json_text, bnn_flip_params* p, bnn_scene_spec* spec, char* warnings, size_t cap) {
+    if(!json_text || !p || !spec) return BNN_ERR_INVALID_ARG;
+    try {
+        bnn_flip_default_params(p);
+        const bnn::json::Value root = bnn::json::Parser(json_text).parse();
+        const std::string      w    = bnn::scene_from_json(root, *p, *spec);
+        if(warnings && cap) {
+            std::strncpy(warnings, w.c_str(), cap - 1);
+            warnings[cap - 1] = 0;
+        }
+        return BNN_OK;
+    } catch(const bnn::SceneLoadError& e) {
+        g_createError = e.what();
+        return e.code;
+    } catch(const std::exception& e) {
+        g_createError = e.what();
+        return BNN_ERR_INVALID_ARG;
+    }
+}
+
 bnn_status bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_flip** out) {
-    (void)json_text;
-    (void)seed;
-    if(out) *out = nullptr;
-    g_createError = "scene JSON loading is not built yet";
-    return BNN_ERR_UNSUPPORTED;
+    if(!out) return BNN_ERR_INVALID_ARG;
+    *out = nullptr;
+    bnn_flip_params p;
+    bnn_scene_spec  sp;
+    bnn_status      st = bnn_scene_from_json(json_text, &p, &sp, nullptr, 0);
+    if(st != BNN_OK) return st;
+    st = bnn_flip_create(&p, out);
+    if(st != BNN_OK) return st;
+    st = bnn_flip_generate_scene(*out, &sp, seed);
+    if(st != BNN_OK) {
+        g_createError = bnn_flip_last_error(*out);
+        bnn_flip_destroy(*out);
+        *out = nullptr;
+    }
+    return st;
 }
 
 bnn_status bnn_flip_destroy(bnn_flip* h) {

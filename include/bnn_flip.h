@@ -92,6 +92,7 @@ typedef struct bnn_flip_params {
     float    mg_omega;                      /* MG: scale of the coarse-grid correction (1.8; the
                                                piecewise-constant coarse space under-corrects) */
     int32_t  mg_coarsest_sweeps;            /* MG: symmetric sweeps on the coarsest level (24) */
+    uint32_t start_frame, final_frame;      /* GlobalParameters StartFrame / FinalFrame (1, 1) */
 } bnn_flip_params;
 
 /* Seeded scene generator input (replaces ParticleGenerator, ParticleGenerator.hpp:87-136): one
@@ -160,6 +161,7 @@ typedef struct bnn_flip_stats {
     double   stage_ms[BNN_NUM_STAGES];    /* last step, HIP events on the solver stream (timing=1) */
     double   stage_bytes[BNN_NUM_STAGES]; /* algorithmic bytes of the last step per stage */
     uint64_t steps;                /* steps since creation */
+    uint32_t finished_frame;       /* GlobalParameters::finishedFrame (ParticleSolverData.h:50) */
     double   kernel_ms[BNN_NUM_KERNEL_TIMERS];       /* summed launch durations, last step */
     uint64_t kernel_launches[BNN_NUM_KERNEL_TIMERS]; /* launches timed, last step */
 } bnn_flip_stats;
@@ -170,8 +172,13 @@ typedef struct bnn_flip bnn_flip;
 void        bnn_flip_default_params(bnn_flip_params* p);
 bnn_status  bnn_flip_create(const bnn_flip_params* p, bnn_flip** out);
 /* scene JSON in the reference format (GlobalParameters / SimulationParameters /
- * ParticleGenerators with Box or Sphere geometry), ParticleSolver.cpp:30-168 */
+ * ParticleGenerators with Box or Sphere geometry), ParticleSolver.cpp:30-168: creates the solver
+ * and generates the particles with the seeded generator */
 bnn_status  bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_flip** out);
+/* host-only: parse a scene JSON into parameters (defaults first) and a generator spec; warnings
+ * (unknown keys, ignored IO options) are written to `warnings` (may be NULL) */
+bnn_status  bnn_scene_from_json(const char* json_text, bnn_flip_params* p, bnn_scene_spec* spec, char* warnings,
+                                size_t warnings_cap);
 bnn_status  bnn_flip_destroy(bnn_flip* h);
 const char* bnn_flip_last_error(const bnn_flip* h);
 bnn_status  bnn_flip_get_params(const bnn_flip* h, bnn_flip_params* p);
