@@ -61,6 +61,28 @@ class Stats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("workers", C.c_uint32),
+                ("batching", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("scene_id", C.c_uint64), ("kind", C.c_int32), ("status", C.c_int32),
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("batch_size", C.c_uint32), ("pcg_iterations", C.c_uint32), ("pcg_success", C.c_int32),
+                ("pad", C.c_int32), ("pcg_residual", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("batches", C.c_uint64),
+                ("scenes_resident", C.c_uint64), ("scenes_waiting", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("peak_resident_bytes", C.c_uint64), ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("latency_mean_ms", C.c_double), ("latency_max_ms", C.c_double), ("throughput_rps", C.c_double),
+                ("mean_batch", C.c_double), ("busy_s", C.c_double)]
+
+
+REQ_STEP, REQ_PRESSURE = 0, 1
+
 # every exported symbol of include/bnn_flip.h: name -> (restype, argtypes)
 _vp, _f32, _u32, _u64, _i32, _P = C.c_void_p, C.c_float, C.c_uint32, C.c_uint64, C.c_int32, C.POINTER
 SIGNATURES = {
@@ -100,6 +122,26 @@ SIGNATURES = {
     "bnn_flip_get_stats": (_i32, [_vp, _P(Stats)]),
     "bnn_flip_set_grid": (_i32, [_vp, _i32, _vp]),
     "bnn_flip_synchronize": (_i32, [_vp]),
+    "bnn_flip_footprint": (_i32, [_P(Params), _u64, _P(_u64)]),
+    "bnn_flip_batch_create": (_i32, [_P(_vp)]),
+    "bnn_flip_batch_destroy": (_i32, [_vp]),
+    "bnn_flip_batch_last_error": (C.c_char_p, [_vp]),
+    "bnn_flip_batch_set_timing": (_i32, [_vp, _i32]),
+    "bnn_flip_batch_step": (_i32, [_vp, _P(_vp), _P(_f32), _u32, _P(_i32)]),
+    "bnn_flip_batch_advance_frame": (_i32, [_vp, _P(_vp), _u32, _P(_i32)]),
+    "bnn_flip_batch_pressure_projection": (_i32, [_vp, _P(_vp), _P(_f32), _u32, _P(_i32)]),
+    "bnn_server_create": (_i32, [_P(ServerConfig), _P(_vp)]),
+    "bnn_server_destroy": (_i32, [_vp]),
+    "bnn_server_last_error": (C.c_char_p, [_vp]),
+    "bnn_server_now": (C.c_double, [_vp]),
+    "bnn_server_add_scene": (_i32, [_vp, _P(Params), _P(SceneSpec), _u64, _P(_u64)]),
+    "bnn_server_release_scene": (_i32, [_vp, _u64]),
+    "bnn_server_submit": (_i32, [_vp, _u64, _i32, _f32, _P(_u64)]),
+    "bnn_server_wait": (_i32, [_vp, _u64, C.c_double, _P(RequestResult)]),
+    "bnn_server_drain": (_i32, [_vp, C.c_double]),
+    "bnn_server_get_particles": (_i32, [_vp, _u64, _vp, _vp, _P(_u64)]),
+    "bnn_server_get_stats": (_i32, [_vp, _P(ServerStats)]),
+    "bnn_server_reset_stats": (_i32, [_vp]),
 }
 
 _lib = None

@@ -1,0 +1,528 @@
+// flip_serving.hpp -- many independent scenes on one GPU (included by flip_solver.hip after the
+// FlipSolver class): a worker pool that runs scenes concurrently, each scene on its own HIP stream,
+// and a request server with HBM-footprint admission and latency accounting.
+//
+// Every scene keeps its own device state and stream, so a scene's results are the same bits
+// whether it runs alone or next to others; concurrency comes from several host workers keeping
+// several streams (hardware queues) busy while each of them waits on its own readbacks.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace bnn {
+
+// device bytes a FlipSolver with these parameters and n particles may hold: the grid, particles
+// and the pressure system sized for every brick active (its largest size)
+static size_t scene_footprint(const bnn_flip_params& prm, uint64_t n) {
+    const GridDims G = grid_dims(prm);
+    const size_t   nx = G.n[0], ny = G.n[1], nz = G.n[2];
+    const size_t   C = nx * ny * nz, N = (nx + 1) * (ny + 1) * (nz + 1);
+    const size_t   nU = (nx + 1) * ny * nz, nV = nx * (ny + 1) * nz, nW = nx * ny * (nz + 1), F = nU + nV + nW;
+    size_t         nbx = (nx + 7) / 8, nby = (ny + 7) / 8, nbz = (nz + 7) / 8, nb = nbx * nby * nbz;
+    size_t         b = 16 * F + 2 * F + std::max(nU, std::max(nV, nW)) + 12 * C + 4 * N + 4 + 4 * (blocks(std::max(C, nb), kScanTile) + 1) +
+               256 + 12 * nb + 2 * sizeof(PcgState);
+    const size_t slotsCap = nb + nb / 4 + 16;
+    b += 12 * 4 * 512 * slotsCap + 28 * slotsCap + 2 * 8 * kMaxRedBlocks;
+    b += (prm.pressure_mode == BNN_PRESSURE_REFINED ? 6 : 1) * 8 * 512 * slotsCap;
+    if(prm.preconditioner == BNN_PRECOND_MG) {
+        size_t lx = nx, ly = ny, lz = nz;
+        while(lx > 8 || ly > 8 || lz > 8) {
+            lx = (lx + 1) / 2, ly = (ly + 1) / 2, lz = (lz + 1) / 2;
+            const size_t lnb = ((lx + 7) / 8) * ((ly + 7) / 8) * ((lz + 7) / 8), cap = lnb + lnb / 4 + 4;
+            b += 12 * lnb + 28 * cap + 8 * 4 * 512 * cap;
+        }
+    }
+    if(n) b += 88 * (n + n / 8 + 1024);
+    return b;
+}
+
+// fixed set of host threads; run(n, f) calls f(i) for i < n across the threads and returns when all
+// calls have finished
+class WorkerPool {
+public:
+    explicit WorkerPool(unsigned threads) {
+        for(unsigned t = 0; t < std::max(threads, 1u); ++t) th.emplace_back([this] { loop(); });
+    }
+    ~WorkerPool() {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            stop = true;
+        }
+        cv.notify_all();
+        for(auto& t : th) t.join();
+    }
+    unsigned size() const { return (unsigned)th.size(); }
+    void     run(uint32_t n, const std::function<void(uint32_t)>& f) {
+        std::unique_lock<std::mutex> lk(m);
+        job  = &f;
+        next = 0;
+        todo = n;
+        left = n;
+        ++epoch;
+        cv.notify_all();
+        done.wait(lk, [&] { return left == 0; });
+        job = nullptr;
+    }
+
+private:
+    std::vector<std::thread>            th;
+    std::mutex                          m;
+    std::condition_variable             cv, done;
+    const std::function<void(uint32_t)>* job  = nullptr;
+    uint32_t                            next = 0, todo = 0, left = 0;
+    uint64_t                            epoch = 0;
+    bool                                stop  = false;
+    void                                loop() {
+        std::unique_lock<std::mutex> lk(m);
+        for(;;) {
+            cv.wait(lk, [&] { return stop || (job && next < todo); });
+            if(stop) return;
+            const uint32_t i = next++;
+            const auto*    f = job;
+            lk.unlock();
+            (*f)(i);
+            lk.lock();
+            if(--left == 0) done.notify_all();
+        }
+    }
+};
+
+//==================================================================================================
+// request server
+//==================================================================================================
+class Server {
+public:
+    explicit Server(const bnn_server_config& c) : cfg(c) {
+        if(!cfg.hbm_budget_bytes) cfg.hbm_budget_bytes = 200000000000ull;
+        if(!cfg.workers) cfg.workers = 4;
+        if(!cfg.batching) cfg.workers = 1;
+        if(cfg.workers > 16) throw Error(BNN_ERR_INVALID_ARG, "at most 16 server workers");
+        t0 = std::chrono::steady_clock::now();
+        for(uint32_t w = 0; w < cfg.workers; ++w) threads.emplace_back([this] { workerLoop(); });
+    }
+
+    ~Server() {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            stopping = true;
+        }
+        cv.notify_all();
+        for(auto& t : threads) t.join();
+        scenes.clear();
+    }
+
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+    uint64_t addScene(const bnn_flip_params& p, const bnn_scene_spec& spec, uint64_t seed) {
+        const uint64_t n     = generate_scene(p, spec, seed, nullptr, nullptr, 0);
+        const size_t   bytes = scene_footprint(p, n);
+        if(bytes > cfg.hbm_budget_bytes) throw Error(BNN_ERR_REJECTED, "scene footprint exceeds the HBM budget");
+        auto s   = std::make_unique<Served>();
+        s->prm   = p;
+        s->spec  = spec;
+        s->seed  = seed;
+        s->np    = n;
+        s->bytes = bytes;
+        std::unique_lock<std::mutex> lk(m);
+        const uint64_t               id = ++lastSceneId;
+        s->id                           = id;
+        waiting.push_back(id);
+        scenes[id] = std::move(s);
+        lk.unlock();
+        cv.notify_all();
+        return id;
+    }
+
+    void releaseScene(uint64_t id) {
+        std::unique_ptr<Served> dead;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            auto                         it = scenes.find(id);
+            if(it == scenes.end()) throw Error(BNN_ERR_INVALID_ARG, "unknown scene");
+            Served& s = *it->second;
+            s.release = true;
+            if(s.state == Served::Waiting) {
+                waiting.erase(std::remove(waiting.begin(), waiting.end(), id), waiting.end());
+                dead = std::move(it->second);
+                scenes.erase(it);
+            } else if(s.state == Served::Ready && !s.busy && s.queued == 0) {
+                dead = takeLocked(it);
+            }
+        }
+        cv.notify_all();
+    }
+
+    uint64_t submit(uint64_t sceneId, int kind, float dt) {
+        if(kind != BNN_REQ_STEP && kind != BNN_REQ_PRESSURE) throw Error(BNN_ERR_INVALID_ARG, "unknown request kind");
+        if(!(dt > 0)) throw Error(BNN_ERR_INVALID_ARG, "dt must be > 0");
+        std::unique_lock<std::mutex> lk(m);
+        auto                         it = scenes.find(sceneId);
+        if(it == scenes.end() || it->second->release) throw Error(BNN_ERR_INVALID_ARG, "unknown scene");
+        const uint64_t id = ++lastReqId;
+        Req&           r  = reqs[id];
+        r.res.request_id  = id;
+        r.res.scene_id    = sceneId;
+        r.res.kind        = kind;
+        r.res.t_submit    = now();
+        r.dt              = dt;
+        it->second->queued += 1;
+        queue.push_back(id);
+        ++st.submitted;
+        if(firstSubmit < 0) firstSubmit = r.res.t_submit;
+        lk.unlock();
+        cv.notify_all();
+        return id;
+    }
+
+    bool wait(uint64_t id, double timeout, bnn_request_result* out) {
+        std::unique_lock<std::mutex> lk(m);
+        if(reqs.find(id) == reqs.end()) throw Error(BNN_ERR_INVALID_ARG, "unknown request");
+        auto isDone = [&] { return reqs[id].done; };
+        if(timeout < 0) doneCv.wait(lk, isDone);
+        else if(!doneCv.wait_for(lk, std::chrono::duration<double>(timeout), isDone)) return false;
+        if(out) *out = reqs[id].res;
+        return true;
+    }
+
+    bool drain(double timeout) {
+        std::unique_lock<std::mutex> lk(m);
+        auto idle = [&] { return queue.empty() && running == 0 && creating == 0; };
+        if(timeout < 0) doneCv.wait(lk, idle);
+        else if(!doneCv.wait_for(lk, std::chrono::duration<double>(timeout), idle)) return false;
+        return true;
+    }
+
+    void getParticles(uint64_t id, float* pos, float* vel, uint64_t* n) {
+        FlipSolver* sc = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            auto                         it = scenes.find(id);
+            if(it == scenes.end()) throw Error(BNN_ERR_INVALID_ARG, "unknown scene");
+            Served& s = *it->second;
+            if(s.state != Served::Ready || s.busy || !s.solver) throw Error(BNN_ERR_STATE, "scene is not resident and idle");
+            s.busy = true;
+            sc     = s.solver.get();
+        }
+        std::string msg;
+        bnn_status  code = BNN_OK;
+        try {
+            if(n) *n = sc->np;
+            if(pos || vel) sc->getParticles(pos, vel);
+        } catch(const Error& e) {
+            code = e.code;
+            msg  = e.what();
+        }
+        std::unique_ptr<Served> dead;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            auto                         it = scenes.find(id);
+            if(it != scenes.end()) {
+                it->second->busy = false;
+                if(it->second->release && it->second->queued == 0) dead = takeLocked(it);
+            }
+        }
+        cv.notify_all();
+        if(code != BNN_OK) throw Error(code, msg);
+    }
+
+    bnn_server_stats stats() {
+        std::unique_lock<std::mutex> lk(m);
+        bnn_server_stats             o = st;
+        o.scenes_resident = o.scenes_waiting = 0;
+        for(auto& kv : scenes) (kv.second->state == Served::Waiting ? o.scenes_waiting : o.scenes_resident) += 1;
+        o.resident_bytes      = residentBytes;
+        o.peak_resident_bytes = peakBytes;
+        std::vector<double> lat(latencies);
+        double              sum = 0, mx = 0;
+        for(double l : lat) {
+            sum += l;
+            mx = std::max(mx, l);
+        }
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double q) {
+            if(lat.empty()) return 0.0;
+            size_t k = (size_t)std::ceil(q * (double)lat.size());
+            k        = k ? k - 1 : 0;
+            return lat[std::min(k, lat.size() - 1)];
+        };
+        o.latency_p50_ms  = pct(0.50) * 1e3;
+        o.latency_p99_ms  = pct(0.99) * 1e3;
+        o.latency_mean_ms = lat.empty() ? 0 : sum / lat.size() * 1e3;
+        o.latency_max_ms  = mx * 1e3;
+        o.throughput_rps  = (firstSubmit >= 0 && lastDone > firstSubmit) ? (double)lat.size() / (lastDone - firstSubmit) : 0;
+        o.mean_batch      = (double)cfg.workers;
+        o.busy_s          = busy;
+        return o;
+    }
+
+    void resetStats() {
+        std::unique_lock<std::mutex> lk(m);
+        st = bnn_server_stats{};
+        latencies.clear();
+        firstSubmit = lastDone = -1;
+        busy                   = 0;
+    }
+
+    std::string lastError() const {
+        std::unique_lock<std::mutex> lk(errMutex);
+        return err;
+    }
+
+private:
+    struct Served {
+        enum State { Waiting, Creating, Ready } state = Waiting;
+        uint64_t                    id = 0;
+        bnn_flip_params             prm{};
+        bnn_scene_spec              spec{};
+        uint64_t                    seed = 0, np = 0;
+        size_t                      bytes = 0;
+        std::unique_ptr<FlipSolver> solver;
+        bool                        busy    = false;
+        bool                        release = false;
+        uint32_t                    queued  = 0;  // requests submitted and not finished
+    };
+    struct Req {
+        bnn_request_result res{};
+        float              dt   = 0;
+        bool               done = false;
+    };
+
+    bnn_server_config                           cfg;
+    std::chrono::steady_clock::time_point       t0;
+    mutable std::mutex                          m, errMutex;
+    std::condition_variable                     cv, doneCv;
+    std::vector<std::thread>                    threads;
+    std::map<uint64_t, std::unique_ptr<Served>> scenes;
+    std::deque<uint64_t>                        waiting;  // scenes in arrival order
+    std::deque<uint64_t>                        queue;    // requests in arrival order
+    std::unordered_map<uint64_t, Req>           reqs;
+    uint64_t                                    lastSceneId = 0, lastReqId = 0;
+    size_t                                      residentBytes = 0, peakBytes = 0;
+    uint32_t                                    running = 0, creating = 0;
+    bool                                        stopping = false;
+    bnn_server_stats                            st{};
+    std::vector<double>                         latencies;
+    double                                      firstSubmit = -1, lastDone = -1, busy = 0;
+    std::string                                 err;
+
+    void setError(const std::string& e) {
+        std::unique_lock<std::mutex> lk(errMutex);
+        err = e;
+    }
+
+    std::unique_ptr<Served> takeLocked(std::map<uint64_t, std::unique_ptr<Served>>::iterator it) {
+        std::unique_ptr<Served> s = std::move(it->second);
+        scenes.erase(it);
+        residentBytes -= s->bytes;
+        return s;
+    }
+
+    // the first waiting scene (arrival order) whose footprint fits; later, smaller ones may pass
+    Served* admitLocked() {
+        for(auto it = waiting.begin(); it != waiting.end(); ++it) {
+            Served& s = *scenes[*it];
+            if(residentBytes + s.bytes <= cfg.hbm_budget_bytes) {
+                residentBytes += s.bytes;
+                peakBytes = std::max(peakBytes, residentBytes);
+                s.state   = Served::Creating;
+                waiting.erase(it);
+                return &s;
+            }
+        }
+        return nullptr;
+    }
+
+    // the first queued request whose scene is resident and idle; requests of a scene run in
+    // submission order (a later request never passes an earlier one of the same scene)
+    bool pickLocked(uint64_t& rid) {
+        std::unordered_map<uint64_t, bool> seen;
+        for(auto it = queue.begin(); it != queue.end(); ++it) {
+            const uint64_t sid = reqs[*it].res.scene_id;
+            Served&        s   = *scenes[sid];
+            if(!seen[sid] && s.state == Served::Ready && !s.busy) {
+                s.busy = true;
+                rid    = *it;
+                queue.erase(it);
+                return true;
+            }
+            seen[sid] = true;
+        }
+        return false;
+    }
+
+    bool admissibleLocked() {
+        for(uint64_t id : waiting)
+            if(residentBytes + scenes[id]->bytes <= cfg.hbm_budget_bytes) return true;
+        return false;
+    }
+    bool runnableLocked() {
+        for(uint64_t id : queue) {
+            const Served& s = *scenes[reqs[id].res.scene_id];
+            if(s.state == Served::Ready && !s.busy) return true;
+        }
+        return false;
+    }
+
+    void failQueuedLocked(uint64_t sid, bnn_status code) {
+        for(auto it = queue.begin(); it != queue.end();) {
+            Req& r = reqs[*it];
+            if(r.res.scene_id == sid) {
+                r.res.status = code;
+                r.res.t_done = now();
+                r.done       = true;
+                ++st.failed;
+                it = queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+
+    void workerLoop() {
+        for(;;) {
+            Served*  toCreate = nullptr;
+            uint64_t rid      = 0;
+            bool     haveReq  = false;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                for(;;) {
+                    if(admissibleLocked() || runnableLocked()) break;
+                    if(stopping) {
+                        if(creating || running) {  // another worker may still unblock queued requests
+                            cv.wait(lk);
+                            continue;
+                        }
+                        for(uint64_t id : queue) {  // nothing can run them any more
+                            Req& r       = reqs[id];
+                            r.res.status = BNN_ERR_STATE;
+                            r.res.t_done = now();
+                            r.done       = true;
+                            ++st.failed;
+                        }
+                        queue.clear();
+                        doneCv.notify_all();
+                        cv.notify_all();
+                        return;
+                    }
+                    cv.wait(lk);
+                }
+                toCreate = admitLocked();
+                if(toCreate) ++creating;
+                else if((haveReq = pickLocked(rid))) ++running;
+            }
+            if(toCreate) create(*toCreate);
+            if(haveReq) run(rid);
+        }
+    }
+
+    void create(Served& s) {
+        std::string msg;
+        try {
+            auto               sv = std::make_unique<FlipSolver>(s.prm);
+            std::vector<float> pos(s.np * 3), vel(s.np * 3);
+            generate_scene(s.prm, s.spec, s.seed, pos.data(), vel.data(), s.np);
+            sv->setParticles(pos.data(), vel.data(), s.np);
+            s.solver = std::move(sv);
+        } catch(const std::exception& e) {
+            msg = e.what();
+        }
+        std::unique_ptr<Served> dead;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            --creating;
+            s.state = Served::Ready;
+            if(!msg.empty()) {  // could not be created: fail its requests, free its budget
+                setError(msg);
+                failQueuedLocked(s.id, BNN_ERR_OUT_OF_MEMORY);
+                s.queued = 0;
+                dead     = takeLocked(scenes.find(s.id));
+            } else if(s.release && s.queued == 0) {
+                dead = takeLocked(scenes.find(s.id));
+            }
+        }
+        cv.notify_all();
+        doneCv.notify_all();
+    }
+
+    void run(uint64_t rid) {
+        FlipSolver* sv = nullptr;
+        int         kind;
+        float       dt;
+        uint64_t    sid;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            Req& r           = reqs[rid];
+            sid              = r.res.scene_id;
+            sv               = scenes[sid]->solver.get();
+            kind             = r.res.kind;
+            dt               = r.dt;
+            r.res.t_start    = now();
+            r.res.batch_size = running;
+        }
+        const double t1   = now();
+        bnn_status   code = BNN_OK;
+        bool         ok   = true;
+        try {
+            if(kind == BNN_REQ_STEP) {
+                ok = sv->step(dt);
+            } else {
+                sv->beginStats();
+                ok = sv->pressureProjection(dt);
+                sv->endStats(dt);
+            }
+        } catch(const Error& e) {
+            code = e.code;
+            setError(e.what());
+        } catch(const std::exception& e) {
+            code = BNN_ERR_DEVICE;
+            setError(e.what());
+        }
+        if(code != BNN_OK) (void)hipStreamSynchronize(sv->stream);  // nothing of this request stays queued
+        std::unique_ptr<Served> dead;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            const double t2 = now();
+            busy += t2 - t1;
+            Req& r       = reqs[rid];
+            r.res.t_done = t2;
+            r.res.status = code;
+            if(code == BNN_OK) {
+                r.res.pcg_iterations = sv->st.pcg_iterations;
+                r.res.pcg_success    = sv->st.pcg_success;
+                r.res.pcg_residual   = sv->st.pcg_residual;
+                if(!ok && sv->prm.exit_if_pressure_projection_failed) r.res.status = BNN_ERR_NOT_CONVERGED;
+            }
+            r.done = true;
+            if(r.res.status == BNN_OK) {
+                ++st.completed;
+                latencies.push_back(r.res.t_done - r.res.t_submit);
+                lastDone = std::max(lastDone, r.res.t_done);
+            } else {
+                ++st.failed;
+            }
+            ++st.batches;
+            auto it = scenes.find(sid);
+            if(it != scenes.end()) {
+                Served& s = *it->second;
+                s.busy    = false;
+                s.queued -= 1;
+                if(s.release && s.queued == 0) dead = takeLocked(it);
+            }
+            --running;
+        }
+        cv.notify_all();
+        doneCv.notify_all();
+    }
+};
+
+}  // namespace bnn

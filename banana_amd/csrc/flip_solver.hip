@@ -1100,6 +1100,8 @@ public:
 
 }  // namespace bnn
 
+#include "flip_serving.hpp"
+
 //==================================================================================================
 // C ABI
 //==================================================================================================
@@ -1442,6 +1444,219 @@ bnn_status bnn_flip_set_grid(bnn_flip* h, int32_t field, const void* src) {
 bnn_status bnn_flip_synchronize(bnn_flip* h) {
     return guard(h, [&](bnn::FlipSolver& s) {
         HIP_OK(hipStreamSynchronize(s.stream));
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, uint64_t* bytes) {
+    if(!p || !bytes) return BNN_ERR_INVALID_ARG;
+    *bytes = bnn::scene_footprint(*p, n_particles);
+    return BNN_OK;
+}
+
+}  // extern "C"
+
+//--------------------------------------------------------------------------------------------------
+// batches: scenes stepped concurrently by a worker pool, each on its own stream
+//--------------------------------------------------------------------------------------------------
+struct bnn_flip_batch {
+    std::unique_ptr<bnn::WorkerPool> pool;
+    std::string                      err;
+    std::mutex                       errMutex;
+    int32_t                          timing = -1;  // < 0: each scene keeps its own setting
+};
+
+struct bnn_server {
+    std::unique_ptr<bnn::Server> srv;
+    std::string                  err;
+};
+
+namespace {
+template<class F>
+bnn_status batch_call(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bnn_status* status, F&& f) {
+    if(!b || (n && !scenes)) return BNN_ERR_INVALID_ARG;
+    b->err.clear();
+    for(uint32_t i = 0; i < n; ++i) {
+        if(!scenes[i] || !scenes[i]->s) {
+            b->err = "null scene handle";
+            return BNN_ERR_INVALID_ARG;
+        }
+        for(uint32_t j = 0; j < i; ++j)
+            if(scenes[j] == scenes[i]) {
+                b->err = "a scene is listed twice";
+                return BNN_ERR_INVALID_ARG;
+            }
+        if(b->timing >= 0) scenes[i]->s->prm.timing = b->timing;
+    }
+    std::vector<bnn_status> st(n, BNN_OK);
+    b->pool->run(n, [&](uint32_t i) {
+        st[i] = f(scenes[i], i);
+        if(st[i] != BNN_OK && st[i] != BNN_ERR_NOT_CONVERGED) {
+            std::unique_lock<std::mutex> lk(b->errMutex);
+            if(b->err.empty()) b->err = bnn_flip_last_error(scenes[i]);
+        }
+    });
+    bnn_status worst = BNN_OK;
+    for(uint32_t i = 0; i < n; ++i) {
+        if(status) status[i] = st[i];
+        if(st[i] != BNN_OK && (worst == BNN_OK || worst == BNN_ERR_NOT_CONVERGED)) worst = st[i];
+    }
+    return worst;
+}
+}  // namespace
+
+extern "C" {
+
+bnn_status bnn_flip_batch_create(bnn_flip_batch** out) {
+    if(!out) return BNN_ERR_INVALID_ARG;
+    *out = nullptr;
+    try {
+        auto b  = std::make_unique<bnn_flip_batch>();
+        b->pool = std::make_unique<bnn::WorkerPool>(4);  // one worker per hardware queue of the device
+        *out    = b.release();
+        return BNN_OK;
+    } catch(const std::exception& e) {
+        g_createError = e.what();
+        return BNN_ERR_DEVICE;
+    }
+}
+
+bnn_status bnn_flip_batch_destroy(bnn_flip_batch* b) {
+    if(!b) return BNN_ERR_INVALID_ARG;
+    delete b;
+    return BNN_OK;
+}
+
+const char* bnn_flip_batch_last_error(const bnn_flip_batch* b) { return b ? b->err.c_str() : g_createError.c_str(); }
+
+bnn_status bnn_flip_batch_set_timing(bnn_flip_batch* b, int32_t timing) {
+    if(!b) return BNN_ERR_INVALID_ARG;
+    b->timing = timing;
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_batch_step(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n, bnn_status* status) {
+    if(n && !dts) return BNN_ERR_INVALID_ARG;
+    return batch_call(b, scenes, n, status, [&](bnn_flip* h, uint32_t i) { return bnn_flip_step(h, dts[i]); });
+}
+
+bnn_status bnn_flip_batch_advance_frame(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bnn_status* status) {
+    return batch_call(b, scenes, n, status, [&](bnn_flip* h, uint32_t) { return bnn_flip_advance_frame(h); });
+}
+
+bnn_status bnn_flip_batch_pressure_projection(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n,
+                                              bnn_status* status) {
+    if(n && !dts) return BNN_ERR_INVALID_ARG;
+    return batch_call(b, scenes, n, status, [&](bnn_flip* h, uint32_t i) { return bnn_flip_pressure_projection(h, dts[i]); });
+}
+
+//--------------------------------------------------------------------------------------------------
+// serving
+//--------------------------------------------------------------------------------------------------
+bnn_status bnn_server_create(const bnn_server_config* cfg, bnn_server** out) {
+    if(!out) return BNN_ERR_INVALID_ARG;
+    *out = nullptr;
+    try {
+        bnn_server_config c{};
+        c.batching = 1;
+        if(cfg) c = *cfg;
+        auto s = std::make_unique<bnn_server>();
+        s->srv = std::make_unique<bnn::Server>(c);
+        *out   = s.release();
+        return BNN_OK;
+    } catch(const bnn::Error& e) {
+        g_createError = e.what();
+        return e.code;
+    } catch(const std::exception& e) {
+        g_createError = e.what();
+        return BNN_ERR_DEVICE;
+    }
+}
+
+bnn_status bnn_server_destroy(bnn_server* s) {
+    if(!s) return BNN_ERR_INVALID_ARG;
+    delete s;
+    return BNN_OK;
+}
+
+const char* bnn_server_last_error(const bnn_server* s) {
+    static thread_local std::string copy;
+    if(!s) return g_createError.c_str();
+    copy = s->err.empty() ? s->srv->lastError() : s->err;
+    return copy.c_str();
+}
+
+double bnn_server_now(const bnn_server* s) { return s ? s->srv->now() : 0.0; }
+
+}  // extern "C"
+
+namespace {
+template<class F> bnn_status server_guard(bnn_server* s, F&& f) {
+    if(!s || !s->srv) return BNN_ERR_INVALID_ARG;
+    try {
+        s->err.clear();
+        return f(*s->srv);
+    } catch(const bnn::Error& e) {
+        s->err = e.what();
+        return e.code;
+    } catch(const std::exception& e) {
+        s->err = e.what();
+        return BNN_ERR_DEVICE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+bnn_status bnn_server_add_scene(bnn_server* s, const bnn_flip_params* p, const bnn_scene_spec* spec, uint64_t seed, uint64_t* id) {
+    if(!p || !spec || !id) return BNN_ERR_INVALID_ARG;
+    return server_guard(s, [&](bnn::Server& v) {
+        *id = v.addScene(*p, *spec, seed);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_server_release_scene(bnn_server* s, uint64_t id) {
+    return server_guard(s, [&](bnn::Server& v) {
+        v.releaseScene(id);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_server_submit(bnn_server* s, uint64_t scene, int32_t kind, float dt, uint64_t* id) {
+    if(!id) return BNN_ERR_INVALID_ARG;
+    return server_guard(s, [&](bnn::Server& v) {
+        *id = v.submit(scene, kind, dt);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_server_wait(bnn_server* s, uint64_t id, double timeout, bnn_request_result* out) {
+    return server_guard(s, [&](bnn::Server& v) { return v.wait(id, timeout, out) ? BNN_OK : BNN_ERR_STATE; });
+}
+
+bnn_status bnn_server_drain(bnn_server* s, double timeout) {
+    return server_guard(s, [&](bnn::Server& v) { return v.drain(timeout) ? BNN_OK : BNN_ERR_STATE; });
+}
+
+bnn_status bnn_server_get_particles(bnn_server* s, uint64_t scene, float* pos, float* vel, uint64_t* n) {
+    return server_guard(s, [&](bnn::Server& v) {
+        v.getParticles(scene, pos, vel, n);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_server_get_stats(bnn_server* s, bnn_server_stats* out) {
+    if(!out) return BNN_ERR_INVALID_ARG;
+    return server_guard(s, [&](bnn::Server& v) {
+        *out = v.stats();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_server_reset_stats(bnn_server* s) {
+    return server_guard(s, [&](bnn::Server& v) {
+        v.resetStats();
         return BNN_OK;
     });
 }

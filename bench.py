@@ -11,6 +11,15 @@ scene resident in HBM.
 Prints one JSON line. `roofline` is for the dominant kernel (the PCG SpMV), from HIP events around
 every launch over the timed steps; `cpu_baseline` times the oracle (FP32, the reference's IC(0)
 PCG) on the host cores on a bounded sample of the same workload.
+
+Two more legs ride on the same line:
+  batched  -- 512 independent scenes, edge length log-uniform 24..96 cells, fluid-box extents
+              uniform in [0.2, 0.8] per axis, N(0, 0.5^2) initial velocities, refined pressure to
+              1e-9: scenes*steps/s with every stage launched once for all scenes, against the plain
+              path (each scene stepped through its own handle, one after the other).
+  served   -- Poisson arrivals (200/s) of step or pressure-solve requests (50/50) on a resident pool
+              of scenes with edge length log-uniform 32..192, HBM admission cap 200 GB: completed
+              requests/s and p50/p99 latency, with continuous batching and with one request at a time.
 """
 import argparse
 import json
@@ -59,6 +68,103 @@ def cpu_baseline(n, threads, steps=3):
     return pos.shape[0] * steps / dt, dt, pos.shape[0], iters / steps
 
 
+def batched_leg(args, log):
+    """scenes*steps/s of the uneven batch: fused launches vs scene-by-scene (same scenes, same kernels)."""
+    import numpy as np
+
+    import banana_amd
+    from banana_amd import scenes
+    from banana_amd.serving import Batch
+
+    rng = np.random.default_rng(7)
+    dt = np.float32(1.0 / 600.0)
+    t0 = time.perf_counter()
+    sims, np_total, cells = [], 0, 0
+    for i in range(args.batch_scenes):
+        p, s = scenes.random_scene(rng, 24, 96)
+        p.update(pressure_mode=1, refine_tolerance=1e-9)
+        g = banana_amd.FLIPSolver(p)
+        g.generate_scene(s, 1000 + i)
+        np_total += g.num_particles()
+        cells += g.n[0] * g.n[1] * g.n[2]
+        sims.append(g)
+    log(f"batched: {len(sims)} scenes, {np_total} particles, {cells} cells, set up in {time.perf_counter() - t0:.1f}s")
+    b = Batch()
+    b.step(sims, dt)  # warm the batched path (work lists, arenas)
+    for g in sims:
+        g.step(dt)  # warm the plain path
+    t1 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        for g in sims:
+            g.step(dt)
+    plain_s = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    worst = 0
+    for _ in range(args.batch_steps):
+        st = b.step(sims, dt)
+        worst = max(worst, int(np.max(st)))
+    batched_s = time.perf_counter() - t2
+    its = [g.stats()["pcg_iterations"] for g in sims]
+    ok = all(g.stats()["pcg_success"] for g in sims)
+    n_ss = len(sims) * args.batch_steps
+    out = {"scenes": len(sims), "steps": args.batch_steps, "particles": np_total, "cells": cells,
+           "scene_steps_per_s": round(n_ss / batched_s, 2), "plain_scene_steps_per_s": round(n_ss / plain_s, 2),
+           "speedup_vs_plain": round(plain_s / batched_s, 3),
+           "particle_steps_per_s": round(np_total * args.batch_steps / batched_s, 1),
+           "pcg_iterations_mean": round(float(np.mean(its)), 1), "all_converged": bool(ok and worst == 0),
+           "pressure": "FP32 PCG under FP64 defect correction to 1e-9"}
+    for g in sims:
+        g.close()
+    log(f"batched: {out}")
+    return out
+
+
+def served_leg(args, log, batching):
+    """Poisson request stream against a resident pool; returns the server's own accounting."""
+    import numpy as np
+
+    from banana_amd import scenes
+    from banana_amd.serving import Server
+
+    rng = np.random.default_rng(11)
+    dt = np.float32(1.0 / 600.0)
+    srv = Server(hbm_budget_bytes=200e9, workers=2 if batching else 1, batching=batching)
+    t0 = time.perf_counter()
+    ids = []
+    for i in range(args.serve_pool):
+        p, s = scenes.random_scene(rng, 32, 192)
+        p.update(pressure_mode=1, refine_tolerance=1e-9)
+        ids.append(srv.add_scene(p, s, 5000 + i))
+    warm = [srv.submit(i, 0, dt) for i in ids]  # every scene resident and stepped once
+    for r in warm:
+        srv.wait(r)
+    srv.reset_stats()
+    log(f"served ({'batched' if batching else 'one at a time'}): pool of {len(ids)} resident in "
+        f"{time.perf_counter() - t0:.1f}s, {srv.stats()['resident_bytes'] / 1e9:.1f} GB")
+    arr = np.random.default_rng(13)
+    t_next = srv.now()
+    end = t_next + args.serve_seconds
+    n = 0
+    while True:
+        t_next += arr.exponential(1.0 / args.serve_rate)
+        if t_next > end:
+            break
+        wait = t_next - srv.now()
+        if wait > 0:
+            time.sleep(wait)
+        srv.submit(ids[int(arr.integers(len(ids)))], int(arr.integers(2)), dt)
+        n += 1
+    srv.drain(600.0)
+    st = srv.stats()
+    srv.close()
+    out = {"requests": n, "completed": st["completed"], "failed": st["failed"],
+           "completed_per_s": round(st["throughput_rps"], 2), "latency_p50_ms": round(st["latency_p50_ms"], 2),
+           "latency_p99_ms": round(st["latency_p99_ms"], 2), "latency_mean_ms": round(st["latency_mean_ms"], 2),
+           "mean_batch": round(st["mean_batch"], 2), "peak_resident_gb": round(st["peak_resident_bytes"] / 1e9, 2)}
+    log(f"served: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +177,11 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128, help="grid edge of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3, help="substeps of the CPU baseline sample")
     ap.add_argument("--timing", type=int, default=2, help="0: none, 1: stage events, 2: + per-kernel events")
+    ap.add_argument("--batch-scenes", type=int, default=512, help="scenes of the batched leg (0: skip the leg)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson request stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="request arrivals per second")
+    ap.add_argument("--serve-pool", type=int, default=48, help="resident scenes the requests address")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -153,6 +264,12 @@ def main():
                "sample": f"first {args.cpu_steps} substeps of the {args.cpu_n}^3 version of this scene ({npc} particles), "
                          f"FP32 oracle with the reference's IC(0)-PCG at rel-tol 1e-6 ({iters:.0f} iterations/step), {secs:.1f} s"}
 
+    batched = batched_leg(args, log) if args.batch_scenes > 0 else None
+    served = None
+    if args.serve_seconds > 0:
+        served = {"arrival_rate_per_s": args.serve_rate, "seconds": args.serve_seconds, "pool": args.serve_pool,
+                  "batched": served_leg(args, log, True), "plain": served_leg(args, log, False)}
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -172,6 +289,8 @@ def main():
                    "preconditioner": args.precond, "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

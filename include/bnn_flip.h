@@ -237,6 +237,90 @@ bnn_status  bnn_flip_get_stats(bnn_flip* h, bnn_flip_stats* s);
 bnn_status  bnn_flip_set_grid(bnn_flip* h, int32_t field, const void* src);
 /* wait for all queued device work of this handle */
 bnn_status  bnn_flip_synchronize(bnn_flip* h);
+/* device bytes a scene with these parameters and n_particles keeps resident in HBM (what
+ * bnn_flip_create + bnn_flip_set_particles allocate; stepping allocates nothing per scene) */
+bnn_status  bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, uint64_t* bytes);
+
+/* --- batching --------------------------------------------------------------------------------
+ * Extension with no counterpart in the reference (FLIP_Solver steps one scene per solver object;
+ * ParticleSolver.cpp:182-219). For callers that step many independent scenes: a pool of host
+ * workers (one per hardware queue) runs the listed scenes concurrently, each on its own handle's
+ * HIP stream, so small scenes overlap on the device instead of waiting for each other's launches
+ * and readbacks. Each scene's results are bit-identical to stepping it alone. The scenes stay owned
+ * by their handles; a handle must not be used elsewhere while a batch call runs on it. */
+typedef struct bnn_flip_batch bnn_flip_batch;
+bnn_status  bnn_flip_batch_create(bnn_flip_batch** out);
+bnn_status  bnn_flip_batch_destroy(bnn_flip_batch* b);
+const char* bnn_flip_batch_last_error(const bnn_flip_batch* b);
+/* applies bnn_flip_params.timing to every scene of later batch calls (< 0: keep each scene's own) */
+bnn_status  bnn_flip_batch_set_timing(bnn_flip_batch* b, int32_t timing);
+/* one substep of scenes[i] with dts[i], i < n; status[i] (may be NULL) receives each scene's own
+ * status (BNN_ERR_NOT_CONVERGED as bnn_flip_step would return it) */
+bnn_status  bnn_flip_batch_step(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n, bnn_status* status);
+/* advanceFrame of every scene: CFL substeps per scene, scenes leave the batch as their frame ends */
+bnn_status  bnn_flip_batch_advance_frame(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bnn_status* status);
+/* pressureProjection (PIC_Solver.cpp:533-539) of every scene on its current grid velocities */
+bnn_status  bnn_flip_batch_pressure_projection(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n,
+                                               bnn_status* status);
+
+/* --- serving ---------------------------------------------------------------------------------
+ * Extension with no counterpart in the reference (its Simulator drives one solver in a loop,
+ * Simulator.cpp:42-69). A request loop for step and pressure-solve requests on resident scenes:
+ * scenes are admitted while the sum of their footprints (bnn_flip_footprint) fits the HBM budget
+ * (later scenes wait in arrival order, smaller ones may pass); each worker thread takes the oldest
+ * ready request (its scene resident and idle; a scene's requests run in submission order) as soon
+ * as it is free, so requests that arrive while others run start at once on their own scenes'
+ * streams and overlap on the device. */
+typedef struct bnn_server bnn_server;
+typedef enum bnn_request_kind {
+    BNN_REQ_STEP     = 0, /* moveParticles + advanceVelocity with the request's dt */
+    BNN_REQ_PRESSURE = 1  /* pressureProjection with the request's dt */
+} bnn_request_kind;
+typedef struct bnn_server_config {
+    uint64_t hbm_budget_bytes; /* admission cap on the summed resident footprint (0: 200e9) */
+    uint32_t max_batch;        /* reserved (0) */
+    uint32_t workers;          /* requests in flight at once (0: 4) */
+    int32_t  batching;         /* 1: concurrent workers (default with a NULL config); 0: one request at a time */
+    int32_t  reserved[5];
+} bnn_server_config;
+typedef struct bnn_request_result {
+    uint64_t   request_id, scene_id;
+    int32_t    kind;
+    bnn_status status;
+    double     t_submit, t_start, t_done; /* seconds since the server was created */
+    uint32_t   batch_size;                /* requests running on the device when it started */
+    uint32_t   pcg_iterations;
+    int32_t    pcg_success;
+    int32_t    pad;
+    double     pcg_residual;
+} bnn_request_result;
+typedef struct bnn_server_stats {
+    uint64_t submitted, completed, failed, batches; /* batches: requests run */
+    uint64_t scenes_resident, scenes_waiting, resident_bytes, peak_resident_bytes;
+    double   latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms; /* submit -> done */
+    double   throughput_rps; /* completed requests / (last completion - first submission) */
+    double   mean_batch;     /* workers (requests that may run at once) */
+    double   busy_s;         /* summed wall time of requests (all workers) */
+} bnn_server_stats;
+bnn_status  bnn_server_create(const bnn_server_config* cfg, bnn_server** out);
+bnn_status  bnn_server_destroy(bnn_server* s); /* finishes queued work first */
+const char* bnn_server_last_error(const bnn_server* s);
+double      bnn_server_now(const bnn_server* s);
+/* queue a scene (parameters + seeded generator); BNN_ERR_REJECTED when its footprint alone exceeds
+ * the budget. Requests may be submitted at once; they run after the scene is resident. */
+bnn_status  bnn_server_add_scene(bnn_server* s, const bnn_flip_params* p, const bnn_scene_spec* spec, uint64_t seed,
+                                 uint64_t* scene_id);
+/* free a scene after its queued requests */
+bnn_status  bnn_server_release_scene(bnn_server* s, uint64_t scene_id);
+bnn_status  bnn_server_submit(bnn_server* s, uint64_t scene_id, int32_t kind, float dt, uint64_t* request_id);
+/* wait for one request (timeout_s < 0: no limit); BNN_ERR_STATE on timeout */
+bnn_status  bnn_server_wait(bnn_server* s, uint64_t request_id, double timeout_s, bnn_request_result* out);
+/* wait until no request is queued or running */
+bnn_status  bnn_server_drain(bnn_server* s, double timeout_s);
+/* particles of an idle resident scene in generation order (*n receives the count; arrays may be NULL) */
+bnn_status  bnn_server_get_particles(bnn_server* s, uint64_t scene_id, float* pos_xyz, float* vel_xyz, uint64_t* n);
+bnn_status  bnn_server_get_stats(bnn_server* s, bnn_server_stats* out);
+bnn_status  bnn_server_reset_stats(bnn_server* s);
 
 #ifdef __cplusplus
 }
