@@ -510,6 +510,232 @@ __global__ void __launch_bounds__(kBlock) k_fluid_sdf(Particles P, const uint32_
 }
 
 //==================================================================================================
+// LDS-staged gathers. A work-group owns an 8x8x4 tile of cells (fluid SDF) or of face nodes (P2G);
+// every thread's 3x3x3 cell window lies inside the tile grown by one cell, so the work-group first
+// copies the positions of all particles of those 10x10x6 cells into LDS (each halo row of cells
+// holds one contiguous run of cell-sorted particles) and then runs the same per-cell loops, in the
+// same order, on LDS. Results are bit-identical to reading the particles from global memory, which
+// is what a tile holding more than kTileCap particles does.
+// Tiles are handed out so that each XCD (blocks are dispatched round-robin over 8 XCDs) works on a
+// contiguous run of tiles and shares halo reads through its own L2.
+//==================================================================================================
+constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
+constexpr int kTileThreads = kTileX * kTileY * kTileZ;  // 256
+constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2, kHaloZ = kTileZ + 2;
+constexpr int kHaloCells = kHaloX * kHaloY * kHaloZ;  // 600
+constexpr int kHaloRows  = kHaloY * kHaloZ;           // 60
+constexpr int kTileCap   = 4608;                      // staged particles per tile (~8 per halo cell)
+
+struct TileLds {
+    float    x[kTileCap], y[kTileCap], z[kTileCap];
+    uint32_t begin[kHaloCells];   // LDS index of a halo cell's first particle
+    uint32_t gbegin[kHaloCells];  // global index of that particle
+    uint32_t count[kHaloCells];
+    uint32_t rowBase[kHaloRows + 1], rowStart[kHaloRows];
+};
+
+__device__ __forceinline__ void tile_of_block(int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
+    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blockIdx.x, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
+    const uint32_t t = xcd * q + min(xcd, r) + b / 8u;
+    tx = (int)(t % (uint32_t)ntx);
+    ty = (int)((t / (uint32_t)ntx) % (uint32_t)nty);
+    tz = (int)(t / (uint32_t)(ntx * nty));
+}
+
+// halo cell (x, y, z) of the tile with origin (x0, y0, z0)
+__device__ __forceinline__ int halo_index(int x, int y, int z, int x0, int y0, int z0) {
+    return ((z - z0 + 1) * kHaloY + (y - y0 + 1)) * kHaloX + (x - x0 + 1);
+}
+
+// stage the tile's particles; false (uniform over the block) when they do not fit in LDS
+__device__ bool stage_tile(TileLds& L, const GridGeom& g, const uint32_t* __restrict__ cs, const float* __restrict__ px,
+                           const float* __restrict__ py, const float* __restrict__ pz, int x0, int y0, int z0) {
+    const int tid = threadIdx.x;
+    const int xlo = max(x0 - 1, 0), xhi = min(x0 + kTileX, g.nx - 1);
+    if(tid < kHaloRows) {
+        const int y = y0 - 1 + tid % kHaloY, z = z0 - 1 + tid / kHaloY;
+        uint32_t  s = 0, n = 0;
+        if(y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
+            const size_t r = lin3(0, y, z, g.nx, g.ny);
+            s              = cs[r + xlo];
+            n              = cs[r + xhi + 1] - s;
+        }
+        L.rowStart[tid] = s;
+        L.rowBase[tid]  = n;
+    }
+    __syncthreads();
+    if(tid == 0) {
+        uint32_t acc = 0;
+        for(int r = 0; r < kHaloRows; ++r) {
+            const uint32_t n = L.rowBase[r];
+            L.rowBase[r]     = acc;
+            acc += n;
+        }
+        L.rowBase[kHaloRows] = acc;
+    }
+    __syncthreads();
+    const uint32_t total = L.rowBase[kHaloRows];
+    for(int h = tid; h < kHaloCells; h += kTileThreads) {
+        const int hx = h % kHaloX, hy = (h / kHaloX) % kHaloY, hz = h / (kHaloX * kHaloY);
+        const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+        uint32_t  b = 0, gb = 0, n = 0;
+        if(x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
+            const size_t c   = lin3(x, y, z, g.nx, g.ny);
+            gb               = cs[c];
+            n                = cs[c + 1] - gb;
+            const int row    = hy + kHaloY * hz;
+            b                = L.rowBase[row] + (gb - L.rowStart[row]);
+        }
+        L.begin[h]  = b;
+        L.gbegin[h] = gb;
+        L.count[h]  = n;
+    }
+    if(total > (uint32_t)kTileCap) {
+        __syncthreads();
+        return false;
+    }
+    for(uint32_t i = tid; i < total; i += kTileThreads) {
+        int lo = 0, hi = kHaloRows - 1;  // last row whose LDS base is <= i (it holds particle i)
+        while(lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if(L.rowBase[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t gi = L.rowStart[lo] + (i - L.rowBase[lo]);
+        L.x[i]            = px[gi];
+        L.y[i]            = py[gi];
+        L.z[i]            = pz[gi];
+    }
+    __syncthreads();
+    return true;
+}
+
+// computeFluidSDF (PIC_Solver.cpp:601-689): same arithmetic and order as k_fluid_sdf
+__global__ void __launch_bounds__(kTileThreads) k_fluid_sdf_tiled(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g,
+                                                                  const float* __restrict__ bsdf, float* __restrict__ phi) {
+    __shared__ TileLds L;
+    const int ntx = (g.nx + kTileX - 1) / kTileX, nty = (g.ny + kTileY - 1) / kTileY, ntz = (g.nz + kTileZ - 1) / kTileZ;
+    int       tx, ty, tz;
+    tile_of_block(ntx, nty, ntz, tx, ty, tz);
+    const int  x0 = tx * kTileX, y0 = ty * kTileY, z0 = tz * kTileZ;
+    const bool staged = stage_tile(L, g, cell_start, P.px, P.py, P.pz, x0, y0, z0);
+    const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
+    if(i >= g.nx || j >= g.ny || k >= g.nz) return;
+    const float sx   = ((float)i + 0.5f) * g.dx + g.bminx;
+    const float sy   = ((float)j + 0.5f) * g.dx + g.bminy;
+    const float sz   = ((float)k + 0.5f) * g.dx + g.bminz;
+    float       best = g.dx * 3.0f;
+    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
+        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y)
+            for(int x = max(i - 1, 0); x <= min(i + 1, g.nx - 1); ++x) {
+                const bool boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
+                auto       visit    = [&](float ppx, float ppy, float ppz) {
+                    if(boundary) {
+                        // particles clamped into a boundary cell: use their true (unclamped) window
+                        const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
+                                  ck = (int)((ppz - g.bminz) / g.dx);
+                        if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) return;
+                    }
+                    const float v = len3f(sx - ppx, sy - ppy, sz - ppz) - g.sdfRadius;
+                    if(v < best) best = v;
+                };
+                if(staged) {
+                    const int      h  = halo_index(x, y, z, x0, y0, z0);
+                    const uint32_t pb = L.begin[h], pe = pb + L.count[h];
+                    for(uint32_t p = pb; p < pe; ++p) visit(L.x[p], L.y[p], L.z[p]);
+                } else {
+                    const size_t   c  = lin3(x, y, z, g.nx, g.ny);
+                    const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
+                    for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
+                }
+            }
+    if(best < g.halfdx) {
+        const int   sxn = g.nx + 1, syn = g.ny + 1;
+        auto        B   = [&](int a, int c, int e) { return bsdf[lin3(a, c, e, sxn, syn)]; };
+        const float s   = 0.125f * (B(i, j, k) + B(i + 1, j, k) + B(i, j + 1, k) + B(i + 1, j + 1, k) + B(i, j, k + 1) +
+                                  B(i + 1, j, k + 1) + B(i, j + 1, k + 1) + B(i + 1, j + 1, k + 1));
+        if(s < 0) best = -g.halfdx;
+    }
+    phi[lin3(i, j, k, g.nx, g.ny)] = best;
+}
+
+// mapParticles2Grid (FLIP_Solver.cpp:57-95): same arithmetic and order as k_p2g; a work-group owns
+// an 8x8x4 tile of face nodes (node (i,j,k) carries u, v and w at (i,j,k)); positions come from LDS,
+// velocities of contributing particles from global memory
+__global__ void __launch_bounds__(kTileThreads) k_p2g_tiled(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
+    __shared__ TileLds L;
+    const int ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
+    const int ntx = (ex + kTileX - 1) / kTileX, nty = (ey + kTileY - 1) / kTileY, ntz = (ez + kTileZ - 1) / kTileZ;
+    int       tx, ty, tz;
+    tile_of_block(ntx, nty, ntz, tx, ty, tz);
+    const int  x0 = tx * kTileX, y0 = ty * kTileY, z0 = tz * kTileZ;
+    const bool staged = stage_tile(L, g, cell_start, P.gx, P.gy, P.gz, x0, y0, z0);
+    const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
+    if(i >= ex || j >= ey || k >= ez) return;
+    const bool doU = j < g.ny && k < g.nz;
+    const bool doV = i < g.nx && k < g.nz;
+    const bool doW = i < g.nx && j < g.ny;
+    float      mu = 0.f, wu = 0.f, mv = 0.f, wv = 0.f, mw = 0.f, ww = 0.f;
+    auto       visit = [&](float gx, float gy, float gz, uint32_t q) {
+        if(doU) {
+            float a, b, c;
+            if(stencil_weight(gx, g.nx + 1, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) && stencil_weight(gz - 0.5f, g.nz, k, c)) {
+                const float wt = a * b * c;
+                mu += wt * P.vx[q];
+                wu += wt;
+            }
+        }
+        if(doV) {
+            float a, b, c;
+            if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy, g.ny + 1, j, b) && stencil_weight(gz - 0.5f, g.nz, k, c)) {
+                const float wt = a * b * c;
+                mv += wt * P.vy[q];
+                wv += wt;
+            }
+        }
+        if(doW) {
+            float a, b, c;
+            if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) && stencil_weight(gz, g.nz + 1, k, c)) {
+                const float wt = a * b * c;
+                mw += wt * P.vz[q];
+                ww += wt;
+            }
+        }
+    };
+    const int xa = max(i - 1, 0), xb = min(i + 1, g.nx - 1);
+    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
+        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y) {
+            if(staged) {  // cells xa..xb of a halo row are one contiguous LDS run
+                const int      ha = halo_index(xa, y, z, x0, y0, z0), hb = halo_index(xb, y, z, x0, y0, z0);
+                const uint32_t pb = L.begin[ha], pe = L.begin[hb] + L.count[hb], gofs = L.gbegin[ha] - pb;
+                for(uint32_t p = pb; p < pe; ++p) visit(L.x[p], L.y[p], L.z[p], p + gofs);
+            } else {
+                const size_t   r  = lin3(0, y, z, g.nx, g.ny);
+                const uint32_t pb = cell_start[r + xa], pe = cell_start[r + xb + 1];
+                for(uint32_t p = pb; p < pe; ++p) visit(P.gx[p], P.gy[p], P.gz[p], p);
+            }
+        }
+    if(doU) {
+        const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
+        const bool   ok = wu > kFltMin;
+        out.u[id]       = ok ? mu / wu : 0.f;
+        out.vu[id]      = ok ? 1 : 0;
+    }
+    if(doV) {
+        const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
+        const bool   ok = wv > kFltMin;
+        out.v[id]       = ok ? mv / wv : 0.f;
+        out.vv[id]      = ok ? 1 : 0;
+    }
+    if(doW) {
+        const size_t id = lin3(i, j, k, g.nx, g.ny);
+        const bool   ok = ww > kFltMin;
+        out.w[id]       = ok ? mw / ww : 0.f;
+        out.vw[id]      = ok ? 1 : 0;
+    }
+}
+
+//==================================================================================================
 // Pressure bricks
 //==================================================================================================
 // one block (512 threads) per brick: flag bricks holding a fluid cell, count fluid cells

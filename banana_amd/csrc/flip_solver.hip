@@ -494,11 +494,11 @@ public:
     // FLIP_Solver.cpp:57-95
     void mapParticles2Grid() {
         requireBinned();
-        FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
-        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+        FaceOut        out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
+        const unsigned tiles = blocks(g.nx + 1, kTileX) * blocks(g.ny + 1, kTileY) * blocks(g.nz + 1, kTileZ);
         ktBegin(BNN_KT_P2G);
-        hipLaunchKernelGGL(k_p2g, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(), g,
-                           out);
+        hipLaunchKernelGGL(k_p2g_tiled, dim3(tiles), dim3(kTileThreads), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
+                           g, out);
         ktEnd(BNN_KT_P2G);
     }
 
@@ -581,8 +581,9 @@ public:
     void computeFluidSDF() {  // PIC_Solver.cpp:601-689
         requireBinned();
         ktBegin(BNN_KT_FLUID_SDF);
-        hipLaunchKernelGGL(k_fluid_sdf, dim3(blocks(nC)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
-                           g, (const float*)bsdf.as<float>(), phi.as<float>());
+        const unsigned tiles = blocks(g.nx, kTileX) * blocks(g.ny, kTileY) * blocks(g.nz, kTileZ);
+        hipLaunchKernelGGL(k_fluid_sdf_tiled, dim3(tiles), dim3(kTileThreads), 0, stream, cur->view(),
+                           (const uint32_t*)cellStart.as<uint32_t>(), g, (const float*)bsdf.as<float>(), phi.as<float>());
         ktEnd(BNN_KT_FLUID_SDF);
     }
 

@@ -128,7 +128,7 @@ def served_leg(args, log, batching):
 
     rng = np.random.default_rng(11)
     dt = np.float32(1.0 / 600.0)
-    srv = Server(hbm_budget_bytes=200e9, workers=2 if batching else 1, batching=batching)
+    srv = Server(hbm_budget_bytes=200e9, workers=4 if batching else 1, batching=batching)
     t0 = time.perf_counter()
     ids = []
     for i in range(args.serve_pool):
