@@ -90,28 +90,56 @@ __device__ __forceinline__ int cell_axis(float p, float bmin, float dx, int n) {
     return c < 0 ? 0 : (c > n - 1 ? n - 1 : c);  // Grid::getValidCellIdx clamp
 }
 
+// Runs of equal keys inside a wavefront (particles arrive almost cell-sorted from the previous
+// substep, so a run usually covers a whole cell): the run's first lane does one atomic for the run.
+// Returns the run length in the leader lane, 0 elsewhere; `leader` gets the run's first lane.
+__device__ __forceinline__ uint32_t key_run(uint32_t c, bool active, int& leader) {
+    const int      lane  = (int)(threadIdx.x & 63);
+    const uint32_t prev  = __shfl_up(c, 1, 64);
+    const bool     first = active && (lane == 0 || prev != c);
+    const uint64_t heads = __ballot(first);
+    const uint64_t live  = __ballot(active);
+    const uint64_t upTo  = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    leader               = 63 - __clzll(heads & upTo);
+    if(!first) return 0u;
+    const uint64_t after = heads & ~upTo;           // later run heads
+    const int      end   = after ? __ffsll((unsigned long long)after) - 1 : 64 - __clzll(live);
+    return (uint32_t)(end - lane);
+}
+
 __global__ void __launch_bounds__(kBlock) k_cell_key(const float* __restrict__ px, const float* __restrict__ py,
                                                      const float* __restrict__ pz, uint32_t n, GridGeom g,
                                                      uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    const int      i = cell_axis(px[p], g.bminx, g.dx, g.nx);
-    const int      j = cell_axis(py[p], g.bminy, g.dx, g.ny);
-    const int      k = cell_axis(pz[p], g.bminz, g.dx, g.nz);
-    const uint32_t c = (uint32_t)lin3(i, j, k, g.nx, g.ny);
-    key[p]           = c;
-    atomicAdd(&count[c], 1u);
+    const uint32_t p      = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool     active = p < n;
+    uint32_t       c      = 0xFFFFFFFFu;
+    if(active) {
+        const int i = cell_axis(px[p], g.bminx, g.dx, g.nx);
+        const int j = cell_axis(py[p], g.bminy, g.dx, g.ny);
+        const int k = cell_axis(pz[p], g.bminz, g.dx, g.nz);
+        c           = (uint32_t)lin3(i, j, k, g.nx, g.ny);
+        key[p]      = c;
+    }
+    int            leader;
+    const uint32_t len = key_run(c, active, leader);
+    if(len) atomicAdd(&count[c], len);
 }
 
-// counts return to zero here (each particle takes one slot), so the count array needs no clearing
+// counts return to zero here (each particle takes one slot), so the count array needs no clearing;
+// a run's lanes take consecutive slots in lane order
 __global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t* __restrict__ key, uint32_t n,
                                                     const uint32_t* __restrict__ cell_start, uint32_t* __restrict__ count,
                                                     uint32_t* __restrict__ perm) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    const uint32_t c    = key[p];
-    const uint32_t slot = cell_start[c] + atomicSub(&count[c], 1u) - 1u;
-    perm[slot]          = p;
+    const uint32_t p      = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool     active = p < n;
+    const uint32_t c      = active ? key[p] : 0xFFFFFFFFu;
+    int            leader;
+    const uint32_t len  = key_run(c, active, leader);
+    uint32_t       left = 0;  // count before the run's subtraction
+    if(len) left = atomicSub(&count[c], len);
+    left = __shfl(left, leader, 64);
+    const uint32_t runLen = __shfl(len, leader, 64);
+    if(active) perm[cell_start[c] + left - runLen + (uint32_t)((int)(threadIdx.x & 63) - leader)] = p;
 }
 
 // make each cell's segment ascending: the counting sort becomes stable (ties by particle index)
@@ -315,37 +343,52 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __r
 }
 
 //==================================================================================================
-// extrapolateVelocity: one Jacobi layer of one component (PIC_Solver.cpp:992-1090)
+// extrapolateVelocity: one Jacobi layer of one component (PIC_Solver.cpp:992-1090), in place.
+// An invalid interior face takes the mean of its valid neighbours (order +x,-x,+y,-y,+z,-z) and
+// becomes valid and extrapolated. `val` holds stamps: 0 invalid, 1 valid on entry, l + 2 for a face
+// filled in layer l, and layer l reads the neighbours valid before it (stamps 1 .. l + 1). A face
+// filled in this very layer reads as 0 or l + 2 to its neighbours, so concurrent writes never feed the
+// same layer and faces valid before a layer never change: the values equal the reference's
+// two-buffer Jacobi bit for bit without copying the faces a layer leaves alone.
+// Layer 0 also clears the extrapolated flag of every face it does not fill.
 //==================================================================================================
-__global__ void __launch_bounds__(kBlock) k_extrap_layer(const float* __restrict__ in, float* __restrict__ out,
-                                                         const uint8_t* __restrict__ vin, uint8_t* __restrict__ vout,
-                                                         uint8_t* __restrict__ ext, int sx, int sy, int sz, int firstLayer) {
+__global__ void __launch_bounds__(kBlock) k_extrap_stamp(float* __restrict__ a, uint8_t* __restrict__ val, uint8_t* __restrict__ ext,
+                                                         int sx, int sy, int sz, int layer) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t N = (size_t)sx * sy * sz;
-    if(t >= N) return;
-    const int  i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
-    const bool interior = i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2;
-    const uint8_t v0   = vin[t];
-    if(interior && !v0) {
-        const size_t sxy   = (size_t)sx * sy;
-        float        sum   = 0.f;
-        uint32_t     count = 0;
-        if(vin[t + 1]) { sum += in[t + 1]; ++count; }
-        if(vin[t - 1]) { sum += in[t - 1]; ++count; }
-        if(vin[t + sx]) { sum += in[t + sx]; ++count; }
-        if(vin[t - sx]) { sum += in[t - sx]; ++count; }
-        if(vin[t + sxy]) { sum += in[t + sxy]; ++count; }
-        if(vin[t - sxy]) { sum += in[t - sxy]; ++count; }
-        if(count > 0) {
-            out[t]  = sum / (float)count;
-            vout[t] = 1;
-            ext[t]  = 1;
-            return;
+    if(t >= (size_t)sx * sy * sz) return;
+    bool filled = false;
+    if(val[t] == 0) {
+        const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+        if(i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2) {
+            const size_t  sxy = (size_t)sx * sy;
+            const uint8_t lim = (uint8_t)(layer + 1);
+            auto          ok  = [&](size_t n) {
+                const uint8_t s = val[n];
+                return s >= 1 && s <= lim;
+            };
+            float    sum   = 0.f;
+            uint32_t count = 0;
+            if(ok(t + 1)) { sum += a[t + 1]; ++count; }
+            if(ok(t - 1)) { sum += a[t - 1]; ++count; }
+            if(ok(t + sx)) { sum += a[t + sx]; ++count; }
+            if(ok(t - sx)) { sum += a[t - sx]; ++count; }
+            if(ok(t + sxy)) { sum += a[t + sxy]; ++count; }
+            if(ok(t - sxy)) { sum += a[t - sxy]; ++count; }
+            if(count > 0) {
+                a[t]   = sum / (float)count;
+                val[t] = (uint8_t)(layer + 2);
+                ext[t] = 1;
+                filled = true;
+            }
         }
     }
-    out[t]  = in[t];
-    vout[t] = v0;
-    if(firstLayer) ext[t] = 0;
+    if(layer == 0 && !filled) ext[t] = 0;
+}
+
+// after the last layer: stamps back to valid flags
+__global__ void __launch_bounds__(kBlock) k_extrap_finish(uint8_t* __restrict__ val, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(t < n && val[t] > 1) val[t] = 1;
 }
 
 //==================================================================================================
@@ -510,8 +553,8 @@ __global__ void __launch_bounds__(kBlock) k_fluid_sdf(Particles P, const uint32_
 }
 
 //==================================================================================================
-// LDS-staged gathers. A work-group owns an 8x8x4 tile of cells (fluid SDF) or of face nodes (P2G);
-// every thread's 3x3x3 cell window lies inside the tile grown by one cell, so the work-group first
+// LDS-staged gather for the fluid SDF. A work-group owns an 8x8x4 tile of cells; every thread's
+// 3x3x3 cell window lies inside the tile grown by one cell, so the work-group first
 // copies the positions of all particles of those 10x10x6 cells into LDS (each halo row of cells
 // holds one contiguous run of cell-sorted particles) and then runs the same per-cell loops, in the
 // same order, on LDS. Results are bit-identical to reading the particles from global memory, which
@@ -657,82 +700,6 @@ __global__ void __launch_bounds__(kTileThreads) k_fluid_sdf_tiled(Particles P, c
         if(s < 0) best = -g.halfdx;
     }
     phi[lin3(i, j, k, g.nx, g.ny)] = best;
-}
-
-// mapParticles2Grid (FLIP_Solver.cpp:57-95): same arithmetic and order as k_p2g; a work-group owns
-// an 8x8x4 tile of face nodes (node (i,j,k) carries u, v and w at (i,j,k)); positions come from LDS,
-// velocities of contributing particles from global memory
-__global__ void __launch_bounds__(kTileThreads) k_p2g_tiled(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
-    __shared__ TileLds L;
-    const int ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
-    const int ntx = (ex + kTileX - 1) / kTileX, nty = (ey + kTileY - 1) / kTileY, ntz = (ez + kTileZ - 1) / kTileZ;
-    int       tx, ty, tz;
-    tile_of_block(ntx, nty, ntz, tx, ty, tz);
-    const int  x0 = tx * kTileX, y0 = ty * kTileY, z0 = tz * kTileZ;
-    const bool staged = stage_tile(L, g, cell_start, P.gx, P.gy, P.gz, x0, y0, z0);
-    const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
-    if(i >= ex || j >= ey || k >= ez) return;
-    const bool doU = j < g.ny && k < g.nz;
-    const bool doV = i < g.nx && k < g.nz;
-    const bool doW = i < g.nx && j < g.ny;
-    float      mu = 0.f, wu = 0.f, mv = 0.f, wv = 0.f, mw = 0.f, ww = 0.f;
-    auto       visit = [&](float gx, float gy, float gz, uint32_t q) {
-        if(doU) {
-            float a, b, c;
-            if(stencil_weight(gx, g.nx + 1, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) && stencil_weight(gz - 0.5f, g.nz, k, c)) {
-                const float wt = a * b * c;
-                mu += wt * P.vx[q];
-                wu += wt;
-            }
-        }
-        if(doV) {
-            float a, b, c;
-            if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy, g.ny + 1, j, b) && stencil_weight(gz - 0.5f, g.nz, k, c)) {
-                const float wt = a * b * c;
-                mv += wt * P.vy[q];
-                wv += wt;
-            }
-        }
-        if(doW) {
-            float a, b, c;
-            if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) && stencil_weight(gz, g.nz + 1, k, c)) {
-                const float wt = a * b * c;
-                mw += wt * P.vz[q];
-                ww += wt;
-            }
-        }
-    };
-    const int xa = max(i - 1, 0), xb = min(i + 1, g.nx - 1);
-    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
-        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y) {
-            if(staged) {  // cells xa..xb of a halo row are one contiguous LDS run
-                const int      ha = halo_index(xa, y, z, x0, y0, z0), hb = halo_index(xb, y, z, x0, y0, z0);
-                const uint32_t pb = L.begin[ha], pe = L.begin[hb] + L.count[hb], gofs = L.gbegin[ha] - pb;
-                for(uint32_t p = pb; p < pe; ++p) visit(L.x[p], L.y[p], L.z[p], p + gofs);
-            } else {
-                const size_t   r  = lin3(0, y, z, g.nx, g.ny);
-                const uint32_t pb = cell_start[r + xa], pe = cell_start[r + xb + 1];
-                for(uint32_t p = pb; p < pe; ++p) visit(P.gx[p], P.gy[p], P.gz[p], p);
-            }
-        }
-    if(doU) {
-        const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
-        const bool   ok = wu > kFltMin;
-        out.u[id]       = ok ? mu / wu : 0.f;
-        out.vu[id]      = ok ? 1 : 0;
-    }
-    if(doV) {
-        const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
-        const bool   ok = wv > kFltMin;
-        out.v[id]       = ok ? mv / wv : 0.f;
-        out.vv[id]      = ok ? 1 : 0;
-    }
-    if(doW) {
-        const size_t id = lin3(i, j, k, g.nx, g.ny);
-        const bool   ok = ww > kFltMin;
-        out.w[id]       = ok ? mw / ww : 0.f;
-        out.vw[id]      = ok ? 1 : 0;
-    }
 }
 
 //==================================================================================================
@@ -1303,19 +1270,52 @@ __device__ __forceinline__ int cell_color(const LevelDev& L, int s, int l) {
     return ((bx * 8 + (l & 7)) + (by * 8 + ((l >> 3) & 7)) + (bz * 8 + (l >> 6))) & 1;
 }
 
-// one red-black half sweep: cells of `color` get x = (b - sum a x_nbr) / diag. With zeroInit the
-// other colour is set to 0 first (x starts at 0 in every V-cycle), which needs no neighbour reads.
-__global__ void __launch_bounds__(512) k_mg_smooth(LevelDev L, int color, int zeroInit, const PcgState* __restrict__ st) {
+// Brick-hybrid Gauss-Seidel: one block per brick runs `sweeps` red-black sweeps (black-red with
+// `reverse`) on its 512 cells in LDS, with the x of neighbouring bricks held at their values on entry
+// (Jacobi between bricks). For the brick's own system A_BB the red-black and black-red sequences are
+// transposes of each other, so the pre- and post-smoothers are adjoint and the V-cycle stays
+// symmetric. The starting values come from xin (or are 0 with zeroInit: every V-cycle starts from
+// x = 0) and the result goes to L.x; xin is never L.x, so no block reads what another writes.
+__global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, const float* __restrict__ xin, int sweeps, int reverse,
+                                                         int zeroInit, const PcgState* __restrict__ st) {
+    __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
     if(st && st->done) return;
+    const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const int c = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
     for(uint32_t slot = blockIdx.x; slot < L.nSlots; slot += gridDim.x) {
-        const int    s = (int)slot, l = threadIdx.x;
-        const size_t o = (size_t)s * 512 + l;
-        if(cell_color(L, s, l) != color) {
-            if(zeroInit) L.x[o] = 0.f;
-            continue;
-        }
-        const float off = zeroInit ? 0.f : level_offdiag_sum(L, s, l, L.x);
-        L.x[o]          = (L.b[o] - off) * L.invdiag[o];
+        const int     s   = (int)slot;
+        const size_t  o   = (size_t)s * 512 + l;
+        const int32_t ixm = nbr_index(L.nbr, s, l, 0), ixp = nbr_index(L.nbr, s, l, 1);
+        const int32_t iym = nbr_index(L.nbr, s, l, 2), iyp = nbr_index(L.nbr, s, l, 3);
+        const int32_t izm = nbr_index(L.nbr, s, l, 4), izp = nbr_index(L.nbr, s, l, 5);
+        const float   czm = izm >= 0 ? L.cz[izm] : 0.f, cym = iym >= 0 ? L.cy[iym] : 0.f, cxm = ixm >= 0 ? L.cx[ixm] : 0.f;
+        const float   cxp = L.cx[o], cyp = L.cy[o], czp = L.cz[o];
+        const float   bv = L.b[o], inv = L.invdiag[o];
+        xs[c]            = zeroInit ? 0.f : xin[o];
+        if(lx == 0) xs[c - 1] = (zeroInit || ixm < 0) ? 0.f : xin[ixm];
+        if(lx == 7) xs[c + 1] = (zeroInit || ixp < 0) ? 0.f : xin[ixp];
+        if(ly == 0) xs[c - 10] = (zeroInit || iym < 0) ? 0.f : xin[iym];
+        if(ly == 7) xs[c + 10] = (zeroInit || iyp < 0) ? 0.f : xin[iyp];
+        if(lz == 0) xs[c - 100] = (zeroInit || izm < 0) ? 0.f : xin[izm];
+        if(lz == 7) xs[c + 100] = (zeroInit || izp < 0) ? 0.f : xin[izp];
+        const int color = cell_color(L, s, l);
+        __syncthreads();
+        for(int it = 0; it < sweeps; ++it)
+            for(int half = 0; half < 2; ++half) {
+                if(color == (reverse ? 1 - half : half)) {
+                    float acc = 0.f;
+                    if(izm >= 0) acc += czm * xs[c - 100];
+                    if(iym >= 0) acc += cym * xs[c - 10];
+                    if(ixm >= 0) acc += cxm * xs[c - 1];
+                    if(ixp >= 0) acc += cxp * xs[c + 1];
+                    if(iyp >= 0) acc += cyp * xs[c + 10];
+                    if(izp >= 0) acc += czp * xs[c + 100];
+                    xs[c] = (bv - acc) * inv;
+                }
+                __syncthreads();
+            }
+        L.x[o] = xs[c];
+        __syncthreads();  // xs is reused by the block's next brick
     }
 }
 
@@ -1346,7 +1346,8 @@ __global__ void __launch_bounds__(512) k_mg_restrict(LevelDev F, LevelDev C, con
     }
 }
 
-// x_fine += omega * x_coarse(parent) (P x_c, scaled)
+// r_fine = x_fine + omega * x_coarse(parent) (P x_c, scaled): the corrected x goes to F.r (free
+// after the restriction) as the post-smoother's starting values
 __global__ void __launch_bounds__(512) k_mg_prolong(LevelDev F, LevelDev C, float omega, const PcgState* __restrict__ st) {
     if(st && st->done) return;
     for(uint32_t slot = blockIdx.x; slot < F.nSlots; slot += gridDim.x) {
@@ -1355,9 +1356,12 @@ __global__ void __launch_bounds__(512) k_mg_prolong(LevelDev F, LevelDev C, floa
         const int    bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
         const int    i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
         const size_t o = (size_t)s * 512 + l;
-        if(F.invdiag[o] == 0.f) continue;  // not an unknown
-        const int32_t c = level_cell_index(C, i >> 1, j >> 1, k >> 1);
-        if(c >= 0) F.x[o] += omega * C.x[c];
+        float        xv = F.x[o];
+        if(F.invdiag[o] != 0.f) {  // an unknown
+            const int32_t c = level_cell_index(C, i >> 1, j >> 1, k >> 1);
+            if(c >= 0) xv += omega * C.x[c];
+        }
+        F.r[o] = xv;
     }
 }
 

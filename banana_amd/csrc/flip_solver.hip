@@ -186,7 +186,7 @@ public:
     // grid
     size_t nU = 0, nV = 0, nW = 0, nC = 0, nN = 0;
     DevBuf u, v, w, ut, vt, wt, uo, vo, wo, wu, wv, ww;
-    DevBuf vu, vv, vw, vtmp, eu, ew;
+    DevBuf vu, vv, vw, eu, ew;
     DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters;
     bool     weightsDirty    = true;   // boundary SDF changed since the weights were computed
     bool     weightsComputed = false;  // computeCellWeights has run at least once
@@ -298,7 +298,6 @@ public:
         eu.reserve(nU);
         ev_.reserve(std::max(nV, (size_t)1));
         ew.reserve(nW);
-        vtmp.reserve(std::max(nU, std::max(nV, nW)));
         phi.reserve(nC * 4);
         bsdf.reserve(nN * 4);
         cellStart.reserve((nC + 1) * 4);
@@ -311,7 +310,7 @@ public:
         pst.reserve(sizeof(PcgState));
         ost.reserve(sizeof(OuterState));
         // zero everything the reference zero-fills at resize (PIC_Data.cpp:101-117)
-        for(auto* b : {&u, &v, &w, &ut, &vt, &wt, &uo, &vo, &wo, &wu, &wv, &ww, &vu, &vv, &vw, &eu, &ev_, &ew, &vtmp, &phi, &bsdf,
+        for(auto* b : {&u, &v, &w, &ut, &vt, &wt, &uo, &vo, &wo, &wu, &wv, &ww, &vu, &vv, &vw, &eu, &ev_, &ew, &phi, &bsdf,
                        &cellStart, &cellCount, &counters, &pst, &ost})
             HIP_OK(hipMemsetAsync(b->p, 0, b->bytes, stream));
         weightsDirty = true;
@@ -494,31 +493,26 @@ public:
     // FLIP_Solver.cpp:57-95
     void mapParticles2Grid() {
         requireBinned();
-        FaceOut        out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
-        const unsigned tiles = blocks(g.nx + 1, kTileX) * blocks(g.ny + 1, kTileY) * blocks(g.nz + 1, kTileZ);
+        FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
+        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
         ktBegin(BNN_KT_P2G);
-        hipLaunchKernelGGL(k_p2g_tiled, dim3(tiles), dim3(kTileThreads), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
-                           g, out);
+        hipLaunchKernelGGL(k_p2g, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(), g,
+                           out);
         ktEnd(BNN_KT_P2G);
     }
 
-    // PIC_Solver.cpp:985-1090: 4 layers per component, ping-pong with the temp arrays
+    // PIC_Solver.cpp:985-1090: 4 Jacobi layers per component, in place with layer stamps
     void extrapolateVelocity() {
         struct C {
-            DevBuf *a, *t, *val, *ext;
+            DevBuf *a, *val, *ext;
             int     sx, sy, sz;
-        } comps[3] = {{&u, &ut, &vu, &eu, g.nx + 1, g.ny, g.nz}, {&v, &vt, &vv, &ev_, g.nx, g.ny + 1, g.nz}, {&w, &wt, &vw, &ew, g.nx, g.ny, g.nz + 1}};
+        } comps[3] = {{&u, &vu, &eu, g.nx + 1, g.ny, g.nz}, {&v, &vv, &ev_, g.nx, g.ny + 1, g.nz}, {&w, &vw, &ew, g.nx, g.ny, g.nz + 1}};
         for(auto& c : comps) {
             const size_t n = (size_t)c.sx * c.sy * c.sz;
-            for(int layer = 0; layer < 4; ++layer) {
-                const bool even = (layer % 2) == 0;
-                float*     in   = even ? c.a->as<float>() : c.t->as<float>();
-                float*     out  = even ? c.t->as<float>() : c.a->as<float>();
-                uint8_t*   vin  = even ? c.val->as<uint8_t>() : vtmp.as<uint8_t>();
-                uint8_t*   vout = even ? vtmp.as<uint8_t>() : c.val->as<uint8_t>();
-                hipLaunchKernelGGL(k_extrap_layer, dim3(blocks(n)), dim3(kBlock), 0, stream, (const float*)in, out,
-                                   (const uint8_t*)vin, vout, c.ext->as<uint8_t>(), c.sx, c.sy, c.sz, layer == 0 ? 1 : 0);
-            }
+            for(int layer = 0; layer < 4; ++layer)
+                hipLaunchKernelGGL(k_extrap_stamp, dim3(blocks(n)), dim3(kBlock), 0, stream, c.a->as<float>(), c.val->as<uint8_t>(),
+                                   c.ext->as<uint8_t>(), c.sx, c.sy, c.sz, layer);
+            hipLaunchKernelGGL(k_extrap_finish, dim3(blocks(n)), dim3(kBlock), 0, stream, c.val->as<uint8_t>(), n);
         }
     }
 
@@ -696,17 +690,14 @@ public:
 
     static unsigned gridFor(uint32_t slots) { return std::min<unsigned>(std::max<uint32_t>(slots, 1u), (unsigned)kMaxRedBlocks); }
 
-    // z = M^-1 r on the fine level (b = r, x = z); symmetric V-cycle
+    // z = M^-1 r on the fine level (b = r, x = z); symmetric V-cycle with the brick-hybrid smoother
     void vcycle(const PcgState* stp) {
         const int   nu    = prm.mg_sweeps;
         const float omega = prm.mg_omega;
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = gridFor(L.nSlots);
-            for(int sw = 0; sw < nu; ++sw) {
-                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 0, sw == 0 ? 1 : 0, stp);
-                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 1, 0, stp);
-            }
+            hipLaunchKernelGGL(k_mg_smooth_brick, dim3(G), dim3(512), 0, stream, L, (const float*)nullptr, nu, 0, 1, stp);
             hipLaunchKernelGGL(k_mg_residual, dim3(G), dim3(512), 0, stream, L, stp);
             hipLaunchKernelGGL(k_mg_restrict, dim3(gridFor(C.nSlots)), dim3(512), 0, stream, L, C, stp);
         }
@@ -720,10 +711,7 @@ public:
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = gridFor(L.nSlots);
             hipLaunchKernelGGL(k_mg_prolong, dim3(G), dim3(512), 0, stream, L, C, omega, stp);
-            for(int sw = 0; sw < nu; ++sw) {
-                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 1, 0, stp);
-                hipLaunchKernelGGL(k_mg_smooth, dim3(G), dim3(512), 0, stream, L, 0, 0, stp);
-            }
+            hipLaunchKernelGGL(k_mg_smooth_brick, dim3(G), dim3(512), 0, stream, L, (const float*)L.r, nu, 1, 0, stp);
         }
     }
 
@@ -1159,7 +1147,7 @@ void bnn_flip_default_params(bnn_flip_params* p) {
     p->refine_tolerance               = 1e-10;
     p->preconditioner                 = BNN_PRECOND_MG;
     p->timing                         = 0;
-    p->mg_sweeps                      = 2;
+    p->mg_sweeps                      = 4;
     p->mg_omega                       = 1.8f;
     p->mg_coarsest_sweeps             = 24;
     p->start_frame                    = 1;

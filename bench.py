@@ -173,6 +173,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge in cells (256 = the headline workload)")
     ap.add_argument("--pressure", choices=["refined", "fp32"], default="refined")
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
+    ap.add_argument("--mg-sweeps", type=int, default=None, help="smoothing sweeps per multigrid level (library default when omitted)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="grid edge of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3, help="substeps of the CPU baseline sample")
@@ -197,6 +198,8 @@ def main():
     params, spec = scenes.dam_break(args.n, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
     params.update(pressure_mode=1 if args.pressure == "refined" else 0, refine_tolerance=1e-10, timing=args.timing,
                   preconditioner=1 if args.precond == "mg" else 0)
+    if args.mg_sweeps is not None:
+        params["mg_sweeps"] = args.mg_sweeps
     dt = np.float32(1.0 / 600.0)
     t0 = time.perf_counter()
     pos, vel = banana_amd.generate_scene_host(params, spec, 2024)

@@ -87,8 +87,9 @@ typedef struct bnn_flip_params {
     int32_t  preconditioner;                /* bnn_preconditioner (MG) */
     int32_t  timing;                        /* 1: record per-stage HIP events (stats.stage_ms),
                                                2: also per-launch events of the timed kernels */
-    int32_t  mg_sweeps;                     /* MG: red-black sweeps before and after the coarse
-                                               correction on each level (2) */
+    int32_t  mg_sweeps;                     /* MG: red-black sweeps inside each 8^3 brick (neighbour
+                                               bricks held) before and after the coarse correction
+                                               on each level (4) */
     float    mg_omega;                      /* MG: scale of the coarse-grid correction (1.8; the
                                                piecewise-constant coarse space under-corrects) */
     int32_t  mg_coarsest_sweeps;            /* MG: symmetric sweeps on the coarsest level (24) */
