@@ -286,14 +286,18 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __r
     const bool doV = i < g.nx && k < g.nz;
     const bool doW = i < g.nx && j < g.ny;
     float mu = 0.f, wu = 0.f, mv = 0.f, wv = 0.f, mw = 0.f, ww = 0.f;
-    const int x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1);
+    // Rows and cells that cannot hold a corner of a component's stencil are not tested for it (u: x
+    // cells i-1..i; v: rows y <= j; w: rows z <= k); the contributing particles and their order
+    // are unchanged.
+    const int x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1), xu = min(i, g.nx - 1);
     for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
         for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y) {
-            const size_t   r  = lin3(0, y, z, g.nx, g.ny);
-            const uint32_t pb = cell_start[r + x0], pe = cell_start[r + x1 + 1];
+            const size_t   r    = lin3(0, y, z, g.nx, g.ny);
+            const uint32_t pb   = cell_start[r + x0], pe = cell_start[r + x1 + 1], pu = cell_start[r + xu + 1];
+            const bool     rowV = doV && y <= j, rowW = doW && z <= k;
             for(uint32_t p = pb; p < pe; ++p) {
                 const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
-                if(doU) {
+                if(doU && p < pu) {
                     float a, b, c;
                     if(stencil_weight(gx, g.nx + 1, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) &&
                        stencil_weight(gz - 0.5f, g.nz, k, c)) {
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __r
                         wu += wt;
                     }
                 }
-                if(doV) {
+                if(rowV) {
                     float a, b, c;
                     if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy, g.ny + 1, j, b) &&
                        stencil_weight(gz - 0.5f, g.nz, k, c)) {
@@ -311,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __r
                         wv += wt;
                     }
                 }
-                if(doW) {
+                if(rowW) {
                     float a, b, c;
                     if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) &&
                        stencil_weight(gz, g.nz + 1, k, c)) {
@@ -664,34 +668,49 @@ __global__ void __launch_bounds__(kTileThreads) k_fluid_sdf_tiled(Particles P, c
     const bool staged = stage_tile(L, g, cell_start, P.px, P.py, P.pz, x0, y0, z0);
     const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
     if(i >= g.nx || j >= g.ny || k >= g.nz) return;
-    const float sx   = ((float)i + 0.5f) * g.dx + g.bminx;
-    const float sy   = ((float)j + 0.5f) * g.dx + g.bminy;
-    const float sz   = ((float)k + 0.5f) * g.dx + g.bminz;
-    float       best = g.dx * 3.0f;
-    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
-        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y)
-            for(int x = max(i - 1, 0); x <= min(i + 1, g.nx - 1); ++x) {
-                const bool boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
-                auto       visit    = [&](float ppx, float ppy, float ppz) {
-                    if(boundary) {
-                        // particles clamped into a boundary cell: use their true (unclamped) window
-                        const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
-                                  ck = (int)((ppz - g.bminz) / g.dx);
-                        if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) return;
+    const float sx      = ((float)i + 0.5f) * g.dx + g.bminx;
+    const float sy      = ((float)j + 0.5f) * g.dx + g.bminy;
+    const float sz      = ((float)k + 0.5f) * g.dx + g.bminz;
+    float       best    = g.dx * 3.0f;
+    float       bestLen = 3.40282347e+38f;  // smallest |centre - p| so far
+    // The own cell first, then its 6 face, 12 edge and 8 corner neighbours: no particle of a class-c
+    // neighbour is nearer to the centre than 0, 0.5, 0.7071 or 0.8660 dx (less 1e-3 dx for the
+    // rounding of particle-to-cell assignment), so once the nearest particle found beats a class's
+    // bound, that class and all farther ones cannot lower the minimum. min() is exact, so visiting
+    // fewer cells gives the same bits.
+    const float bound[4] = {0.f, 0.499f * g.dx, 0.7061068f * g.dx, 0.8650254f * g.dx};
+    for(int cls = 0; cls < 4; ++cls) {
+        if(bestLen < bound[cls]) break;
+        for(int dz = -1; dz <= 1; ++dz)
+            for(int dy = -1; dy <= 1; ++dy)
+                for(int dxo = -1; dxo <= 1; ++dxo) {
+                    if(abs(dz) + abs(dy) + abs(dxo) != cls) continue;
+                    const int x = i + dxo, y = j + dy, z = k + dz;
+                    if(x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                    const bool boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
+                    auto       visit    = [&](float ppx, float ppy, float ppz) {
+                        if(boundary) {
+                            // particles clamped into a boundary cell: use their true (unclamped) window
+                            const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
+                                      ck = (int)((ppz - g.bminz) / g.dx);
+                            if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) return;
+                        }
+                        const float len = len3f(sx - ppx, sy - ppy, sz - ppz);
+                        const float v   = len - g.sdfRadius;
+                        if(v < best) best = v;
+                        if(len < bestLen) bestLen = len;
+                    };
+                    if(staged) {
+                        const int      h  = halo_index(x, y, z, x0, y0, z0);
+                        const uint32_t pb = L.begin[h], pe = pb + L.count[h];
+                        for(uint32_t p = pb; p < pe; ++p) visit(L.x[p], L.y[p], L.z[p]);
+                    } else {
+                        const size_t   c  = lin3(x, y, z, g.nx, g.ny);
+                        const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
+                        for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
                     }
-                    const float v = len3f(sx - ppx, sy - ppy, sz - ppz) - g.sdfRadius;
-                    if(v < best) best = v;
-                };
-                if(staged) {
-                    const int      h  = halo_index(x, y, z, x0, y0, z0);
-                    const uint32_t pb = L.begin[h], pe = pb + L.count[h];
-                    for(uint32_t p = pb; p < pe; ++p) visit(L.x[p], L.y[p], L.z[p]);
-                } else {
-                    const size_t   c  = lin3(x, y, z, g.nx, g.ny);
-                    const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
-                    for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
                 }
-            }
+    }
     if(best < g.halfdx) {
         const int   sxn = g.nx + 1, syn = g.ny + 1;
         auto        B   = [&](int a, int c, int e) { return bsdf[lin3(a, c, e, sxn, syn)]; };
