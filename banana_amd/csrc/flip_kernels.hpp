@@ -178,6 +178,158 @@ __global__ void __launch_bounds__(kBlock) k_permute(Particles in, Particles out,
     out.gz[i]        = (z - g.bminz) / g.dx;
 }
 
+//==================================================================================================
+// Z-order sort of the particles (PIC_Solver.cpp:315-328 -> NeighborSearch::z_sort,
+// NeighborSearch.cpp:45-86). Keys are built in the caller's particle order; a stable LSD radix sort
+// on 8-bit digits gives the permutation; the particles' external ids are renamed by it (storage on
+// the device stays cell-sorted, so no particle data moves).
+//==================================================================================================
+// NeighborSearch::cell_index (NeighborSearch.cpp:47-55) and the shift of z_value (:60-70): cell of
+// x - 1.2345 through the reciprocal cell size (floor for negative offsets), then minus (INT_MIN + 1).
+__device__ __forceinline__ uint64_t zsort_coord(float x, float inv) {
+    const float tmp = x - 1.2345f;
+    const int   k   = tmp >= 0.f ? (int)(inv * tmp) : (int)(inv * tmp) - 1;
+    return (uint64_t)((int64_t)k + 2147483647ll);
+}
+// bit b of v (b < 21) to bit 3b
+__device__ __forceinline__ uint64_t spread_by_3(uint64_t v) {
+    v &= 0x1FFFFFull;
+    v = (v | v << 32) & 0x001F00000000FFFFull;
+    v = (v | v << 16) & 0x001F0000FF0000FFull;
+    v = (v | v << 8) & 0x100F00F00F00F00Full;
+    v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+// morton3D_64_encode (Morton3D.h:54-66) keeps, in 64 bits, bit b of coordinate d at bit 3b + d:
+// 22 bits of x, 21 of y and z.
+__device__ __forceinline__ uint64_t zsort_key(float x, float y, float z, float inv) {
+    const uint64_t cx = zsort_coord(x, inv), cy = zsort_coord(y, inv), cz = zsort_coord(z, inv);
+    return spread_by_3(cx) | spread_by_3(cy) << 1 | spread_by_3(cz) << 2 | ((cx >> 21) & 1ull) << 63;
+}
+
+// keys in external order, split in 32-bit halves; values = external index
+__global__ void __launch_bounds__(kBlock) k_zsort_keys(Particles P, uint32_t n, float inv, uint32_t* __restrict__ klo,
+                                                       uint32_t* __restrict__ khi, uint32_t* __restrict__ val) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    const uint64_t k = zsort_key(P.px[p], P.py[p], P.pz[p], inv);
+    const uint32_t e = P.id[p];
+    klo[e]           = (uint32_t)k;
+    khi[e]           = (uint32_t)(k >> 32);
+    val[e]           = e;
+}
+
+// AND and OR of all keys (digits on which every key agrees need no pass); acc = {and, or}, preset
+// to {~0, 0}
+__global__ void __launch_bounds__(kBlock) k_zsort_andor(const uint32_t* __restrict__ klo, const uint32_t* __restrict__ khi,
+                                                        uint32_t n, unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long sa[kBlock / 64], so[kBlock / 64];
+    unsigned long long a = ~0ull, o = 0ull;
+    for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long k = (unsigned long long)khi[i] << 32 | klo[i];
+        a &= k;
+        o |= k;
+    }
+    for(int off = 32; off > 0; off >>= 1) {
+        a &= __shfl_xor(a, off, 64);
+        o |= __shfl_xor(o, off, 64);
+    }
+    if((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = a;
+        so[threadIdx.x >> 6] = o;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        for(int w = 1; w < kBlock / 64; ++w) {
+            a &= sa[w];
+            o |= so[w];
+        }
+        atomicAnd(&acc[0], a);
+        atomicOr(&acc[1], o);
+    }
+}
+
+constexpr int kRsItems = 16;                  // keys per thread per tile
+constexpr int kRsTile  = kBlock * kRsItems;   // keys per tile (one workgroup)
+static_assert(kBlock == 256, "the radix passes use one thread per digit value");
+
+// digit histogram of one tile, stored digit-major: hist[d * ntiles + tile]
+__global__ void __launch_bounds__(kBlock) k_rs_hist(const uint32_t* __restrict__ kw, uint32_t n, int shift, uint32_t ntiles,
+                                                    uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    for(int c = 0; c < kRsItems; ++c) {
+        const size_t i = base + (size_t)c * kBlock + threadIdx.x;
+        if(i < n) atomicAdd(&h[(kw[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// stable scatter of one tile: offs = exclusive scan of hist; inside the tile, keys keep their order
+// (wave ballots give the rank among equal digits, earlier waves and chunks come first)
+__global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint32_t* __restrict__ klo, const uint32_t* __restrict__ khi,
+                                                       const uint32_t* __restrict__ val, uint32_t n, int word, int shift,
+                                                       uint32_t ntiles, const uint32_t* __restrict__ offs,
+                                                       uint32_t* __restrict__ olo, uint32_t* __restrict__ ohi,
+                                                       uint32_t* __restrict__ oval) {
+    constexpr int       kWaves = kBlock / 64;
+    __shared__ uint32_t next[256];
+    __shared__ uint32_t wcnt[kWaves][256];
+    const int           t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    next[t] = offs[(size_t)t * ntiles + blockIdx.x];
+    for(int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    for(int c = 0; c < kRsItems; ++c) {
+        const size_t i   = (size_t)blockIdx.x * kRsTile + (size_t)c * kBlock + t;
+        const bool   act = i < n;
+        uint32_t     lo = 0, hi = 0, v = 0, d = 0;
+        if(act) {
+            lo = klo[i];
+            hi = khi[i];
+            v  = val[i];
+            d  = ((word ? hi : lo) >> shift) & 255u;
+        }
+        uint64_t peers = __ballot(act);
+        for(int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if(act && rank == 0) wcnt[wv][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if(act) {
+            uint32_t pos = next[d] + rank;
+            for(int w = 0; w < wv; ++w) pos += wcnt[w][d];
+            olo[pos]  = lo;
+            ohi[pos]  = hi;
+            oval[pos] = v;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        for(int w = 0; w < kWaves; ++w) {
+            tot += wcnt[w][t];
+            wcnt[w][t] = 0;
+        }
+        next[t] += tot;
+        __syncthreads();
+    }
+}
+
+// rank[perm[q]] = q, then every particle's external id becomes its rank
+__global__ void __launch_bounds__(kBlock) k_zsort_rank(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ rank) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if(q < n) rank[perm[q]] = q;
+}
+__global__ void __launch_bounds__(kBlock) k_zsort_rename(uint32_t* __restrict__ id, uint32_t n, const uint32_t* __restrict__ rank) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p < n) id[p] = rank[id[p]];
+}
+
 //--------------------------------------------------------------------------------------------------
 // device-wide exclusive scan of uint32 (3 phases); tile = 256 threads x 8 items
 //--------------------------------------------------------------------------------------------------

@@ -432,6 +432,7 @@ private:
             std::vector<float> pos(s.np * 3), vel(s.np * 3);
             generate_scene(s.prm, s.spec, s.seed, pos.data(), vel.data(), s.np);
             sv->setParticles(pos.data(), vel.data(), s.np);
+            sv->sortParticles();  // as bnn_flip_generate_scene
             s.solver = std::move(sv);
         } catch(const std::exception& e) {
             msg = e.what();

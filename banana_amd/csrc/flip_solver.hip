@@ -181,6 +181,7 @@ public:
     ParticleBufs bufA, bufB;
     ParticleBufs *cur = &bufA, *alt = &bufB;
     DevBuf       key, perm;
+    DevBuf       zsHist, zsAcc;  // z-sort digit histograms / offsets, key AND-OR
     bool         binned = false;  // cell_start / g* valid for the current particle positions
 
     // grid
@@ -270,6 +271,7 @@ public:
         for(int d = 0; d < 3; ++d)
             if(!(prm.box_max[d] > prm.box_min[d])) throw Error(BNN_ERR_INVALID_ARG, "SimulationDomainBox: BoxMax must exceed BoxMin");
         if(prm.advection_steps < 1) throw Error(BNN_ERR_INVALID_ARG, "AdvectionSteps must be >= 1");
+        if(prm.enable_sort_particle && prm.sort_frequency < 1) throw Error(BNN_ERR_INVALID_ARG, "SortFrequency must be >= 1");
         if(prm.correct_position) throw Error(BNN_ERR_UNSUPPORTED, "CorrectPosition (PIC_Solver.cpp:377-485) is not implemented");
         if(prm.gravity_type < 0 || prm.gravity_type > 3) throw Error(BNN_ERR_INVALID_ARG, "GravityType must be 0..3");
         if(prm.preconditioner != BNN_PRECOND_JACOBI && prm.preconditioner != BNN_PRECOND_MG)
@@ -1003,6 +1005,56 @@ public:
         return ok;
     }
 
+    // PIC_Solver.cpp:315-328: z-sort at frame 0 and then every SortFrequency frames, if enabled
+    void sortParticles() {
+        if(!prm.enable_sort_particle || (finishedFrame > 0 && (finishedFrame + 1) % prm.sort_frequency != 0)) return;
+        zSortParticles();
+    }
+
+    // NeighborSearch::z_sort + PointSet::sort_field (NeighborSearch.cpp:72-86, PointSet.h:114-131):
+    // stable by the caller's index (the reference's tbb::parallel_sort leaves ties unordered). The
+    // spare particle buffer holds the keys; only the external ids change.
+    void zSortParticles() {
+        if(!np) return;
+        const uint32_t n      = (uint32_t)np;
+        const uint32_t ntiles = (uint32_t)((np + kRsTile - 1) / kRsTile);
+        const size_t   nh     = (size_t)256 * ntiles;
+        zsHist.reserve(nh * 4 * 2);
+        zsAcc.reserve(16);
+        scanPart.reserve(std::max<size_t>(scanPart.bytes, (blocks(nh, kScanTile) + 1) * 4));
+        uint32_t*      lo[2]  = {alt->px.as<uint32_t>(), alt->vx.as<uint32_t>()};
+        uint32_t*      hi[2]  = {alt->py.as<uint32_t>(), alt->vy.as<uint32_t>()};
+        uint32_t*      val[2] = {alt->pz.as<uint32_t>(), alt->vz.as<uint32_t>()};
+        const float    inv    = (float)(1.0 / (double)prm.cell_size);  // NeighborSearch.cpp:33
+        hipLaunchKernelGGL(k_zsort_keys, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), n, inv, lo[0], hi[0], val[0]);
+        const unsigned long long init[2] = {~0ull, 0ull};
+        HIP_OK(hipMemcpyAsync(zsAcc.p, init, 16, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_zsort_andor, dim3(std::min<unsigned>(blocks(np), 1024)), dim3(kBlock), 0, stream,
+                           (const uint32_t*)lo[0], (const uint32_t*)hi[0], n, zsAcc.as<unsigned long long>());
+        HIP_OK(hipMemcpyAsync(hostU32, zsAcc.p, 16, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        uint64_t acc[2];
+        std::memcpy(acc, hostU32, 16);
+        const uint64_t varying = acc[0] ^ acc[1];
+        uint32_t*      hist    = zsHist.as<uint32_t>();
+        uint32_t*      offs    = hist + nh;
+        int            s       = 0;
+        for(int d = 0; d < 8; ++d) {
+            if(((varying >> (8 * d)) & 0xFFull) == 0) continue;  // every key has the same digit here
+            const int word = d >> 2, shift = (d & 3) * 8;
+            hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kBlock), 0, stream, (const uint32_t*)(word ? hi[s] : lo[s]), n, shift,
+                               ntiles, hist);
+            scan(hist, offs, nh, nullptr);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kBlock), 0, stream, (const uint32_t*)lo[s], (const uint32_t*)hi[s],
+                               (const uint32_t*)val[s], n, word, shift, ntiles, (const uint32_t*)offs, lo[1 - s], hi[1 - s], val[1 - s]);
+            s = 1 - s;
+        }
+        uint32_t* rank = alt->gx.as<uint32_t>();
+        hipLaunchKernelGGL(k_zsort_rank, dim3(blocks(np)), dim3(kBlock), 0, stream, (const uint32_t*)val[s], n, rank);
+        hipLaunchKernelGGL(k_zsort_rename, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->id.as<uint32_t>(), n, (const uint32_t*)rank);
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
     // readback of a per-cell pressure-system vector in the reference's activeCellIdx order
     template<class T> void gatherActive(const DevBuf& vec, double* dst, uint64_t* nActive) {
         std::vector<float>   ph(nC);
@@ -1266,6 +1318,7 @@ bnn_status bnn_flip_generate_scene(bnn_flip* h, const bnn_scene_spec* spec, uint
         std::vector<float> pos(n * 3), vel(n * 3);
         bnn::generate_scene(s.prm, *spec, seed, pos.data(), vel.data(), n);
         s.setParticles(pos.data(), vel.data(), n);
+        s.sortParticles();  // PIC_Solver::generateParticles sorts the new particles (PIC_Solver.cpp:57)
         return BNN_OK;
     });
 }
@@ -1294,7 +1347,7 @@ bnn_status bnn_flip_advance_frame(bnn_flip* h) {
 bnn_status bnn_flip_do_frame_simulation(bnn_flip* h, uint32_t frame) {
     (void)frame;
     return guard(h, [&](bnn::FlipSolver& s) {
-        if(s.prm.enable_sort_particle) throw bnn::Error(BNN_ERR_UNSUPPORTED, "EnableSortParticle (z-sort) is not built yet");
+        s.sortParticles();
         return solve_status(s, s.advanceFrame());
     });
 }
@@ -1316,8 +1369,8 @@ bnn_status bnn_flip_move_particles(bnn_flip* h, float dt) {
 
 bnn_status bnn_flip_sort_particles(bnn_flip* h) {
     return guard(h, [&](bnn::FlipSolver& s) {
-        if(!s.prm.enable_sort_particle) return BNN_OK;  // PIC_Solver.cpp:316-318
-        throw bnn::Error(BNN_ERR_UNSUPPORTED, "EnableSortParticle (z-sort) is not built yet");
+        s.sortParticles();
+        return BNN_OK;
     });
 }
 

@@ -200,6 +200,9 @@ def main():
                   preconditioner=1 if args.precond == "mg" else 0)
     if args.mg_sweeps is not None:
         params["mg_sweeps"] = args.mg_sweeps
+    # the z-sort only runs when called (set_particles does not sort; steps never do); it is timed
+    # once after the timed steps
+    params.update(enable_sort_particle=1, sort_frequency=10)
     dt = np.float32(1.0 / 600.0)
     t0 = time.perf_counter()
     pos, vel = banana_amd.generate_scene_host(params, spec, 2024)
@@ -242,6 +245,13 @@ def main():
     elapsed = time.perf_counter() - t_start
     np_ = sim.num_particles()
     value = np_ * args.steps / elapsed
+
+    zsort_ms = []
+    for _ in range(2):  # the first call also allocates its histogram buffer
+        tz = time.perf_counter()
+        sim.sort_particles()  # synchronous
+        zsort_ms.append(1e3 * (time.perf_counter() - tz))
+    log(f"z-sort of {np_} particles: {zsort_ms[-1]:.2f} ms")
 
     roofline = None
     if spmv_launches and spmv_ms > 0:
@@ -294,6 +304,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "zsort": {"ms_per_call": round(zsort_ms[-1], 3), "particles": np_},
     }
     print(json.dumps(out), flush=True)
 
