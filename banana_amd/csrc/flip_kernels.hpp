@@ -429,6 +429,109 @@ __device__ __forceinline__ bool stencil_weight(float q, int i_high, int target, 
     return false;
 }
 
+//--------------------------------------------------------------------------------------------------
+// P2G as a gather that streams along z. A thread owns the nodes (i, j, k0 .. k0+kP2gSeg-1) of all
+// three face grids and reads each particle row z = k0-1 .. k0+kP2gSeg once: a particle binned to
+// row z reaches only nodes z-1 .. z+1 along z (its z base is z-1 or z for the half-shifted stencil,
+// z for the node-aligned one, clamps included), which a three-node register window holds. Node k
+// still receives rows k-1, k, k+1 in that order, each row's y rows in order and each y row's
+// particles in cell-sorted order, with the weight (wx*wy)*wz and the momentum w*v rounded before
+// each add (no contraction): the per-node sums are those of a per-node gather over its 27 cells.
+//--------------------------------------------------------------------------------------------------
+constexpr int kP2gSeg = 16;
+
+struct P2gAcc {
+    float mu, wu, mv, wv, mw, ww;
+};
+
+// weights (1-f) at base and f at base+1; d = base - (z-1) picks the window slots (0,1) or (1,2).
+// A zero ab (no contribution) adds exact zeros.
+__device__ __forceinline__ void p2g_window_add(int d, float ab, float f, float v, float& m0, float& w0, float& m1, float& w1,
+                                               float& m2, float& w2) {
+#pragma clang fp contract(off)
+    const float wl = ab * (1.0f - f), wh = ab * f;
+    const bool  lo = d == 0;
+    const float c0 = lo ? wl : 0.f, c1 = lo ? wh : wl, c2 = lo ? 0.f : wh;
+    m0 += c0 * v;
+    w0 += c0;
+    m1 += c1 * v;
+    w1 += c1;
+    m2 += c2 * v;
+    w2 += c2;
+}
+
+__device__ __forceinline__ void p2g_emit(const P2gAcc& a, int i, int j, int k, bool uCol, bool vCol, bool wCol, const GridGeom& g,
+                                         const FaceOut& out) {
+    if(uCol && k < g.nz) {
+        const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
+        const bool   ok = a.wu > kFltMin;
+        out.u[id]       = ok ? a.mu / a.wu : 0.f;
+        out.vu[id]      = ok ? 1 : 0;
+    }
+    if(vCol && k < g.nz) {
+        const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
+        const bool   ok = a.wv > kFltMin;
+        out.v[id]       = ok ? a.mv / a.wv : 0.f;
+        out.vv[id]      = ok ? 1 : 0;
+    }
+    if(wCol) {
+        const size_t id = lin3(i, j, k, g.nx, g.ny);
+        const bool   ok = a.ww > kFltMin;
+        out.w[id]       = ok ? a.mw / a.ww : 0.f;
+        out.vw[id]      = ok ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
+#pragma clang fp contract(off)
+    const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
+    const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
+    const uint32_t nseg = (uint32_t)(ez + kP2gSeg - 1) / kP2gSeg;
+    const uint32_t t    = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= cols * nseg) return;
+    const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
+    const int  k0 = seg * kP2gSeg, k1 = min(k0 + kP2gSeg, ez);  // nodes [k0, k1)
+    const bool uCol = j < g.ny, vCol = i < g.nx, wCol = i < g.nx && j < g.ny;
+    const int  x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1), xu = min(i, g.nx - 1);
+    const int  y0 = max(j - 1, 0), y1 = min(j + 1, g.ny - 1);
+    const int  zs = max(k0 - 1, 0), ze = min(k1, g.nz - 1);
+    P2gAcc     a0 = {}, a1 = {}, a2 = {};  // nodes z-1, z, z+1 while row z is read
+    for(int z = zs; z <= ze; ++z) {
+        for(int y = y0; y <= y1; ++y) {
+            const size_t   r    = lin3(0, y, z, g.nx, g.ny);
+            const uint32_t pb   = cell_start[r + x0], pe = cell_start[r + x1 + 1], pu = cell_start[r + xu + 1];
+            const bool     rowU = uCol, rowV = vCol && y <= j, rowW = wCol;
+            for(uint32_t p = pb; p < pe; ++p) {
+                const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+                float       axn = 0.f, axh = 0.f, byn = 0.f, byh = 0.f;
+                const bool  oxn = stencil_weight(gx, g.nx + 1, i, axn);
+                const bool  oxh = stencil_weight(gx - 0.5f, g.nx, i, axh);
+                const bool  oyn = stencil_weight(gy, g.ny + 1, j, byn);
+                const bool  oyh = stencil_weight(gy - 0.5f, g.ny, j, byh);
+                int         kh, kn;
+                float       fh, fn;
+                bary(gz - 0.5f, g.nz, kh, fh);
+                bary(gz, g.nz + 1, kn, fn);
+                const int  dh = kh - (z - 1), dn = kn - (z - 1);
+                const bool okU = rowU && p < pu && oxn && oyh;
+                const bool okV = rowV && oxh && oyn;
+                const bool okW = rowW && oxh && oyh;
+                const float vx = okU ? P.vx[p] : 0.f, vy = okV ? P.vy[p] : 0.f, vz = okW ? P.vz[p] : 0.f;
+                p2g_window_add(dh, okU ? axn * byh : 0.f, fh, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
+                p2g_window_add(dh, okV ? axh * byn : 0.f, fh, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
+                p2g_window_add(dn, okW ? axh * byh : 0.f, fn, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
+            }
+        }
+        if(z - 1 >= k0) p2g_emit(a0, i, j, z - 1, uCol, vCol, wCol, g, out);  // node z-1 is complete
+        a0 = a1;
+        a1 = a2;
+        a2 = P2gAcc{};
+    }
+    // nodes ze and ze+1 receive no further rows
+    if(ze >= k0 && ze < k1) p2g_emit(a0, i, j, ze, uCol, vCol, wCol, g, out);
+    if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
+}
+
 __global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
     const uint32_t t  = blockIdx.x * blockDim.x + threadIdx.x;
     const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;

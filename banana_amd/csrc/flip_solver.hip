@@ -496,10 +496,10 @@ public:
     void mapParticles2Grid() {
         requireBinned();
         FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
-        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (((size_t)g.nz + 1 + kP2gSeg - 1) / kP2gSeg);  // columns x segments
         ktBegin(BNN_KT_P2G);
-        hipLaunchKernelGGL(k_p2g, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(), g,
-                           out);
+        hipLaunchKernelGGL(k_p2g_col, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
+                           g, out);
         ktEnd(BNN_KT_P2G);
     }
 
