@@ -532,75 +532,6 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t*
     if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
 }
 
-__global__ void __launch_bounds__(kBlock) k_p2g(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
-    const uint32_t t  = blockIdx.x * blockDim.x + threadIdx.x;
-    const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
-    if(t >= (uint32_t)(ex * ey * ez)) return;
-    const int i = t % ex, j = (t / ex) % ey, k = t / (ex * ey);
-    const bool doU = j < g.ny && k < g.nz;
-    const bool doV = i < g.nx && k < g.nz;
-    const bool doW = i < g.nx && j < g.ny;
-    float mu = 0.f, wu = 0.f, mv = 0.f, wv = 0.f, mw = 0.f, ww = 0.f;
-    // Rows and cells that cannot hold a corner of a component's stencil are not tested for it (u: x
-    // cells i-1..i; v: rows y <= j; w: rows z <= k); the contributing particles and their order
-    // are unchanged.
-    const int x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1), xu = min(i, g.nx - 1);
-    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
-        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y) {
-            const size_t   r    = lin3(0, y, z, g.nx, g.ny);
-            const uint32_t pb   = cell_start[r + x0], pe = cell_start[r + x1 + 1], pu = cell_start[r + xu + 1];
-            const bool     rowV = doV && y <= j, rowW = doW && z <= k;
-            for(uint32_t p = pb; p < pe; ++p) {
-                const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
-                if(doU && p < pu) {
-                    float a, b, c;
-                    if(stencil_weight(gx, g.nx + 1, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) &&
-                       stencil_weight(gz - 0.5f, g.nz, k, c)) {
-                        const float wt = a * b * c;
-                        mu += wt * P.vx[p];
-                        wu += wt;
-                    }
-                }
-                if(rowV) {
-                    float a, b, c;
-                    if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy, g.ny + 1, j, b) &&
-                       stencil_weight(gz - 0.5f, g.nz, k, c)) {
-                        const float wt = a * b * c;
-                        mv += wt * P.vy[p];
-                        wv += wt;
-                    }
-                }
-                if(rowW) {
-                    float a, b, c;
-                    if(stencil_weight(gx - 0.5f, g.nx, i, a) && stencil_weight(gy - 0.5f, g.ny, j, b) &&
-                       stencil_weight(gz, g.nz + 1, k, c)) {
-                        const float wt = a * b * c;
-                        mw += wt * P.vz[p];
-                        ww += wt;
-                    }
-                }
-            }
-        }
-    if(doU) {
-        const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
-        const bool   ok = wu > kFltMin;
-        out.u[id]       = ok ? mu / wu : 0.f;
-        out.vu[id]      = ok ? 1 : 0;
-    }
-    if(doV) {
-        const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
-        const bool   ok = wv > kFltMin;
-        out.v[id]       = ok ? mv / wv : 0.f;
-        out.vv[id]      = ok ? 1 : 0;
-    }
-    if(doW) {
-        const size_t id = lin3(i, j, k, g.nx, g.ny);
-        const bool   ok = ww > kFltMin;
-        out.w[id]       = ok ? mw / ww : 0.f;
-        out.vw[id]      = ok ? 1 : 0;
-    }
-}
-
 //==================================================================================================
 // extrapolateVelocity: one Jacobi layer of one component (PIC_Solver.cpp:992-1090), in place.
 // An invalid interior face takes the mean of its valid neighbours (order +x,-x,+y,-y,+z,-z) and
@@ -1544,14 +1475,28 @@ __device__ __forceinline__ int cell_color(const LevelDev& L, int s, int l) {
     return ((bx * 8 + (l & 7)) + (by * 8 + ((l >> 3) & 7)) + (bz * 8 + (l >> 6))) & 1;
 }
 
+// xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c)
+__device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __restrict__ xpre, const LevelDev& C, float omega,
+                                              int32_t o, int i, int j, int k) {
+    float xv = xpre[o];
+    if(F.invdiag[o] != 0.f) {
+        const int32_t c = level_cell_index(C, i >> 1, j >> 1, k >> 1);
+        if(c >= 0) xv += omega * C.x[c];
+    }
+    return xv;
+}
+
 // Brick-hybrid Gauss-Seidel: one block per brick runs `sweeps` red-black sweeps (black-red with
 // `reverse`) on its 512 cells in LDS, with the x of neighbouring bricks held at their values on entry
 // (Jacobi between bricks). For the brick's own system A_BB the red-black and black-red sequences are
 // transposes of each other, so the pre- and post-smoothers are adjoint and the V-cycle stays
-// symmetric. The starting values come from xin (or are 0 with zeroInit: every V-cycle starts from
-// x = 0) and the result goes to L.x; xin is never L.x, so no block reads what another writes.
-__global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, const float* __restrict__ xin, int sweeps, int reverse,
-                                                         int zeroInit, const PcgState* __restrict__ st) {
+// symmetric. Pre-smoother (kProlong == 0): starts from x = 0 (every V-cycle does) and writes L.r.
+// Post-smoother (kProlong == 1): starts from the coarse-grid-corrected L.r + omega * P x_coarse,
+// formed on the fly for the brick and its halo, and writes L.x. Input and output are different
+// arrays, so no block reads what another writes.
+template<int kProlong>
+__global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, int reverse,
+                                                         const PcgState* __restrict__ st) {
     __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
     if(st && st->done) return;
     const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
@@ -1565,14 +1510,27 @@ __global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, const float
         const float   czm = izm >= 0 ? L.cz[izm] : 0.f, cym = iym >= 0 ? L.cy[iym] : 0.f, cxm = ixm >= 0 ? L.cx[ixm] : 0.f;
         const float   cxp = L.cx[o], cyp = L.cy[o], czp = L.cz[o];
         const float   bv = L.b[o], inv = L.invdiag[o];
-        xs[c]            = zeroInit ? 0.f : xin[o];
-        if(lx == 0) xs[c - 1] = (zeroInit || ixm < 0) ? 0.f : xin[ixm];
-        if(lx == 7) xs[c + 1] = (zeroInit || ixp < 0) ? 0.f : xin[ixp];
-        if(ly == 0) xs[c - 10] = (zeroInit || iym < 0) ? 0.f : xin[iym];
-        if(ly == 7) xs[c + 10] = (zeroInit || iyp < 0) ? 0.f : xin[iyp];
-        if(lz == 0) xs[c - 100] = (zeroInit || izm < 0) ? 0.f : xin[izm];
-        if(lz == 7) xs[c + 100] = (zeroInit || izp < 0) ? 0.f : xin[izp];
-        const int color = cell_color(L, s, l);
+        const int     b  = L.slotBrick[s];
+        const int     i = (b % L.nbx) * 8 + lx, j = ((b / L.nbx) % L.nby) * 8 + ly, k = (b / (L.nbx * L.nby)) * 8 + lz;
+        if(kProlong) {
+            const float* xp = L.r;
+            xs[c]           = mg_prolonged(L, xp, C, omega, (int32_t)o, i, j, k);
+            if(lx == 0) xs[c - 1] = ixm < 0 ? 0.f : mg_prolonged(L, xp, C, omega, ixm, i - 1, j, k);
+            if(lx == 7) xs[c + 1] = ixp < 0 ? 0.f : mg_prolonged(L, xp, C, omega, ixp, i + 1, j, k);
+            if(ly == 0) xs[c - 10] = iym < 0 ? 0.f : mg_prolonged(L, xp, C, omega, iym, i, j - 1, k);
+            if(ly == 7) xs[c + 10] = iyp < 0 ? 0.f : mg_prolonged(L, xp, C, omega, iyp, i, j + 1, k);
+            if(lz == 0) xs[c - 100] = izm < 0 ? 0.f : mg_prolonged(L, xp, C, omega, izm, i, j, k - 1);
+            if(lz == 7) xs[c + 100] = izp < 0 ? 0.f : mg_prolonged(L, xp, C, omega, izp, i, j, k + 1);
+        } else {
+            xs[c] = 0.f;
+            if(lx == 0) xs[c - 1] = 0.f;
+            if(lx == 7) xs[c + 1] = 0.f;
+            if(ly == 0) xs[c - 10] = 0.f;
+            if(ly == 7) xs[c + 10] = 0.f;
+            if(lz == 0) xs[c - 100] = 0.f;
+            if(lz == 7) xs[c + 100] = 0.f;
+        }
+        const int color = (i + j + k) & 1;
         __syncthreads();
         for(int it = 0; it < sweeps; ++it)
             for(int half = 0; half < 2; ++half) {
@@ -1588,54 +1546,39 @@ __global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, const float
                 }
                 __syncthreads();
             }
-        L.x[o] = xs[c];
+        (kProlong ? L.x : L.r)[o] = xs[c];
         __syncthreads();  // xs is reused by the block's next brick
     }
 }
 
-// r = b - A x
-__global__ void __launch_bounds__(512) k_mg_residual(LevelDev L, const PcgState* __restrict__ st) {
+// r = b - A x on level F with the pre-smoothed x (held in F.r), summed over each 2x2x2 group into
+// the coarse right-hand side (P^T r). A group's eight cells lie in one fine brick, so the block of a
+// fine brick writes the 64 coarse cells it covers; r itself is never stored. Coarse cells without an
+// active child brick keep the zeros they get when the hierarchy is built.
+__global__ void __launch_bounds__(512) k_mg_residual_restrict(LevelDev F, LevelDev C, const PcgState* __restrict__ st) {
+    __shared__ float rs[512];
     if(st && st->done) return;
-    for(uint32_t slot = blockIdx.x; slot < L.nSlots; slot += gridDim.x) {
-        const int    s = (int)slot, l = threadIdx.x;
-        const size_t o = (size_t)s * 512 + l;
-        L.r[o]         = L.b[o] - (L.diag[o] * L.x[o] + level_offdiag_sum(L, s, l, L.x));
-    }
-}
-
-// b_coarse(C) = sum of the fine residuals of C's 8 children (P^T r)
-__global__ void __launch_bounds__(512) k_mg_restrict(LevelDev F, LevelDev C, const PcgState* __restrict__ st) {
-    if(st && st->done) return;
-    for(uint32_t slot = blockIdx.x; slot < C.nSlots; slot += gridDim.x) {
-        const int s = (int)slot, l = threadIdx.x;
-        const int b = C.slotBrick[s];
-        const int bx = b % C.nbx, by = (b / C.nbx) % C.nby, bz = b / (C.nbx * C.nby);
-        const int I = bx * 8 + (l & 7), J = by * 8 + ((l >> 3) & 7), K = bz * 8 + (l >> 6);
-        float     sum = 0.f;
-        for(int c = 0; c < 8; ++c) {
-            const int32_t f = level_cell_index(F, 2 * I + (c & 1), 2 * J + ((c >> 1) & 1), 2 * K + ((c >> 2) & 1));
-            if(f >= 0) sum += F.r[f];
-        }
-        C.b[(size_t)s * 512 + l] = sum;
-    }
-}
-
-// r_fine = x_fine + omega * x_coarse(parent) (P x_c, scaled): the corrected x goes to F.r (free
-// after the restriction) as the post-smoother's starting values
-__global__ void __launch_bounds__(512) k_mg_prolong(LevelDev F, LevelDev C, float omega, const PcgState* __restrict__ st) {
-    if(st && st->done) return;
+    const int l = threadIdx.x;
     for(uint32_t slot = blockIdx.x; slot < F.nSlots; slot += gridDim.x) {
-        const int    s = (int)slot, l = threadIdx.x;
-        const int    b = F.slotBrick[s];
-        const int    bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
-        const int    i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+        const int    s = (int)slot;
         const size_t o = (size_t)s * 512 + l;
-        float        xv = F.x[o];
-        if(F.invdiag[o] != 0.f) {  // an unknown
-            const int32_t c = level_cell_index(C, i >> 1, j >> 1, k >> 1);
-            if(c >= 0) xv += omega * C.x[c];
+        rs[l]          = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
+        __syncthreads();
+        if(l < 64) {
+            const int b  = F.slotBrick[s];
+            const int bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+            const int qx = l & 3, qy = (l >> 2) & 3, qz = l >> 4;  // coarse cell inside the brick's 4x4x4
+            const int I = bx * 4 + qx, J = by * 4 + qy, K = bz * 4 + qz;
+            float     sum = 0.f;
+            for(int ch = 0; ch < 8; ++ch) {
+                const int fi = 2 * I + (ch & 1), fj = 2 * J + ((ch >> 1) & 1), fk = 2 * K + ((ch >> 2) & 1);
+                if(fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
+                sum += rs[((fk & 7) * 8 + (fj & 7)) * 8 + (fi & 7)];
+            }
+            const int32_t ci = level_cell_index(C, I, J, K);
+            if(ci >= 0) C.b[ci] = sum;
         }
-        F.r[o] = xv;
+        __syncthreads();  // rs is reused by the block's next brick
     }
 }
 

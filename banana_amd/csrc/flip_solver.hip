@@ -685,23 +685,23 @@ public:
             const LevelDev cur = levelDev(l);
             hipLaunchKernelGGL(k_mg_coarsen, dim3(L.nSlots), dim3(512), 0, stream, prev, cur, L.diag.as<float>(), L.cx.as<float>(),
                                L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>());
+            // the restriction writes only coarse cells under an active fine brick; the others stay 0
+            HIP_OK(hipMemsetAsync(L.b.p, 0, (size_t)L.nSlots * 512 * 4, stream));
             prevSlots = L.nSlots;
             ++mgLevels;
         }
     }
 
-    static unsigned gridFor(uint32_t slots) { return std::min<unsigned>(std::max<uint32_t>(slots, 1u), (unsigned)kMaxRedBlocks); }
-
-    // z = M^-1 r on the fine level (b = r, x = z); symmetric V-cycle with the brick-hybrid smoother
+    // z = M^-1 r on the fine level (b = r, x = z); symmetric V-cycle with the brick-hybrid smoother.
+    // One block per brick. On each level the pre-smoothed x lives in L.r and the final x in L.x.
     void vcycle(const PcgState* stp) {
         const int   nu    = prm.mg_sweeps;
         const float omega = prm.mg_omega;
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
-            const unsigned G = gridFor(L.nSlots);
-            hipLaunchKernelGGL(k_mg_smooth_brick, dim3(G), dim3(512), 0, stream, L, (const float*)nullptr, nu, 0, 1, stp);
-            hipLaunchKernelGGL(k_mg_residual, dim3(G), dim3(512), 0, stream, L, stp);
-            hipLaunchKernelGGL(k_mg_restrict, dim3(gridFor(C.nSlots)), dim3(512), 0, stream, L, C, stp);
+            const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
+            hipLaunchKernelGGL(k_mg_smooth_brick<0>, dim3(G), dim3(512), 0, stream, L, C, omega, nu, 0, stp);
+            hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(512), 0, stream, L, C, stp);
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
         if(Lc.nSlots == 1) {
@@ -711,9 +711,7 @@ public:
         }
         for(int l = mgLevels - 2; l >= 0; --l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
-            const unsigned G = gridFor(L.nSlots);
-            hipLaunchKernelGGL(k_mg_prolong, dim3(G), dim3(512), 0, stream, L, C, omega, stp);
-            hipLaunchKernelGGL(k_mg_smooth_brick, dim3(G), dim3(512), 0, stream, L, (const float*)L.r, nu, 1, 0, stp);
+            hipLaunchKernelGGL(k_mg_smooth_brick<1>, dim3(std::max<uint32_t>(L.nSlots, 1u)), dim3(512), 0, stream, L, C, omega, nu, 1, stp);
         }
     }
 
