@@ -1475,12 +1475,43 @@ __device__ __forceinline__ int cell_color(const LevelDev& L, int s, int l) {
     return ((bx * 8 + (l & 7)) + (by * 8 + ((l >> 3) & 7)) + (bz * 8 + (l >> 6))) & 1;
 }
 
+// Launch order -> brick slot, a bijection on [0, n) that gives each XCD (the blocks with equal
+// b % 8) one contiguous range of slots: slots are ordered by brick index, so a brick's neighbours,
+// whose faces the stencils read, mostly sit in the same XCD's L2.
+__device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
+    const uint32_t xcd = b % 8u, q = n / 8u, r = n % 8u;
+    return xcd * q + min(xcd, r) + b / 8u;
+}
+
+// The coarse bricks a fine brick's cells and halo can have as parents: its own parent (index 6) and
+// that parent's six neighbours (-x, +x, -y, +y, -z, +z), or -1.
+struct ParentSlots {
+    int32_t s[7];
+};
+__device__ __forceinline__ ParentSlots parent_slots(const LevelDev& C, int bx, int by, int bz) {
+    ParentSlots P;
+    const int32_t cs = C.brickSlot[((bz >> 1) * C.nby + (by >> 1)) * C.nbx + (bx >> 1)];
+    for(int d = 0; d < 6; ++d) P.s[d] = cs >= 0 ? C.nbr[6 * cs + d] : -1;
+    P.s[6] = cs;
+    return P;
+}
+// level_cell_index(C, i >> 1, j >> 1, k >> 1) for a fine cell (i, j, k) whose coarse brick is the
+// parent of the fine brick at (bx, by, bz) or one of that parent's face neighbours
+__device__ __forceinline__ int32_t parent_cell(const LevelDev& C, const ParentSlots& P, int bx, int by, int bz, int i, int j, int k) {
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    if(i < 0 || j < 0 || k < 0 || I >= C.nx || J >= C.ny || K >= C.nz) return -1;
+    const int dx = (I >> 3) - (bx >> 1), dy = (J >> 3) - (by >> 1), dz = (K >> 3) - (bz >> 1);
+    const int d  = dx < 0 ? 0 : dx > 0 ? 1 : dy < 0 ? 2 : dy > 0 ? 3 : dz < 0 ? 4 : dz > 0 ? 5 : 6;
+    const int32_t cs = P.s[d];
+    return cs < 0 ? -1 : cs * 512 + ((K & 7) * 8 + (J & 7)) * 8 + (I & 7);
+}
+
 // xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c)
-__device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __restrict__ xpre, const LevelDev& C, float omega,
-                                              int32_t o, int i, int j, int k) {
+__device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __restrict__ xpre, const LevelDev& C,
+                                              const ParentSlots& P, int bx, int by, int bz, float omega, int32_t o, int i, int j, int k) {
     float xv = xpre[o];
     if(F.invdiag[o] != 0.f) {
-        const int32_t c = level_cell_index(C, i >> 1, j >> 1, k >> 1);
+        const int32_t c = parent_cell(C, P, bx, by, bz, i, j, k);
         if(c >= 0) xv += omega * C.x[c];
     }
     return xv;
@@ -1501,8 +1532,8 @@ __global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, LevelDev C,
     if(st && st->done) return;
     const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
     const int c = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
-    for(uint32_t slot = blockIdx.x; slot < L.nSlots; slot += gridDim.x) {
-        const int     s   = (int)slot;
+    for(uint32_t v = blockIdx.x; v < L.nSlots; v += gridDim.x) {
+        const int     s   = (int)xcd_slot(v, L.nSlots);
         const size_t  o   = (size_t)s * 512 + l;
         const int32_t ixm = nbr_index(L.nbr, s, l, 0), ixp = nbr_index(L.nbr, s, l, 1);
         const int32_t iym = nbr_index(L.nbr, s, l, 2), iyp = nbr_index(L.nbr, s, l, 3);
@@ -1511,16 +1542,18 @@ __global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, LevelDev C,
         const float   cxp = L.cx[o], cyp = L.cy[o], czp = L.cz[o];
         const float   bv = L.b[o], inv = L.invdiag[o];
         const int     b  = L.slotBrick[s];
-        const int     i = (b % L.nbx) * 8 + lx, j = ((b / L.nbx) % L.nby) * 8 + ly, k = (b / (L.nbx * L.nby)) * 8 + lz;
+        const int     bx = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
+        const int     i = bx * 8 + lx, j = by * 8 + ly, k = bz * 8 + lz;
         if(kProlong) {
-            const float* xp = L.r;
-            xs[c]           = mg_prolonged(L, xp, C, omega, (int32_t)o, i, j, k);
-            if(lx == 0) xs[c - 1] = ixm < 0 ? 0.f : mg_prolonged(L, xp, C, omega, ixm, i - 1, j, k);
-            if(lx == 7) xs[c + 1] = ixp < 0 ? 0.f : mg_prolonged(L, xp, C, omega, ixp, i + 1, j, k);
-            if(ly == 0) xs[c - 10] = iym < 0 ? 0.f : mg_prolonged(L, xp, C, omega, iym, i, j - 1, k);
-            if(ly == 7) xs[c + 10] = iyp < 0 ? 0.f : mg_prolonged(L, xp, C, omega, iyp, i, j + 1, k);
-            if(lz == 0) xs[c - 100] = izm < 0 ? 0.f : mg_prolonged(L, xp, C, omega, izm, i, j, k - 1);
-            if(lz == 7) xs[c + 100] = izp < 0 ? 0.f : mg_prolonged(L, xp, C, omega, izp, i, j, k + 1);
+            const float*      xp = L.r;
+            const ParentSlots P  = parent_slots(C, bx, by, bz);
+            xs[c]                = mg_prolonged(L, xp, C, P, bx, by, bz, omega, (int32_t)o, i, j, k);
+            if(lx == 0) xs[c - 1] = ixm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixm, i - 1, j, k);
+            if(lx == 7) xs[c + 1] = ixp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixp, i + 1, j, k);
+            if(ly == 0) xs[c - 10] = iym < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iym, i, j - 1, k);
+            if(ly == 7) xs[c + 10] = iyp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iyp, i, j + 1, k);
+            if(lz == 0) xs[c - 100] = izm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izm, i, j, k - 1);
+            if(lz == 7) xs[c + 100] = izp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izp, i, j, k + 1);
         } else {
             xs[c] = 0.f;
             if(lx == 0) xs[c - 1] = 0.f;
@@ -1559,14 +1592,15 @@ __global__ void __launch_bounds__(512) k_mg_residual_restrict(LevelDev F, LevelD
     __shared__ float rs[512];
     if(st && st->done) return;
     const int l = threadIdx.x;
-    for(uint32_t slot = blockIdx.x; slot < F.nSlots; slot += gridDim.x) {
-        const int    s = (int)slot;
+    for(uint32_t v = blockIdx.x; v < F.nSlots; v += gridDim.x) {
+        const int    s = (int)xcd_slot(v, F.nSlots);
         const size_t o = (size_t)s * 512 + l;
-        rs[l]          = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
+        const int    b = F.slotBrick[s];
+        const int    bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+        const int32_t cs = C.brickSlot[((bz >> 1) * C.nby + (by >> 1)) * C.nbx + (bx >> 1)];  // the parent brick
+        rs[l]            = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
         __syncthreads();
         if(l < 64) {
-            const int b  = F.slotBrick[s];
-            const int bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
             const int qx = l & 3, qy = (l >> 2) & 3, qz = l >> 4;  // coarse cell inside the brick's 4x4x4
             const int I = bx * 4 + qx, J = by * 4 + qy, K = bz * 4 + qz;
             float     sum = 0.f;
@@ -1575,8 +1609,7 @@ __global__ void __launch_bounds__(512) k_mg_residual_restrict(LevelDev F, LevelD
                 if(fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
                 sum += rs[((fk & 7) * 8 + (fj & 7)) * 8 + (fi & 7)];
             }
-            const int32_t ci = level_cell_index(C, I, J, K);
-            if(ci >= 0) C.b[ci] = sum;
+            if(cs >= 0 && I < C.nx && J < C.ny && K < C.nz) C.b[(size_t)cs * 512 + ((K & 7) * 8 + (J & 7)) * 8 + (I & 7)] = sum;
         }
         __syncthreads();  // rs is reused by the block's next brick
     }

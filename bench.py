@@ -175,6 +175,9 @@ def main():
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
     ap.add_argument("--mg-sweeps", type=int, default=None, help="smoothing sweeps per multigrid level (library default when omitted)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="after the steps, copy this many bytes device-to-device once (a known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--cpu-n", type=int, default=128, help="grid edge of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3, help="substeps of the CPU baseline sample")
     ap.add_argument("--timing", type=int, default=2, help="0: none, 1: stage events, 2: + per-kernel events")
@@ -252,6 +255,15 @@ def main():
         sim.sort_particles()  # synchronous
         zsort_ms.append(1e3 * (time.perf_counter() - tz))
     log(f"z-sort of {np_} particles: {zsort_ms[-1]:.2f} ms")
+    if args.copy_calibration > 0:
+        import torch
+        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} bytes device-to-device")
+        del src, dst
 
     roofline = None
     if spmv_launches and spmv_ms > 0:
