@@ -165,6 +165,25 @@ def served_leg(args, log, batching):
     return out
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (its copy kernel reads and writes
+    exactly nbytes), to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # 3: device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,14 +275,8 @@ def main():
         zsort_ms.append(1e3 * (time.perf_counter() - tz))
     log(f"z-sort of {np_} particles: {zsort_ms[-1]:.2f} ms")
     if args.copy_calibration > 0:
-        import torch
-        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        copy_calibration(args.copy_calibration)
         log(f"copy calibration: {args.copy_calibration} bytes device-to-device")
-        del src, dst
 
     roofline = None
     if spmv_launches and spmv_ms > 0:
