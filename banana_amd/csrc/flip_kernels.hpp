@@ -1144,6 +1144,16 @@ __global__ void __launch_bounds__(1024) k_pcg_init_fin(PcgState* st, const doubl
 }
 
 // s_out = z + beta s_in ; q = A s_out ; partial s_out . q
+// the slots of this block when the grid shares n slots: one contiguous chunk per block, and the
+// chunks of one XCD's blocks (equal blockIdx % 8) adjacent, so that brick faces read from
+// neighbouring bricks are mostly in that XCD's L2
+__device__ __forceinline__ void block_chunk(uint32_t n, uint32_t& s0, uint32_t& s1) {
+    const uint32_t G = gridDim.x, b = blockIdx.x, xcd = b % 8u, q = G / 8u, r = G % 8u;
+    const uint32_t v = xcd * q + min(xcd, r) + b / 8u;
+    s0               = (uint32_t)(((uint64_t)n * v) / G);
+    s1               = (uint32_t)(((uint64_t)n * (v + 1)) / G);
+}
+
 __global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nbr, const float* __restrict__ z,
                                                   const float* __restrict__ sin, float* __restrict__ sout, float* __restrict__ q,
                                                   const float* __restrict__ diag, const float* __restrict__ cx,
@@ -1154,7 +1164,9 @@ __global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nb
     const float beta = (float)st->beta;
     const int   l    = threadIdx.x;
     double      d    = 0;
-    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
+    uint32_t    s0, s1;
+    block_chunk(nSlots, s0, s1);
+    for(uint32_t slot = s0; slot < s1; ++slot) {
         const int     s   = (int)slot;
         const size_t  o   = (size_t)s * 512 + l;
         auto          S   = [&](int32_t idx) { return z[idx] + beta * sin[idx]; };
@@ -1407,6 +1419,7 @@ struct LevelDev {
     const float *  diag, *cx, *cy, *cz, *invdiag;
     float *        x, *b, *r;
     uint32_t       nSlots;
+    const int32_t* ptab;  // per slot: parent coarse brick and its six neighbours (8 ints), or null
 };
 
 __device__ __forceinline__ int32_t level_cell_index(const LevelDev& L, int i, int j, int k) {
@@ -1484,15 +1497,24 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
 }
 
 // The coarse bricks a fine brick's cells and halo can have as parents: its own parent (index 6) and
-// that parent's six neighbours (-x, +x, -y, +y, -z, +z), or -1.
+// that parent's six neighbours (-x, +x, -y, +y, -z, +z), or -1. Stored per fine slot (8 ints, the
+// last unused) when the hierarchy is built.
 struct ParentSlots {
     int32_t s[7];
 };
-__device__ __forceinline__ ParentSlots parent_slots(const LevelDev& C, int bx, int by, int bz) {
-    ParentSlots P;
+__global__ void __launch_bounds__(kBlock) k_mg_parent_table(LevelDev F, LevelDev C, int32_t* __restrict__ tab) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if(s >= F.nSlots) return;
+    const int     b  = F.slotBrick[s];
+    const int     bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
     const int32_t cs = C.brickSlot[((bz >> 1) * C.nby + (by >> 1)) * C.nbx + (bx >> 1)];
-    for(int d = 0; d < 6; ++d) P.s[d] = cs >= 0 ? C.nbr[6 * cs + d] : -1;
-    P.s[6] = cs;
+    for(int d = 0; d < 6; ++d) tab[8 * s + d] = cs >= 0 ? C.nbr[6 * cs + d] : -1;
+    tab[8 * s + 6] = cs;
+    tab[8 * s + 7] = -1;
+}
+__device__ __forceinline__ ParentSlots parent_slots(const int32_t* __restrict__ tab, int s) {
+    ParentSlots P;
+    for(int d = 0; d < 7; ++d) P.s[d] = tab[8 * s + d];
     return P;
 }
 // level_cell_index(C, i >> 1, j >> 1, k >> 1) for a fine cell (i, j, k) whose coarse brick is the
@@ -1517,69 +1539,99 @@ __device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __
     return xv;
 }
 
-// Brick-hybrid Gauss-Seidel: one block per brick runs `sweeps` red-black sweeps (black-red with
-// `reverse`) on its 512 cells in LDS, with the x of neighbouring bricks held at their values on entry
-// (Jacobi between bricks). For the brick's own system A_BB the red-black and black-red sequences are
-// transposes of each other, so the pre- and post-smoothers are adjoint and the V-cycle stays
-// symmetric. Pre-smoother (kProlong == 0): starts from x = 0 (every V-cycle does) and writes L.r.
-// Post-smoother (kProlong == 1): starts from the coarse-grid-corrected L.r + omega * P x_coarse,
-// formed on the fly for the brick and its halo, and writes L.x. Input and output are different
-// arrays, so no block reads what another writes.
+// one cell of the brick smoother: its LDS index and its row of A (a coupling to a missing neighbour
+// is 0 and that neighbour's halo entry holds 0, so the six terms are added unconditionally)
+struct SmoothCell {
+    int   c;
+    float czm, cym, cxm, cxp, cyp, czp, bv, inv;
+};
+
 template<int kProlong>
-__global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, int reverse,
-                                                         const PcgState* __restrict__ st) {
+__device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, const LevelDev& C, const ParentSlots& P, float omega, int s,
+                                                        int l, int bx, int by, int bz, float* xs) {
+    const int     lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const int     c  = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
+    const size_t  o  = (size_t)s * 512 + l;
+    const int32_t ixm = nbr_index(L.nbr, s, l, 0), ixp = nbr_index(L.nbr, s, l, 1);
+    const int32_t iym = nbr_index(L.nbr, s, l, 2), iyp = nbr_index(L.nbr, s, l, 3);
+    const int32_t izm = nbr_index(L.nbr, s, l, 4), izp = nbr_index(L.nbr, s, l, 5);
+    SmoothCell    R;
+    R.c   = c;
+    R.czm = izm >= 0 ? L.cz[izm] : 0.f;
+    R.cym = iym >= 0 ? L.cy[iym] : 0.f;
+    R.cxm = ixm >= 0 ? L.cx[ixm] : 0.f;
+    R.cxp = ixp >= 0 ? L.cx[o] : 0.f;
+    R.cyp = iyp >= 0 ? L.cy[o] : 0.f;
+    R.czp = izp >= 0 ? L.cz[o] : 0.f;
+    R.bv  = L.b[o];
+    R.inv = L.invdiag[o];
+    if(kProlong) {
+        const float* xp = L.r;
+        const int    i = bx * 8 + lx, j = by * 8 + ly, k = bz * 8 + lz;
+        xs[c]          = mg_prolonged(L, xp, C, P, bx, by, bz, omega, (int32_t)o, i, j, k);
+        if(lx == 0) xs[c - 1] = ixm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixm, i - 1, j, k);
+        if(lx == 7) xs[c + 1] = ixp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixp, i + 1, j, k);
+        if(ly == 0) xs[c - 10] = iym < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iym, i, j - 1, k);
+        if(ly == 7) xs[c + 10] = iyp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iyp, i, j + 1, k);
+        if(lz == 0) xs[c - 100] = izm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izm, i, j, k - 1);
+        if(lz == 7) xs[c + 100] = izp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izp, i, j, k + 1);
+    } else {
+        xs[c] = 0.f;
+        if(lx == 0) xs[c - 1] = 0.f;
+        if(lx == 7) xs[c + 1] = 0.f;
+        if(ly == 0) xs[c - 10] = 0.f;
+        if(ly == 7) xs[c + 10] = 0.f;
+        if(lz == 0) xs[c - 100] = 0.f;
+        if(lz == 7) xs[c + 100] = 0.f;
+    }
+    return R;
+}
+
+__device__ __forceinline__ void smooth_cell_update(const SmoothCell& R, float* xs) {
+    float acc = 0.f;
+    acc += R.czm * xs[R.c - 100];
+    acc += R.cym * xs[R.c - 10];
+    acc += R.cxm * xs[R.c - 1];
+    acc += R.cxp * xs[R.c + 1];
+    acc += R.cyp * xs[R.c + 10];
+    acc += R.czp * xs[R.c + 100];
+    xs[R.c] = (R.bv - acc) * R.inv;
+}
+
+// Brick-hybrid Gauss-Seidel: one block per brick runs `sweeps` red-black sweeps (pre-smoother) or
+// black-red sweeps (post-smoother) on its 512 cells in LDS, with the x of neighbouring bricks held at
+// their values on entry (Jacobi between bricks). For the brick's own system A_BB the red-black and
+// black-red sequences are transposes of each other, so the pre- and post-smoothers are adjoint and
+// the V-cycle stays symmetric. Pre-smoother (kProlong == 0): starts from x = 0 (every V-cycle does)
+// and writes L.r. Post-smoother (kProlong == 1): starts from the coarse-grid-corrected
+// L.r + omega * P x_coarse, formed on the fly for the brick and its halo, and writes L.x. Input and
+// output are different arrays, so no block reads what another writes. Each of the 256 threads owns
+// an x-adjacent pair of cells, one of each colour, so every thread updates one cell per half-sweep.
+template<int kProlong>
+__global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, const PcgState* __restrict__ st) {
     __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
     if(st && st->done) return;
-    const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-    const int c = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
+    const int t = threadIdx.x, ly = (t >> 2) & 7, lz = t >> 5;
+    const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
+    const int lr = ((ly + lz) & 1) ? l0 + 1 : l0, lb = ((ly + lz) & 1) ? l0 : l0 + 1;  // red (colour 0), black
     for(uint32_t v = blockIdx.x; v < L.nSlots; v += gridDim.x) {
-        const int     s   = (int)xcd_slot(v, L.nSlots);
-        const size_t  o   = (size_t)s * 512 + l;
-        const int32_t ixm = nbr_index(L.nbr, s, l, 0), ixp = nbr_index(L.nbr, s, l, 1);
-        const int32_t iym = nbr_index(L.nbr, s, l, 2), iyp = nbr_index(L.nbr, s, l, 3);
-        const int32_t izm = nbr_index(L.nbr, s, l, 4), izp = nbr_index(L.nbr, s, l, 5);
-        const float   czm = izm >= 0 ? L.cz[izm] : 0.f, cym = iym >= 0 ? L.cy[iym] : 0.f, cxm = ixm >= 0 ? L.cx[ixm] : 0.f;
-        const float   cxp = L.cx[o], cyp = L.cy[o], czp = L.cz[o];
-        const float   bv = L.b[o], inv = L.invdiag[o];
-        const int     b  = L.slotBrick[s];
-        const int     bx = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
-        const int     i = bx * 8 + lx, j = by * 8 + ly, k = bz * 8 + lz;
-        if(kProlong) {
-            const float*      xp = L.r;
-            const ParentSlots P  = parent_slots(C, bx, by, bz);
-            xs[c]                = mg_prolonged(L, xp, C, P, bx, by, bz, omega, (int32_t)o, i, j, k);
-            if(lx == 0) xs[c - 1] = ixm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixm, i - 1, j, k);
-            if(lx == 7) xs[c + 1] = ixp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixp, i + 1, j, k);
-            if(ly == 0) xs[c - 10] = iym < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iym, i, j - 1, k);
-            if(ly == 7) xs[c + 10] = iyp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iyp, i, j + 1, k);
-            if(lz == 0) xs[c - 100] = izm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izm, i, j, k - 1);
-            if(lz == 7) xs[c + 100] = izp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izp, i, j, k + 1);
-        } else {
-            xs[c] = 0.f;
-            if(lx == 0) xs[c - 1] = 0.f;
-            if(lx == 7) xs[c + 1] = 0.f;
-            if(ly == 0) xs[c - 10] = 0.f;
-            if(ly == 7) xs[c + 10] = 0.f;
-            if(lz == 0) xs[c - 100] = 0.f;
-            if(lz == 7) xs[c + 100] = 0.f;
-        }
-        const int color = (i + j + k) & 1;
+        const int   s  = (int)xcd_slot(v, L.nSlots);
+        const int   b  = L.slotBrick[s];
+        const int   bx = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
+        ParentSlots P;
+        if(kProlong) P = parent_slots(L.ptab, s);
+        const SmoothCell R = smooth_cell_setup<kProlong>(L, C, P, omega, s, lr, bx, by, bz, xs);
+        const SmoothCell K = smooth_cell_setup<kProlong>(L, C, P, omega, s, lb, bx, by, bz, xs);
         __syncthreads();
-        for(int it = 0; it < sweeps; ++it)
-            for(int half = 0; half < 2; ++half) {
-                if(color == (reverse ? 1 - half : half)) {
-                    float acc = 0.f;
-                    if(izm >= 0) acc += czm * xs[c - 100];
-                    if(iym >= 0) acc += cym * xs[c - 10];
-                    if(ixm >= 0) acc += cxm * xs[c - 1];
-                    if(ixp >= 0) acc += cxp * xs[c + 1];
-                    if(iyp >= 0) acc += cyp * xs[c + 10];
-                    if(izp >= 0) acc += czp * xs[c + 100];
-                    xs[c] = (bv - acc) * inv;
-                }
-                __syncthreads();
-            }
-        (kProlong ? L.x : L.r)[o] = xs[c];
+        for(int it = 0; it < sweeps; ++it) {
+            smooth_cell_update(kProlong ? K : R, xs);
+            __syncthreads();
+            smooth_cell_update(kProlong ? R : K, xs);
+            __syncthreads();
+        }
+        float* out                   = kProlong ? L.x : L.r;
+        out[(size_t)s * 512 + lr]    = xs[R.c];
+        out[(size_t)s * 512 + lb]    = xs[K.c];
         __syncthreads();  // xs is reused by the block's next brick
     }
 }
@@ -1587,21 +1639,24 @@ __global__ void __launch_bounds__(512) k_mg_smooth_brick(LevelDev L, LevelDev C,
 // r = b - A x on level F with the pre-smoothed x (held in F.r), summed over each 2x2x2 group into
 // the coarse right-hand side (P^T r). A group's eight cells lie in one fine brick, so the block of a
 // fine brick writes the 64 coarse cells it covers; r itself is never stored. Coarse cells without an
-// active child brick keep the zeros they get when the hierarchy is built.
-__global__ void __launch_bounds__(512) k_mg_residual_restrict(LevelDev F, LevelDev C, const PcgState* __restrict__ st) {
+// active child brick keep the zeros they get when the hierarchy is built. 256 threads, two cells each.
+__global__ void __launch_bounds__(256) k_mg_residual_restrict(LevelDev F, LevelDev C, const PcgState* __restrict__ st) {
     __shared__ float rs[512];
     if(st && st->done) return;
-    const int l = threadIdx.x;
+    const int t = threadIdx.x;
     for(uint32_t v = blockIdx.x; v < F.nSlots; v += gridDim.x) {
-        const int    s = (int)xcd_slot(v, F.nSlots);
-        const size_t o = (size_t)s * 512 + l;
-        const int    b = F.slotBrick[s];
-        const int    bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
-        const int32_t cs = C.brickSlot[((bz >> 1) * C.nby + (by >> 1)) * C.nbx + (bx >> 1)];  // the parent brick
-        rs[l]            = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
+        const int s = (int)xcd_slot(v, F.nSlots);
+        const int b = F.slotBrick[s];
+        const int bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+        for(int h = 0; h < 2; ++h) {
+            const int    l = t + 256 * h;
+            const size_t o = (size_t)s * 512 + l;
+            rs[l]          = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
+        }
         __syncthreads();
-        if(l < 64) {
-            const int qx = l & 3, qy = (l >> 2) & 3, qz = l >> 4;  // coarse cell inside the brick's 4x4x4
+        if(t < 64) {
+            const int32_t cs = F.ptab[8 * s + 6];  // the parent brick
+            const int qx = t & 3, qy = (t >> 2) & 3, qz = t >> 4;  // coarse cell inside the brick's 4x4x4
             const int I = bx * 4 + qx, J = by * 4 + qy, K = bz * 4 + qz;
             float     sum = 0.f;
             for(int ch = 0; ch < 8; ++ch) {

@@ -620,8 +620,9 @@ public:
     struct MgLevel {
         int      nx = 0, ny = 0, nz = 0, nbx = 0, nby = 0, nbz = 0;
         uint32_t nb = 0, nSlots = 0, slotCap = 0;
-        DevBuf   flag, scan, brickSlot, slotBrick, nbr, diag, cx, cy, cz, inv, x, b, r;
+        DevBuf   flag, scan, brickSlot, slotBrick, nbr, diag, cx, cy, cz, inv, x, b, r, ptab;
     };
+    DevBuf                                ptab0;  // parent table of the fine level
     std::vector<std::unique_ptr<MgLevel>> mg;  // mg[0] unused (the fine level lives in the PCG buffers)
     int                                   mgLevels = 0;
     static constexpr uint32_t             kCoarsestSlots = 1;  // coarsen until one brick remains
@@ -629,14 +630,14 @@ public:
     LevelDev fineLevel() {
         return LevelDev{g.nx, g.ny, g.nz, g.nbx, g.nby, g.nbz, brickSlot.as<int32_t>(), slotBrick.as<int32_t>(), nbrTab.as<int32_t>(),
                         diag32.as<float>(), cx32.as<float>(), cy32.as<float>(), cz32.as<float>(), invd.as<float>(),
-                        z.as<float>(), r.as<float>(), q.as<float>(), nSlots};
+                        z.as<float>(), r.as<float>(), q.as<float>(), nSlots, ptab0.as<int32_t>()};
     }
     LevelDev levelDev(int l) {
         if(l == 0) return fineLevel();
         MgLevel& L = *mg[l];
         return LevelDev{L.nx, L.ny, L.nz, L.nbx, L.nby, L.nbz, L.brickSlot.as<int32_t>(), L.slotBrick.as<int32_t>(), L.nbr.as<int32_t>(),
                         L.diag.as<float>(), L.cx.as<float>(), L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>(),
-                        L.x.as<float>(), L.b.as<float>(), L.r.as<float>(), L.nSlots};
+                        L.x.as<float>(), L.b.as<float>(), L.r.as<float>(), L.nSlots, L.ptab.as<int32_t>()};
     }
 
     void buildHierarchy() {
@@ -687,6 +688,10 @@ public:
                                L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>());
             // the restriction writes only coarse cells under an active fine brick; the others stay 0
             HIP_OK(hipMemsetAsync(L.b.p, 0, (size_t)L.nSlots * 512 * 4, stream));
+            // the finer level's parent table (its bricks' parents in this level)
+            DevBuf& pt = l == 1 ? ptab0 : mg[l - 1]->ptab;
+            pt.reserve((size_t)std::max<uint32_t>(prev.nSlots, 1u) * 32);
+            hipLaunchKernelGGL(k_mg_parent_table, dim3(blocks(prev.nSlots)), dim3(kBlock), 0, stream, prev, levelDev(l), pt.as<int32_t>());
             prevSlots = L.nSlots;
             ++mgLevels;
         }
@@ -700,8 +705,8 @@ public:
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
-            hipLaunchKernelGGL(k_mg_smooth_brick<0>, dim3(G), dim3(512), 0, stream, L, C, omega, nu, 0, stp);
-            hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(512), 0, stream, L, C, stp);
+            hipLaunchKernelGGL(k_mg_smooth_brick<0>, dim3(G), dim3(256), 0, stream, L, C, omega, nu, stp);
+            hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(256), 0, stream, L, C, stp);
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
         if(Lc.nSlots == 1) {
@@ -711,7 +716,7 @@ public:
         }
         for(int l = mgLevels - 2; l >= 0; --l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
-            hipLaunchKernelGGL(k_mg_smooth_brick<1>, dim3(std::max<uint32_t>(L.nSlots, 1u)), dim3(512), 0, stream, L, C, omega, nu, 1, stp);
+            hipLaunchKernelGGL(k_mg_smooth_brick<1>, dim3(std::max<uint32_t>(L.nSlots, 1u)), dim3(256), 0, stream, L, C, omega, nu, stp);
         }
     }
 
