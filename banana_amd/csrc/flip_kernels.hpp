@@ -1174,14 +1174,21 @@ __global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nb
         const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
         const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
         const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
+        // a missing neighbour reads the cell itself with a zero coupling (adds exactly nothing), so
+        // all loads are unconditional
+        const int32_t oi = (int32_t)o;
+        const int32_t jxm = ixm >= 0 ? ixm : oi, jxp = ixp >= 0 ? ixp : oi, jym = iym >= 0 ? iym : oi;
+        const int32_t jyp = iyp >= 0 ? iyp : oi, jzm = izm >= 0 ? izm : oi, jzp = izp >= 0 ? izp : oi;
+        const float   czm = cz[jzm], cym = cy[jym], cxm = cx[jxm], cxp = cx[o], cyp = cy[o], czp = cz[o];
+        const float   szm = S(jzm), sym = S(jym), sxm = S(jxm), sxp = S(jxp), syp = S(jyp), szp = S(jzp);
         float         acc = 0.f;
-        if(izm >= 0) acc += cz[izm] * S(izm);
-        if(iym >= 0) acc += cy[iym] * S(iym);
-        if(ixm >= 0) acc += cx[ixm] * S(ixm);
+        acc += (izm >= 0 ? czm : 0.f) * szm;
+        acc += (iym >= 0 ? cym : 0.f) * sym;
+        acc += (ixm >= 0 ? cxm : 0.f) * sxm;
         acc += diag[o] * sc;
-        if(ixp >= 0) acc += cx[o] * S(ixp);
-        if(iyp >= 0) acc += cy[o] * S(iyp);
-        if(izp >= 0) acc += cz[o] * S(izp);
+        acc += (ixp >= 0 ? cxp : 0.f) * sxp;
+        acc += (iyp >= 0 ? cyp : 0.f) * syp;
+        acc += (izp >= 0 ? czp : 0.f) * szp;
         sout[o] = sc;
         q[o]    = acc;
         d += (double)sc * (double)acc;
@@ -1467,18 +1474,24 @@ __global__ void __launch_bounds__(512) k_mg_coarsen(LevelDev F, LevelDev C, floa
     cinv[o]        = d > 0.f ? 1.0f / d : 0.f;
 }
 
+// sum of the six couplings times x. A missing neighbour reads the cell's own x (finite) with a zero
+// coupling, which adds exactly nothing, so every load is issued before the first use.
 __device__ __forceinline__ float level_offdiag_sum(const LevelDev& L, int s, int l, const float* __restrict__ x) {
     const int32_t ixm = nbr_index(L.nbr, s, l, 0), ixp = nbr_index(L.nbr, s, l, 1);
     const int32_t iym = nbr_index(L.nbr, s, l, 2), iyp = nbr_index(L.nbr, s, l, 3);
     const int32_t izm = nbr_index(L.nbr, s, l, 4), izp = nbr_index(L.nbr, s, l, 5);
-    const size_t  o   = (size_t)s * 512 + l;
+    const int32_t o   = s * 512 + l;
+    const int32_t jxm = ixm >= 0 ? ixm : o, jxp = ixp >= 0 ? ixp : o, jym = iym >= 0 ? iym : o;
+    const int32_t jyp = iyp >= 0 ? iyp : o, jzm = izm >= 0 ? izm : o, jzp = izp >= 0 ? izp : o;
+    const float   czm = L.cz[jzm], cym = L.cy[jym], cxm = L.cx[jxm], cxp = L.cx[o], cyp = L.cy[o], czp = L.cz[o];
+    const float   xzm = x[jzm], xym = x[jym], xxm = x[jxm], xxp = x[jxp], xyp = x[jyp], xzp = x[jzp];
     float         acc = 0.f;
-    if(izm >= 0) acc += L.cz[izm] * x[izm];
-    if(iym >= 0) acc += L.cy[iym] * x[iym];
-    if(ixm >= 0) acc += L.cx[ixm] * x[ixm];
-    if(ixp >= 0) acc += L.cx[o] * x[ixp];
-    if(iyp >= 0) acc += L.cy[o] * x[iyp];
-    if(izp >= 0) acc += L.cz[o] * x[izp];
+    acc += (izm >= 0 ? czm : 0.f) * xzm;
+    acc += (iym >= 0 ? cym : 0.f) * xym;
+    acc += (ixm >= 0 ? cxm : 0.f) * xxm;
+    acc += (ixp >= 0 ? cxp : 0.f) * xxp;
+    acc += (iyp >= 0 ? cyp : 0.f) * xyp;
+    acc += (izp >= 0 ? czp : 0.f) * xzp;
     return acc;
 }
 
@@ -1523,20 +1536,19 @@ __device__ __forceinline__ int32_t parent_cell(const LevelDev& C, const ParentSl
     const int I = i >> 1, J = j >> 1, K = k >> 1;
     if(i < 0 || j < 0 || k < 0 || I >= C.nx || J >= C.ny || K >= C.nz) return -1;
     const int dx = (I >> 3) - (bx >> 1), dy = (J >> 3) - (by >> 1), dz = (K >> 3) - (bz >> 1);
-    const int d  = dx < 0 ? 0 : dx > 0 ? 1 : dy < 0 ? 2 : dy > 0 ? 3 : dz < 0 ? 4 : dz > 0 ? 5 : 6;
-    const int32_t cs = P.s[d];
+    const int32_t cs = dx < 0 ? P.s[0] : dx > 0 ? P.s[1] : dy < 0 ? P.s[2] : dy > 0 ? P.s[3] : dz < 0 ? P.s[4] : dz > 0 ? P.s[5] : P.s[6];
     return cs < 0 ? -1 : cs * 512 + ((K & 7) * 8 + (J & 7)) * 8 + (I & 7);
 }
 
-// xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c)
+// xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c); the three loads
+// are unconditional (coarse cell 0 stands in when there is no parent)
 __device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __restrict__ xpre, const LevelDev& C,
                                               const ParentSlots& P, int bx, int by, int bz, float omega, int32_t o, int i, int j, int k) {
-    float xv = xpre[o];
-    if(F.invdiag[o] != 0.f) {
-        const int32_t c = parent_cell(C, P, bx, by, bz, i, j, k);
-        if(c >= 0) xv += omega * C.x[c];
-    }
-    return xv;
+    const int32_t c  = parent_cell(C, P, bx, by, bz, i, j, k);
+    const float   xc = C.x[c >= 0 ? c : 0];
+    const float   xv = xpre[o];
+    const bool    un = F.invdiag[o] != 0.f && c >= 0;
+    return un ? xv + omega * xc : xv;
 }
 
 // one cell of the brick smoother: its LDS index and its row of A (a coupling to a missing neighbour
@@ -1557,34 +1569,50 @@ __device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, const
     const int32_t izm = nbr_index(L.nbr, s, l, 4), izp = nbr_index(L.nbr, s, l, 5);
     SmoothCell    R;
     R.c   = c;
-    R.czm = izm >= 0 ? L.cz[izm] : 0.f;
-    R.cym = iym >= 0 ? L.cy[iym] : 0.f;
-    R.cxm = ixm >= 0 ? L.cx[ixm] : 0.f;
-    R.cxp = ixp >= 0 ? L.cx[o] : 0.f;
-    R.cyp = iyp >= 0 ? L.cy[o] : 0.f;
-    R.czp = izp >= 0 ? L.cz[o] : 0.f;
+    const float   czm = L.cz[izm >= 0 ? izm : (int32_t)o], cym = L.cy[iym >= 0 ? iym : (int32_t)o];
+    const float   cxm = L.cx[ixm >= 0 ? ixm : (int32_t)o], cxp = L.cx[o], cyp = L.cy[o], czp = L.cz[o];
+    R.czm = izm >= 0 ? czm : 0.f;
+    R.cym = iym >= 0 ? cym : 0.f;
+    R.cxm = ixm >= 0 ? cxm : 0.f;
+    R.cxp = ixp >= 0 ? cxp : 0.f;
+    R.cyp = iyp >= 0 ? cyp : 0.f;
+    R.czp = izp >= 0 ? czp : 0.f;
     R.bv  = L.b[o];
     R.inv = L.invdiag[o];
-    if(kProlong) {
-        const float* xp = L.r;
-        const int    i = bx * 8 + lx, j = by * 8 + ly, k = bz * 8 + lz;
-        xs[c]          = mg_prolonged(L, xp, C, P, bx, by, bz, omega, (int32_t)o, i, j, k);
-        if(lx == 0) xs[c - 1] = ixm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixm, i - 1, j, k);
-        if(lx == 7) xs[c + 1] = ixp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, ixp, i + 1, j, k);
-        if(ly == 0) xs[c - 10] = iym < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iym, i, j - 1, k);
-        if(ly == 7) xs[c + 10] = iyp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, iyp, i, j + 1, k);
-        if(lz == 0) xs[c - 100] = izm < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izm, i, j, k - 1);
-        if(lz == 7) xs[c + 100] = izp < 0 ? 0.f : mg_prolonged(L, xp, C, P, bx, by, bz, omega, izp, i, j, k + 1);
-    } else {
-        xs[c] = 0.f;
-        if(lx == 0) xs[c - 1] = 0.f;
-        if(lx == 7) xs[c + 1] = 0.f;
-        if(ly == 0) xs[c - 10] = 0.f;
-        if(ly == 7) xs[c + 10] = 0.f;
-        if(lz == 0) xs[c - 100] = 0.f;
-        if(lz == 7) xs[c + 100] = 0.f;
-    }
+    xs[c] = kProlong ? mg_prolonged(L, L.r, C, P, bx, by, bz, omega, (int32_t)o, bx * 8 + lx, by * 8 + ly, bz * 8 + lz) : 0.f;
     return R;
+}
+
+// the brick's one-cell halo (6 faces x 64 cells) in LDS: the neighbouring bricks' starting values
+// (0 for the pre-smoother and for missing neighbours). A wave's 64 entries share one face.
+template<int kProlong>
+__device__ __forceinline__ void smooth_halo(const LevelDev& L, const LevelDev& C, const ParentSlots& P, float omega, int s, int bx,
+                                            int by, int bz, float* xs) {
+    for(int h = threadIdx.x; h < 384; h += blockDim.x) {
+        const int d = h >> 6, p = h & 63, a = p & 7, e = p >> 3;
+        int       lx, ly, lz;  // the brick cell next to the halo entry
+        if(d < 2) {
+            lx = d == 0 ? 0 : 7;
+            ly = a;
+            lz = e;
+        } else if(d < 4) {
+            lx = a;
+            ly = d == 2 ? 0 : 7;
+            lz = e;
+        } else {
+            lx = a;
+            ly = e;
+            lz = d == 4 ? 0 : 7;
+        }
+        const int sx = d == 0 ? -1 : d == 1 ? 1 : 0, sy = d == 2 ? -1 : d == 3 ? 1 : 0, sz = d == 4 ? -1 : d == 5 ? 1 : 0;
+        const int c  = ((lz + 1 + sz) * 10 + (ly + 1 + sy)) * 10 + (lx + 1 + sx);
+        float     v  = 0.f;
+        if(kProlong) {
+            const int32_t n = nbr_index(L.nbr, s, (lz * 8 + ly) * 8 + lx, d);
+            if(n >= 0) v = mg_prolonged(L, L.r, C, P, bx, by, bz, omega, n, bx * 8 + lx + sx, by * 8 + ly + sy, bz * 8 + lz + sz);
+        }
+        xs[c] = v;
+    }
 }
 
 __device__ __forceinline__ void smooth_cell_update(const SmoothCell& R, float* xs) {
@@ -1622,6 +1650,7 @@ __global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C,
         if(kProlong) P = parent_slots(L.ptab, s);
         const SmoothCell R = smooth_cell_setup<kProlong>(L, C, P, omega, s, lr, bx, by, bz, xs);
         const SmoothCell K = smooth_cell_setup<kProlong>(L, C, P, omega, s, lb, bx, by, bz, xs);
+        smooth_halo<kProlong>(L, C, P, omega, s, bx, by, bz, xs);
         __syncthreads();
         for(int it = 0; it < sweeps; ++it) {
             smooth_cell_update(kProlong ? K : R, xs);
