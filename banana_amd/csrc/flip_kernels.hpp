@@ -1512,9 +1512,6 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
 // The coarse bricks a fine brick's cells and halo can have as parents: its own parent (index 6) and
 // that parent's six neighbours (-x, +x, -y, +y, -z, +z), or -1. Stored per fine slot (8 ints, the
 // last unused) when the hierarchy is built.
-struct ParentSlots {
-    int32_t s[7];
-};
 __global__ void __launch_bounds__(kBlock) k_mg_parent_table(LevelDev F, LevelDev C, int32_t* __restrict__ tab) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if(s >= F.nSlots) return;
@@ -1525,29 +1522,17 @@ __global__ void __launch_bounds__(kBlock) k_mg_parent_table(LevelDev F, LevelDev
     tab[8 * s + 6] = cs;
     tab[8 * s + 7] = -1;
 }
-__device__ __forceinline__ ParentSlots parent_slots(const int32_t* __restrict__ tab, int s) {
-    ParentSlots P;
-    for(int d = 0; d < 7; ++d) P.s[d] = tab[8 * s + d];
-    return P;
-}
-// level_cell_index(C, i >> 1, j >> 1, k >> 1) for a fine cell (i, j, k) whose coarse brick is the
-// parent of the fine brick at (bx, by, bz) or one of that parent's face neighbours
-__device__ __forceinline__ int32_t parent_cell(const LevelDev& C, const ParentSlots& P, int bx, int by, int bz, int i, int j, int k) {
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
-    if(i < 0 || j < 0 || k < 0 || I >= C.nx || J >= C.ny || K >= C.nz) return -1;
-    const int dx = (I >> 3) - (bx >> 1), dy = (J >> 3) - (by >> 1), dz = (K >> 3) - (bz >> 1);
-    const int32_t cs = dx < 0 ? P.s[0] : dx > 0 ? P.s[1] : dy < 0 ? P.s[2] : dy > 0 ? P.s[3] : dz < 0 ? P.s[4] : dz > 0 ? P.s[5] : P.s[6];
-    return cs < 0 ? -1 : cs * 512 + ((K & 7) * 8 + (J & 7)) * 8 + (I & 7);
-}
 
-// xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c); the three loads
-// are unconditional (coarse cell 0 stands in when there is no parent)
-__device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __restrict__ xpre, const LevelDev& C,
-                                              const ParentSlots& P, int bx, int by, int bz, float omega, int32_t o, int i, int j, int k) {
-    const int32_t c  = parent_cell(C, P, bx, by, bz, i, j, k);
-    const float   xc = C.x[c >= 0 ? c : 0];
+// xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c). cs is the coarse
+// brick holding the parent of fine cell (i, j, k) (-1: none). An unknown's parent always lies inside
+// the coarse grid; for other cells the coarse value is loaded but not used. The loads are
+// unconditional (coarse cell 0 stands in when there is no parent brick).
+__device__ __forceinline__ float mg_prolonged(const LevelDev& F, const float* __restrict__ xpre, const LevelDev& C, int32_t cs,
+                                              float omega, int32_t o, int i, int j, int k) {
+    const int32_t c  = cs < 0 ? 0 : cs * 512 + (((k >> 1) & 7) * 8 + ((j >> 1) & 7)) * 8 + ((i >> 1) & 7);
+    const float   xc = C.x[c];
     const float   xv = xpre[o];
-    const bool    un = F.invdiag[o] != 0.f && c >= 0;
+    const bool    un = F.invdiag[o] != 0.f && cs >= 0;
     return un ? xv + omega * xc : xv;
 }
 
@@ -1559,8 +1544,8 @@ struct SmoothCell {
 };
 
 template<int kProlong>
-__device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, const LevelDev& C, const ParentSlots& P, float omega, int s,
-                                                        int l, int bx, int by, int bz, float* xs) {
+__device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, const LevelDev& C, int32_t parent, float omega, int s, int l,
+                                                        int bx, int by, int bz, float* xs) {
     const int     lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
     const int     c  = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
     const size_t  o  = (size_t)s * 512 + l;
@@ -1579,15 +1564,15 @@ __device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, const
     R.czp = izp >= 0 ? czp : 0.f;
     R.bv  = L.b[o];
     R.inv = L.invdiag[o];
-    xs[c] = kProlong ? mg_prolonged(L, L.r, C, P, bx, by, bz, omega, (int32_t)o, bx * 8 + lx, by * 8 + ly, bz * 8 + lz) : 0.f;
+    xs[c] = kProlong ? mg_prolonged(L, L.r, C, parent, omega, (int32_t)o, bx * 8 + lx, by * 8 + ly, bz * 8 + lz) : 0.f;
     return R;
 }
 
 // the brick's one-cell halo (6 faces x 64 cells) in LDS: the neighbouring bricks' starting values
 // (0 for the pre-smoother and for missing neighbours). A wave's 64 entries share one face.
 template<int kProlong>
-__device__ __forceinline__ void smooth_halo(const LevelDev& L, const LevelDev& C, const ParentSlots& P, float omega, int s, int bx,
-                                            int by, int bz, float* xs) {
+__device__ __forceinline__ void smooth_halo(const LevelDev& L, const LevelDev& C, int32_t parent, float omega, int s, int bx, int by,
+                                            int bz, float* xs) {
     for(int h = threadIdx.x; h < 384; h += blockDim.x) {
         const int d = h >> 6, p = h & 63, a = p & 7, e = p >> 3;
         int       lx, ly, lz;  // the brick cell next to the halo entry
@@ -1609,7 +1594,14 @@ __device__ __forceinline__ void smooth_halo(const LevelDev& L, const LevelDev& C
         float     v  = 0.f;
         if(kProlong) {
             const int32_t n = nbr_index(L.nbr, s, (lz * 8 + ly) * 8 + lx, d);
-            if(n >= 0) v = mg_prolonged(L, L.r, C, P, bx, by, bz, omega, n, bx * 8 + lx + sx, by * 8 + ly + sy, bz * 8 + lz + sz);
+            if(n >= 0) {
+                // the neighbour brick's parent: this brick's parent, or its neighbour across the face
+                // when this brick is on the parent's face (both uniform over the wave)
+                const int  i = bx * 8 + lx + sx, j = by * 8 + ly + sy, k = bz * 8 + lz + sz;
+                const bool cross = ((i >> 4) != (bx >> 1)) || ((j >> 4) != (by >> 1)) || ((k >> 4) != (bz >> 1));
+                const int32_t cs = cross ? L.ptab[8 * s + d] : parent;
+                v = mg_prolonged(L, L.r, C, cs, omega, n, i, j, k);
+            }
         }
         xs[c] = v;
     }
@@ -1646,11 +1638,10 @@ __global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C,
         const int   s  = (int)xcd_slot(v, L.nSlots);
         const int   b  = L.slotBrick[s];
         const int   bx = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
-        ParentSlots P;
-        if(kProlong) P = parent_slots(L.ptab, s);
-        const SmoothCell R = smooth_cell_setup<kProlong>(L, C, P, omega, s, lr, bx, by, bz, xs);
-        const SmoothCell K = smooth_cell_setup<kProlong>(L, C, P, omega, s, lb, bx, by, bz, xs);
-        smooth_halo<kProlong>(L, C, P, omega, s, bx, by, bz, xs);
+        const int32_t parent = kProlong ? L.ptab[8 * s + 6] : -1;
+        const SmoothCell R = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lr, bx, by, bz, xs);
+        const SmoothCell K = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lb, bx, by, bz, xs);
+        smooth_halo<kProlong>(L, C, parent, omega, s, bx, by, bz, xs);
         __syncthreads();
         for(int it = 0; it < sweeps; ++it) {
             smooth_cell_update(kProlong ? K : R, xs);
