@@ -487,7 +487,9 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t*
     const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
     const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
     const uint32_t nseg = (uint32_t)(ez + kP2gSeg - 1) / kP2gSeg;
-    const uint32_t t    = blockIdx.x * blockDim.x + threadIdx.x;
+    // consecutive blocks (neighbouring j rows, which read the same particle rows) stay on one XCD
+    const uint32_t nblk = gridDim.x, b = blockIdx.x, xcd = b % 8u, q = nblk / 8u, rr = nblk % 8u;
+    const uint32_t t    = (xcd * q + min(xcd, rr) + b / 8u) * blockDim.x + threadIdx.x;
     if(t >= cols * nseg) return;
     const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
     const int  k0 = seg * kP2gSeg, k1 = min(k0 + kP2gSeg, ez);  // nodes [k0, k1)
@@ -542,13 +544,47 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t*
 // two-buffer Jacobi bit for bit without copying the faces a layer leaves alone.
 // Layer 0 also clears the extrapolated flag of every face it does not fill.
 //==================================================================================================
+// Faces that no layer can reach: `near` marks the 8^3 face bricks that hold a valid face or touch
+// one that does (26 neighbours). A face outside them is at least 9 face steps from every valid face,
+// so four layers cannot fill it: it is skipped (layer 0 still clears its extrapolated flag).
+__global__ void __launch_bounds__(256) k_extrap_brick_valid(const uint8_t* __restrict__ val, int sx, int sy, int sz,
+                                                            uint8_t* __restrict__ has) {
+    const int nbx = (sx + 7) / 8, nby = (sy + 7) / 8;
+    const int b = blockIdx.x, bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    int       any = 0;
+    for(int l = threadIdx.x; l < 512; l += 256) {
+        const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+        if(i < sx && j < sy && k < sz && val[lin3(i, j, k, sx, sy)] != 0) any = 1;
+    }
+    any = __syncthreads_or(any);
+    if(threadIdx.x == 0) has[b] = (uint8_t)any;
+}
+__global__ void __launch_bounds__(kBlock) k_extrap_brick_near(const uint8_t* __restrict__ has, int nbx, int nby, int nbz,
+                                                              uint8_t* __restrict__ near) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if(b >= nbx * nby * nbz) return;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    uint8_t   v = 0;
+    for(int dz = -1; dz <= 1; ++dz)
+        for(int dy = -1; dy <= 1; ++dy)
+            for(int dx = -1; dx <= 1; ++dx) {
+                const int x = bx + dx, y = by + dy, z = bz + dz;
+                if(x >= 0 && y >= 0 && z >= 0 && x < nbx && y < nby && z < nbz) v |= has[(z * nby + y) * nbx + x];
+            }
+    near[b] = v;
+}
+
 __global__ void __launch_bounds__(kBlock) k_extrap_stamp(float* __restrict__ a, uint8_t* __restrict__ val, uint8_t* __restrict__ ext,
-                                                         int sx, int sy, int sz, int layer) {
+                                                         const uint8_t* __restrict__ near, int sx, int sy, int sz, int layer) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= (size_t)sx * sy * sz) return;
+    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    if(!near[((k >> 3) * ((sy + 7) / 8) + (j >> 3)) * ((sx + 7) / 8) + (i >> 3)]) {
+        if(layer == 0) ext[t] = 0;
+        return;
+    }
     bool filled = false;
     if(val[t] == 0) {
-        const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
         if(i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2) {
             const size_t  sxy = (size_t)sx * sy;
             const uint8_t lim = (uint8_t)(layer + 1);

@@ -188,7 +188,7 @@ public:
     size_t nU = 0, nV = 0, nW = 0, nC = 0, nN = 0;
     DevBuf u, v, w, ut, vt, wt, uo, vo, wo, wu, wv, ww;
     DevBuf vu, vv, vw, eu, ew;
-    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters;
+    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters, extBricks;
     bool     weightsDirty    = true;   // boundary SDF changed since the weights were computed
     bool     weightsComputed = false;  // computeCellWeights has run at least once
     bool     bsdfNonzero     = false;  // the boundary SDF has been computed or set
@@ -294,6 +294,8 @@ public:
         for(auto* b : {&u, &ut, &uo, &wu}) b->reserve(nU * 4);
         for(auto* b : {&v, &vt, &vo, &wv}) b->reserve(nV * 4);
         for(auto* b : {&w, &wt, &wo, &ww}) b->reserve(nW * 4);
+        // face-brick masks of the extrapolation (valid / near a valid face), per component in turn
+        extBricks.reserve(2 * (size_t)((g.nx + 8) / 8 + 1) * ((g.ny + 8) / 8 + 1) * ((g.nz + 8) / 8 + 1));
         vu.reserve(nU);
         vv.reserve(nV);
         vw.reserve(nW);
@@ -510,10 +512,15 @@ public:
             int     sx, sy, sz;
         } comps[3] = {{&u, &vu, &eu, g.nx + 1, g.ny, g.nz}, {&v, &vv, &ev_, g.nx, g.ny + 1, g.nz}, {&w, &vw, &ew, g.nx, g.ny, g.nz + 1}};
         for(auto& c : comps) {
-            const size_t n = (size_t)c.sx * c.sy * c.sz;
+            const size_t n   = (size_t)c.sx * c.sy * c.sz;
+            const int    nbx = (c.sx + 7) / 8, nby = (c.sy + 7) / 8, nbz = (c.sz + 7) / 8, nbk = nbx * nby * nbz;
+            uint8_t*     has = extBricks.as<uint8_t>(), *near = has + nbk;
+            hipLaunchKernelGGL(k_extrap_brick_valid, dim3(nbk), dim3(256), 0, stream, (const uint8_t*)c.val->as<uint8_t>(), c.sx, c.sy,
+                               c.sz, has);
+            hipLaunchKernelGGL(k_extrap_brick_near, dim3(blocks(nbk)), dim3(kBlock), 0, stream, (const uint8_t*)has, nbx, nby, nbz, near);
             for(int layer = 0; layer < 4; ++layer)
                 hipLaunchKernelGGL(k_extrap_stamp, dim3(blocks(n)), dim3(kBlock), 0, stream, c.a->as<float>(), c.val->as<uint8_t>(),
-                                   c.ext->as<uint8_t>(), c.sx, c.sy, c.sz, layer);
+                                   c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer);
             hipLaunchKernelGGL(k_extrap_finish, dim3(blocks(n)), dim3(kBlock), 0, stream, c.val->as<uint8_t>(), n);
         }
     }

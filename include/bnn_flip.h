@@ -208,7 +208,12 @@ bnn_status  bnn_flip_timestep_cfl(bnn_flip* h, float* dt);
 
 /* --- override points of the reference (PIC_Solver.h:64-70, FLIP_Solver.h:49-52) ------------- */
 bnn_status  bnn_flip_move_particles(bnn_flip* h, float dt);          /* PIC_Solver.cpp:353-373 */
-bnn_status  bnn_flip_sort_particles(bnn_flip* h);                     /* PIC_Solver.cpp:315-328 */
+/* PIC_Solver::sortParticles (PIC_Solver.cpp:315-328): with EnableSortParticle, at finishedFrame 0
+ * and whenever (finishedFrame + 1) % SortFrequency == 0, reorders the particles' user order by the
+ * Morton key of NeighborSearch::z_sort (NeighborSearch.cpp:45-86), equal keys keeping their previous
+ * order; otherwise does nothing. bnn_flip_generate_scene sorts new particles the same way
+ * (PIC_Solver.cpp:57); bnn_flip_set_particles does not. Only the readback order changes. */
+bnn_status  bnn_flip_sort_particles(bnn_flip* h);
 bnn_status  bnn_flip_advance_velocity(bnn_flip* h, float dt);        /* FLIP_Solver.cpp:42-53 */
 bnn_status  bnn_flip_map_particles_to_grid(bnn_flip* h);              /* FLIP_Solver.cpp:57-95 */
 bnn_status  bnn_flip_map_grid_to_particles(bnn_flip* h);              /* FLIP_Solver.cpp:99-111 */
