@@ -165,6 +165,20 @@ def served_leg(args, log, batching):
     return out
 
 
+def measured_traffic(grid):
+    """HBM bytes per k_pcg_spmv launch (FETCH_SIZE + WRITE_SIZE counter runs, scaled by a 1 GiB copy)
+    as recorded in profiles/ for this workload's grid, else None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_pcg_spmv_dambreak256_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("grid") != grid:
+        return None
+    return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]
+
+
 def copy_calibration(nbytes):
     """One device-to-device copy of nbytes through the HIP runtime (its copy kernel reads and writes
     exactly nbytes), to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -283,7 +297,7 @@ def main():
         avg_ms = spmv_ms / spmv_launches
         achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
         roofline = {"kernel": "k_pcg_spmv", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(list(sim.n)),
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": spmv_launches,
                     "bytes_per_launch": spmv_bytes / max(pcg_iters, 1)}
 
