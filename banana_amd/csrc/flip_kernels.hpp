@@ -82,6 +82,15 @@ __global__ void __launch_bounds__(kBlock) k_move(Particles P, uint32_t n, VelGri
     }
 }
 
+// (i, j, k) of the linear index t of an sx x sy x sz array (x fastest), in 32-bit arithmetic: every
+// grid array here has fewer than 2^32 entries, and 64-bit division is many instructions on the GPU
+__device__ __forceinline__ void unlin3(uint32_t t, int sx, int sy, int& i, int& j, int& k) {
+    const uint32_t r = t / (uint32_t)sx;
+    i                = (int)(t - r * (uint32_t)sx);
+    j                = (int)(r % (uint32_t)sy);
+    k                = (int)(r / (uint32_t)sy);
+}
+
 //==================================================================================================
 // Binning: stable counting sort by linear cell index (Grid.cpp:92-114, 283-309)
 //==================================================================================================
@@ -578,7 +587,8 @@ __global__ void __launch_bounds__(kBlock) k_extrap_stamp(float* __restrict__ a, 
                                                          const uint8_t* __restrict__ near, int sx, int sy, int sz, int layer) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= (size_t)sx * sy * sz) return;
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
     if(!near[((k >> 3) * ((sy + 7) / 8) + (j >> 3)) * ((sx + 7) / 8) + (i >> 3)]) {
         if(layer == 0) ext[t] = 0;
         return;
@@ -645,7 +655,8 @@ __global__ void __launch_bounds__(kBlock) k_constrain(VelGrid G, GridGeom g, int
     const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
     float        val = cur[t];
     if(ext[t] == 1 && weight[t] < kFltMin) {
-        const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+        int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
         float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
         gp[d]           = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
         float vx, vy, vz, nx, ny, nz;
@@ -684,7 +695,8 @@ __global__ void __launch_bounds__(kBlock) k_gravity_center(float* __restrict__ a
     const int    sx = g.nx + 1, sy = g.ny, sz = g.nz;
     const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= (size_t)sx * sy * sz) return;
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
     const int ax = g.nx + (d == 0), ay = g.ny + (d == 1), az = g.nz + (d == 2);
     if(i >= ax || j >= ay || k >= az) return;
     float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
@@ -706,7 +718,8 @@ __global__ void __launch_bounds__(kBlock) k_weights(const float* __restrict__ b,
     const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
     const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= (size_t)sx * sy * sz) return;
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
     auto      B = [&](int a, int c, int e) { return b[lin3(a, c, e, sx, sy)]; };
     uint32_t  solid = 0;
     if(j < g.ny && k < g.nz) {
@@ -732,7 +745,8 @@ __global__ void __launch_bounds__(kBlock) k_boundary_sdf(float* __restrict__ b, 
     const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
     const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= (size_t)sx * sy * sz) return;
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
     b[t]        = box_sd_out(dom, (float)i * g.dx + g.bminx, (float)j * g.dx + g.bminy, (float)k * g.dx + g.bminz);
 }
 
@@ -741,43 +755,6 @@ __global__ void __launch_bounds__(kBlock) k_boundary_sdf(float* __restrict__ b, 
 // 3x3x3 window holds c of |centre(c) - p| - sdfRadius; min is exact, so the result is bit-identical
 // to the reference's locked scatter. Then the solid extension.
 //==================================================================================================
-__global__ void __launch_bounds__(kBlock) k_fluid_sdf(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g,
-                                                      const float* __restrict__ bsdf, float* __restrict__ phi) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= (uint32_t)(g.nx * g.ny * g.nz)) return;
-    const int   i = t % g.nx, j = (t / g.nx) % g.ny, k = t / (g.nx * g.ny);
-    const float sx = ((float)i + 0.5f) * g.dx + g.bminx;
-    const float sy = ((float)j + 0.5f) * g.dx + g.bminy;
-    const float sz = ((float)k + 0.5f) * g.dx + g.bminz;
-    float       best = g.dx * 3.0f;
-    for(int z = max(k - 1, 0); z <= min(k + 1, g.nz - 1); ++z)
-        for(int y = max(j - 1, 0); y <= min(j + 1, g.ny - 1); ++y)
-            for(int x = max(i - 1, 0); x <= min(i + 1, g.nx - 1); ++x) {
-                const size_t   c        = lin3(x, y, z, g.nx, g.ny);
-                const uint32_t pb       = cell_start[c], pe = cell_start[c + 1];
-                const bool     boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
-                for(uint32_t p = pb; p < pe; ++p) {
-                    const float ppx = P.px[p], ppy = P.py[p], ppz = P.pz[p];
-                    if(boundary) {
-                        // particles clamped into a boundary cell: use their true (unclamped) window
-                        const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
-                                  ck = (int)((ppz - g.bminz) / g.dx);
-                        if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) continue;
-                    }
-                    const float v = len3f(sx - ppx, sy - ppy, sz - ppz) - g.sdfRadius;
-                    if(v < best) best = v;
-                }
-            }
-    if(best < g.halfdx) {
-        const int sxn = g.nx + 1, syn = g.ny + 1;
-        auto      B   = [&](int a, int c, int e) { return bsdf[lin3(a, c, e, sxn, syn)]; };
-        const float s = 0.125f * (B(i, j, k) + B(i + 1, j, k) + B(i, j + 1, k) + B(i + 1, j + 1, k) + B(i, j, k + 1) +
-                                  B(i + 1, j, k + 1) + B(i, j + 1, k + 1) + B(i + 1, j + 1, k + 1));
-        if(s < 0) best = -g.halfdx;
-    }
-    phi[t] = best;
-}
-
 //==================================================================================================
 // LDS-staged gather for the fluid SDF. A work-group owns an 8x8x4 tile of cells; every thread's
 // 3x3x3 cell window lies inside the tile grown by one cell, so the work-group first
@@ -1383,7 +1360,8 @@ __global__ void __launch_bounds__(kBlock) k_update_velocity(float* __restrict__ 
     const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
     const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= (size_t)sx * sy * sz) return;
-    const int i = (int)(t % sx), j = (int)((t / sx) % sy), k = (int)(t / ((size_t)sx * sy));
+    int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
     const int ai = d == 0 ? i : (d == 1 ? j : k);       // index along the face normal
     const int an = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
     bool      upd = false;
@@ -1765,35 +1743,63 @@ __global__ void __launch_bounds__(512) k_pcg_dot_zr(const float* __restrict__ z,
 namespace bnn {
 
 // coarsest level when it is a single brick (no neighbour bricks): x = 0, then `sweeps` red-black
-// sweeps and `sweeps` black-red sweeps with x held in LDS, one thread per cell
-__global__ void __launch_bounds__(512) k_mg_coarsest1(LevelDev L, int sweeps, const PcgState* __restrict__ st) {
-    __shared__ float xs[512];
+// sweeps and `sweeps` black-red sweeps with x held in LDS. One wavefront: each lane owns four red and
+// four black cells (x-adjacent pairs on z = z0, z0+2, z0+4, z0+6), so a half-sweep ends with a
+// single-wave barrier. The LDS array has a zero halo, so couplings to cells outside the brick (zero)
+// are added unconditionally and change nothing.
+__global__ void __launch_bounds__(64) k_mg_coarsest1(LevelDev L, int sweeps, const PcgState* __restrict__ st) {
+    __shared__ float xs[10 * 10 * 10];
     if(st && st->done) return;
-    const int   l = threadIdx.x;
-    const int   lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-    const float b = L.b[l], inv = L.invdiag[l];
-    const float cxp = L.cx[l], cyp = L.cy[l], czp = L.cz[l];
-    const float cxm = lx > 0 ? L.cx[l - 1] : 0.f, cym = ly > 0 ? L.cy[l - 8] : 0.f, czm = lz > 0 ? L.cz[l - 64] : 0.f;
-    const int   color = cell_color(L, 0, l);
-    xs[l]             = 0.f;
-    __syncthreads();
-    for(int it = 0; it < 2 * sweeps; ++it) {
-        for(int half = 0; half < 2; ++half) {
-            const int c = (it < sweeps) ? half : 1 - half;
-            if(color == c) {
-                float off = 0.f;
-                if(lz > 0) off += czm * xs[l - 64];
-                if(ly > 0) off += cym * xs[l - 8];
-                if(lx > 0) off += cxm * xs[l - 1];
-                if(lx < 7) off += cxp * xs[l + 1];
-                if(ly < 7) off += cyp * xs[l + 8];
-                if(lz < 7) off += czp * xs[l + 64];
-                xs[l] = (b - off) * inv;
-            }
-            __syncthreads();
+    const int t = threadIdx.x, px = 2 * (t & 3), ly = (t >> 2) & 7, lz0 = t >> 5;
+    const int shift = (ly + lz0) & 1;  // colour of the even-x cell of every pair of this lane
+    for(int c = t; c < 1000; c += 64) xs[c] = 0.f;
+    SmoothCell R[4], K[4];  // red (colour 0) and black cells
+#pragma unroll
+    for(int q = 0; q < 4; ++q) {
+        const int lz = lz0 + 2 * q;
+#pragma unroll
+        for(int e = 0; e < 2; ++e) {
+            const int  lx = px + e, l = (lz * 8 + ly) * 8 + lx;
+            SmoothCell C;
+            C.c   = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
+            C.czm = lz > 0 ? L.cz[l - 64] : 0.f;
+            C.cym = ly > 0 ? L.cy[l - 8] : 0.f;
+            C.cxm = lx > 0 ? L.cx[l - 1] : 0.f;
+            C.cxp = lx < 7 ? L.cx[l] : 0.f;
+            C.cyp = ly < 7 ? L.cy[l] : 0.f;
+            C.czp = lz < 7 ? L.cz[l] : 0.f;
+            C.bv  = L.b[l];
+            C.inv = L.invdiag[l];
+            if((e ^ shift) == 0) R[q] = C;
+            else K[q] = C;
         }
     }
-    L.x[l] = xs[l];
+    __syncthreads();
+    for(int it = 0; it < 2 * sweeps; ++it) {
+        if(it < sweeps) {  // red-black (uniform branch)
+#pragma unroll
+            for(int q = 0; q < 4; ++q) smooth_cell_update(R[q], xs);
+            __syncthreads();
+#pragma unroll
+            for(int q = 0; q < 4; ++q) smooth_cell_update(K[q], xs);
+        } else {  // black-red
+#pragma unroll
+            for(int q = 0; q < 4; ++q) smooth_cell_update(K[q], xs);
+            __syncthreads();
+#pragma unroll
+            for(int q = 0; q < 4; ++q) smooth_cell_update(R[q], xs);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for(int q = 0; q < 4; ++q) {
+        const int lz = lz0 + 2 * q;
+#pragma unroll
+        for(int e = 0; e < 2; ++e) {
+            const int lx = px + e;
+            L.x[(lz * 8 + ly) * 8 + lx] = xs[((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1)];
+        }
+    }
 }
 
 }  // namespace bnn

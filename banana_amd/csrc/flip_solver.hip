@@ -717,7 +717,7 @@ public:
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
         if(Lc.nSlots == 1) {
-            hipLaunchKernelGGL(k_mg_coarsest1, dim3(1), dim3(512), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
+            hipLaunchKernelGGL(k_mg_coarsest1, dim3(1), dim3(64), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
         } else {
             hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(1024), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
         }
