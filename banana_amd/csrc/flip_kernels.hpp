@@ -1621,6 +1621,32 @@ __device__ __forceinline__ void smooth_halo(const LevelDev& L, const LevelDev& C
     }
 }
 
+// four cells of one colour: their neighbours all have the other colour, so every neighbour value is
+// read before any of the four is written (the reads need not wait for the writes)
+__device__ __forceinline__ void smooth_cells_update4(const SmoothCell* R, float* xs) {
+    float v[4][6];
+#pragma unroll
+    for(int q = 0; q < 4; ++q) {
+        v[q][0] = xs[R[q].c - 100];
+        v[q][1] = xs[R[q].c - 10];
+        v[q][2] = xs[R[q].c - 1];
+        v[q][3] = xs[R[q].c + 1];
+        v[q][4] = xs[R[q].c + 10];
+        v[q][5] = xs[R[q].c + 100];
+    }
+#pragma unroll
+    for(int q = 0; q < 4; ++q) {
+        float acc = 0.f;
+        acc += R[q].czm * v[q][0];
+        acc += R[q].cym * v[q][1];
+        acc += R[q].cxm * v[q][2];
+        acc += R[q].cxp * v[q][3];
+        acc += R[q].cyp * v[q][4];
+        acc += R[q].czp * v[q][5];
+        xs[R[q].c] = (R[q].bv - acc) * R[q].inv;
+    }
+}
+
 __device__ __forceinline__ void smooth_cell_update(const SmoothCell& R, float* xs) {
     float acc = 0.f;
     acc += R.czm * xs[R.c - 100];
@@ -1777,17 +1803,13 @@ __global__ void __launch_bounds__(64) k_mg_coarsest1(LevelDev L, int sweeps, con
     __syncthreads();
     for(int it = 0; it < 2 * sweeps; ++it) {
         if(it < sweeps) {  // red-black (uniform branch)
-#pragma unroll
-            for(int q = 0; q < 4; ++q) smooth_cell_update(R[q], xs);
+            smooth_cells_update4(R, xs);
             __syncthreads();
-#pragma unroll
-            for(int q = 0; q < 4; ++q) smooth_cell_update(K[q], xs);
+            smooth_cells_update4(K, xs);
         } else {  // black-red
-#pragma unroll
-            for(int q = 0; q < 4; ++q) smooth_cell_update(K[q], xs);
+            smooth_cells_update4(K, xs);
             __syncthreads();
-#pragma unroll
-            for(int q = 0; q < 4; ++q) smooth_cell_update(R[q], xs);
+            smooth_cells_update4(R, xs);
         }
         __syncthreads();
     }

@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--pressure", choices=["refined", "fp32"], default="refined")
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
     ap.add_argument("--mg-sweeps", type=int, default=None, help="smoothing sweeps per multigrid level (library default when omitted)")
+    ap.add_argument("--mg-coarsest-sweeps", type=int, default=None,
+                    help="symmetric sweeps of the coarsest multigrid level (library default when omitted)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the steps, copy this many bytes device-to-device once (a known-size kernel that scales "
@@ -236,6 +238,8 @@ def main():
                   preconditioner=1 if args.precond == "mg" else 0)
     if args.mg_sweeps is not None:
         params["mg_sweeps"] = args.mg_sweeps
+    if args.mg_coarsest_sweeps is not None:
+        params["mg_coarsest_sweeps"] = args.mg_coarsest_sweeps
     # the z-sort only runs when called (set_particles does not sort; steps never do); it is timed
     # once after the timed steps
     params.update(enable_sort_particle=1, sort_frequency=10)
