@@ -491,21 +491,25 @@ __device__ __forceinline__ void p2g_emit(const P2gAcc& a, int i, int j, int k, b
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
+// stencil weight of node `target` for coordinate q when no clamp can apply (q's floor is base, the
+// weights (1 - f, f) sit at base and base + 1): the operations of bary + stencil_weight without the
+// clamps, so the value is the same; 0 outside the stencil
+__device__ __forceinline__ float stencil_weight_interior(float q, float target) {
+    const float s = floorf(q), f = q - s;
+    return s == target ? 1.0f - f : (s == target - 1.0f ? f : 0.f);
+}
+
+// one column segment of P2G: nodes (i, j, k0 .. k1-1). kInterior: every cell the segment reads lies
+// in [1, n-2] on each axis, so no barycentric clamp can apply to its particles.
+template<bool kInterior>
+__device__ __forceinline__ void p2g_column(const Particles& P, const uint32_t* __restrict__ cell_start, const GridGeom& g,
+                                           const FaceOut& out, int i, int j, int k0, int k1) {
 #pragma clang fp contract(off)
-    const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
-    const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
-    const uint32_t nseg = (uint32_t)(ez + kP2gSeg - 1) / kP2gSeg;
-    // consecutive blocks (neighbouring j rows, which read the same particle rows) stay on one XCD
-    const uint32_t nblk = gridDim.x, b = blockIdx.x, xcd = b % 8u, q = nblk / 8u, rr = nblk % 8u;
-    const uint32_t t    = (xcd * q + min(xcd, rr) + b / 8u) * blockDim.x + threadIdx.x;
-    if(t >= cols * nseg) return;
-    const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
-    const int  k0 = seg * kP2gSeg, k1 = min(k0 + kP2gSeg, ez);  // nodes [k0, k1)
     const bool uCol = j < g.ny, vCol = i < g.nx, wCol = i < g.nx && j < g.ny;
     const int  x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1), xu = min(i, g.nx - 1);
     const int  y0 = max(j - 1, 0), y1 = min(j + 1, g.ny - 1);
     const int  zs = max(k0 - 1, 0), ze = min(k1, g.nz - 1);
+    const float fi = (float)i, fj = (float)j;
     P2gAcc     a0 = {}, a1 = {}, a2 = {};  // nodes z-1, z, z+1 while row z is read
     for(int z = zs; z <= ze; ++z) {
         for(int y = y0; y <= y1; ++y) {
@@ -514,23 +518,39 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t*
             const bool     rowU = uCol, rowV = vCol && y <= j, rowW = wCol;
             for(uint32_t p = pb; p < pe; ++p) {
                 const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
-                float       axn = 0.f, axh = 0.f, byn = 0.f, byh = 0.f;
-                const bool  oxn = stencil_weight(gx, g.nx + 1, i, axn);
-                const bool  oxh = stencil_weight(gx - 0.5f, g.nx, i, axh);
-                const bool  oyn = stencil_weight(gy, g.ny + 1, j, byn);
-                const bool  oyh = stencil_weight(gy - 0.5f, g.ny, j, byh);
-                int         kh, kn;
-                float       fh, fn;
-                bary(gz - 0.5f, g.nz, kh, fh);
-                bary(gz, g.nz + 1, kn, fn);
-                const int  dh = kh - (z - 1), dn = kn - (z - 1);
-                const bool okU = rowU && p < pu && oxn && oyh;
-                const bool okV = rowV && oxh && oyn;
-                const bool okW = rowW && oxh && oyh;
-                const float vx = okU ? P.vx[p] : 0.f, vy = okV ? P.vy[p] : 0.f, vz = okW ? P.vz[p] : 0.f;
-                p2g_window_add(dh, okU ? axn * byh : 0.f, fh, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
-                p2g_window_add(dh, okV ? axh * byn : 0.f, fh, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
-                p2g_window_add(dn, okW ? axh * byh : 0.f, fn, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
+                float       abU, abV, abW, fh, fn;
+                int         dh, dn;
+                if(kInterior) {
+                    const float axn = stencil_weight_interior(gx, fi), axh = stencil_weight_interior(gx - 0.5f, fi);
+                    const float byn = stencil_weight_interior(gy, fj), byh = stencil_weight_interior(gy - 0.5f, fj);
+                    const float qh = gz - 0.5f, sh = floorf(qh);
+                    fh  = qh - sh;
+                    dh  = sh == (float)(z - 1) ? 0 : 1;
+                    fn  = gz - floorf(gz);
+                    dn  = 1;  // the node-aligned base is the row itself
+                    abU = (rowU && p < pu) ? axn * byh : 0.f;
+                    abV = rowV ? axh * byn : 0.f;
+                    abW = rowW ? axh * byh : 0.f;
+                } else {
+                    float      axn = 0.f, axh = 0.f, byn = 0.f, byh = 0.f;
+                    const bool oxn = stencil_weight(gx, g.nx + 1, i, axn);
+                    const bool oxh = stencil_weight(gx - 0.5f, g.nx, i, axh);
+                    const bool oyn = stencil_weight(gy, g.ny + 1, j, byn);
+                    const bool oyh = stencil_weight(gy - 0.5f, g.ny, j, byh);
+                    int        kh, kn;
+                    bary(gz - 0.5f, g.nz, kh, fh);
+                    bary(gz, g.nz + 1, kn, fn);
+                    dh  = kh - (z - 1);
+                    dn  = kn - (z - 1);
+                    abU = (rowU && p < pu && oxn && oyh) ? axn * byh : 0.f;
+                    abV = (rowV && oxh && oyn) ? axh * byn : 0.f;
+                    abW = (rowW && oxh && oyh) ? axh * byh : 0.f;
+                }
+                // a zero weight adds exact zeros whatever v is, so v is read only when it counts
+                const float vx = abU != 0.f ? P.vx[p] : 0.f, vy = abV != 0.f ? P.vy[p] : 0.f, vz = abW != 0.f ? P.vz[p] : 0.f;
+                p2g_window_add(dh, abU, fh, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
+                p2g_window_add(dh, abV, fh, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
+                p2g_window_add(dn, abW, fn, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
             }
         }
         if(z - 1 >= k0) p2g_emit(a0, i, j, z - 1, uCol, vCol, wCol, g, out);  // node z-1 is complete
@@ -541,6 +561,22 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t*
     // nodes ze and ze+1 receive no further rows
     if(ze >= k0 && ze < k1) p2g_emit(a0, i, j, ze, uCol, vCol, wCol, g, out);
     if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
+}
+
+__global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
+    const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
+    const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
+    const uint32_t nseg = (uint32_t)(ez + kP2gSeg - 1) / kP2gSeg;
+    // consecutive blocks (neighbouring j rows, which read the same particle rows) stay on one XCD
+    const uint32_t nblk = gridDim.x, b = blockIdx.x, xcd = b % 8u, q = nblk / 8u, rr = nblk % 8u;
+    const uint32_t t    = (xcd * q + min(xcd, rr) + b / 8u) * blockDim.x + threadIdx.x;
+    if(t >= cols * nseg) return;
+    const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
+    const int  k0 = seg * kP2gSeg, k1 = min(k0 + kP2gSeg, ez);  // nodes [k0, k1)
+    // cells read: x in [i-1, i+1], y in [j-1, j+1], z in [k0-1, k1]
+    const bool interior = i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k0 >= 2 && k1 <= g.nz - 2;
+    if(interior) p2g_column<true>(P, cell_start, g, out, i, j, k0, k1);
+    else p2g_column<false>(P, cell_start, g, out, i, j, k0, k1);
 }
 
 //==================================================================================================
