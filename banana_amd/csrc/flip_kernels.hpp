@@ -789,32 +789,14 @@ __global__ void __launch_bounds__(kBlock) k_boundary_sdf(float* __restrict__ b, 
 //==================================================================================================
 // computeFluidSDF as a gather (PIC_Solver.cpp:601-689): phi(c) = min over particles whose clamped
 // 3x3x3 window holds c of |centre(c) - p| - sdfRadius; min is exact, so the result is bit-identical
-// to the reference's locked scatter. Then the solid extension.
-//==================================================================================================
-//==================================================================================================
-// LDS-staged gather for the fluid SDF. A work-group owns an 8x8x4 tile of cells; every thread's
-// 3x3x3 cell window lies inside the tile grown by one cell, so the work-group first
-// copies the positions of all particles of those 10x10x6 cells into LDS (each halo row of cells
-// holds one contiguous run of cell-sorted particles) and then runs the same per-cell loops, in the
-// same order, on LDS. Results are bit-identical to reading the particles from global memory, which
-// is what a tile holding more than kTileCap particles does.
-// Tiles are handed out so that each XCD (blocks are dispatched round-robin over 8 XCDs) works on a
-// contiguous run of tiles and shares halo reads through its own L2.
+// to the reference's locked scatter. Then the solid extension. A work-group owns an 8x8x4 tile of
+// cells; tiles are handed out so that each XCD (blocks are dispatched round-robin over 8 XCDs) works
+// on a contiguous run of tiles and shares neighbour-cell reads through its own L2. A cell visits its
+// own cell first and stops as soon as no farther class of neighbour can lower the minimum, so most
+// cells read only their own particles (staging the 10x10x6 halo in LDS cost more than it saved).
 //==================================================================================================
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
 constexpr int kTileThreads = kTileX * kTileY * kTileZ;  // 256
-constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2, kHaloZ = kTileZ + 2;
-constexpr int kHaloCells = kHaloX * kHaloY * kHaloZ;  // 600
-constexpr int kHaloRows  = kHaloY * kHaloZ;           // 60
-constexpr int kTileCap   = 4608;                      // staged particles per tile (~8 per halo cell)
-
-struct TileLds {
-    float    x[kTileCap], y[kTileCap], z[kTileCap];
-    uint32_t begin[kHaloCells];   // LDS index of a halo cell's first particle
-    uint32_t gbegin[kHaloCells];  // global index of that particle
-    uint32_t count[kHaloCells];
-    uint32_t rowBase[kHaloRows + 1], rowStart[kHaloRows];
-};
 
 __device__ __forceinline__ void tile_of_block(int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
     const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blockIdx.x, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
@@ -824,83 +806,12 @@ __device__ __forceinline__ void tile_of_block(int ntx, int nty, int ntz, int& tx
     tz = (int)(t / (uint32_t)(ntx * nty));
 }
 
-// halo cell (x, y, z) of the tile with origin (x0, y0, z0)
-__device__ __forceinline__ int halo_index(int x, int y, int z, int x0, int y0, int z0) {
-    return ((z - z0 + 1) * kHaloY + (y - y0 + 1)) * kHaloX + (x - x0 + 1);
-}
-
-// stage the tile's particles; false (uniform over the block) when they do not fit in LDS
-__device__ bool stage_tile(TileLds& L, const GridGeom& g, const uint32_t* __restrict__ cs, const float* __restrict__ px,
-                           const float* __restrict__ py, const float* __restrict__ pz, int x0, int y0, int z0) {
-    const int tid = threadIdx.x;
-    const int xlo = max(x0 - 1, 0), xhi = min(x0 + kTileX, g.nx - 1);
-    if(tid < kHaloRows) {
-        const int y = y0 - 1 + tid % kHaloY, z = z0 - 1 + tid / kHaloY;
-        uint32_t  s = 0, n = 0;
-        if(y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
-            const size_t r = lin3(0, y, z, g.nx, g.ny);
-            s              = cs[r + xlo];
-            n              = cs[r + xhi + 1] - s;
-        }
-        L.rowStart[tid] = s;
-        L.rowBase[tid]  = n;
-    }
-    __syncthreads();
-    if(tid == 0) {
-        uint32_t acc = 0;
-        for(int r = 0; r < kHaloRows; ++r) {
-            const uint32_t n = L.rowBase[r];
-            L.rowBase[r]     = acc;
-            acc += n;
-        }
-        L.rowBase[kHaloRows] = acc;
-    }
-    __syncthreads();
-    const uint32_t total = L.rowBase[kHaloRows];
-    for(int h = tid; h < kHaloCells; h += kTileThreads) {
-        const int hx = h % kHaloX, hy = (h / kHaloX) % kHaloY, hz = h / (kHaloX * kHaloY);
-        const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
-        uint32_t  b = 0, gb = 0, n = 0;
-        if(x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz) {
-            const size_t c   = lin3(x, y, z, g.nx, g.ny);
-            gb               = cs[c];
-            n                = cs[c + 1] - gb;
-            const int row    = hy + kHaloY * hz;
-            b                = L.rowBase[row] + (gb - L.rowStart[row]);
-        }
-        L.begin[h]  = b;
-        L.gbegin[h] = gb;
-        L.count[h]  = n;
-    }
-    if(total > (uint32_t)kTileCap) {
-        __syncthreads();
-        return false;
-    }
-    for(uint32_t i = tid; i < total; i += kTileThreads) {
-        int lo = 0, hi = kHaloRows - 1;  // last row whose LDS base is <= i (it holds particle i)
-        while(lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if(L.rowBase[mid] <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t gi = L.rowStart[lo] + (i - L.rowBase[lo]);
-        L.x[i]            = px[gi];
-        L.y[i]            = py[gi];
-        L.z[i]            = pz[gi];
-    }
-    __syncthreads();
-    return true;
-}
-
-// computeFluidSDF (PIC_Solver.cpp:601-689): same arithmetic and order as k_fluid_sdf
-__global__ void __launch_bounds__(kTileThreads) k_fluid_sdf_tiled(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g,
-                                                                  const float* __restrict__ bsdf, float* __restrict__ phi) {
-    __shared__ TileLds L;
+__global__ void __launch_bounds__(kTileThreads) k_fluid_sdf(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g,
+                                                            const float* __restrict__ bsdf, float* __restrict__ phi) {
     const int ntx = (g.nx + kTileX - 1) / kTileX, nty = (g.ny + kTileY - 1) / kTileY, ntz = (g.nz + kTileZ - 1) / kTileZ;
     int       tx, ty, tz;
     tile_of_block(ntx, nty, ntz, tx, ty, tz);
     const int  x0 = tx * kTileX, y0 = ty * kTileY, z0 = tz * kTileZ;
-    const bool staged = stage_tile(L, g, cell_start, P.px, P.py, P.pz, x0, y0, z0);
     const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
     if(i >= g.nx || j >= g.ny || k >= g.nz) return;
     const float sx      = ((float)i + 0.5f) * g.dx + g.bminx;
@@ -935,15 +846,9 @@ __global__ void __launch_bounds__(kTileThreads) k_fluid_sdf_tiled(Particles P, c
                         if(v < best) best = v;
                         if(len < bestLen) bestLen = len;
                     };
-                    if(staged) {
-                        const int      h  = halo_index(x, y, z, x0, y0, z0);
-                        const uint32_t pb = L.begin[h], pe = pb + L.count[h];
-                        for(uint32_t p = pb; p < pe; ++p) visit(L.x[p], L.y[p], L.z[p]);
-                    } else {
-                        const size_t   c  = lin3(x, y, z, g.nx, g.ny);
-                        const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
-                        for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
-                    }
+                    const size_t   c  = lin3(x, y, z, g.nx, g.ny);
+                    const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
+                    for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
                 }
     }
     if(best < g.halfdx) {

@@ -585,7 +585,7 @@ public:
         requireBinned();
         ktBegin(BNN_KT_FLUID_SDF);
         const unsigned tiles = blocks(g.nx, kTileX) * blocks(g.ny, kTileY) * blocks(g.nz, kTileZ);
-        hipLaunchKernelGGL(k_fluid_sdf_tiled, dim3(tiles), dim3(kTileThreads), 0, stream, cur->view(),
+        hipLaunchKernelGGL(k_fluid_sdf, dim3(tiles), dim3(kTileThreads), 0, stream, cur->view(),
                            (const uint32_t*)cellStart.as<uint32_t>(), g, (const float*)bsdf.as<float>(), phi.as<float>());
         ktEnd(BNN_KT_FLUID_SDF);
     }
@@ -1211,7 +1211,7 @@ void bnn_flip_default_params(bnn_flip_params* p) {
     p->timing                         = 0;
     p->mg_sweeps                      = 4;
     p->mg_omega                       = 1.8f;
-    p->mg_coarsest_sweeps             = 24;
+    p->mg_coarsest_sweeps             = 8;
     p->start_frame                    = 1;
     p->final_frame                    = 1;
 }

@@ -92,7 +92,7 @@ typedef struct bnn_flip_params {
                                                on each level (4) */
     float    mg_omega;                      /* MG: scale of the coarse-grid correction (1.8; the
                                                piecewise-constant coarse space under-corrects) */
-    int32_t  mg_coarsest_sweeps;            /* MG: symmetric sweeps on the coarsest level (24) */
+    int32_t  mg_coarsest_sweeps;            /* MG: symmetric sweeps on the coarsest level (8) */
     uint32_t start_frame, final_frame;      /* GlobalParameters StartFrame / FinalFrame (1, 1) */
 } bnn_flip_params;
 
