@@ -619,12 +619,12 @@ __global__ void __launch_bounds__(kBlock) k_extrap_brick_near(const uint8_t* __r
     near[b] = v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_extrap_stamp(float* __restrict__ a, uint8_t* __restrict__ val, uint8_t* __restrict__ ext,
-                                                         const uint8_t* __restrict__ near, int sx, int sy, int sz, int layer) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= (size_t)sx * sy * sz) return;
-    int i, j, k;
-    unlin3((uint32_t)t, sx, sy, i, j, k);
+// launched as (ceil(sx / 64), ceil(sy / 4), sz) blocks of 64 x 4 threads: (i, j, k) without divisions
+__global__ void __launch_bounds__(256) k_extrap_stamp(float* __restrict__ a, uint8_t* __restrict__ val, uint8_t* __restrict__ ext,
+                                                      const uint8_t* __restrict__ near, int sx, int sy, int sz, int layer) {
+    const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+    if(i >= sx || j >= sy) return;
+    const size_t t = lin3(i, j, k, sx, sy);
     if(!near[((k >> 3) * ((sy + 7) / 8) + (j >> 3)) * ((sx + 7) / 8) + (i >> 3)]) {
         if(layer == 0) ext[t] = 0;
         return;

@@ -519,8 +519,8 @@ public:
                                c.sz, has);
             hipLaunchKernelGGL(k_extrap_brick_near, dim3(blocks(nbk)), dim3(kBlock), 0, stream, (const uint8_t*)has, nbx, nby, nbz, near);
             for(int layer = 0; layer < 4; ++layer)
-                hipLaunchKernelGGL(k_extrap_stamp, dim3(blocks(n)), dim3(kBlock), 0, stream, c.a->as<float>(), c.val->as<uint8_t>(),
-                                   c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer);
+                hipLaunchKernelGGL(k_extrap_stamp, dim3((c.sx + 63) / 64, (c.sy + 3) / 4, c.sz), dim3(64, 4), 0, stream, c.a->as<float>(),
+                                   c.val->as<uint8_t>(), c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer);
             hipLaunchKernelGGL(k_extrap_finish, dim3(blocks(n)), dim3(kBlock), 0, stream, c.val->as<uint8_t>(), n);
         }
     }
