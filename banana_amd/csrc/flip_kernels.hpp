@@ -1002,7 +1002,7 @@ __global__ void __launch_bounds__(512) k_assemble(const int32_t* __restrict__ sl
 // PCG (PCGSolver.cpp:143-199) with Jacobi preconditioning, FP32 vectors and FP64 scalars.
 // Scalars and the stop flag live in device memory; every kernel returns at once after the stop
 // flag is set, so the host can queue iterations in chunks without reading back per iteration.
-// Reductions: per-block partials, then the last block to finish folds them in fixed order.
+// Reductions: per-block partials, then a one-block kernel folds them in fixed order.
 //==================================================================================================
 struct PcgState {
     double   rho, alpha, beta, tmp;
@@ -1010,10 +1010,10 @@ struct PcgState {
     uint32_t iter, maxIter, done, success;
 };
 
-// Reductions are two-level and deterministic: each block of a main kernel reduces the bricks it
-// owns (bricks b, b + gridDim.x, ...) and writes one partial with a plain store; a one-block fold
-// kernel then combines the partials in index order. Kernel boundaries order the two, so no
-// device-scope fences are needed inside the main kernels.
+// Reductions are two-level and deterministic: each block of a main kernel reduces a fixed set of
+// bricks (b, b + gridDim.x, ... or a contiguous chunk) and writes one partial with a plain store; a
+// one-block fold kernel then combines the partials in index order. Kernel boundaries order the two,
+// so no device-scope fences are needed inside the main kernels.
 constexpr int kMaxRedBlocks = 2048;
 
 template<int NT> __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -1072,11 +1072,12 @@ __global__ void __launch_bounds__(512) k_pcg_init(const float* __restrict__ b, c
     }
 }
 
-__global__ void __launch_bounds__(1024) k_pcg_init_fin(PcgState* st, const double* __restrict__ pa, const double* __restrict__ pb,
-                                                       int n, float relTol, uint32_t maxIter) {
+// pa: na partials of z.r, pb: nb partials of max|b|
+__global__ void __launch_bounds__(1024) k_pcg_init_fin(PcgState* st, const double* __restrict__ pa, int na,
+                                                       const double* __restrict__ pb, int nb, float relTol, uint32_t maxIter) {
     __shared__ double sh[16];
-    const double      A = fold(pa, n, false, sh);
-    const double      B = fold(pb, n, true, sh);
+    const double      A = fold(pa, na, false, sh);
+    const double      B = fold(pb, nb, true, sh);
     if(threadIdx.x != 0) return;
     st->bmax    = (float)B;
     st->res     = (float)B;
@@ -1198,11 +1199,13 @@ __global__ void __launch_bounds__(512) k_pcg_update(const float* __restrict__ s,
 }
 
 // res = max|r| ; stop test (PCGSolver.cpp:182-187) ; beta = rz / rho (:190-194)
-__global__ void __launch_bounds__(1024) k_pcg_beta(PcgState* st, const double* __restrict__ pa, const double* __restrict__ pb, int n) {
+// pa: na partials of z.r, pb: nb partials of max|r|
+__global__ void __launch_bounds__(1024) k_pcg_beta(PcgState* st, const double* __restrict__ pa, int na, const double* __restrict__ pb,
+                                                   int nb) {
     __shared__ double sh[16];
     if(st && st->done) return;
-    const double A = fold(pa, n, false, sh);
-    const double B = fold(pb, n, true, sh);
+    const double A = fold(pa, na, false, sh);
+    const double B = fold(pb, nb, true, sh);
     if(threadIdx.x != 0) return;
     st->iter += 1;
     st->res = (float)B;
@@ -1608,9 +1611,13 @@ __device__ __forceinline__ void smooth_cell_update(const SmoothCell& R, float* x
 // L.r + omega * P x_coarse, formed on the fly for the brick and its halo, and writes L.x. Input and
 // output are different arrays, so no block reads what another writes. Each of the 256 threads owns
 // an x-adjacent pair of cells, one of each colour, so every thread updates one cell per half-sweep.
-template<int kProlong>
-__global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, const PcgState* __restrict__ st) {
+// kDot (finest level, post-smoother): also writes the brick's partial of z.r (z = the result,
+// r = L.b) to pz[slot], the PCG's next rho.
+template<int kProlong, int kDot>
+__global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, const PcgState* __restrict__ st,
+                                                         double* __restrict__ pz) {
     __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
+    __shared__ double sh[4];
     if(st && st->done) return;
     const int t = threadIdx.x, ly = (t >> 2) & 7, lz = t >> 5;
     const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
@@ -1630,9 +1637,14 @@ __global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C,
             smooth_cell_update(kProlong ? R : K, xs);
             __syncthreads();
         }
-        float* out                   = kProlong ? L.x : L.r;
-        out[(size_t)s * 512 + lr]    = xs[R.c];
-        out[(size_t)s * 512 + lb]    = xs[K.c];
+        float*      out             = kProlong ? L.x : L.r;
+        const float zr = xs[R.c], zb = xs[K.c];
+        out[(size_t)s * 512 + lr]   = zr;
+        out[(size_t)s * 512 + lb]   = zb;
+        if(kDot) {
+            const double d = block_sum<256>((double)zr * (double)R.bv + (double)zb * (double)K.bv, sh);
+            if(t == 0) pz[s] = d;
+        }
         __syncthreads();  // xs is reused by the block's next brick
     }
 }

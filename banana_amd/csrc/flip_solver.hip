@@ -199,7 +199,7 @@ public:
     DevBuf   bflag, bscan, brickSlot, slotBrick, nbrTab;
     DevBuf   x, r, z, s0, s1, q, rhs32, diag32, cx32, cy32, cz32, invd;
     DevBuf   rhs64, diag64, cx64, cy64, cz64, x64;
-    DevBuf   pa, pb, pst, ost;
+    DevBuf   pa, pb, pz, pst, ost;  // PCG partials (per block: pa, pb; per brick: pz), device states
     PcgState*   hostSt  = nullptr;  // pinned
     OuterState* hostOst = nullptr;
     uint32_t*   hostU32 = nullptr;
@@ -340,6 +340,7 @@ public:
         nbrTab.reserve((size_t)c * 6 * 4);
         pa.reserve((size_t)kMaxRedBlocks * 8);
         pb.reserve((size_t)kMaxRedBlocks * 8);
+        pz.reserve((size_t)c * 8);
         if(prm.pressure_mode == BNN_PRESSURE_REFINED)
             for(auto* b : {&rhs64, &diag64, &cx64, &cy64, &cz64, &x64}) b->reserve(nv * 8);
         else
@@ -706,13 +707,15 @@ public:
 
     // z = M^-1 r on the fine level (b = r, x = z); symmetric V-cycle with the brick-hybrid smoother.
     // One block per brick. On each level the pre-smoothed x lives in L.r and the final x in L.x.
-    void vcycle(const PcgState* stp) {
+    // Returns true when the fine level's post-smoother also wrote the per-brick partials of z.r to pz
+    // (false when the fine level is the coarsest: the caller then forms z.r itself).
+    bool vcycle(const PcgState* stp) {
         const int   nu    = prm.mg_sweeps;
         const float omega = prm.mg_omega;
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
-            hipLaunchKernelGGL(k_mg_smooth_brick<0>, dim3(G), dim3(256), 0, stream, L, C, omega, nu, stp);
+            hipLaunchKernelGGL((k_mg_smooth_brick<0, 0>), dim3(G), dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
             hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(256), 0, stream, L, C, stp);
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
@@ -723,8 +726,11 @@ public:
         }
         for(int l = mgLevels - 2; l >= 0; --l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
-            hipLaunchKernelGGL(k_mg_smooth_brick<1>, dim3(std::max<uint32_t>(L.nSlots, 1u)), dim3(256), 0, stream, L, C, omega, nu, stp);
+            const dim3     G(std::max<uint32_t>(L.nSlots, 1u));
+            if(l == 0) hipLaunchKernelGGL((k_mg_smooth_brick<1, 1>), G, dim3(256), 0, stream, L, C, omega, nu, stp, pz.as<double>());
+            else hipLaunchKernelGGL((k_mg_smooth_brick<1, 0>), G, dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
         }
+        return mgLevels > 1;
     }
 
     unsigned redBlocks() const { return std::min<unsigned>(std::max<uint32_t>(nSlots, 1u), (unsigned)kMaxRedBlocks); }
@@ -739,13 +745,20 @@ public:
         const int      jac = prm.preconditioner == BNN_PRECOND_JACOBI ? 1 : 0;
         hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(512), 0, stream, b, (const float*)invd.as<float>(), x.as<float>(), r.as<float>(),
                            z.as<float>(), sA, nSlots, dpa, dpb, jac);
-        if(!jac) {
-            vcycle(nullptr);
-            hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(), nSlots,
-                               (const PcgState*)nullptr, dpa);
-        }
-        hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (const double*)dpb, (int)G, relTol,
-                           maxIter);
+        // partials of z.r: per block (G) from k_pcg_init / k_pcg_dot_zr, or per brick from the V-cycle
+        const double* rz  = dpa;
+        int           nrz = (int)G;
+        auto          zr  = [&](const PcgState* stp) {
+            if(vcycle(stp)) {
+                rz  = pz.as<double>();
+                nrz = (int)nSlots;
+            } else {
+                hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(),
+                                   nSlots, stp, dpa);
+            }
+        };
+        if(!jac) zr(nullptr);
+        hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(1024), 0, stream, dst, rz, nrz, (const double*)dpb, (int)G, relTol, maxIter);
         uint32_t chunk = 4, launched = 0;
         for(;;) {
             HIP_OK(hipMemcpyAsync(hostSt, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream));
@@ -766,12 +779,8 @@ public:
                                    (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst,
                                    dpa, dpb, jac);
                 ktEnd(BNN_KT_PCG_UPDATE);
-                if(!jac) {
-                    vcycle(dst);
-                    hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(),
-                                       nSlots, (const PcgState*)dst, dpa);
-                }
-                hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (const double*)dpb, (int)G);
+                if(!jac) zr(dst);
+                hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1024), 0, stream, dst, rz, nrz, (const double*)dpb, (int)G);
                 std::swap(sA, sB);
             }
             launched += n;
