@@ -691,40 +691,26 @@ __global__ void __launch_bounds__(256) k_constrain_faces(VelGrid G, GridGeom g, 
                                                          float* __restrict__ tmp, uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
     const int sx = g.nx + (d == 0), sy = g.ny + (d == 1);
     const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
-    bool      changed = false;
-    size_t    t       = 0;
-    if(i < sx && j < sy) {
-        t = lin3(i, j, k, sx, sy);
-        if(ext[t] == 1 && weight[t] < kFltMin) {
-            float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
-            gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
-            float vx, vy, vz, nx, ny, nz;
-            vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
-            interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
-            const float m2 = nx * nx + ny * ny + nz * nz;
-            if(m2 > kFltMin) {
-                const float s = sqrtf(m2);
-                nx /= s;
-                ny /= s;
-                nz /= s;
-                const float dt = vx * nx + vy * ny + vz * nz;
-                vx -= dt * nx;
-                vy -= dt * ny;
-                vz -= dt * nz;
-                tmp[t]  = d == 0 ? vx : (d == 1 ? vy : vz);
-                changed = true;
-            }
-        }
-    }
-    // one atomic per wavefront (a wavefront is one 64-wide x row of the block)
-    const uint64_t mask = __ballot(changed);
-    if(!mask) return;
-    const int lane   = (int)threadIdx.x;
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t  base   = 0;
-    if(lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if(changed) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)t;
+    if(i >= sx || j >= sy) return;
+    const size_t t = lin3(i, j, k, sx, sy);
+    if(ext[t] != 1 || !(weight[t] < kFltMin)) return;
+    float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+    gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+    float vx, vy, vz, nx, ny, nz;
+    vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
+    interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
+    const float m2 = nx * nx + ny * ny + nz * nz;
+    if(!(m2 > kFltMin)) return;
+    const float s = sqrtf(m2);
+    nx /= s;
+    ny /= s;
+    nz /= s;
+    const float dt = vx * nx + vy * ny + vz * nz;
+    vx -= dt * nx;
+    vy -= dt * ny;
+    vz -= dt * nz;
+    tmp[t]                      = d == 0 ? vx : (d == 1 ? vy : vz);
+    list[atomicAdd(count, 1u)] = (uint32_t)t;
 }
 
 __global__ void __launch_bounds__(kBlock) k_constrain_apply(float* __restrict__ a, const float* __restrict__ tmp,
