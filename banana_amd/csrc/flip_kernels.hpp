@@ -681,45 +681,37 @@ __device__ __forceinline__ void interp_gradient(const float* __restrict__ b, int
     gz = bilerpf(v001 - v000, v101 - v100, v011 - v010, v111 - v110, fi, fj);
 }
 
-// Two passes per component. k_constrain_faces: each face that qualifies (extrapolated, zero weight)
-// and has a nonzero boundary normal gets its constrained value in tmp[t] and its index in `list`
-// (order of the list is irrelevant: each face appears once). k_constrain_apply then writes those
-// values into the component; every other face keeps its value, as with the reference's full copy.
-// Launched as (ceil(sx / 64), ceil(sy / 4), sz) blocks of 64 x 4 threads.
-__global__ void __launch_bounds__(256) k_constrain_faces(VelGrid G, GridGeom g, int d, const uint8_t* __restrict__ ext,
-                                                         const float* __restrict__ weight, const float* __restrict__ bsdf,
-                                                         float* __restrict__ tmp, uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
-    const int sx = g.nx + (d == 0), sy = g.ny + (d == 1);
-    const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
-    if(i >= sx || j >= sy) return;
-    const size_t t = lin3(i, j, k, sx, sy);
-    if(ext[t] != 1 || !(weight[t] < kFltMin)) return;
-    float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
-    gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
-    float vx, vy, vz, nx, ny, nz;
-    vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
-    interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
-    const float m2 = nx * nx + ny * ny + nz * nz;
-    if(!(m2 > kFltMin)) return;
-    const float s = sqrtf(m2);
-    nx /= s;
-    ny /= s;
-    nz /= s;
-    const float dt = vx * nx + vy * ny + vz * nz;
-    vx -= dt * nx;
-    vy -= dt * ny;
-    vz -= dt * nz;
-    tmp[t]                      = d == 0 ? vx : (d == 1 ? vy : vz);
-    list[atomicAdd(count, 1u)] = (uint32_t)t;
-}
-
-__global__ void __launch_bounds__(kBlock) k_constrain_apply(float* __restrict__ a, const float* __restrict__ tmp,
-                                                            const uint32_t* __restrict__ list, const uint32_t* __restrict__ count) {
-    const uint32_t n = *count;
-    for(uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const uint32_t t = list[e];
-        a[t]             = tmp[t];
+__global__ void __launch_bounds__(kBlock) k_constrain(VelGrid G, GridGeom g, int d, const uint8_t* __restrict__ ext,
+                                                      const float* __restrict__ weight, const float* __restrict__ bsdf,
+                                                      float* __restrict__ out) {
+    const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
+    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N  = (size_t)sx * sy * sz;
+    if(t >= N) return;
+    const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
+    float        val = cur[t];
+    if(ext[t] == 1 && weight[t] < kFltMin) {
+        int i, j, k;
+    unlin3((uint32_t)t, sx, sy, i, j, k);
+        float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+        gp[d]           = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+        float vx, vy, vz, nx, ny, nz;
+        vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
+        interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
+        const float m2 = nx * nx + ny * ny + nz * nz;
+        if(m2 > kFltMin) {
+            const float s = sqrtf(m2);
+            nx /= s;
+            ny /= s;
+            nz /= s;
+            const float dt = vx * nx + vy * ny + vz * nz;
+            vx -= dt * nx;
+            vy -= dt * ny;
+            vz -= dt * nz;
+            val = d == 0 ? vx : (d == 1 ? vy : vz);
+        }
     }
+    out[t] = val;
 }
 
 //==================================================================================================

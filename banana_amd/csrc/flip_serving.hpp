@@ -31,7 +31,6 @@ static size_t scene_footprint(const bnn_flip_params& prm, uint64_t n) {
     size_t         nbx = (nx + 7) / 8, nby = (ny + 7) / 8, nbz = (nz + 7) / 8, nb = nbx * nby * nbz;
     size_t         b = 16 * F + 2 * F + 12 * C + 4 * N + 4 + 4 * (blocks(std::max(C, nb), kScanTile) + 1) +
                256 + 12 * nb + 2 * sizeof(PcgState);
-    b += 12 * std::max({nU, nV, nW}) - 4 * F + 16;  // the constrain temporaries are sized to the largest component
     const size_t slotsCap = nb + nb / 4 + 16;
     b += 12 * 4 * 512 * slotsCap + 28 * slotsCap + 2 * 8 * kMaxRedBlocks;
     b += (prm.pressure_mode == BNN_PRESSURE_REFINED ? 6 : 1) * 8 * 512 * slotsCap;

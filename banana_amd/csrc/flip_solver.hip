@@ -188,7 +188,7 @@ public:
     size_t nU = 0, nV = 0, nW = 0, nC = 0, nN = 0;
     DevBuf u, v, w, ut, vt, wt, uo, vo, wo, wu, wv, ww;
     DevBuf vu, vv, vw, eu, ew;
-    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters, extBricks, cstCount;
+    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters, extBricks;
     bool     weightsDirty    = true;   // boundary SDF changed since the weights were computed
     bool     weightsComputed = false;  // computeCellWeights has run at least once
     bool     bsdfNonzero     = false;  // the boundary SDF has been computed or set
@@ -291,12 +291,9 @@ public:
         nC = (size_t)g.nx * g.ny * g.nz;
         nN = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
         nb = (uint32_t)(g.nbx * g.nby * g.nbz);
-        for(auto* b : {&u, &uo, &wu}) b->reserve(nU * 4);
-        for(auto* b : {&v, &vo, &wv}) b->reserve(nV * 4);
-        for(auto* b : {&w, &wo, &ww}) b->reserve(nW * 4);
-        // constrain temporaries: each also holds another component's list of changed faces
-        for(auto* b : {&ut, &vt, &wt}) b->reserve(std::max({nU, nV, nW}) * 4);
-        cstCount.reserve(16);
+        for(auto* b : {&u, &ut, &uo, &wu}) b->reserve(nU * 4);
+        for(auto* b : {&v, &vt, &vo, &wv}) b->reserve(nV * 4);
+        for(auto* b : {&w, &wt, &wo, &ww}) b->reserve(nW * 4);
         // face-brick masks of the extrapolation (valid / near a valid face), per component in turn
         extBricks.reserve(2 * (size_t)((g.nx + 8) / 8 + 1) * ((g.ny + 8) / 8 + 1) * ((g.nz + 8) / 8 + 1));
         vu.reserve(nU);
@@ -552,18 +549,12 @@ public:
         DevBuf* tmp[3] = {&ut, &vt, &wt};
         DevBuf* ext[3] = {&eu, &ev_, &ew};
         DevBuf* wgt[3] = {&wu, &wv, &ww};
-        // component d: values in its own temporary, the list of changed faces in the next one's
-        HIP_OK(hipMemsetAsync(cstCount.p, 0, 16, stream));
+        size_t  cnt[3] = {nU, nV, nW};
         for(int d = 0; d < 3; ++d) {
-            VelGrid   G{u.as<float>(), v.as<float>(), w.as<float>()};
-            const int sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
-            uint32_t* list  = tmp[(d + 1) % 3]->as<uint32_t>();
-            uint32_t* count = cstCount.as<uint32_t>() + d;
-            hipLaunchKernelGGL(k_constrain_faces, dim3((sx + 63) / 64, (sy + 3) / 4, sz), dim3(64, 4), 0, stream, G, g, d,
-                               (const uint8_t*)ext[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(),
-                               tmp[d]->as<float>(), list, count);
-            hipLaunchKernelGGL(k_constrain_apply, dim3(256), dim3(kBlock), 0, stream, vel[d]->as<float>(), (const float*)tmp[d]->as<float>(),
-                               (const uint32_t*)list, (const uint32_t*)count);
+            VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
+            hipLaunchKernelGGL(k_constrain, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, G, g, d, (const uint8_t*)ext[d]->as<uint8_t>(),
+                               (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(), tmp[d]->as<float>());
+            HIP_OK(hipMemcpyAsync(vel[d]->p, tmp[d]->p, cnt[d] * 4, hipMemcpyDeviceToDevice, stream));
         }
     }
 
