@@ -266,6 +266,7 @@ def main():
     pcg_iters = 0
     per_step = []
     stage_ms = {}
+    stage_bytes = {}
     t_start = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
@@ -279,6 +280,8 @@ def main():
         spmv_bytes += spmv_bytes_per_cell() * st["num_active_cells"] * it
         for k, v in st["stage_ms"].items():
             stage_ms[k] = stage_ms.get(k, 0.0) + v
+        for k, v in st["stage_bytes"].items():
+            stage_bytes[k] = stage_bytes.get(k, 0.0) + v
         log(f"step {i + 1}/{args.steps}: {1e3 * per_step[-1]:.1f} ms, pcg {it} (outer {st['outer_iterations']}), "
             f"residual {st['pcg_residual']:.2e}/{st['rhs_max']:.2e}, active cells {st['num_active_cells']}")
     sim.synchronize()
@@ -348,6 +351,12 @@ def main():
         "batched": batched,
         "served": served,
         "zsort": {"ms_per_call": round(zsort_ms[-1], 3), "particles": np_},
+        # per stage: algorithmic bytes (the library's byte model; PCG counts 64 B per fluid cell and
+        # inner iteration, the Jacobi form, so the multigrid's extra traffic is not in it) over the
+        # stage's event time
+        "stage_roofline": {k: {"gbs": round(stage_bytes[k] / (stage_ms[k] * 1e6), 1),
+                               "frac": round(stage_bytes[k] / (stage_ms[k] * 1e6) / HBM_PEAK_GBS, 3)}
+                           for k in stage_ms if stage_ms[k] > 0 and stage_bytes.get(k, 0) > 0},
     }
     print(json.dumps(out), flush=True)
 
