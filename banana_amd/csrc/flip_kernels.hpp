@@ -1502,7 +1502,8 @@ struct SmoothCell {
 };
 
 template<int kProlong>
-__device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, int s, int l, float* xs) {
+__device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, const LevelDev& C, int32_t parent, float omega, int s, int l,
+                                                        int bx, int by, int bz, float* xs) {
     const int     lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
     const int     c  = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
     const size_t  o  = (size_t)s * 512 + l;
@@ -1521,14 +1522,16 @@ __device__ __forceinline__ SmoothCell smooth_cell_setup(const LevelDev& L, int s
     R.czp = izp >= 0 ? czp : 0.f;
     R.bv  = L.b[o];
     R.inv = L.invdiag[o];
-    xs[c] = kProlong ? L.r[o] : 0.f;  // the post-smoother starts from the corrected x (k_mg_prolong)
+    xs[c] = kProlong == 1 ? mg_prolonged(L, L.r, C, parent, omega, (int32_t)o, bx * 8 + lx, by * 8 + ly, bz * 8 + lz)
+                          : (kProlong == 2 ? L.r[o] : 0.f);
     return R;
 }
 
 // the brick's one-cell halo (6 faces x 64 cells) in LDS: the neighbouring bricks' starting values
 // (0 for the pre-smoother and for missing neighbours). A wave's 64 entries share one face.
 template<int kProlong>
-__device__ __forceinline__ void smooth_halo(const LevelDev& L, int s, float* xs) {
+__device__ __forceinline__ void smooth_halo(const LevelDev& L, const LevelDev& C, int32_t parent, float omega, int s, int bx, int by,
+                                            int bz, float* xs) {
     for(int h = threadIdx.x; h < 384; h += blockDim.x) {
         const int d = h >> 6, p = h & 63, a = p & 7, e = p >> 3;
         int       lx, ly, lz;  // the brick cell next to the halo entry
@@ -1548,9 +1551,19 @@ __device__ __forceinline__ void smooth_halo(const LevelDev& L, int s, float* xs)
         const int sx = d == 0 ? -1 : d == 1 ? 1 : 0, sy = d == 2 ? -1 : d == 3 ? 1 : 0, sz = d == 4 ? -1 : d == 5 ? 1 : 0;
         const int c  = ((lz + 1 + sz) * 10 + (ly + 1 + sy)) * 10 + (lx + 1 + sx);
         float     v  = 0.f;
-        if(kProlong) {
+        if(kProlong == 2) {
             const int32_t n = nbr_index(L.nbr, s, (lz * 8 + ly) * 8 + lx, d);
             if(n >= 0) v = L.r[n];
+        } else if(kProlong == 1) {
+            const int32_t n = nbr_index(L.nbr, s, (lz * 8 + ly) * 8 + lx, d);
+            if(n >= 0) {
+                // the neighbour brick's parent: this brick's parent, or its neighbour across the face
+                // when this brick is on the parent's face (both uniform over the wave)
+                const int  i = bx * 8 + lx + sx, j = by * 8 + ly + sy, k = bz * 8 + lz + sz;
+                const bool cross = ((i >> 4) != (bx >> 1)) || ((j >> 4) != (by >> 1)) || ((k >> 4) != (bz >> 1));
+                const int32_t cs = cross ? L.ptab[8 * s + d] : parent;
+                v = mg_prolonged(L, L.r, C, cs, omega, n, i, j, k);
+            }
         }
         xs[c] = v;
     }
@@ -1615,14 +1628,17 @@ __global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, floa
 // their values on entry (Jacobi between bricks). For the brick's own system A_BB the red-black and
 // black-red sequences are transposes of each other, so the pre- and post-smoothers are adjoint and
 // the V-cycle stays symmetric. Pre-smoother (kProlong == 0): starts from x = 0 (every V-cycle does)
-// and writes L.r. Post-smoother (kProlong == 1): starts from L.r after k_mg_prolong added the
-// coarse-grid correction omega * P x_coarse, for the brick and its halo, and writes L.x. Input and
-// output are different arrays, so no block reads what another writes. Each of the 256 threads owns
+// and writes L.r. Post-smoother: starts from the coarse-grid-corrected L.r + omega * P x_coarse for
+// the brick and its halo, formed on the fly (kProlong == 1: no extra launch, for the small coarse
+// levels) or already added by k_mg_prolong (kProlong == 2: the finest level, where a streaming pass
+// is cheaper than the dependent loads), and writes L.x. Input and output are different arrays, so
+// no block reads what another writes. Each of the 256 threads owns
 // an x-adjacent pair of cells, one of each colour, so every thread updates one cell per half-sweep.
 // kDot (finest level, post-smoother): also writes the brick's partial of z.r (z = the result,
 // r = L.b) to pz[slot], the PCG's next rho.
 template<int kProlong, int kDot>
-__global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, int sweeps, const PcgState* __restrict__ st, double* __restrict__ pz) {
+__global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, const PcgState* __restrict__ st,
+                                                         double* __restrict__ pz) {
     __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
     __shared__ double sh[4];
     if(st && st->done) return;
@@ -1630,10 +1646,13 @@ __global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, int sweeps,
     const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
     const int lr = ((ly + lz) & 1) ? l0 + 1 : l0, lb = ((ly + lz) & 1) ? l0 : l0 + 1;  // red (colour 0), black
     for(uint32_t v = blockIdx.x; v < L.nSlots; v += gridDim.x) {
-        const int        s = (int)xcd_slot(v, L.nSlots);
-        const SmoothCell R = smooth_cell_setup<kProlong>(L, s, lr, xs);
-        const SmoothCell K = smooth_cell_setup<kProlong>(L, s, lb, xs);
-        smooth_halo<kProlong>(L, s, xs);
+        const int     s      = (int)xcd_slot(v, L.nSlots);
+        const int     b      = L.slotBrick[s];
+        const int     bx     = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
+        const int32_t parent = kProlong == 1 ? L.ptab[8 * s + 6] : -1;
+        const SmoothCell R = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lr, bx, by, bz, xs);
+        const SmoothCell K = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lb, bx, by, bz, xs);
+        smooth_halo<kProlong>(L, C, parent, omega, s, bx, by, bz, xs);
         __syncthreads();
         for(int it = 0; it < sweeps; ++it) {
             smooth_cell_update(kProlong ? K : R, xs);

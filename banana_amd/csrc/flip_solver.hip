@@ -715,7 +715,7 @@ public:
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
-            hipLaunchKernelGGL((k_mg_smooth_brick<0, 0>), dim3(G), dim3(256), 0, stream, L, nu, stp, (double*)nullptr);
+            hipLaunchKernelGGL((k_mg_smooth_brick<0, 0>), dim3(G), dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
             hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(256), 0, stream, L, C, stp);
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
@@ -727,9 +727,12 @@ public:
         for(int l = mgLevels - 2; l >= 0; --l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const dim3     G(std::max<uint32_t>(L.nSlots, 1u));
-            hipLaunchKernelGGL(k_mg_prolong, G, dim3(256), 0, stream, L, C, omega, stp);
-            if(l == 0) hipLaunchKernelGGL((k_mg_smooth_brick<1, 1>), G, dim3(256), 0, stream, L, nu, stp, pz.as<double>());
-            else hipLaunchKernelGGL((k_mg_smooth_brick<1, 0>), G, dim3(256), 0, stream, L, nu, stp, (double*)nullptr);
+            if(l == 0) {  // finest level: correction as its own streaming pass, then smooth and form z.r
+                hipLaunchKernelGGL(k_mg_prolong, G, dim3(256), 0, stream, L, C, omega, stp);
+                hipLaunchKernelGGL((k_mg_smooth_brick<2, 1>), G, dim3(256), 0, stream, L, C, omega, nu, stp, pz.as<double>());
+            } else {
+                hipLaunchKernelGGL((k_mg_smooth_brick<1, 0>), G, dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
+            }
         }
         return mgLevels > 1;
     }
