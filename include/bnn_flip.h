@@ -142,7 +142,8 @@ typedef enum bnn_stage {
 #define BNN_NUM_KERNEL_TIMERS 4
 typedef enum bnn_kernel_timer {
     BNN_KT_PCG_SPMV   = 0, /* s = z + beta s; q = A s; s.q */
-    BNN_KT_PCG_UPDATE = 1, /* x, r updates; max|r|; z = M^-1 r; z.r */
+    BNN_KT_PCG_UPDATE = 1, /* x, r updates; max|r| (Jacobi: also z = M^-1 r and z.r; with the
+                              multigrid preconditioner z.r comes from the V-cycle) */
     BNN_KT_P2G        = 2,
     BNN_KT_FLUID_SDF  = 3
 } bnn_kernel_timer;
