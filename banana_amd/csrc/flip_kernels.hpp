@@ -447,7 +447,7 @@ __device__ __forceinline__ bool stencil_weight(float q, int i_high, int target, 
 // particles in cell-sorted order, with the weight (wx*wy)*wz and the momentum w*v rounded before
 // each add (no contraction): the per-node sums are those of a per-node gather over its 27 cells.
 //--------------------------------------------------------------------------------------------------
-constexpr int kP2gSeg = 16;
+constexpr int kP2gSeg = 8;
 
 struct P2gAcc {
     float mu, wu, mv, wv, mw, ww;
