@@ -152,6 +152,8 @@ public:
             Served& s = *it->second;
             s.release = true;
             if(s.state == Served::Waiting) {
+                // never admitted: its queued requests cannot run any more
+                failQueuedLocked(id, BNN_ERR_STATE);
                 waiting.erase(std::remove(waiting.begin(), waiting.end(), id), waiting.end());
                 dead = std::move(it->second);
                 scenes.erase(it);
@@ -160,6 +162,7 @@ public:
             }
         }
         cv.notify_all();
+        doneCv.notify_all();
     }
 
     uint64_t submit(uint64_t sceneId, int kind, float dt) {
@@ -327,10 +330,17 @@ private:
         return s;
     }
 
+    Served* findLocked(uint64_t id) {
+        auto it = scenes.find(id);
+        return it == scenes.end() ? nullptr : it->second.get();
+    }
+
     // the first waiting scene (arrival order) whose footprint fits; later, smaller ones may pass
     Served* admitLocked() {
         for(auto it = waiting.begin(); it != waiting.end(); ++it) {
-            Served& s = *scenes[*it];
+            Served* sp = findLocked(*it);
+            if(!sp) continue;
+            Served& s = *sp;
             if(residentBytes + s.bytes <= cfg.hbm_budget_bytes) {
                 residentBytes += s.bytes;
                 peakBytes = std::max(peakBytes, residentBytes);
@@ -348,8 +358,9 @@ private:
         std::unordered_map<uint64_t, bool> seen;
         for(auto it = queue.begin(); it != queue.end(); ++it) {
             const uint64_t sid = reqs[*it].res.scene_id;
-            Served&        s   = *scenes[sid];
-            if(!seen[sid] && s.state == Served::Ready && !s.busy) {
+            Served*        sp  = findLocked(sid);
+            if(sp && !seen[sid] && sp->state == Served::Ready && !sp->busy) {
+                Served& s = *sp;
                 s.busy = true;
                 rid    = *it;
                 queue.erase(it);
@@ -361,14 +372,16 @@ private:
     }
 
     bool admissibleLocked() {
-        for(uint64_t id : waiting)
-            if(residentBytes + scenes[id]->bytes <= cfg.hbm_budget_bytes) return true;
+        for(uint64_t id : waiting) {
+            const Served* s = findLocked(id);
+            if(s && residentBytes + s->bytes <= cfg.hbm_budget_bytes) return true;
+        }
         return false;
     }
     bool runnableLocked() {
         for(uint64_t id : queue) {
-            const Served& s = *scenes[reqs[id].res.scene_id];
-            if(s.state == Served::Ready && !s.busy) return true;
+            const Served* s = findLocked(reqs[id].res.scene_id);
+            if(s && s->state == Served::Ready && !s->busy) return true;
         }
         return false;
     }
@@ -464,7 +477,19 @@ private:
             std::unique_lock<std::mutex> lk(m);
             Req& r           = reqs[rid];
             sid              = r.res.scene_id;
-            sv               = scenes[sid]->solver.get();
+            Served* sp       = findLocked(sid);  // busy scenes are never erased
+            if(!sp || !sp->solver) {              // cannot happen while pickLocked marks it busy
+                r.res.status = BNN_ERR_STATE;
+                r.res.t_done = now();
+                r.done       = true;
+                ++st.failed;
+                --running;
+                lk.unlock();
+                cv.notify_all();
+                doneCv.notify_all();
+                return;
+            }
+            sv               = sp->solver.get();
             kind             = r.res.kind;
             dt               = r.dt;
             r.res.t_start    = now();

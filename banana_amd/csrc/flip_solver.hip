@@ -352,6 +352,9 @@ public:
     void setParticles(const float* pos, const float* vel, uint64_t n) {
         if(n > 0xFFFFFFF0ull) throw Error(BNN_ERR_INVALID_ARG, "too many particles for 32-bit particle ids");
         if(n && (!pos || !vel)) throw Error(BNN_ERR_INVALID_ARG, "null particle array");
+        for(uint64_t i = 0; i < 3 * n; ++i)
+            if(!std::isfinite(pos[i]) || !std::isfinite(vel[i]))
+                throw Error(BNN_ERR_INVALID_ARG, "particle " + std::to_string(i / 3) + " has a non-finite position or velocity");
         ensureParticles(std::max<uint64_t>(n, 1));
         std::vector<float> tmp(n);
         std::vector<uint32_t> ids(n);

@@ -1,0 +1,308 @@
+"""The stated workloads at their full sizes on the device.
+
+  config 2  128^3 dam break with walls, FP32 mode (FP32 vectors, FP64 PCG scalars), rel-tol 1e-6:
+            step 1 against the FP32 oracle (the reference's IC(0)-PCG) and the FP64 oracle, then 19
+            more steps; the same step in refined mode against the FP64 oracle at <= 1e-9.
+  config 3  256^3 dam break + falling sphere (33.2M particles), refined pressure to 1e-10: binning
+            and fluid SDF / labels bit-exact against the oracle; after 10 steps the FP64 residual
+            of the pressure system, rebuilt on the host from the grid fields with the reference's
+            assembly rule (PIC_Solver.cpp:693-893), is <= 1e-10 max|b|, and the projected velocity
+            is divergence-free to that bound plus FP32 rounding.
+  config 4  512 uneven scenes (24..96 cells), refined to 1e-9: every scene converges in the batch,
+            and sampled scenes stepped alone give the same bits.
+  config 5  the request server on scenes of 32..192 cells under Poisson arrivals at 200/s: every
+            request completes, and sampled scenes replayed alone give the same bits.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from banana_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+DT = np.float32(1.0 / 600.0)
+HOST_THREADS = 16
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# --------------------------------------------------------------------------------------------
+# config 2
+# --------------------------------------------------------------------------------------------
+@pytest.mark.timeout(600)
+def test_config2_dambreak128_step1_vs_oracles(oracle, bnn):
+    params, spec = scenes.dam_break(128, boundary_sdf_at_init=True, cg_tol=1e-6)
+    pos, vel = bnn.generate_scene_host(params, spec, 2)
+    assert pos.shape[0] == 4021248
+    t0 = time.perf_counter()
+    o32 = oracle.Oracle(params, precision=32, threads=HOST_THREADS)
+    o32.set_particles(pos, vel)
+    o32.step(DT)
+    it32, _, ok32 = o32.pcg_stats()
+    _, p32 = o32.system()
+    _, v32 = o32.particles()
+    o64 = oracle.Oracle(params, precision=64, threads=HOST_THREADS)
+    o64.set_particles(pos, vel)
+    o64.set_cg(1e-14, 20000)
+    o64.step(DT)
+    _, p64 = o64.system()
+    _, v64 = o64.particles()
+    assert ok32 and o64.pcg_stats()[2]
+    say(f"config 2 oracles: {time.perf_counter() - t0:.1f} s, FP32 IC(0) {it32} iterations, FP64 {o64.pcg_stats()[0]}")
+    # FP32 mode on the device
+    g = bnn.FLIPSolver(params, pressure_mode=0)
+    g.set_particles(pos, vel)
+    g.step(DT)
+    st = g.stats()
+    assert st["pcg_success"] == 1 and st["pcg_iterations"] > 0
+    pg = g.pressure()
+    _, vg = g.particles()
+    e_p, e_p_ref = rel_l2(pg, p64), rel_l2(p32, p64)
+    e_v, e_v_ref, e_v32 = rel_l2(vg, v64), rel_l2(v32, v64), rel_l2(vg, v32)
+    say(f"config 2 FP32 mode: {st['pcg_iterations']} iterations; pressure vs FP64 {e_p:.3e} (FP32 oracle {e_p_ref:.3e}); "
+        f"velocity vs FP64 {e_v:.3e} (FP32 oracle {e_v_ref:.3e}), vs FP32 oracle {e_v32:.3e}")
+    # Both FP32 solves stop at max|r| < 1e-6 max|b| and carry their own truncation error: the device
+    # must be within 1e-5 or within 1.5x of the reference algorithm's own error, whichever is larger
+    assert e_p <= max(1e-5, 1.5 * e_p_ref)
+    assert e_v <= max(1e-5, 1.5 * e_v_ref)
+    # refined mode: the same step against the FP64 oracle
+    h = bnn.FLIPSolver(params, pressure_mode=1, refine_tolerance=1e-11)
+    h.set_particles(pos, vel)
+    h.step(DT)
+    assert h.stats()["pcg_success"] == 1
+    e_pr = rel_l2(h.pressure(), p64)
+    e_vr = rel_l2(h.particles()[1], v64)
+    say(f"config 2 refined: pressure vs FP64 {e_pr:.3e}, velocity vs FP64 {e_vr:.3e}")
+    assert e_pr <= 1e-9
+    assert e_vr <= 1e-6
+    h.close()
+    # 19 more FP32-mode steps (20 in all): every solve converges, particles stay finite in the box
+    for s in range(19):
+        g.step(DT)
+        assert g.stats()["pcg_success"] == 1, s
+    p, v = g.particles()
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+    assert p.min() >= 0.0 and p.max() <= 1.0
+    say(f"config 2: 20 steps, last {g.stats()['pcg_iterations']} iterations")
+    g.close()
+
+
+# --------------------------------------------------------------------------------------------
+# config 3
+# --------------------------------------------------------------------------------------------
+def _fraction_inside(a, b):
+    """MathHelpers.h:804-822 for a fluid cell a < 0 next to b"""
+    return np.where(b < 0, 1.0, a / (a - b))
+
+
+def host_residual(phi, wu, wv, ww, p_active, b_active, dt):
+    """r = b - A p in FP64 over the interior fluid cells, A rebuilt from phi and the face weights with
+    the reference's rule: per face w*dt to the diagonal (divided by max(0.01, theta) towards air),
+    -w*dt couplings to interior fluid neighbours. Returns (max|r|, max|b|, r on the grid)."""
+    nz, ny, nx = phi.shape
+    fluid = phi < 0
+    P = np.zeros(phi.shape)
+    B = np.zeros(phi.shape)
+    P[fluid] = p_active
+    B[fluid] = b_active
+    dt = float(dt)
+    c = (slice(1, -1),) * 3
+    pc = phi[c].astype(np.float64)
+    interior = np.zeros(phi.shape, bool)
+    interior[c] = True
+    Ap = np.zeros(pc.shape)
+    diag = np.zeros(pc.shape)
+    # (neighbour slice, face weight slice) for -x, +x, -y, +y, -z, +z
+    nb = [((slice(1, -1), slice(1, -1), slice(0, -2)), wu[1:-1, 1:-1, 1:-2]),
+          ((slice(1, -1), slice(1, -1), slice(2, None)), wu[1:-1, 1:-1, 2:-1]),
+          ((slice(1, -1), slice(0, -2), slice(1, -1)), wv[1:-1, 1:-2, 1:-1]),
+          ((slice(1, -1), slice(2, None), slice(1, -1)), wv[1:-1, 2:-1, 1:-1]),
+          ((slice(0, -2), slice(1, -1), slice(1, -1)), ww[1:-2, 1:-1, 1:-1]),
+          ((slice(2, None), slice(1, -1), slice(1, -1)), ww[2:-1, 1:-1, 1:-1])]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for sl, w in nb:
+            pn = phi[sl].astype(np.float64)
+            term = w.astype(np.float64) * dt
+            nfl = pn < 0
+            th = np.maximum(_fraction_inside(pc, pn), 0.01)
+            diag += np.where(nfl, term, term / th)
+            Ap -= np.where(nfl & interior[sl], term * P[sl], 0.0)
+    Ap += diag * P[c]
+    mask = fluid[c]
+    r = np.where(mask, B[c] - Ap, 0.0)
+    return np.abs(r).max(), np.abs(B[c][mask]).max(), r
+
+
+def weighted_divergence(u, v, w, wu, wv, ww):
+    """sum over a cell's faces of +-w * velocity (the negative of the assembled right-hand side)"""
+    f = lambda a: a.astype(np.float64)
+    return (f(wu[:, :, 1:]) * f(u[:, :, 1:]) - f(wu[:, :, :-1]) * f(u[:, :, :-1]) +
+            f(wv[:, 1:, :]) * f(v[:, 1:, :]) - f(wv[:, :-1, :]) * f(v[:, :-1, :]) +
+            f(ww[1:, :, :]) * f(w[1:, :, :]) - f(ww[:-1, :, :]) * f(w[:-1, :, :]))
+
+
+@pytest.mark.timeout(900)
+def test_config3_dambreak256_sphere(oracle, bnn):
+    params, spec = scenes.dam_break(256, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
+    pos, vel = bnn.generate_scene_host(params, spec, 2024)
+    say(f"config 3: {pos.shape[0]} particles")
+    assert pos.shape[0] > 33_000_000
+    g = bnn.FLIPSolver(params, pressure_mode=1, refine_tolerance=1e-10)
+    g.set_particles(pos, vel)
+    assert g.n == (260, 260, 260)
+    # binning: keys, offsets and permutation bit-exact
+    g.bin_particles()
+    perm, start = g.bins()
+    _, perm_o, start_o = oracle.bin_particles(pos, g.bmin, g.dx, g.n)
+    assert np.array_equal(start, start_o)
+    assert np.array_equal(perm, perm_o)
+    del perm, perm_o, start, start_o
+    say("config 3: binning bit-exact")
+    # fluid SDF and labels bit-exact (walls on: the solid extension acts in the band)
+    g.compute_fluid_sdf()
+    o = oracle.Oracle(params, precision=32, threads=HOST_THREADS)
+    o.set_particles(pos, vel)
+    o.compute_fluid_sdf()
+    phi_g = g.grid("phi")
+    phi_o = o.field(oracle.F_PHI).astype(np.float32)
+    assert np.array_equal(phi_g.view(np.uint32), phi_o.view(np.uint32))
+    del o, phi_o
+    say(f"config 3: fluid SDF bit-exact, {(phi_g < 0).sum()} fluid cells")
+    # 10 refined steps
+    for s in range(10):
+        g.step(DT)
+        st = g.stats()
+        assert st["pcg_success"] == 1, s
+        assert st["pcg_residual"] <= 1e-10 * st["rhs_max"], s
+    say(f"config 3: 10 steps, last {st['pcg_iterations']} inner iterations in {st['outer_iterations']} FP64 passes, "
+        f"device residual {st['pcg_residual']:.3e} / {st['rhs_max']:.3e}")
+    # one more projection on the current velocities, checked on the host
+    g.pressure_projection(DT)
+    st = g.stats()
+    assert st["pcg_success"] == 1
+    phi = g.grid("phi")
+    wu, wv, ww = g.grid("weight_u"), g.grid("weight_v"), g.grid("weight_w")
+    rmax, bmax, _ = host_residual(phi, wu, wv, ww, g.pressure(), g.rhs(), DT)
+    say(f"config 3: host FP64 residual {rmax:.3e}, max|b| {bmax:.3e} (device {st['pcg_residual']:.3e})")
+    assert bmax > 0 and rmax <= 1e-10 * bmax
+    u, v, w = g.grid("u"), g.grid("v"), g.grid("w")
+    div = weighted_divergence(u, v, w, wu, wv, ww)[1:-1, 1:-1, 1:-1]
+    mag = (np.abs(wu[:, :, 1:] * u[:, :, 1:]) + np.abs(wu[:, :, :-1] * u[:, :, :-1]) + np.abs(wv[:, 1:, :] * v[:, 1:, :]) +
+           np.abs(wv[:, :-1, :] * v[:, :-1, :]) + np.abs(ww[1:, :, :] * w[1:, :, :]) + np.abs(ww[:-1, :, :] * w[:-1, :, :]))[1:-1, 1:-1, 1:-1]
+    fl = phi[1:-1, 1:-1, 1:-1] < 0
+    # the FP32 velocity update rounds each face to 2^-24 relative
+    bound = 1e-10 * bmax + 4 * 2.0 ** -24 * mag[fl].astype(np.float64)
+    assert np.all(np.abs(div[fl]) <= bound)
+    say(f"config 3: max |div u| over fluid cells {np.abs(div[fl]).max():.3e}")
+    g.close()
+
+
+# --------------------------------------------------------------------------------------------
+# config 4
+# --------------------------------------------------------------------------------------------
+def config4_scenes(n=512):
+    rng = np.random.default_rng(7)
+    out = []
+    for i in range(n):
+        p, s = scenes.random_scene(rng, 24, 96)
+        p.update(pressure_mode=1, refine_tolerance=1e-9)
+        out.append((p, s, 1000 + i))
+    return out
+
+
+@pytest.mark.timeout(600)
+def test_config4_512_uneven_scenes(bnn):
+    from banana_amd.serving import Batch
+
+    cases = config4_scenes()
+    sims = []
+    for p, s, seed in cases:
+        g = bnn.FLIPSolver(p)
+        g.generate_scene(s, seed)
+        sims.append(g)
+    say(f"config 4: {len(sims)} scenes, {sum(g.num_particles() for g in sims)} particles")
+    b = Batch()
+    for step in range(2):
+        status = b.step(sims, DT)
+        assert np.all(status == 0), step
+        for i, g in enumerate(sims):
+            st = g.stats()
+            assert st["pcg_success"] == 1 and st["pcg_residual"] <= 1e-9 * st["rhs_max"], (step, i)
+    sample = list(range(0, len(cases), 64)) + [len(cases) - 1]
+    for i in sample:
+        p, s, seed = cases[i]
+        a = bnn.FLIPSolver(p)
+        a.generate_scene(s, seed)
+        a.step(DT)
+        a.step(DT)
+        pa, va = a.particles()
+        pb, vb = sims[i].particles()
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb), i
+        assert a.stats()["pcg_iterations"] == sims[i].stats()["pcg_iterations"], i
+        a.close()
+    say(f"config 4: all converged; scenes {sample} bit-equal alone")
+    for g in sims:
+        g.close()
+
+
+# --------------------------------------------------------------------------------------------
+# config 5
+# --------------------------------------------------------------------------------------------
+@pytest.mark.timeout(600)
+def test_config5_served_poisson_stream(bnn):
+    from banana_amd.serving import Server
+
+    rng = np.random.default_rng(11)
+    cases = []
+    for i in range(12):
+        p, s = scenes.random_scene(rng, 32, 192)
+        p.update(pressure_mode=1, refine_tolerance=1e-9)
+        cases.append((p, s, 5000 + i))
+    srv = Server(hbm_budget_bytes=200e9)
+    ids = [srv.add_scene(p, s, seed) for p, s, seed in cases]
+    arr = np.random.default_rng(13)
+    t_next = srv.now()
+    end = t_next + 4.0
+    sent = {sid: [] for sid in ids}
+    rids = []
+    while True:
+        t_next += arr.exponential(1.0 / 200.0)
+        if t_next > end:
+            break
+        wait = t_next - srv.now()
+        if wait > 0:
+            time.sleep(wait)
+        sid = ids[int(arr.integers(len(ids)))]
+        kind = int(arr.integers(2))
+        rids.append(srv.submit(sid, kind, DT))
+        sent[sid].append(kind)
+    srv.drain(300.0)
+    res = [srv.wait(r, 10.0) for r in rids]
+    assert all(r["status"] == 0 for r in res)
+    st = srv.stats()
+    assert st["completed"] == len(rids) and st["failed"] == 0
+    say(f"config 5: {len(rids)} requests, p50 {st['latency_p50_ms']:.2f} ms, p99 {st['latency_p99_ms']:.2f} ms, "
+        f"{st['throughput_rps']:.1f} req/s")
+    # replay sampled scenes alone, request by request
+    for k in (0, 3, 7, 11):
+        p, s, seed = cases[k]
+        a = bnn.FLIPSolver(p)
+        a.generate_scene(s, seed)
+        for kind in sent[ids[k]]:
+            (a.step if kind == 0 else a.pressure_projection)(DT)
+        pa, va = a.particles()
+        pb, vb = srv.particles(ids[k])
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb), k
+        a.close()
+    srv.close()

@@ -140,15 +140,21 @@ def test_constrain_with_walls_bit_exact(oracle, bnn, weights_computed):
         g.compute_cell_weights()
     o.map_particles_to_grid()
     o.extrapolate_velocity()
+    before = {}
     for c, f, fv, fe in (("u", oracle.F_U, oracle.F_VALID_U, oracle.F_EXTRAP_U), ("v", oracle.F_V, oracle.F_VALID_V, oracle.F_EXTRAP_V),
                          ("w", oracle.F_W, oracle.F_VALID_W, oracle.F_EXTRAP_W)):
-        g.set_grid(c, o.field(f))
+        before[c] = o.field(f).astype(np.float32)
+        g.set_grid(c, before[c])
         g.set_grid("extrap_" + c, o.field(fe))
     g.constrain_grid_velocity()
     o.constrain_grid_velocity()
     changed = 0
     for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
-        assert np.array_equal(g.grid(c), o.field(f).astype(np.float32)), c
+        after = g.grid(c)
+        assert np.array_equal(after, o.field(f).astype(np.float32)), c
+        changed += int(np.count_nonzero(after != before[c]))
+    # the case moves faces: every extrapolated face before the first weights, the wall faces after
+    assert changed > (1000 if not weights_computed else 0), changed
 
 
 def test_move_particles_bit_exact(oracle, bnn):
@@ -301,7 +307,10 @@ def test_step_plain_fp32_mode(oracle, bnn, case):
     e_dev, e_ref = rel_l2(vg, v64), rel_l2(vo, v64)
     print(f"plain FP32 step: device vs FP64 {e_dev:.3e}, FP32 oracle vs FP64 {e_ref:.3e}, device vs FP32 oracle {rel_l2(vg, vo):.3e}")
     assert g.stats()["pcg_success"] == 1
-    assert e_dev <= 1e-2 and e_ref <= 1e-2
+    # Both solves stop at max|r| < 1e-6 max|rhs| and keep that truncation error; the device (MG-PCG,
+    # FP64 scalars) must be as close to the FP64 answer as the reference algorithm (IC(0)-PCG, FP32
+    # scalars) is, within a factor 1.5, or within the 1e-5 bar.
+    assert e_dev <= max(1e-5, 1.5 * e_ref)
 
 
 @pytest.mark.parametrize("n", [32, 64, 160])
