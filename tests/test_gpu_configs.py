@@ -84,8 +84,43 @@ def test_config2_dambreak128_step1_vs_oracles(oracle, bnn):
     e_pr = rel_l2(h.pressure(), p64)
     e_vr = rel_l2(h.particles()[1], v64)
     say(f"config 2 refined: pressure vs FP64 {e_pr:.3e}, velocity vs FP64 {e_vr:.3e}")
-    assert e_pr <= 1e-9
+    # The FP64 oracle's whole step splats and extrapolates in FP64, the device in FP32 (the reference's
+    # precision), so the two pressure systems differ by the FP32 rounding of the face velocities: that
+    # rounding, not the solve, is the floor of this comparison.
+    assert e_pr <= 1e-6 and e_pr <= 0.01 * e_p_ref
     assert e_vr <= 1e-6
+    h.close()
+    # refined mode on exactly the pre-projection grid state of the oracle: the FP64 solve of the same
+    # system, where the bar is 1e-9
+    o32 = oracle.Oracle(params, precision=32, threads=HOST_THREADS)
+    o32.set_particles(pos, vel)
+    o32.compute_cell_weights()
+    o32.map_particles_to_grid()
+    o32.extrapolate_velocity()
+    o32.constrain_grid_velocity()
+    o32.backup_grid_velocity()
+    o32.add_gravity(DT)
+    o32.compute_fluid_sdf()
+    state = {f: o32.field(f) for f in (oracle.F_U, oracle.F_V, oracle.F_W, oracle.F_WEIGHT_U, oracle.F_WEIGHT_V,
+                                       oracle.F_WEIGHT_W, oracle.F_PHI, oracle.F_BOUNDARY_SDF)}
+    o64 = oracle.Oracle(params, precision=64, threads=HOST_THREADS)
+    for f, a in state.items():
+        o64.set_field(f, a)
+    o64.compute_system(DT)
+    o64.set_cg(1e-15, 20000)
+    o64.solve_system()
+    assert o64.pcg_stats()[2]
+    _, p64s = o64.system()
+    h = bnn.FLIPSolver(params, pressure_mode=1, refine_tolerance=1e-11)
+    h.set_particles(pos, vel)
+    for c, f in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W)):
+        h.set_grid(c, state[f])
+    h.pressure_projection(DT)
+    assert np.array_equal(h.grid("phi"), state[oracle.F_PHI].astype(np.float32))
+    e_ps = rel_l2(h.pressure(), p64s)
+    say(f"config 2 refined, same system as the FP64 oracle: pressure rel L2 {e_ps:.3e} "
+        f"({h.stats()['pcg_iterations']} inner iterations, {o64.pcg_stats()[0]} FP64 oracle iterations)")
+    assert e_ps <= 1e-9
     h.close()
     # 19 more FP32-mode steps (20 in all): every solve converges, particles stay finite in the box
     for s in range(19):
