@@ -106,6 +106,7 @@ SIGNATURES = {
     "bnn_flip_timestep_cfl": (_i32, [_vp, _P(_f32)]),
     "bnn_flip_move_particles": (_i32, [_vp, _f32]),
     "bnn_flip_sort_particles": (_i32, [_vp]),
+    "bnn_flip_sort_particles_noalloc": (_i32, [_vp]),
     "bnn_flip_advance_velocity": (_i32, [_vp, _f32]),
     "bnn_flip_map_particles_to_grid": (_i32, [_vp]),
     "bnn_flip_map_grid_to_particles": (_i32, [_vp]),

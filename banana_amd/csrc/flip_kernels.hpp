@@ -329,6 +329,14 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint32_t* __restric
     }
 }
 
+// identity elements of the AND / OR key accumulators
+__global__ void k_zsort_acc_init(unsigned long long* __restrict__ acc) {
+    if(threadIdx.x == 0) {
+        acc[0] = ~0ull;
+        acc[1] = 0ull;
+    }
+}
+
 // rank[perm[q]] = q, then every particle's external id becomes its rank
 __global__ void __launch_bounds__(kBlock) k_zsort_rank(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ rank) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,6 +35,30 @@ struct Error : std::runtime_error {
     } while(0)
 
 static inline unsigned blocks(size_t n, unsigned bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
+
+// scratch of the z-sort: 256-bin digit histograms and their scanned offsets per 4096-key tile, the
+// AND / OR key accumulators, and the scan's per-tile partials
+struct ZsortScratch {
+    uint32_t*           hist;  // 2 * nh words: histograms, then offsets
+    unsigned long long* acc;   // 2 words
+    uint32_t*           part;  // partWords words
+};
+struct ZsortLayout {
+    size_t nh, partWords;
+    size_t histOff, accOff, partOff, bytes;  // offsets into one contiguous block (256-B aligned)
+};
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+static inline ZsortLayout zsort_layout(uint64_t np) {
+    ZsortLayout L{};
+    const size_t ntiles = (size_t)((np + kRsTile - 1) / kRsTile);
+    L.nh                = 256 * ntiles;
+    L.partWords         = blocks(L.nh, kScanTile) + 1;
+    L.accOff            = 0;
+    L.histOff           = 256;
+    L.partOff           = L.histOff + align256(L.nh * 8);
+    L.bytes             = L.partOff + align256(L.partWords * 4);
+    return L;
+}
 
 // device buffer, grows on demand (contents are not preserved across growth)
 struct DevBuf {
@@ -203,6 +228,7 @@ public:
     PcgState*   hostSt  = nullptr;  // pinned
     OuterState* hostOst = nullptr;
     uint32_t*   hostU32 = nullptr;
+    int         device  = 0;  // the device current at creation (owns the stream and every buffer)
     bool        systemValid = false;
     bool        lastRefined = false;
 
@@ -248,6 +274,7 @@ public:
             grav[1] = -9.81f;
             grav[2] = 0;
         }
+        HIP_OK(hipGetDevice(&device));
         HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         allocGrid();
         HIP_OK(hipHostMalloc((void**)&hostSt, sizeof(PcgState), hipHostMallocDefault));
@@ -450,9 +477,10 @@ public:
     }
 
     // ---------------------------------------------------------------------------------------------
-    void scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total) {
+    // part: ceil(n / kScanTile) + 1 words of partials (the solver's own buffer when null)
+    void scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, uint32_t* part = nullptr) {
         const unsigned nt = blocks(n, kScanTile);
-        uint32_t*      part = scanPart.as<uint32_t>();
+        if(!part) part = scanPart.as<uint32_t>();
         hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, part);
         hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, stream, part, nt, total);
         hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, stream, in, n, (const uint32_t*)part, out);
@@ -1032,9 +1060,11 @@ public:
     }
 
     // PIC_Solver.cpp:315-328: z-sort at frame 0 and then every SortFrequency frames, if enabled
+    bool sortDue() const {
+        return prm.enable_sort_particle && !(finishedFrame > 0 && (finishedFrame + 1) % prm.sort_frequency != 0);
+    }
     void sortParticles() {
-        if(!prm.enable_sort_particle || (finishedFrame > 0 && (finishedFrame + 1) % prm.sort_frequency != 0)) return;
-        zSortParticles();
+        if(sortDue()) zSortParticles();
     }
 
     // NeighborSearch::z_sort + PointSet::sort_field (NeighborSearch.cpp:72-86, PointSet.h:114-131):
@@ -1042,27 +1072,34 @@ public:
     // spare particle buffer holds the keys; only the external ids change.
     void zSortParticles() {
         if(!np) return;
+        const ZsortLayout L = zsort_layout(np);
+        zsHist.reserve(L.nh * 4 * 2);
+        zsAcc.reserve(16);
+        scanPart.reserve(std::max<size_t>(scanPart.bytes, L.partWords * 4));
+        zSortWith(ZsortScratch{zsHist.as<uint32_t>(), zsAcc.as<unsigned long long>(), scanPart.as<uint32_t>()});
+    }
+
+    // the z-sort with its scratch (histograms + offsets, key accumulators, scan partials) given,
+    // sized by zsort_layout(np); allocates nothing
+    void zSortWith(const ZsortScratch& sc) {
+        if(!np) return;
         const uint32_t n      = (uint32_t)np;
         const uint32_t ntiles = (uint32_t)((np + kRsTile - 1) / kRsTile);
         const size_t   nh     = (size_t)256 * ntiles;
-        zsHist.reserve(nh * 4 * 2);
-        zsAcc.reserve(16);
-        scanPart.reserve(std::max<size_t>(scanPart.bytes, (blocks(nh, kScanTile) + 1) * 4));
         uint32_t*      lo[2]  = {alt->px.as<uint32_t>(), alt->vx.as<uint32_t>()};
         uint32_t*      hi[2]  = {alt->py.as<uint32_t>(), alt->vy.as<uint32_t>()};
         uint32_t*      val[2] = {alt->pz.as<uint32_t>(), alt->vz.as<uint32_t>()};
         const float    inv    = (float)(1.0 / (double)prm.cell_size);  // NeighborSearch.cpp:33
         hipLaunchKernelGGL(k_zsort_keys, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), n, inv, lo[0], hi[0], val[0]);
-        const unsigned long long init[2] = {~0ull, 0ull};
-        HIP_OK(hipMemcpyAsync(zsAcc.p, init, 16, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_zsort_acc_init, dim3(1), dim3(64), 0, stream, sc.acc);
         hipLaunchKernelGGL(k_zsort_andor, dim3(std::min<unsigned>(blocks(np), 1024)), dim3(kBlock), 0, stream,
-                           (const uint32_t*)lo[0], (const uint32_t*)hi[0], n, zsAcc.as<unsigned long long>());
-        HIP_OK(hipMemcpyAsync(hostU32, zsAcc.p, 16, hipMemcpyDeviceToHost, stream));
+                           (const uint32_t*)lo[0], (const uint32_t*)hi[0], n, sc.acc);
+        HIP_OK(hipMemcpyAsync(hostU32, sc.acc, 16, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
         uint64_t acc[2];
         std::memcpy(acc, hostU32, 16);
         const uint64_t varying = acc[0] ^ acc[1];
-        uint32_t*      hist    = zsHist.as<uint32_t>();
+        uint32_t*      hist    = sc.hist;
         uint32_t*      offs    = hist + nh;
         int            s       = 0;
         for(int d = 0; d < 8; ++d) {
@@ -1070,7 +1107,7 @@ public:
             const int word = d >> 2, shift = (d & 3) * 8;
             hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kBlock), 0, stream, (const uint32_t*)(word ? hi[s] : lo[s]), n, shift,
                                ntiles, hist);
-            scan(hist, offs, nh, nullptr);
+            scan(hist, offs, nh, nullptr, sc.part);
             hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kBlock), 0, stream, (const uint32_t*)lo[s], (const uint32_t*)hi[s],
                                (const uint32_t*)val[s], n, word, shift, ntiles, (const uint32_t*)offs, lo[1 - s], hi[1 - s], val[1 - s]);
             s = 1 - s;
@@ -1195,6 +1232,37 @@ template<class F> bnn_status guard(bnn_flip* h, F&& f) {
     }
 }
 thread_local std::string g_createError;
+}  // namespace
+
+//--------------------------------------------------------------------------------------------------
+// Scratch of the allocation-free z-sort (bnn_flip_sort_particles_noalloc): one array per device,
+// held until the process ends. Host code never uses the name `scratch` as an address (on the host it
+// is a stand-in, not device memory): each device's address comes from hipGetSymbolAddress, taken
+// once with that device current.
+//--------------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[BNN_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratchMutex;
+void*         g_scratchPtr[kMaxDevices] = {};
+
+void* scratch_for_device(int dev) {
+    if(dev < 0 || dev >= kMaxDevices) throw bnn::Error(BNN_ERR_DEVICE, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    if(!g_scratchPtr[dev]) {
+        int cur = 0;
+        HIP_OK(hipGetDevice(&cur));
+        if(cur != dev) HIP_OK(hipSetDevice(dev));
+        void*            ptr = nullptr;
+        const hipError_t e   = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if(cur != dev) (void)hipSetDevice(cur);
+        if(e != hipSuccess || !ptr)
+            throw bnn::Error(BNN_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchPtr[dev] = ptr;
+    }
+    return g_scratchPtr[dev];
+}
 }  // namespace
 
 extern "C" {
@@ -1389,6 +1457,23 @@ bnn_status bnn_flip_timestep_cfl(bnn_flip* h, float* dt) {
 bnn_status bnn_flip_move_particles(bnn_flip* h, float dt) {
     return guard(h, [&](bnn::FlipSolver& s) {
         s.moveParticles(dt);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_sort_particles_noalloc(bnn_flip* h) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!s.sortDue() || !s.np) return BNN_OK;
+        const bnn::ZsortLayout L = bnn::zsort_layout(s.np);
+        if(L.bytes > (size_t)BNN_NOALLOC_SCRATCH_BYTES)
+            throw bnn::Error(BNN_ERR_INVALID_ARG, "bnn_flip_sort_particles_noalloc: " + std::to_string(s.np) +
+                                                      " particles need " + std::to_string(L.bytes) +
+                                                      " bytes of scratch, more than BNN_NOALLOC_SCRATCH_BYTES (" +
+                                                      std::to_string((size_t)BNN_NOALLOC_SCRATCH_BYTES) + ")");
+        unsigned char* base = static_cast<unsigned char*>(scratch_for_device(s.device));
+        s.zSortWith(bnn::ZsortScratch{reinterpret_cast<uint32_t*>(base + L.histOff),
+                                      reinterpret_cast<unsigned long long*>(base + L.accOff),
+                                      reinterpret_cast<uint32_t*>(base + L.partOff)});
         return BNN_OK;
     });
 }

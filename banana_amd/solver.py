@@ -152,6 +152,10 @@ class FLIPSolver:
     def sort_particles(self):
         self._c(self.L.bnn_flip_sort_particles(self.h))
 
+    def sort_particles_noalloc(self):
+        """The same z-sort with its scratch in the library's static per-device array (no allocation)."""
+        self._c(self.L.bnn_flip_sort_particles_noalloc(self.h))
+
     def advance_velocity(self, dt):
         return self._c(self.L.bnn_flip_advance_velocity(self.h, dt))
 

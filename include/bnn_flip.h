@@ -215,6 +215,16 @@ bnn_status  bnn_flip_move_particles(bnn_flip* h, float dt);          /* PIC_Solv
  * order; otherwise does nothing. bnn_flip_generate_scene sorts new particles the same way
  * (PIC_Solver.cpp:57); bnn_flip_set_particles does not. Only the readback order changes. */
 bnn_status  bnn_flip_sort_particles(bnn_flip* h);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the same z-sort as
+ * bnn_flip_sort_particles (same rule, same result), with its scratch (digit histograms, key
+ * accumulators, scan partials) taken from a static per-device array of BNN_NOALLOC_SCRATCH_BYTES
+ * instead of buffers the handle allocates; calls no hipMalloc / hipFree. Limit: 32 MiB of scratch,
+ * i.e. at most 67,088,384 particles; beyond that it returns BNN_ERR_INVALID_ARG. All calls on one
+ * device share the array, so calls that overlap in time on one device (different handles, different
+ * threads) race: serialise them. */
+#define BNN_NOALLOC_SCRATCH_BYTES (32u << 20)
+bnn_status  bnn_flip_sort_particles_noalloc(bnn_flip* h);
 bnn_status  bnn_flip_advance_velocity(bnn_flip* h, float dt);        /* FLIP_Solver.cpp:42-53 */
 bnn_status  bnn_flip_map_particles_to_grid(bnn_flip* h);              /* FLIP_Solver.cpp:57-95 */
 bnn_status  bnn_flip_map_grid_to_particles(bnn_flip* h);              /* FLIP_Solver.cpp:99-111 */
