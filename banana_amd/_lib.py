@@ -12,10 +12,12 @@ BNN_OK = 0
 STATUS = {0: "BNN_OK", 1: "BNN_ERR_INVALID_ARG", 2: "BNN_ERR_NOT_CONVERGED", 3: "BNN_ERR_DEVICE",
           4: "BNN_ERR_OUT_OF_MEMORY", 5: "BNN_ERR_UNSUPPORTED", 6: "BNN_ERR_STATE", 7: "BNN_ERR_REJECTED"}
 PRESSURE_FP32, PRESSURE_REFINED = 0, 1
-PRECOND_JACOBI, PRECOND_MG = 0, 1
+PRECOND_JACOBI, PRECOND_MG, PRECOND_IC0 = 0, 1, 2
+TRANSFER_FLIP, TRANSFER_APIC, TRANSFER_AFLIP = 0, 1, 2
 NUM_STAGES = 12
-NUM_KERNEL_TIMERS = 4
-KERNEL_TIMER_NAMES = ["pcg_spmv", "pcg_update", "p2g", "fluid_sdf"]
+NUM_KERNEL_TIMERS = 13
+KERNEL_TIMER_NAMES = ["pcg_spmv", "pcg_update", "p2g", "fluid_sdf", "g2p", "move", "mg_pre", "mg_restrict", "mg_prolong",
+                      "mg_post", "mg_coarse", "extrap", "permute"]
 STAGE_NAMES = ["move", "bin", "p2g", "extrapolate", "constrain", "weights", "phi", "assembly", "pcg", "update",
                "g2p", "other"]
 
@@ -37,6 +39,8 @@ class Params(C.Structure):
         ("sort_frequency", C.c_uint32), ("pressure_mode", C.c_int32), ("refine_tolerance", C.c_double),
         ("preconditioner", C.c_int32), ("timing", C.c_int32), ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float),
         ("mg_coarsest_sweeps", C.c_int32), ("start_frame", C.c_uint32), ("final_frame", C.c_uint32),
+        ("repulsive_force_stiffness", C.c_float), ("random_seed", C.c_uint64), ("refine_inner_tolerance", C.c_float),
+        ("transfer_scheme", C.c_int32),
     ]
 
 
@@ -59,6 +63,12 @@ class Stats(C.Structure):
         ("finished_frame", C.c_uint32),
         ("kernel_ms", C.c_double * NUM_KERNEL_TIMERS), ("kernel_launches", C.c_uint64 * NUM_KERNEL_TIMERS),
     ]
+
+
+class IoParams(C.Structure):
+    _fields_ = [("data_path", C.c_char * 512), ("memory_state_folder", C.c_char * 128), ("frame_data_folder", C.c_char * 128),
+                ("load_memory_state", C.c_int32), ("save_memory_state", C.c_int32), ("save_frame_data", C.c_int32),
+                ("frame_per_state", C.c_uint32), ("save_velocity", C.c_int32), ("save_object_index", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
@@ -105,6 +115,9 @@ SIGNATURES = {
     "bnn_flip_do_frame_simulation": (_i32, [_vp, _u32]),
     "bnn_flip_timestep_cfl": (_i32, [_vp, _P(_f32)]),
     "bnn_flip_move_particles": (_i32, [_vp, _f32]),
+    "bnn_flip_correct_particle_positions": (_i32, [_vp, _f32]),
+    "bnn_flip_get_affine": (_i32, [_vp, _vp]),
+    "bnn_flip_set_affine": (_i32, [_vp, _vp]),
     "bnn_flip_sort_particles": (_i32, [_vp]),
     "bnn_flip_sort_particles_noalloc": (_i32, [_vp]),
     "bnn_flip_advance_velocity": (_i32, [_vp, _f32]),
@@ -119,10 +132,21 @@ SIGNATURES = {
     "bnn_flip_get_grid": (_i32, [_vp, _i32, _vp]),
     "bnn_flip_get_pressure": (_i32, [_vp, _vp, _P(_u64)]),
     "bnn_flip_get_rhs": (_i32, [_vp, _vp, _P(_u64)]),
+    "bnn_flip_apply_preconditioner": (_i32, [_vp, _vp, _vp]),
+    "bnn_flip_get_ic0_factor": (_i32, [_vp, _vp, _P(_u64)]),
     "bnn_flip_get_bins": (_i32, [_vp, _vp, _vp]),
     "bnn_flip_get_stats": (_i32, [_vp, _P(Stats)]),
     "bnn_flip_set_grid": (_i32, [_vp, _i32, _vp]),
     "bnn_flip_synchronize": (_i32, [_vp]),
+    "bnn_flip_set_timing": (_i32, [_vp, _i32]),
+    "bnn_data_file_path": (_i32, [C.c_char_p, C.c_char_p, _i32, C.c_char_p, C.c_size_t]),
+    "bnn_latest_state_index": (_i32, [C.c_char_p, C.c_char_p, _u32, _u32]),
+    "bnn_flip_save_frame_data": (_i32, [_vp, C.c_char_p, _i32, _i32]),
+    "bnn_flip_save_memory_state": (_i32, [_vp, C.c_char_p]),
+    "bnn_flip_load_memory_state": (_i32, [_vp, C.c_char_p, _u32]),
+    "bnn_flip_get_object_index": (_i32, [_vp, _vp, _P(_u32)]),
+    "bnn_io_default_params": (None, [_P(IoParams)]),
+    "bnn_io_from_json": (_i32, [C.c_char_p, _P(IoParams)]),
     "bnn_flip_footprint": (_i32, [_P(Params), _u64, _P(_u64)]),
     "bnn_flip_batch_create": (_i32, [_P(_vp)]),
     "bnn_flip_batch_destroy": (_i32, [_vp]),

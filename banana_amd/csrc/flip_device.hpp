@@ -92,6 +92,44 @@ __device__ __forceinline__ float interp3_diff(const float* __restrict__ a, const
     return lerpf(bilerpf(v000, v100, v010, v110, fi, fj), bilerpf(v001, v101, v011, v111, fi, fj), fk);
 }
 
+// ArrayHelpers.cpp:275-302 (interpolateGradientValue): the gradient of the trilinear interpolant of
+// the eight corner values over the cell, the corner terms added in the reference's order, divided
+// by the cell size
+__device__ __forceinline__ void grad_value8(float v000, float v001, float v010, float v011, float v100, float v101, float v110, float v111,
+                                            float fi, float fj, float fk, float dx, float& gx, float& gy, float& gz) {
+#pragma clang fp contract(off)
+    const float a = 1.0f - fi, b = 1.0f - fj, c = 1.0f - fk;
+    gx = v000 * (-b * c) + v001 * (-b * fk) + v010 * (-fj * c) + v011 * (-fj * fk) + v100 * (b * c) + v101 * (b * fk) + v110 * (fj * c) +
+         v111 * (fj * fk);
+    gy = v000 * (-a * c) + v001 * (-a * fk) + v010 * (a * c) + v011 * (a * fk) + v100 * (-fi * c) + v101 * (-fi * fk) + v110 * (fi * c) +
+         v111 * (fi * fk);
+    gz = v000 * (-a * b) + v001 * (a * b) + v010 * (-a * fj) + v011 * (a * fj) + v100 * (-fi * b) + v101 * (fi * b) + v110 * (-fi * fj) +
+         v111 * (fi * fj);
+    gx /= dx;
+    gy /= dx;
+    gz /= dx;
+}
+
+// interpolateGradientValue of grid g (dims sx, sy, sz) at grid coordinate (x, y, z); with o, of the
+// difference g - o formed per corner (the dVels array of computeChangesGridVelocity)
+__device__ __forceinline__ void interp3_grad(const float* __restrict__ g, const float* __restrict__ o, int sx, int sy, int sz, float x,
+                                             float y, float z, float dx, float& gx, float& gy, float& gz) {
+    int   i, j, k;
+    float fi, fj, fk;
+    bary(x, sx, i, fi);
+    bary(y, sy, j, fj);
+    bary(z, sz, k, fk);
+    const size_t b = lin3(i, j, k, sx, sy), sxy = (size_t)sx * sy;
+    const size_t i000 = b, i001 = b + sxy, i010 = b + sx, i011 = b + sxy + sx, i100 = b + 1, i101 = b + sxy + 1, i110 = b + sx + 1,
+                 i111 = b + sxy + sx + 1;
+    if(o) {
+        grad_value8(g[i000] - o[i000], g[i001] - o[i001], g[i010] - o[i010], g[i011] - o[i011], g[i100] - o[i100], g[i101] - o[i101],
+                    g[i110] - o[i110], g[i111] - o[i111], fi, fj, fk, dx, gx, gy, gz);
+    } else {
+        grad_value8(g[i000], g[i001], g[i010], g[i011], g[i100], g[i101], g[i110], g[i111], fi, fj, fk, dx, gx, gy, gz);
+    }
+}
+
 // MathHelpers.h:804-821
 template<class T> __device__ __forceinline__ T fraction_inside2(T l, T r) {
     if(l < 0 && r < 0) return T(1.0);

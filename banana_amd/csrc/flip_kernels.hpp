@@ -23,6 +23,11 @@ struct VelGrid {
     const float *u, *v, *w;
 };
 
+// APIC / AFLIP affine matrices, nine arrays in the particles' storage order: c[3 d + e] = C_d[e]
+struct Affine {
+    float* c[9];
+};
+
 __device__ __forceinline__ void vel_from_grid(const VelGrid& G, const GridGeom& g, float gpx, float gpy, float gpz, float& ux,
                                               float& uy, float& uz) {
     ux = interp3(G.u, g.nx + 1, g.ny, g.nz, gpx, gpy - 0.5f, gpz - 0.5f);
@@ -329,6 +334,100 @@ __global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint32_t* __restric
     }
 }
 
+//==================================================================================================
+// PIC_Solver::correctParticlePositions (PIC_Solver.cpp:377-485), 3-D: each particle is pushed away
+// from the other particles of the 27 cells around its cell (Grid::getCellIdx, not clamped; cells
+// outside the grid skipped) with w = smooth_kernel(d^2, r^2) (MathHelpers.h:661-664, (t*t)*t here),
+// spring += w (p - q) / d; a pair closer than the threshold gets a random push (threshold / dt) u
+// with u in [-1, 1)^3 from a hash of (seed, p, q, component), p and q being the particles' user
+// indices. Then advectionSteps times pos += spring * K followed by the domain-box projection
+// (BoundaryObject::constrainToBoundary(ppos), BoundaryObject.cpp:55-67). Neighbours are visited cell
+// by cell (k, j, i ascending) and, in a cell, in storage order. Output to separate arrays (the
+// neighbours' positions are read while others are written).
+//==================================================================================================
+__device__ __forceinline__ uint64_t mix64d(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ float rand11(uint64_t seed, uint32_t p, uint32_t q, int comp) {
+    const uint64_t h = mix64d(seed ^ mix64d(((uint64_t)p << 32 | q) * 4u + (uint64_t)comp));
+    return (float)(uint32_t)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;
+}
+
+struct CorrectParams {
+    float    radius2, threshold, threshold2, push, K;  // push = threshold / dt
+    uint32_t advSteps;
+    uint64_t seed;
+};
+
+__global__ void __launch_bounds__(kBlock) k_correct_positions(Particles P, const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                              GridGeom g, BoxDev dom, CorrectParams c, float* __restrict__ ox,
+                                                              float* __restrict__ oy, float* __restrict__ oz) {
+#pragma clang fp contract(off)
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    const float    x = P.px[p], y = P.py[p], z = P.pz[p];
+    const uint32_t pid = P.id[p];
+    const int      ci = (int)((x - g.bminx) / g.dx), cj = (int)((y - g.bminy) / g.dx), ck = (int)((z - g.bminz) / g.dx);
+    float          sx = 0.f, sy = 0.f, sz = 0.f;
+    for(int k = ck - 1; k <= ck + 1; ++k) {
+        if(k < 0 || k >= g.nz) continue;
+        for(int j = cj - 1; j <= cj + 1; ++j) {
+            if(j < 0 || j >= g.ny) continue;
+            for(int i = ci - 1; i <= ci + 1; ++i) {
+                if(i < 0 || i >= g.nx) continue;
+                const size_t   cell = lin3(i, j, k, g.nx, g.ny);
+                const uint32_t qb = cell_start[cell], qe = cell_start[cell + 1];
+                for(uint32_t q = qb; q < qe; ++q) {
+                    if(q == p) continue;
+                    const float dx = x - P.px[q], dy = y - P.py[q], dz = z - P.pz[q];
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    const float x2 = d2 / c.radius2;
+                    const float t  = 1.0f - x2;
+                    const float w  = x2 > 1.0f ? 0.f : (t * t) * t;
+                    if(d2 > c.threshold2) {
+                        const float s = sqrtf(d2);
+                        sx += w * dx / s;
+                        sy += w * dy / s;
+                        sz += w * dz / s;
+                    } else {
+                        const uint32_t qid = P.id[q];
+                        sx += c.push * rand11(c.seed, pid, qid, 0);
+                        sy += c.push * rand11(c.seed, pid, qid, 1);
+                        sz += c.push * rand11(c.seed, pid, qid, 2);
+                    }
+                }
+            }
+        }
+    }
+    float nx_ = x, ny_ = y, nz_ = z;
+    for(uint32_t it = 0; it < c.advSteps; ++it) {
+        nx_ += sx * c.K;
+        ny_ += sy * c.K;
+        nz_ += sz * c.K;
+        const float phi = box_sd_out(dom, nx_, ny_, nz_);
+        if(phi < 0) {
+            float gx, gy, gz;
+            box_grad_out(dom, nx_, ny_, nz_, gx, gy, gz);
+            const float m2 = gx * gx + gy * gy + gz * gz;
+            if(m2 > kFltMin) {
+                const float s = sqrtf(m2);
+                gx /= s;
+                gy /= s;
+                gz /= s;
+                nx_ -= phi * gx;
+                ny_ -= phi * gy;
+                nz_ -= phi * gz;
+            }
+        }
+    }
+    ox[p] = nx_;
+    oy[p] = ny_;
+    oz[p] = nz_;
+}
+
 // identity elements of the AND / OR key accumulators
 __global__ void k_zsort_acc_init(unsigned long long* __restrict__ acc) {
     if(threadIdx.x == 0) {
@@ -462,19 +561,31 @@ struct P2gAcc {
 };
 
 // weights (1-f) at base and f at base+1; d = base - (z-1) picks the window slots (0,1) or (1,2).
-// A zero ab (no contribution) adds exact zeros.
-__device__ __forceinline__ void p2g_window_add(int d, float ab, float f, float v, float& m0, float& w0, float& m1, float& w1,
-                                               float& m2, float& w2) {
+// vlo / vhi: the velocity carried to the base node and to base+1 (the particle's velocity; with the
+// affine term of APIC / AFLIP, v + dot(C_d, node - particle) per node). A zero ab (no
+// contribution) adds exact zeros.
+__device__ __forceinline__ void p2g_window_add(int d, float ab, float f, float vlo, float vhi, float& m0, float& w0, float& m1,
+                                               float& w1, float& m2, float& w2) {
 #pragma clang fp contract(off)
     const float wl = ab * (1.0f - f), wh = ab * f;
     const bool  lo = d == 0;
     const float c0 = lo ? wl : 0.f, c1 = lo ? wh : wl, c2 = lo ? 0.f : wh;
-    m0 += c0 * v;
+    const float v1 = lo ? vhi : vlo;
+    m0 += c0 * vlo;
     w0 += c0;
-    m1 += c1 * v;
+    m1 += c1 * v1;
     w1 += c1;
-    m2 += c2 * v;
+    m2 += c2 * vhi;
     w2 += c2;
+}
+
+// v_d + dot(C_d, gpos - ppos) (APIC_Solver.cpp:73-76) for the node at grid index (xi + ex, yi + ey,
+// zi + ez): gpos = getWorldCoordinate = (index + extra) * dx + bMin, the dot product in glm's order
+__device__ __forceinline__ float affine_vel(float v, const float* c, float xi, float yi, float zi, float ex, float ey, float ez,
+                                            const GridGeom& g, float px, float py, float pz) {
+#pragma clang fp contract(off)
+    const float gxw = (xi + ex) * g.dx + g.bminx, gyw = (yi + ey) * g.dx + g.bminy, gzw = (zi + ez) * g.dx + g.bminz;
+    return v + ((c[0] * (gxw - px) + c[1] * (gyw - py)) + c[2] * (gzw - pz));
 }
 
 __device__ __forceinline__ void p2g_emit(const P2gAcc& a, int i, int j, int k, bool uCol, bool vCol, bool wCol, const GridGeom& g,
@@ -509,8 +620,8 @@ __device__ __forceinline__ float stencil_weight_interior(float q, float target) 
 
 // one column segment of P2G: nodes (i, j, k0 .. k1-1). kInterior: every cell the segment reads lies
 // in [1, n-2] on each axis, so no barycentric clamp can apply to its particles.
-template<bool kInterior>
-__device__ __forceinline__ void p2g_column(const Particles& P, const uint32_t* __restrict__ cell_start, const GridGeom& g,
+template<bool kInterior, bool kAffine>
+__device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, const uint32_t* __restrict__ cell_start, const GridGeom& g,
                                            const FaceOut& out, int i, int j, int k0, int k1) {
 #pragma clang fp contract(off)
     const bool uCol = j < g.ny, vCol = i < g.nx, wCol = i < g.nx && j < g.ny;
@@ -556,9 +667,31 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const uint32_t* _
                 }
                 // a zero weight adds exact zeros whatever v is, so v is read only when it counts
                 const float vx = abU != 0.f ? P.vx[p] : 0.f, vy = abV != 0.f ? P.vy[p] : 0.f, vz = abW != 0.f ? P.vz[p] : 0.f;
-                p2g_window_add(dh, abU, fh, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
-                p2g_window_add(dh, abV, fh, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
-                p2g_window_add(dn, abW, fn, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
+                if(kAffine) {
+                    // the base node along z of the half-shifted (u, v) and node-aligned (w) stencils
+                    const float zh = (float)(z - 1 + dh), zn = (float)(z - 1 + dn);
+                    const float px = P.px[p], py = P.py[p], pz = P.pz[p];
+                    float       cu[3] = {0.f, 0.f, 0.f}, cv[3] = {0.f, 0.f, 0.f}, cw[3] = {0.f, 0.f, 0.f};
+                    if(abU != 0.f)
+                        for(int e = 0; e < 3; ++e) cu[e] = A.c[e][p];
+                    if(abV != 0.f)
+                        for(int e = 0; e < 3; ++e) cv[e] = A.c[3 + e][p];
+                    if(abW != 0.f)
+                        for(int e = 0; e < 3; ++e) cw[e] = A.c[6 + e][p];
+                    p2g_window_add(dh, abU, fh, affine_vel(vx, cu, fi, fj, zh, 0.f, 0.5f, 0.5f, g, px, py, pz),
+                                   affine_vel(vx, cu, fi, fj, zh + 1.0f, 0.f, 0.5f, 0.5f, g, px, py, pz), a0.mu, a0.wu, a1.mu, a1.wu,
+                                   a2.mu, a2.wu);
+                    p2g_window_add(dh, abV, fh, affine_vel(vy, cv, fi, fj, zh, 0.5f, 0.f, 0.5f, g, px, py, pz),
+                                   affine_vel(vy, cv, fi, fj, zh + 1.0f, 0.5f, 0.f, 0.5f, g, px, py, pz), a0.mv, a0.wv, a1.mv, a1.wv,
+                                   a2.mv, a2.wv);
+                    p2g_window_add(dn, abW, fn, affine_vel(vz, cw, fi, fj, zn, 0.5f, 0.5f, 0.f, g, px, py, pz),
+                                   affine_vel(vz, cw, fi, fj, zn + 1.0f, 0.5f, 0.5f, 0.f, g, px, py, pz), a0.mw, a0.ww, a1.mw, a1.ww,
+                                   a2.mw, a2.ww);
+                } else {
+                    p2g_window_add(dh, abU, fh, vx, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
+                    p2g_window_add(dh, abV, fh, vy, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
+                    p2g_window_add(dn, abW, fn, vz, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
+                }
             }
         }
         if(z - 1 >= k0) p2g_emit(a0, i, j, z - 1, uCol, vCol, wCol, g, out);  // node z-1 is complete
@@ -571,7 +704,9 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const uint32_t* _
     if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
 }
 
-__global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g, FaceOut out) {
+template<bool kAffine>
+__global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, Affine A, const uint32_t* __restrict__ cell_start, GridGeom g,
+                                                    FaceOut out) {
     const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
     const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
     const uint32_t nseg = (uint32_t)(ez + kP2gSeg - 1) / kP2gSeg;
@@ -583,8 +718,8 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, const uint32_t*
     const int  k0 = seg * kP2gSeg, k1 = min(k0 + kP2gSeg, ez);  // nodes [k0, k1)
     // cells read: x in [i-1, i+1], y in [j-1, j+1], z in [k0-1, k1]
     const bool interior = i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k0 >= 2 && k1 <= g.nz - 2;
-    if(interior) p2g_column<true>(P, cell_start, g, out, i, j, k0, k1);
-    else p2g_column<false>(P, cell_start, g, out, i, j, k0, k1);
+    if(interior) p2g_column<true, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
+    else p2g_column<false, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
 }
 
 //==================================================================================================
@@ -725,6 +860,13 @@ __global__ void __launch_bounds__(kBlock) k_constrain(VelGrid G, GridGeom g, int
 //==================================================================================================
 // addGravity (PIC_Solver.cpp:505-529): Earth / Directional add a constant per component
 //==================================================================================================
+// c = a * b elementwise
+__global__ void __launch_bounds__(kBlock) k_mul_arrays(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ c,
+                                                       size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) c[i] = a[i] * b[i];
+}
+
 __global__ void __launch_bounds__(kBlock) k_add_const(float* __restrict__ a, size_t n, float add, int subtract) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(t >= n) return;
@@ -1356,6 +1498,56 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Particles P, uint32_t n, VelGrid
     P.vz[p]        = a * uz + ratio * (P.vz[p] + dz);
 }
 
+//==================================================================================================
+// APIC / AFLIP (APIC_Solver.cpp:42-125, AFLIP_Solver.cpp:43-140): each particle carries an affine
+// matrix C (row d = the gradient of velocity component d), stored as nine arrays in the particles'
+// storage order and moved with them by the binning permute.
+//==================================================================================================
+__global__ void __launch_bounds__(kBlock) k_permute_affine(Affine in, Affine out, const uint32_t* __restrict__ perm, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const uint32_t s = perm[i];
+#pragma unroll
+    for(int k = 0; k < 9; ++k) out.c[k][i] = in.c[k][s];
+}
+
+// APIC mapGrid2Particles (APIC_Solver.cpp:94-115): v = the interpolated grid velocity, C_d = the
+// gradient of component d at gridPos - extra_d. AFLIP (AFLIP_Solver.cpp:94-140): both blended like
+// FLIP, lerp(grid, particle + change, PIC_FLIP_ratio), the changes taken from vel - backup.
+template<int kScheme>
+__global__ void __launch_bounds__(kBlock) k_g2p_affine(Particles P, Affine A, uint32_t n, VelGrid G, OldGrid O, GridGeom g, float ratio) {
+#pragma clang fp contract(off)
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= n) return;
+    const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+    float       v[3], gc[9];
+    vel_from_grid(G, g, gx, gy, gz, v[0], v[1], v[2]);
+    interp3_grad(G.u, nullptr, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, gc[0], gc[1], gc[2]);
+    interp3_grad(G.v, nullptr, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f, g.dx, gc[3], gc[4], gc[5]);
+    interp3_grad(G.w, nullptr, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz, g.dx, gc[6], gc[7], gc[8]);
+    if(kScheme == 1) {
+        P.vx[p] = v[0];
+        P.vy[p] = v[1];
+        P.vz[p] = v[2];
+#pragma unroll
+        for(int k = 0; k < 9; ++k) A.c[k][p] = gc[k];
+        return;
+    }
+    float dc[9];
+    interp3_grad(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, dc[0], dc[1], dc[2]);
+    interp3_grad(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f, g.dx, dc[3], dc[4], dc[5]);
+    interp3_grad(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz, g.dx, dc[6], dc[7], dc[8]);
+    const float dvx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
+    const float dvy = interp3_diff(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f);
+    const float dvz = interp3_diff(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz);
+    const float a   = 1.0f - ratio;
+    P.vx[p]         = a * v[0] + ratio * (P.vx[p] + dvx);
+    P.vy[p]         = a * v[1] + ratio * (P.vy[p] + dvy);
+    P.vz[p]         = a * v[2] + ratio * (P.vz[p] + dvz);
+#pragma unroll
+    for(int k = 0; k < 9; ++k) A.c[k][p] = a * gc[k] + ratio * (A.c[k][p] + dc[k]);
+}
+
 // max |a| over an array (timestepCFL, PIC_Solver.cpp:342-349); result via atomicMax on the bits
 __global__ void __launch_bounds__(kBlock) k_maxabs(const float* __restrict__ a, size_t n, uint32_t* out) {
     __shared__ float sh[kBlock / 64];
@@ -1735,6 +1927,114 @@ __global__ void __launch_bounds__(1024) k_mg_coarsest(LevelDev L, int sweeps, co
 }
 
 // partial z.r for the PCG (after the V-cycle wrote z)
+//==================================================================================================
+// The reference's preconditioner: natural-order incomplete Cholesky IC(0) with the 0.25 pivot
+// safeguard (PCGSolver.cpp:398-466, minDiagonalRatio PCGSolver.h:88-89) and its two triangular
+// solves (PCGSolver.cpp:243-269), level-scheduled over the hyperplanes i + j + k = h. On the 7-point
+// stencil every lower neighbour of a cell (-x, -y, -z) lies on plane h - 1 and every upper neighbour
+// on h + 1, so one launch per plane, in plane order, reproduces the sequential recurrences; each
+// cell applies its neighbours' terms in the reference's order (by ascending natural index: -z, -y,
+// -x going forward; +x, +y, +z coming back), with the same FP32 roundings. 3n - 2 dependent launches
+// per sweep: latency-bound, used to compare iterates with the reference's preconditioner, not for
+// speed. Cells outside the system (solid, air, the outer layer) have a zero diagonal and a zero
+// factor, so they contribute nothing and get z = 0.
+//==================================================================================================
+struct Ic0Dev {
+    const int32_t* brickSlot;
+    const float *  diag, *cx, *cy, *cz;  // A: diagonal and the couplings to +x, +y, +z
+    float *        inv, *lx, *ly, *lz;   // 1/sqrt(pivot); L entries below the diagonal (column-scaled)
+};
+
+__device__ __forceinline__ int32_t sys_cell(const int32_t* __restrict__ brickSlot, const GridGeom& g, int i, int j, int k) {
+    if(i < 0 || j < 0 || k < 0) return -1;
+    const int32_t s = brickSlot[((k >> 3) * g.nby + (j >> 3)) * g.nbx + (i >> 3)];
+    return s < 0 ? -1 : s * 512 + ((k & 7) * 8 + (j & 7)) * 8 + (i & 7);
+}
+
+// the cell (i, j, k = h - i - j) of plane h owned by this thread, or -1
+__device__ __forceinline__ int32_t ic0_plane_cell(const Ic0Dev& D, const GridGeom& g, int h, int& i, int& j, int& k) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= (uint32_t)g.nx * (uint32_t)g.ny) return -1;
+    i = (int)(t % (uint32_t)g.nx);
+    j = (int)(t / (uint32_t)g.nx);
+    k = h - i - j;
+    if(k < 0 || k >= g.nz) return -1;
+    return sys_cell(D.brickSlot, g, i, j, k);
+}
+
+// pivot d = a_oo - L(o,-z)^2 - L(o,-y)^2 - L(o,-x)^2 (the column updates of the lower neighbours,
+// in their processing order), inv = 1/sqrt(d), or 1/sqrt(a_oo) when d < 0.25 a_oo; then the column
+// below the diagonal is scaled: L(+x,o) = a(o,+x) * inv, and likewise +y, +z
+__global__ void __launch_bounds__(kBlock) k_ic0_factor_plane(Ic0Dev D, GridGeom g, int h) {
+#pragma clang fp contract(off)
+    int           i, j, k;
+    const int32_t o = ic0_plane_cell(D, g, h, i, j, k);
+    if(o < 0) return;
+    const float a = D.diag[o];
+    float       inv;
+    if(a < kFltMin) {
+        inv = a;  // row skipped by the factorisation (PCGSolver.cpp:419-421): its couplings are zero
+    } else {
+        const int32_t mz = sys_cell(D.brickSlot, g, i, j, k - 1), my = sys_cell(D.brickSlot, g, i, j - 1, k),
+                      mx = sys_cell(D.brickSlot, g, i - 1, j, k);
+        float d = a;
+        if(mz >= 0) {
+            const float l = D.lz[mz];
+            d -= l * l;
+        }
+        if(my >= 0) {
+            const float l = D.ly[my];
+            d -= l * l;
+        }
+        if(mx >= 0) {
+            const float l = D.lx[mx];
+            d -= l * l;
+        }
+        inv = d < 0.25f * a ? 1.0f / sqrtf(a) : 1.0f / sqrtf(d);
+    }
+    D.inv[o] = inv;
+    const bool skipped = a < kFltMin;  // an unscaled zero column
+    D.lx[o]  = skipped ? D.cx[o] : D.cx[o] * inv;
+    D.ly[o]  = skipped ? D.cy[o] : D.cy[o] * inv;
+    D.lz[o]  = skipped ? D.cz[o] : D.cz[o] * inv;
+}
+
+// forward solve, plane h: y_o = (((r_o - L(o,-z) y_-z) - L(o,-y) y_-y) - L(o,-x) y_-x) * inv_o
+__global__ void __launch_bounds__(kBlock) k_ic0_lower_plane(Ic0Dev D, GridGeom g, int h, const float* __restrict__ r, float* __restrict__ y,
+                                                            const PcgState* __restrict__ st) {
+#pragma clang fp contract(off)
+    if(st && st->done) return;
+    int           i, j, k;
+    const int32_t o = ic0_plane_cell(D, g, h, i, j, k);
+    if(o < 0) return;
+    const int32_t mz = sys_cell(D.brickSlot, g, i, j, k - 1), my = sys_cell(D.brickSlot, g, i, j - 1, k),
+                  mx = sys_cell(D.brickSlot, g, i - 1, j, k);
+    float v = r[o];
+    if(mz >= 0) v -= D.lz[mz] * y[mz];
+    if(my >= 0) v -= D.ly[my] * y[my];
+    if(mx >= 0) v -= D.lx[mx] * y[mx];
+    y[o] = v * D.inv[o];
+}
+
+// backward solve in place, plane h (descending): z_o = (((y_o - L(+x,o) z_+x) - L(+y,o) z_+y) -
+// L(+z,o) z_+z) * inv_o
+__global__ void __launch_bounds__(kBlock) k_ic0_upper_plane(Ic0Dev D, GridGeom g, int h, float* __restrict__ z,
+                                                            const PcgState* __restrict__ st) {
+#pragma clang fp contract(off)
+    if(st && st->done) return;
+    int           i, j, k;
+    const int32_t o = ic0_plane_cell(D, g, h, i, j, k);
+    if(o < 0) return;
+    const int32_t px = i + 1 < g.nx ? sys_cell(D.brickSlot, g, i + 1, j, k) : -1;
+    const int32_t py = j + 1 < g.ny ? sys_cell(D.brickSlot, g, i, j + 1, k) : -1;
+    const int32_t pz = k + 1 < g.nz ? sys_cell(D.brickSlot, g, i, j, k + 1) : -1;
+    float         v  = z[o];
+    if(px >= 0) v -= D.lx[o] * z[px];
+    if(py >= 0) v -= D.ly[o] * z[py];
+    if(pz >= 0) v -= D.lz[o] * z[pz];
+    z[o] = v * D.inv[o];
+}
+
 __global__ void __launch_bounds__(512) k_pcg_dot_zr(const float* __restrict__ z, const float* __restrict__ r, uint32_t nSlots,
                                                     const PcgState* __restrict__ st, double* __restrict__ pa) {
     __shared__ double sh[8];

@@ -13,10 +13,12 @@
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <sys/stat.h>
 #include <string>
 #include <vector>
 
 #include "../../include/bnn_flip.h"
+#include "flip_io.hpp"
 #include "scene_json.hpp"
 
 namespace bnn {
@@ -85,9 +87,17 @@ struct DevBuf {
 
 struct ParticleBufs {
     DevBuf px, py, pz, vx, vy, vz, gx, gy, gz, id;
-    void   reserve(size_t n) {
+    DevBuf c[9];  // APIC / AFLIP affine matrices (allocated only for those schemes)
+    void   reserve(size_t n, bool affine) {
         for(DevBuf* b : {&px, &py, &pz, &vx, &vy, &vz, &gx, &gy, &gz}) b->reserve(n * 4);
         id.reserve(n * 4);
+        if(affine)
+            for(DevBuf& b : c) b.reserve(n * 4);
+    }
+    Affine affine() const {
+        Affine a;
+        for(int k = 0; k < 9; ++k) a.c[k] = c[k].as<float>();
+        return a;
     }
     Particles view() const {
         return Particles{px.as<float>(), py.as<float>(), pz.as<float>(), vx.as<float>(), vy.as<float>(),
@@ -128,7 +138,7 @@ static GridDims grid_dims(const bnn_flip_params& p) {
 }
 
 static uint64_t generate_scene(const bnn_flip_params& prm, const bnn_scene_spec& sp, uint64_t seed, float* pos, float* vel,
-                               uint64_t cap) {
+                               uint64_t cap, uint16_t* obj = nullptr) {
     const GridDims G   = grid_dims(prm);
     const int      ppa = sp.ppa > 0 ? sp.ppa : 2;
     const double   h   = (double)G.dx / ppa;
@@ -185,6 +195,7 @@ static uint64_t generate_scene(const bnn_flip_params& prm, const bnn_scene_spec&
                             vel[count * 3 + d] = (float)v0;
                         }
                     }
+                    if(obj && pos && count < cap) obj[count] = (uint16_t)o;  // generator index (ParticleSolverData.h objectIndex)
                     ++count;
                 }
     }
@@ -225,10 +236,17 @@ public:
     DevBuf   x, r, z, s0, s1, q, rhs32, diag32, cx32, cy32, cz32, invd;
     DevBuf   rhs64, diag64, cx64, cy64, cz64, x64;
     DevBuf   pa, pb, pz, pst, ost;  // PCG partials (per block: pa, pb; per brick: pz), device states
+    DevBuf   icInv, icLx, icLy, icLz;  // IC(0) factor (BNN_PRECOND_IC0 only)
     PcgState*   hostSt  = nullptr;  // pinned
     OuterState* hostOst = nullptr;
     uint32_t*   hostU32 = nullptr;
     int         device  = 0;  // the device current at creation (owns the stream and every buffer)
+    // ParticleSimulationData::objectIndex / nObjects: the generator of each particle, in user order
+    // (empty: every particle 0)
+    std::vector<uint16_t> objIndex;
+    uint32_t              nObjects = 1;
+    std::vector<uint16_t> objScratch;  // same size as objIndex: the z-sort's reordering (no allocation there)
+    std::vector<uint32_t> rankHost;
     bool        systemValid = false;
     bool        lastRefined = false;
 
@@ -299,9 +317,12 @@ public:
             if(!(prm.box_max[d] > prm.box_min[d])) throw Error(BNN_ERR_INVALID_ARG, "SimulationDomainBox: BoxMax must exceed BoxMin");
         if(prm.advection_steps < 1) throw Error(BNN_ERR_INVALID_ARG, "AdvectionSteps must be >= 1");
         if(prm.enable_sort_particle && prm.sort_frequency < 1) throw Error(BNN_ERR_INVALID_ARG, "SortFrequency must be >= 1");
-        if(prm.correct_position) throw Error(BNN_ERR_UNSUPPORTED, "CorrectPosition (PIC_Solver.cpp:377-485) is not implemented");
         if(prm.gravity_type < 0 || prm.gravity_type > 3) throw Error(BNN_ERR_INVALID_ARG, "GravityType must be 0..3");
-        if(prm.preconditioner != BNN_PRECOND_JACOBI && prm.preconditioner != BNN_PRECOND_MG)
+        if(prm.transfer_scheme < BNN_TRANSFER_FLIP || prm.transfer_scheme > BNN_TRANSFER_AFLIP)
+            throw Error(BNN_ERR_INVALID_ARG, "unknown transfer scheme");
+        if(!(prm.refine_inner_tolerance > 0.f && prm.refine_inner_tolerance < 1.f))
+            throw Error(BNN_ERR_INVALID_ARG, "refine_inner_tolerance must be in (0, 1)");
+        if(prm.preconditioner != BNN_PRECOND_JACOBI && prm.preconditioner != BNN_PRECOND_MG && prm.preconditioner != BNN_PRECOND_IC0)
             throw Error(BNN_ERR_INVALID_ARG, "unknown preconditioner");
         if(prm.preconditioner == BNN_PRECOND_MG && (prm.mg_sweeps < 1 || prm.mg_sweeps > 8 || prm.mg_coarsest_sweeps < 1 ||
                                                     prm.mg_coarsest_sweeps > 512 || !(prm.mg_omega > 0.f) || prm.mg_omega >= 2.f))
@@ -351,8 +372,8 @@ public:
     void ensureParticles(uint64_t n) {
         if(n <= cap) return;
         const uint64_t c = n + n / 8 + 1024;
-        bufA.reserve(c);
-        bufB.reserve(c);
+        bufA.reserve(c, affineScheme());
+        bufB.reserve(c, affineScheme());
         key.reserve(c * 4);
         perm.reserve(c * 4);
         cap = c;
@@ -368,6 +389,8 @@ public:
         pa.reserve((size_t)kMaxRedBlocks * 8);
         pb.reserve((size_t)kMaxRedBlocks * 8);
         pz.reserve((size_t)c * 8);
+        if(prm.preconditioner == BNN_PRECOND_IC0)
+            for(auto* b : {&icInv, &icLx, &icLy, &icLz}) b->reserve(nv * 4);
         if(prm.pressure_mode == BNN_PRESSURE_REFINED)
             for(auto* b : {&rhs64, &diag64, &cx64, &cy64, &cz64, &x64}) b->reserve(nv * 8);
         else
@@ -395,10 +418,53 @@ public:
         }
         for(uint64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
         if(n) HIP_OK(hipMemcpyAsync(P.id.p, ids.data(), n * 4, hipMemcpyHostToDevice, stream));
+        if(affineScheme() && n)  // new particles start with C = 0 (APIC_Data: the matrices are zero-filled)
+            for(DevBuf& b : P.c) HIP_OK(hipMemsetAsync(b.p, 0, n * 4, stream));
         HIP_OK(hipMemsetAsync(cellCount.p, 0, cellCount.bytes, stream));
         HIP_OK(hipStreamSynchronize(stream));
         np     = n;
         binned = false;
+        objIndex.clear();
+        objScratch.clear();
+        rankHost.clear();
+        nObjects = 1;
+    }
+    void setObjectIndices(std::vector<uint16_t>&& obj, uint32_t nObj) {
+        if(obj.size() != np) throw Error(BNN_ERR_INVALID_ARG, "object_index has the wrong particle count");
+        objIndex.swap(obj);
+        objScratch.assign(np, 0);
+        rankHost.assign(np, 0);
+        nObjects = nObj;
+    }
+
+    bool affineScheme() const { return prm.transfer_scheme != BNN_TRANSFER_FLIP; }
+
+    // the affine matrices in the caller's order, 9 floats per particle (row d = C_d)
+    void getAffine(float* c) {
+        if(!affineScheme()) throw Error(BNN_ERR_STATE, "the transfer scheme has no affine matrices (FLIP)");
+        if(!np) return;
+        std::vector<uint32_t> ids(np);
+        std::vector<float>    a(np);
+        HIP_OK(hipMemcpyAsync(ids.data(), cur->id.p, np * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for(int k = 0; k < 9; ++k) {
+            HIP_OK(hipMemcpyAsync(a.data(), cur->c[k].p, np * 4, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            for(uint64_t i = 0; i < np; ++i) c[9 * (uint64_t)ids[i] + k] = a[i];
+        }
+    }
+    void setAffine(const float* c) {
+        if(!affineScheme()) throw Error(BNN_ERR_STATE, "the transfer scheme has no affine matrices (FLIP)");
+        if(!np) return;
+        std::vector<uint32_t> ids(np);
+        std::vector<float>    a(np);
+        HIP_OK(hipMemcpyAsync(ids.data(), cur->id.p, np * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for(int k = 0; k < 9; ++k) {
+            for(uint64_t i = 0; i < np; ++i) a[i] = c[9 * (uint64_t)ids[i] + k];
+            HIP_OK(hipMemcpyAsync(cur->c[k].p, a.data(), np * 4, hipMemcpyHostToDevice, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+        }
     }
 
     void getParticles(float* pos, float* vel) {
@@ -496,8 +562,35 @@ public:
     void moveParticles(float dt) {
         if(!np) return;
         VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
+        ktBegin(BNN_KT_MOVE);
         hipLaunchKernelGGL(k_move, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, g, dom, dt,
                            prm.advection_steps, prm.reflect_velocity_at_boundary, prm.boundary_reflection_multiplier);
+        ktEnd(BNN_KT_MOVE);
+        binned = false;
+    }
+
+    uint64_t correctSeed() const { return mix64(prm.random_seed ^ mix64(st.steps + 1)); }
+
+    // PIC_Solver::correctParticlePositions (PIC_Solver.cpp:377-485): bins the current positions,
+    // writes the pushed positions to the spare buffer and copies them back
+    void correctParticlePositions(float dt) {
+        if(!np) return;
+        requireBinned();
+        const float   pr     = prm.cell_size / prm.ratio_cell_size_pradius;  // ParticleSolverData.cpp:232
+        const float   radius = 2.0f * pr / (float)std::sqrt(3);
+        CorrectParams c;
+        c.radius2    = radius * radius;
+        c.threshold  = 0.01f * radius;
+        c.threshold2 = c.threshold * c.threshold;
+        c.push       = c.threshold / dt;
+        c.K          = radius * (dt / (float)prm.advection_steps) * prm.repulsive_force_stiffness;
+        c.advSteps   = prm.advection_steps;
+        c.seed       = correctSeed();
+        hipLaunchKernelGGL(k_correct_positions, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
+                           (uint32_t)np, g, dom, c, alt->px.as<float>(), alt->py.as<float>(), alt->pz.as<float>());
+        HIP_OK(hipMemcpyAsync(cur->px.p, alt->px.p, np * 4, hipMemcpyDeviceToDevice, stream));
+        HIP_OK(hipMemcpyAsync(cur->py.p, alt->py.p, np * 4, hipMemcpyDeviceToDevice, stream));
+        HIP_OK(hipMemcpyAsync(cur->pz.p, alt->pz.p, np * 4, hipMemcpyDeviceToDevice, stream));
         binned = false;
     }
 
@@ -516,8 +609,13 @@ public:
                            cellStart.as<uint32_t>(), cellCount.as<uint32_t>(), perm.as<uint32_t>());
         hipLaunchKernelGGL(k_segsort, dim3(blocks(nC)), dim3(kBlock), 0, stream, cellStart.as<uint32_t>(), (uint32_t)nC,
                            perm.as<uint32_t>());
+        ktBegin(BNN_KT_PERMUTE);
         hipLaunchKernelGGL(k_permute, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), alt->view(), perm.as<uint32_t>(),
                            (uint32_t)np, g);
+        if(affineScheme())
+            hipLaunchKernelGGL(k_permute_affine, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->affine(), alt->affine(),
+                               (const uint32_t*)perm.as<uint32_t>(), (uint32_t)np);
+        ktEnd(BNN_KT_PERMUTE);
         std::swap(cur, alt);
         binned = true;
     }
@@ -532,8 +630,12 @@ public:
         FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
         const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (((size_t)g.nz + 1 + kP2gSeg - 1) / kP2gSeg);  // columns x segments
         ktBegin(BNN_KT_P2G);
-        hipLaunchKernelGGL(k_p2g_col, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
-                           g, out);
+        if(affineScheme())  // APIC_Solver.cpp:52-91, AFLIP_Solver.cpp:48-90
+            hipLaunchKernelGGL(k_p2g_col<true>, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), cur->affine(),
+                               (const uint32_t*)cellStart.as<uint32_t>(), g, out);
+        else
+            hipLaunchKernelGGL(k_p2g_col<false>, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), Affine{},
+                               (const uint32_t*)cellStart.as<uint32_t>(), g, out);
         ktEnd(BNN_KT_P2G);
     }
 
@@ -550,9 +652,12 @@ public:
             hipLaunchKernelGGL(k_extrap_brick_valid, dim3(nbk), dim3(256), 0, stream, (const uint8_t*)c.val->as<uint8_t>(), c.sx, c.sy,
                                c.sz, has);
             hipLaunchKernelGGL(k_extrap_brick_near, dim3(blocks(nbk)), dim3(kBlock), 0, stream, (const uint8_t*)has, nbx, nby, nbz, near);
-            for(int layer = 0; layer < 4; ++layer)
+            for(int layer = 0; layer < 4; ++layer) {
+                ktBegin(BNN_KT_EXTRAP);
                 hipLaunchKernelGGL(k_extrap_stamp, dim3((c.sx + 63) / 64, (c.sy + 3) / 4, c.sz), dim3(64, 4), 0, stream, c.a->as<float>(),
                                    c.val->as<uint8_t>(), c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer);
+                ktEnd(BNN_KT_EXTRAP);
+            }
             hipLaunchKernelGGL(k_extrap_finish, dim3(blocks(n)), dim3(kBlock), 0, stream, c.val->as<uint8_t>(), n);
         }
     }
@@ -746,26 +851,58 @@ public:
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
+            ktBegin(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
             hipLaunchKernelGGL((k_mg_smooth_brick<0, 0>), dim3(G), dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
+            ktEnd(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
+            ktBegin(l == 0 ? BNN_KT_MG_RESTRICT : BNN_KT_MG_COARSE);
             hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(256), 0, stream, L, C, stp);
+            ktEnd(l == 0 ? BNN_KT_MG_RESTRICT : BNN_KT_MG_COARSE);
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
+        ktBegin(BNN_KT_MG_COARSE);
         if(Lc.nSlots == 1) {
             hipLaunchKernelGGL(k_mg_coarsest1, dim3(1), dim3(64), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
         } else {
             hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(1024), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
         }
+        ktEnd(BNN_KT_MG_COARSE);
         for(int l = mgLevels - 2; l >= 0; --l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const dim3     G(std::max<uint32_t>(L.nSlots, 1u));
             if(l == 0) {  // finest level: correction as its own streaming pass, then smooth and form z.r
+                ktBegin(BNN_KT_MG_PROLONG);
                 hipLaunchKernelGGL(k_mg_prolong, G, dim3(256), 0, stream, L, C, omega, stp);
+                ktEnd(BNN_KT_MG_PROLONG);
+                ktBegin(BNN_KT_MG_POST);
                 hipLaunchKernelGGL((k_mg_smooth_brick<2, 1>), G, dim3(256), 0, stream, L, C, omega, nu, stp, pz.as<double>());
+                ktEnd(BNN_KT_MG_POST);
             } else {
+                ktBegin(BNN_KT_MG_COARSE);
                 hipLaunchKernelGGL((k_mg_smooth_brick<1, 0>), G, dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
+                ktEnd(BNN_KT_MG_COARSE);
             }
         }
         return mgLevels > 1;
+    }
+
+    // IC(0) of the reference, one launch per hyperplane i + j + k = h (h = 0 .. nx + ny + nz - 3)
+    Ic0Dev ic0Dev() {
+        return Ic0Dev{brickSlot.as<int32_t>(), diag32.as<float>(), cx32.as<float>(), cy32.as<float>(), cz32.as<float>(),
+                      icInv.as<float>(), icLx.as<float>(), icLy.as<float>(), icLz.as<float>()};
+    }
+    int ic0Planes() const { return g.nx + g.ny + g.nz - 2; }
+    void formIC0() {  // PCGSolver.cpp:398-466 (formPreconditioner at the start of every solve)
+        const Ic0Dev   D  = ic0Dev();
+        const unsigned nb = blocks((size_t)g.nx * g.ny);
+        for(int h = 0; h < ic0Planes(); ++h) hipLaunchKernelGGL(k_ic0_factor_plane, dim3(nb), dim3(kBlock), 0, stream, D, g, h);
+    }
+    // z = (L L^T)^-1 r (PCGSolver.cpp:243-269): forward planes ascending into z, then backward in place
+    void applyIC0(const float* rv, float* zv, const PcgState* stp) {
+        const Ic0Dev   D  = ic0Dev();
+        const unsigned nb = blocks((size_t)g.nx * g.ny);
+        const int      H  = ic0Planes();
+        for(int h = 0; h < H; ++h) hipLaunchKernelGGL(k_ic0_lower_plane, dim3(nb), dim3(kBlock), 0, stream, D, g, h, rv, zv, stp);
+        for(int h = H - 1; h >= 0; --h) hipLaunchKernelGGL(k_ic0_upper_plane, dim3(nb), dim3(kBlock), 0, stream, D, g, h, zv, stp);
     }
 
     unsigned redBlocks() const { return std::min<unsigned>(std::max<uint32_t>(nSlots, 1u), (unsigned)kMaxRedBlocks); }
@@ -783,8 +920,14 @@ public:
         // partials of z.r: per block (G) from k_pcg_init / k_pcg_dot_zr, or per brick from the V-cycle
         const double* rz  = dpa;
         int           nrz = (int)G;
+        const bool    ic0 = prm.preconditioner == BNN_PRECOND_IC0;
+        if(ic0) formIC0();
         auto          zr  = [&](const PcgState* stp) {
-            if(vcycle(stp)) {
+            if(ic0) {
+                applyIC0(r.as<float>(), z.as<float>(), stp);
+                hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(),
+                                   nSlots, stp, dpa);
+            } else if(vcycle(stp)) {
                 rz  = pz.as<double>();
                 nrz = (int)nSlots;
             } else {
@@ -851,7 +994,7 @@ public:
         }
         // FP64 defect correction around FP32 PCG: r = b - A x in FP64, solve A d = r in FP32, x += d
         const double   tol      = prm.refine_tolerance > 0 ? prm.refine_tolerance : 1e-10;
-        const float    innerTol = 1e-4f;
+        const float    innerTol = prm.refine_inner_tolerance;
         const uint32_t maxOuter = 30;
         const unsigned G        = redBlocks();
         uint32_t       total = 0, outer = 0;
@@ -940,16 +1083,41 @@ public:
         requireBinned();
         VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
         OldGrid O{uo.as<float>(), vo.as<float>(), wo.as<float>()};
-        hipLaunchKernelGGL(k_g2p, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, O, g, prm.pic_flip_ratio);
+        ktBegin(BNN_KT_G2P);
+        if(prm.transfer_scheme == BNN_TRANSFER_APIC)
+            hipLaunchKernelGGL(k_g2p_affine<1>, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), cur->affine(), (uint32_t)np, G, O, g,
+                               prm.pic_flip_ratio);
+        else if(prm.transfer_scheme == BNN_TRANSFER_AFLIP)
+            hipLaunchKernelGGL(k_g2p_affine<2>, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), cur->affine(), (uint32_t)np, G, O, g,
+                               prm.pic_flip_ratio);
+        else
+            hipLaunchKernelGGL(k_g2p, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, O, g, prm.pic_flip_ratio);
+        ktEnd(BNN_KT_G2P);
     }
 
-    bool advanceVelocity(float dt) {  // FLIP_Solver.cpp:42-53
+    bool advanceVelocity(float dt) {  // FLIP_Solver.cpp:42-53 (FLIP, AFLIP); APIC_Solver.cpp:41-48 (APIC)
         stageBegin(BNN_STAGE_BIN);
         requireBinned();
         stageEnd(BNN_STAGE_BIN);
         stageBegin(BNN_STAGE_P2G);
         mapParticles2Grid();
         stageEnd(BNN_STAGE_P2G);
+        if(prm.transfer_scheme == BNN_TRANSFER_APIC) {  // no extrapolation, constrain or backup before the solve
+            stageBegin(BNN_STAGE_CONSTRAIN);
+            addGravity(dt);
+            stageEnd(BNN_STAGE_CONSTRAIN);
+            bool ok = pressureProjection(dt);
+            stageBegin(BNN_STAGE_EXTRAPOLATE);
+            extrapolateVelocity();
+            stageEnd(BNN_STAGE_EXTRAPOLATE);
+            stageBegin(BNN_STAGE_CONSTRAIN);
+            constrainGridVelocity();
+            stageEnd(BNN_STAGE_CONSTRAIN);
+            stageBegin(BNN_STAGE_G2P);
+            mapGrid2Particles();
+            stageEnd(BNN_STAGE_G2P);
+            return ok;
+        }
         stageBegin(BNN_STAGE_EXTRAPOLATE);
         extrapolateVelocity();
         stageEnd(BNN_STAGE_EXTRAPOLATE);
@@ -994,7 +1162,18 @@ public:
         st.stage_bytes[BNN_STAGE_WEIGHTS]     = 0;
         st.stage_bytes[BNN_STAGE_PHI]         = 12 * Np + 13 * C + 4 * V;
         st.stage_bytes[BNN_STAGE_ASSEMBLY]    = 9 * C + 4 * C + 8 * F + 24 * A;
-        st.stage_bytes[BNN_STAGE_PCG]         = 64 * A * st.pcg_iterations;
+        // PCG per inner iteration: SpMV 32 B and update 24 B per fluid cell, plus the preconditioner:
+        // Jacobi 8 B (fused into the update); the V-cycle's four finest passes 89 B (pre-smoother 24,
+        // residual-restriction 24.5, correction 12.5, post-smoother 28) and the coarser levels about
+        // 89/7 B (each level 1/8 of the one above), once per iteration and once per inner solve; the
+        // two IC(0) sweeps 2 x 24 B. FP64 defect correction: 72 B per outer pass.
+        const double it = st.pcg_iterations, outer = st.outer_iterations;
+        double       pcgB = 56 * A * it;
+        if(prm.preconditioner == BNN_PRECOND_JACOBI) pcgB += 8 * A * it;
+        else if(prm.preconditioner == BNN_PRECOND_MG) pcgB += (89 + 89.0 / 7) * A * (it + outer);
+        else pcgB += 48 * A * (it + outer);
+        if(lastRefined) pcgB += 72 * A * (outer + 1);
+        st.stage_bytes[BNN_STAGE_PCG]         = pcgB;
         st.stage_bytes[BNN_STAGE_UPDATE]      = 4 * C + 4 * A + 13 * F;
         st.stage_bytes[BNN_STAGE_G2P]         = 36 * Np + 8 * F + 12 * F;
         st.stage_bytes[BNN_STAGE_OTHER]       = 0;
@@ -1011,6 +1190,11 @@ public:
         stageBegin(BNN_STAGE_MOVE);
         moveParticles(dt);
         stageEnd(BNN_STAGE_MOVE);
+        if(prm.correct_position) {  // PIC_Solver.cpp:295 (after moveParticles, before advanceVelocity)
+            stageBegin(BNN_STAGE_OTHER);
+            correctParticlePositions(dt);
+            stageEnd(BNN_STAGE_OTHER);
+        }
         stageBegin(BNN_STAGE_BIN);
         binParticles();
         stageEnd(BNN_STAGE_BIN);
@@ -1115,7 +1299,57 @@ public:
         uint32_t* rank = alt->gx.as<uint32_t>();
         hipLaunchKernelGGL(k_zsort_rank, dim3(blocks(np)), dim3(kBlock), 0, stream, (const uint32_t*)val[s], n, rank);
         hipLaunchKernelGGL(k_zsort_rename, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->id.as<uint32_t>(), n, (const uint32_t*)rank);
+        if(objIndex.size() == np && rankHost.size() == np && objScratch.size() == np) {
+            // the user order changed: rank[old user index] = new user index
+            HIP_OK(hipMemcpyAsync(rankHost.data(), rank, np * 4, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            for(uint64_t i = 0; i < np; ++i) objScratch[rankHost[i]] = objIndex[i];
+            objIndex.swap(objScratch);
+        }
         HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // positions in the brick layout of the active cells, in the reference's activeCellIdx order
+    std::vector<int64_t> activeOffsets() {
+        std::vector<float>   ph(nC);
+        std::vector<int32_t> bs(nb);
+        HIP_OK(hipMemcpyAsync(ph.data(), phi.p, nC * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(bs.data(), brickSlot.p, nb * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        std::vector<int64_t> off;
+        for(int k = 0; k < g.nz; ++k)
+            for(int j = 0; j < g.ny; ++j)
+                for(int i = 0; i < g.nx; ++i) {
+                    if(!(ph[((size_t)k * g.ny + j) * g.nx + i] < 0)) continue;
+                    const int32_t s = bs[((k >> 3) * g.nby + (j >> 3)) * g.nbx + (i >> 3)];
+                    off.push_back(s < 0 ? -1 : (int64_t)s * 512 + ((k & 7) * 8 + (j & 7)) * 8 + (i & 7));
+                }
+        return off;
+    }
+
+    // z = M^-1 r on the current system, r and z in activeCellIdx order; uses the PCG's r, z, q
+    void applyPreconditioner(const double* rin, double* zout) {
+        const std::vector<int64_t> off = activeOffsets();
+        if(!nSlots) {
+            for(size_t c = 0; c < off.size(); ++c) zout[c] = 0;
+            return;
+        }
+        std::vector<float> rv((size_t)nSlots * 512, 0.f), zv((size_t)nSlots * 512);
+        for(size_t c = 0; c < off.size(); ++c)
+            if(off[c] >= 0) rv[off[c]] = (float)rin[c];
+        HIP_OK(hipMemcpyAsync(r.p, rv.data(), rv.size() * 4, hipMemcpyHostToDevice, stream));
+        if(prm.preconditioner == BNN_PRECOND_IC0) {
+            formIC0();
+            applyIC0(r.as<float>(), z.as<float>(), nullptr);
+        } else if(prm.preconditioner == BNN_PRECOND_MG) {
+            vcycle(nullptr);  // writes z (with a single level, the coarsest solve alone does)
+        } else {
+            hipLaunchKernelGGL(k_mul_arrays, dim3(blocks(rv.size())), dim3(kBlock), 0, stream, (const float*)invd.as<float>(),
+                               (const float*)r.as<float>(), z.as<float>(), rv.size());
+        }
+        HIP_OK(hipMemcpyAsync(zv.data(), z.p, zv.size() * 4, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for(size_t c = 0; c < off.size(); ++c) zout[c] = off[c] >= 0 ? (double)zv[off[c]] : 0.0;
     }
 
     // readback of a per-cell pressure-system vector in the reference's activeCellIdx order
@@ -1192,6 +1426,127 @@ public:
         }
         HIP_OK(hipMemcpyAsync(dst->p, src, bytes, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // frame output and memory states in the reference's BananaParticleData files
+    float particleRadius() const { return prm.cell_size / prm.ratio_cell_size_pradius; }  // ParticleSolverData.cpp:232
+
+    // ParticleHelpers::compress (ParticleHelpers.cpp:47-79): per-component min / max over the
+    // particles, then (uint16)(65535 * ((v - min) / (max - min))) in float; the attribute's bytes are
+    // the 3 minima, the 3 maxima (float) and the 3 uint16 of every particle
+    static std::vector<uint8_t> compress3(const std::vector<float>& v, uint64_t n) {
+        float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+        for(int d = 0; d < 3 && n; ++d) {
+            mn[d] = mx[d] = v[d];
+            for(uint64_t i = 1; i < n; ++i) {
+                mn[d] = std::min(mn[d], v[3 * i + d]);
+                mx[d] = std::max(mx[d], v[3 * i + d]);
+            }
+        }
+        std::vector<uint8_t> out(24 + 6 * n);
+        std::memcpy(out.data(), mn, 12);
+        std::memcpy(out.data() + 12, mx, 12);
+        uint16_t* q = reinterpret_cast<uint16_t*>(out.data() + 24);
+        for(int d = 0; d < 3; ++d) {
+            const float diff = mx[d] - mn[d];
+            for(uint64_t i = 0; i < n; ++i) {
+                // a component with max == min has no spread to encode (the reference divides by zero)
+                const float t = diff > 0.f ? 65535.0f * ((v[3 * i + d] - mn[d]) / diff) : 0.f;
+                q[3 * i + d]  = (uint16_t)t;
+            }
+        }
+        return out;
+    }
+
+    std::vector<uint16_t> objectIndices() const { return objIndex.size() == np ? objIndex : std::vector<uint16_t>(np, 0); }
+
+    // PIC_Solver::saveFrameData (PIC_Solver.cpp:232-262): particle radius, compressed positions and,
+    // as the OptionalSavingData keys ask, compressed velocities and the object indices
+    void saveFrameData(const std::string& path, bool withVelocity, bool withObjectIndex) {
+        BpdFile f;
+        f.nParticles     = np;
+        const float rad  = particleRadius();
+        f.addFixed<float>("particle_radius", "real", 1, &rad);
+        std::vector<float> pos(3 * np), vel(withVelocity ? 3 * np : 0);
+        getParticles(pos.data(), withVelocity ? vel.data() : nullptr);
+        f.addRaw(false, "particle_position", "compressed_real", 4, 3, compress3(pos, np));
+        if(withVelocity) f.addRaw(false, "particle_velocity", "compressed_real", 4, 3, compress3(vel, np));
+        if(withObjectIndex) {
+            const std::vector<uint16_t> obj = objectIndices();
+            f.addFixed<uint32_t>("NObjects", "uint", 1, &nObjects);
+            f.addParticle<uint16_t>("object_index", "uint16", 1, obj.data());
+        }
+        std::string err;
+        if(!bpd_write(path, std::move(f), err)) throw Error(BNN_ERR_INVALID_ARG, err);
+    }
+
+    // PIC_Solver::saveMemoryState (PIC_Solver.cpp:206-228): grid resolution, the three face-velocity
+    // grids, particle radius, object count; per particle: object index, position, velocity
+    void saveMemoryState(const std::string& path) {
+        BpdFile f;
+        f.nParticles          = np;
+        const uint32_t res[3] = {(uint32_t)g.nx, (uint32_t)g.ny, (uint32_t)g.nz};
+        f.addFixed<uint32_t>("grid_resolution", "uint", 3, res);
+        const DevBuf* vg[3]   = {&u, &v, &w};
+        const size_t  cnt[3]  = {nU, nV, nW};
+        for(int d = 0; d < 3; ++d) {
+            std::vector<float> a(cnt[d]);
+            HIP_OK(hipMemcpyAsync(a.data(), vg[d]->p, cnt[d] * 4, hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            f.addFixed<float>("grid_velocity_" + std::to_string(d), "real", (uint32_t)cnt[d], a.data());
+        }
+        const float rad = particleRadius();
+        f.addFixed<uint32_t>("NObjects", "uint", 1, &nObjects);
+        f.addFixed<float>("particle_radius", "real", 1, &rad);
+        std::vector<float> pos(3 * np), vel(3 * np);
+        getParticles(pos.data(), vel.data());
+        const std::vector<uint16_t> obj = objectIndices();
+        f.addParticle<uint16_t>("object_index", "uint16", 1, obj.data());
+        f.addParticle<float>("particle_position", "real", 3, pos.data());
+        f.addParticle<float>("particle_velocity", "real", 3, vel.data());
+        std::string err;
+        if(!bpd_write(path, std::move(f), err)) throw Error(BNN_ERR_INVALID_ARG, err);
+    }
+
+    // PIC_Solver::loadMemoryState (PIC_Solver.cpp:158-202) from one file; `frame` becomes the
+    // finished frame, as the reference sets it to the loaded state's index
+    void loadMemoryState(const std::string& path, uint32_t frame) {
+        BpdFile     f;
+        std::string err;
+        if(!bpd_read(path, f, err)) throw Error(BNN_ERR_INVALID_ARG, err);
+        auto need = [&](const char* name, size_t bytes) -> const BpdAttr& {
+            const BpdAttr* a = f.find(name);
+            if(!a) throw Error(BNN_ERR_INVALID_ARG, path + ": no attribute " + name);
+            if(a->data.size() != bytes)
+                throw Error(BNN_ERR_INVALID_ARG, path + ": attribute " + name + " has " + std::to_string(a->data.size()) + " bytes, expected " +
+                                                     std::to_string(bytes));
+            return *a;
+        };
+        uint32_t res[3];
+        std::memcpy(res, need("grid_resolution", 12).data.data(), 12);
+        if(res[0] != (uint32_t)g.nx || res[1] != (uint32_t)g.ny || res[2] != (uint32_t)g.nz)
+            throw Error(BNN_ERR_INVALID_ARG, path + ": grid resolution differs from the scene's");
+        float rad;
+        std::memcpy(&rad, need("particle_radius", 4).data.data(), 4);
+        if(!(std::fabs(rad - particleRadius()) < 1.1920929e-07f))  // __BNN_REQUIRE_APPROX_NUMBERS(.., MEpsilon)
+            throw Error(BNN_ERR_INVALID_ARG, path + ": particle radius differs from the scene's");
+        uint32_t nobj;
+        std::memcpy(&nobj, need("NObjects", 4).data.data(), 4);
+        const uint64_t n   = f.nParticles;
+        const BpdAttr& pa_ = need("particle_position", 12 * n);
+        const BpdAttr& va_ = need("particle_velocity", 12 * n);
+        const BpdAttr& oa_ = need("object_index", 2 * n);
+        const BpdAttr* gv[3] = {&need("grid_velocity_0", nU * 4), &need("grid_velocity_1", nV * 4), &need("grid_velocity_2", nW * 4)};
+        setParticles(reinterpret_cast<const float*>(pa_.data.data()), reinterpret_cast<const float*>(va_.data.data()), n);
+        std::vector<uint16_t> obj(n);
+        if(n) std::memcpy(obj.data(), oa_.data.data(), 2 * n);
+        setObjectIndices(std::move(obj), nobj);
+        setGrid(BNN_FIELD_U, gv[0]->data.data());
+        setGrid(BNN_FIELD_V, gv[1]->data.data());
+        setGrid(BNN_FIELD_W, gv[2]->data.data());
+        finishedFrame    = frame;
+        st.finished_frame = frame;
     }
 
     void getBins(uint32_t* permOut, uint32_t* startOut) {
@@ -1292,6 +1647,8 @@ void bnn_flip_default_params(bnn_flip_params* p) {
     p->pressure_mode                  = BNN_PRESSURE_FP32;
     p->refine_tolerance               = 1e-10;
     p->preconditioner                 = BNN_PRECOND_MG;
+    p->repulsive_force_stiffness      = 50.f;  // ParticleSolverData.h:143
+    p->refine_inner_tolerance         = 1e-4f;
     p->timing                         = 0;
     p->mg_sweeps                      = 4;
     p->mg_omega                       = 1.8f;
@@ -1408,10 +1765,12 @@ bnn_status bnn_generate_scene_host(const bnn_flip_params* p, const bnn_scene_spe
 bnn_status bnn_flip_generate_scene(bnn_flip* h, const bnn_scene_spec* spec, uint64_t seed) {
     if(!spec) return BNN_ERR_INVALID_ARG;
     return guard(h, [&](bnn::FlipSolver& s) {
-        const uint64_t     n = bnn::generate_scene(s.prm, *spec, seed, nullptr, nullptr, 0);
-        std::vector<float> pos(n * 3), vel(n * 3);
-        bnn::generate_scene(s.prm, *spec, seed, pos.data(), vel.data(), n);
+        const uint64_t        n = bnn::generate_scene(s.prm, *spec, seed, nullptr, nullptr, 0);
+        std::vector<float>    pos(n * 3), vel(n * 3);
+        std::vector<uint16_t> obj(n);
+        bnn::generate_scene(s.prm, *spec, seed, pos.data(), vel.data(), n, obj.data());
         s.setParticles(pos.data(), vel.data(), n);
+        s.setObjectIndices(std::move(obj), (uint32_t)std::max(0, std::min(spec->n_boxes, 8)) + (uint32_t)std::max(0, std::min(spec->n_spheres, 8)));
         s.sortParticles();  // PIC_Solver::generateParticles sorts the new particles (PIC_Solver.cpp:57)
         return BNN_OK;
     });
@@ -1450,6 +1809,30 @@ bnn_status bnn_flip_timestep_cfl(bnn_flip* h, float* dt) {
     if(!dt) return BNN_ERR_INVALID_ARG;
     return guard(h, [&](bnn::FlipSolver& s) {
         *dt = s.timestepCFL();
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_affine(bnn_flip* h, float* c) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!c && s.np) throw bnn::Error(BNN_ERR_INVALID_ARG, "null destination");
+        s.getAffine(c);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_set_affine(bnn_flip* h, const float* c) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!c && s.np) throw bnn::Error(BNN_ERR_INVALID_ARG, "null source");
+        s.setAffine(c);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_correct_particle_positions(bnn_flip* h, float dt) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!(dt > 0)) throw bnn::Error(BNN_ERR_INVALID_ARG, "dt must be > 0");
+        if(s.prm.correct_position) s.correctParticlePositions(dt);
         return BNN_OK;
     });
 }
@@ -1568,6 +1951,112 @@ bnn_status bnn_flip_get_rhs(bnn_flip* h, double* dst, uint64_t* n_active) {
     return guard(h, [&](bnn::FlipSolver& s) {
         if(!s.systemValid) throw bnn::Error(BNN_ERR_STATE, "no pressure system yet");
         s.gatherActive<double>(s.rhs64, dst, n_active);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_apply_preconditioner(bnn_flip* h, const double* r, double* z) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!r || !z) throw bnn::Error(BNN_ERR_INVALID_ARG, "null vector");
+        if(!s.systemValid) throw bnn::Error(BNN_ERR_STATE, "no pressure system yet");
+        s.applyPreconditioner(r, z);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_ic0_factor(bnn_flip* h, double* dst, uint64_t* n_active) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(s.prm.preconditioner != BNN_PRECOND_IC0) throw bnn::Error(BNN_ERR_STATE, "the preconditioner is not IC(0)");
+        if(!s.systemValid) throw bnn::Error(BNN_ERR_STATE, "no pressure system yet");
+        if(s.nSlots) s.formIC0();
+        s.gatherActive<float>(s.icInv, dst, n_active);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_data_file_path(const char* data_path, const char* folder, int32_t frame, char* out, size_t cap) {
+    if(!data_path || !folder || !out || !cap) return BNN_ERR_INVALID_ARG;
+    const std::string p = bnn::bpd_file_path(data_path, folder, frame);
+    if(p.size() + 1 > cap) return BNN_ERR_INVALID_ARG;
+    std::memcpy(out, p.c_str(), p.size() + 1);
+    return BNN_OK;
+}
+
+int32_t bnn_latest_state_index(const char* data_path, const char* folder, uint32_t start_frame, uint32_t final_frame) {
+    if(!data_path || !folder) return -1;
+    auto exists = [&](int32_t i) {
+        struct stat sb;
+        return ::stat(bnn::bpd_file_path(data_path, folder, i).c_str(), &sb) == 0;
+    };
+    const int32_t last = (int32_t)start_frame - 1;  // PIC_Solver.cpp:165-167
+    if(last > 1 && exists(last)) return last;
+    for(int32_t i = (int32_t)final_frame; i >= 1; --i)  // DataIO::getLatestFileIndex (DataIO.h:90-98)
+        if(exists(i)) return i;
+    return -1;
+}
+
+bnn_status bnn_flip_save_frame_data(bnn_flip* h, const char* path, int32_t save_velocity, int32_t save_object_index) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!path) throw bnn::Error(BNN_ERR_INVALID_ARG, "null path");
+        s.saveFrameData(path, save_velocity != 0, save_object_index != 0);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_save_memory_state(bnn_flip* h, const char* path) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!path) throw bnn::Error(BNN_ERR_INVALID_ARG, "null path");
+        s.saveMemoryState(path);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_load_memory_state(bnn_flip* h, const char* path, uint32_t frame) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(!path) throw bnn::Error(BNN_ERR_INVALID_ARG, "null path");
+        s.loadMemoryState(path, frame);
+        return BNN_OK;
+    });
+}
+
+bnn_status bnn_flip_get_object_index(bnn_flip* h, uint16_t* dst, uint32_t* n_objects) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(dst) {
+            const std::vector<uint16_t> o = s.objectIndices();
+            if(!o.empty()) std::memcpy(dst, o.data(), o.size() * 2);
+        }
+        if(n_objects) *n_objects = s.nObjects;
+        return BNN_OK;
+    });
+}
+
+void bnn_io_default_params(bnn_io_params* io) {
+    if(!io) return;
+    std::memset(io, 0, sizeof(*io));
+    std::strcpy(io->data_path, "./SimData");  // ParticleSolverData.h:56-64
+    std::strcpy(io->memory_state_folder, "MemoryState");
+    std::strcpy(io->frame_data_folder, "FrameData");
+    io->load_memory_state = 1;
+    io->frame_per_state   = 1;
+}
+
+bnn_status bnn_io_from_json(const char* json_text, bnn_io_params* io) {
+    if(!json_text || !io) return BNN_ERR_INVALID_ARG;
+    bnn_io_default_params(io);
+    try {
+        bnn::io_from_json(bnn::json::Parser(json_text).parse(), *io);
+    } catch(const bnn::SceneLoadError& e) {
+        return e.code;
+    } catch(const std::exception&) {
+        return BNN_ERR_INVALID_ARG;
+    }
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_set_timing(bnn_flip* h, int32_t timing) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        if(timing < 0 || timing > 2) throw bnn::Error(BNN_ERR_INVALID_ARG, "timing must be 0, 1 or 2");
+        s.prm.timing = timing;
         return BNN_OK;
     });
 }

@@ -208,11 +208,17 @@ inline std::string scene_from_json(const json::Value& root, bnn_flip_params& p, 
     if(!gp || gp->kind != Value::Object) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: GlobalParameters is required");
     if(!sm || sm->kind != Value::Object) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: SimulationParameters is required");
 
-    // ParticleSolverFactory::createSolverFromJSon (ParticleSolverFactory.cpp:59-75): only this solver
+    // ParticleSolverFactory::createSolverFromJSon (ParticleSolverFactory.cpp:59-75): the 3-D PIC-family
+    // solvers whose step this library provides, by their factory names (FLIP_Solver.h:37,
+    // APIC_Solver.h, AFLIP_Solver.h)
     if(const Value* s = gp->find("Solver")) {
-        if(s->kind != Value::String || s->str != "FLIP_3DSolver")
-            throw SceneLoadError(BNN_ERR_UNSUPPORTED, "scene JSON: solver '" + (s->kind == Value::String ? s->str : std::string("?")) +
-                                                          "' is not provided (this library is FLIP_3DSolver)");
+        const std::string name = s->kind == Value::String ? s->str : std::string("?");
+        if(name == "FLIP_3DSolver") p.transfer_scheme = BNN_TRANSFER_FLIP;
+        else if(name == "APIC_3DSolver") p.transfer_scheme = BNN_TRANSFER_APIC;
+        else if(name == "AFLIP_3DSolver") p.transfer_scheme = BNN_TRANSFER_AFLIP;
+        else
+            throw SceneLoadError(BNN_ERR_UNSUPPORTED, "scene JSON: solver '" + name +
+                                                          "' is not provided (this library is FLIP_3DSolver, APIC_3DSolver, AFLIP_3DSolver)");
     }
     auto num = [&](const Value& o, const char* k, auto& dst) -> bool {
         const Value* v = o.find(k);
@@ -253,10 +259,6 @@ inline std::string scene_from_json(const json::Value& root, bnn_flip_params& p, 
     num(*gp, "ApplyGravity", p.apply_gravity);
     num(*gp, "EnableSortParticle", p.enable_sort_particle);
     num(*gp, "SortFrequency", p.sort_frequency);
-    for(const char* io : {"SaveFrameData", "SaveMemoryState", "LoadMemoryState"}) {
-        int32_t on = 0;
-        if(num(*gp, io, on) && on) note(std::string("GlobalParameters: '") + io + "' (file IO) is not provided; ignored");
-    }
 
     // SimulationParameters (ParticleSolverData.cpp:148-211, PIC_Data.cpp:33-38)
     static const std::set<std::string> sKnown = {
@@ -275,6 +277,7 @@ inline std::string scene_from_json(const json::Value& root, bnn_flip_params& p, 
     num(*sm, "NExpandCells", p.n_expand_cells);
     num(*sm, "AdvectionSteps", p.advection_steps);
     num(*sm, "CorrectPosition", p.correct_position);
+    num(*sm, "RepulsiveForceStiffness", p.repulsive_force_stiffness);
     num(*sm, "CGRelativeTolerance", p.cg_relative_tolerance);
     num(*sm, "MaxCGIteration", p.max_cg_iteration);
     num(*sm, "PIC_FLIP_ratio", p.pic_flip_ratio);
@@ -355,6 +358,38 @@ inline std::string scene_from_json(const json::Value& root, bnn_flip_params& p, 
         }
     }
     return warn;
+}
+
+// The file IO keys of GlobalParameters (ParticleSolverData.cpp:48-56): data path, folders, load /
+// save switches, frames per state, and the OptionalSavingData list. Wrongly typed values keep their
+// defaults.
+inline void io_from_json(const json::Value& root, bnn_io_params& io) {
+    using json::Value;
+    if(root.kind != Value::Object) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: the top level must be an object");
+    const Value* gp = root.find("GlobalParameters");
+    if(!gp || gp->kind != Value::Object) return;
+    auto str = [&](const char* k, char* dst, size_t cap) {
+        const Value* v = gp->find(k);
+        if(v && v->kind == Value::String && v->str.size() + 1 <= cap) std::memcpy(dst, v->str.c_str(), v->str.size() + 1);
+    };
+    auto flag = [&](const char* k, int32_t& dst) {
+        const Value* v = gp->find(k);
+        if(v && v->kind == Value::Bool) dst = v->b ? 1 : 0;
+        if(v && v->kind == Value::Number) dst = v->num != 0 ? 1 : 0;
+    };
+    str("DataPath", io.data_path, sizeof(io.data_path));
+    str("MemoryStateDataFolder", io.memory_state_folder, sizeof(io.memory_state_folder));
+    str("FrameDataFolder", io.frame_data_folder, sizeof(io.frame_data_folder));
+    flag("LoadMemoryState", io.load_memory_state);
+    flag("SaveMemoryState", io.save_memory_state);
+    flag("SaveFrameData", io.save_frame_data);
+    if(const Value* v = gp->find("FramePerState"); v && v->kind == Value::Number && v->num >= 1) io.frame_per_state = (uint32_t)v->num;
+    if(const Value* v = gp->find("OptionalSavingData"); v && v->kind == Value::Array)
+        for(const Value& e : v->arr) {
+            if(e.kind != Value::String) continue;
+            if(e.str == "ParticleVelocity") io.save_velocity = 1;
+            if(e.str == "ObjectIndex") io.save_object_index = 1;
+        }
 }
 
 }  // namespace bnn

@@ -4,6 +4,7 @@ Method names follow the reference's override points (PIC_Solver.h:64-70, FLIP_So
 ParticleSolver.h:51-64); every call goes through libbnn_flip.so to the gfx950 kernels.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -50,6 +51,30 @@ def make_spec(d):
             s.sphere_center[i][a] = b["center"][a]
             s.sphere_vel[i][a] = b.get("vel", (0, 0, 0))[a]
     return s
+
+
+def io_from_json(text):
+    """The file IO keys of a scene's GlobalParameters (bnn_io_from_json) as a dict."""
+    L = _lib.load()
+    io = _lib.IoParams()
+    st = L.bnn_io_from_json(text.encode() if isinstance(text, str) else text, C.byref(io))
+    if st != 0:
+        raise BnnError(st, "bnn_io_from_json failed")
+    return {k: (getattr(io, k).decode() if isinstance(getattr(io, k), bytes) else getattr(io, k)) for k, _ in _lib.IoParams._fields_}
+
+
+def data_file_path(data_path, folder, frame):
+    """DataIO::getFilePath: <data_path>/<folder>/frame.%04d.bnn"""
+    L = _lib.load()
+    buf = C.create_string_buffer(4096)
+    st = L.bnn_data_file_path(os.fsencode(data_path), os.fsencode(folder), frame, buf, len(buf))
+    if st != 0:
+        raise BnnError(st, "bnn_data_file_path failed")
+    return buf.value.decode()
+
+
+def latest_state_index(data_path, folder, start_frame, final_frame):
+    return int(_lib.load().bnn_latest_state_index(os.fsencode(data_path), os.fsencode(folder), start_frame, final_frame))
 
 
 def generate_scene_host(params, spec, seed):
@@ -146,6 +171,20 @@ class FLIPSolver:
         self._c(self.L.bnn_flip_timestep_cfl(self.h, C.byref(dt)))
         return float(dt.value)
 
+    def affine(self):
+        """APIC / AFLIP: the particles' affine matrices, shape (n, 3, 3), row d = the gradient of v_d."""
+        c = np.zeros((self.num_particles(), 3, 3), np.float32)
+        self._c(self.L.bnn_flip_get_affine(self.h, _ptr(c)))
+        return c
+
+    def set_affine(self, c):
+        c = np.ascontiguousarray(c, np.float32)
+        assert c.shape == (self.num_particles(), 3, 3)
+        self._c(self.L.bnn_flip_set_affine(self.h, _ptr(c)))
+
+    def correct_particle_positions(self, dt):
+        self._c(self.L.bnn_flip_correct_particle_positions(self.h, dt))
+
     def move_particles(self, dt):
         self._c(self.L.bnn_flip_move_particles(self.h, dt))
 
@@ -223,6 +262,41 @@ class FLIPSolver:
         self._c(self.L.bnn_flip_get_rhs(self.h, None, C.byref(n)))
         out = np.zeros(n.value, np.float64)
         self._c(self.L.bnn_flip_get_rhs(self.h, _ptr(out), C.byref(n)))
+        return out
+
+    def save_frame_data(self, path, save_velocity=False, save_object_index=False):
+        """PIC_Solver::saveFrameData: a BananaParticleData frame file (compressed positions)."""
+        self._c(self.L.bnn_flip_save_frame_data(self.h, os.fsencode(path), int(save_velocity), int(save_object_index)))
+
+    def save_memory_state(self, path):
+        self._c(self.L.bnn_flip_save_memory_state(self.h, os.fsencode(path)))
+
+    def load_memory_state(self, path, frame):
+        self._c(self.L.bnn_flip_load_memory_state(self.h, os.fsencode(path), frame))
+
+    def object_index(self):
+        n = self.num_particles()
+        out = np.zeros(n, np.uint16)
+        nobj = C.c_uint32()
+        self._c(self.L.bnn_flip_get_object_index(self.h, _ptr(out), C.byref(nobj)))
+        return out, int(nobj.value)
+
+    def set_timing(self, timing):
+        """0: no events, 1: stage events, 2: also per-kernel events (from the next step on)."""
+        self._c(self.L.bnn_flip_set_timing(self.h, timing))
+
+    def apply_preconditioner(self, r):
+        """z = M^-1 r on the last assembled system (active-cell order, like pressure())."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        self._c(self.L.bnn_flip_apply_preconditioner(self.h, _ptr(r), _ptr(z)))
+        return z
+
+    def ic0_factor(self):
+        n = C.c_uint64()
+        self._c(self.L.bnn_flip_get_ic0_factor(self.h, None, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        self._c(self.L.bnn_flip_get_ic0_factor(self.h, _ptr(out), C.byref(n)))
         return out
 
     def bins(self):

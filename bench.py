@@ -8,18 +8,23 @@ loop to 1e-10 relative residual. A step = one full FLIP substep (move, bin, P2G,
 constrain, gravity, fluid SDF, assembly, PCG, velocity update, extrapolate, constrain, G2P) with the
 scene resident in HBM.
 
-Prints one JSON line. `roofline` is for the dominant kernel (the PCG SpMV), from HIP events around
-every launch over the timed steps; `cpu_baseline` times the oracle (FP32, the reference's IC(0)
-PCG) on the host cores on a bounded sample of the same workload.
+Prints one JSON line. `value` and `ms_per_step` come from the timed steps, run without any event
+instrumentation. A second, instrumented pass (--profile-steps) records a HIP event pair around every
+launch of the main kernels: `kernels` lists each one's launches, average duration, algorithmic bytes
+(KERNEL_BYTES below) and achieved GB/s; `roofline` is the entry with the largest total time.
+`cpu_baseline` times the oracle (FP32, the reference's IC(0)-PCG) on the host cores on bounded
+samples: the 128^3 version of this scene, and the dam-break configurations at 64^3 (1 and 10 steps,
+both boundary modes, all threads and one thread) and 128^3 (1 step).
 
 Two more legs ride on the same line:
   batched  -- 512 independent scenes, edge length log-uniform 24..96 cells, fluid-box extents
               uniform in [0.2, 0.8] per axis, N(0, 0.5^2) initial velocities, refined pressure to
-              1e-9: scenes*steps/s with every stage launched once for all scenes, against the plain
-              path (each scene stepped through its own handle, one after the other).
+              1e-9: scenes*steps/s of a WorkerPool of host threads, each stepping its share of the
+              scenes on its own stream, against the plain path (each scene stepped through its own
+              handle, one after the other).
   served   -- Poisson arrivals (200/s) of step or pressure-solve requests (50/50) on a resident pool
               of scenes with edge length log-uniform 32..192, HBM admission cap 200 GB: completed
-              requests/s and p50/p99 latency, with continuous batching and with one request at a time.
+              requests/s and p50/p99 latency, with four workers and with one.
 """
 import argparse
 import json
@@ -38,34 +43,121 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def spmv_bytes_per_cell():
-    # k_pcg_spmv per active cell: read z, s (8 B) + diag, cx, cy, cz (16 B); write s, q (8 B)
-    return 32.0
+# Algorithmic bytes per launch of each timed kernel: every input element it must read once and every
+# output element it writes once, at storage precision (neighbour re-reads, atomics and temporaries not
+# counted). Np particles, F faces (u+v+w), C cells, V nodes, A fluid cells (rows of the system).
+KERNEL_BYTES = {
+    "p2g": lambda Np, F, C, V, A: 24 * Np + 5 * F,          # grid coords + velocities; velocity + valid
+    "g2p": lambda Np, F, C, V, A: 36 * Np + 8 * F,          # coords + velocities in/out; grid and backup
+    "move": lambda Np, F, C, V, A: 24 * Np + 4 * F,         # positions in/out; grid velocities
+    "fluid_sdf": lambda Np, F, C, V, A: 12 * Np + 8 * C + 4 * V,  # coords, cell ranges, phi; boundary SDF
+    "permute": lambda Np, F, C, V, A: 72 * Np,              # order + 7 words in, 7 words + coords out
+    "pcg_spmv": lambda Np, F, C, V, A: 32 * A,              # z, s, 4 coefficients in; s, q out
+    "pcg_update": lambda Np, F, C, V, A: 24 * A,            # x, r, s, q in; x, r out
+    "mg_pre": lambda Np, F, C, V, A: 24 * A,                # 3 couplings, b, 1/diag in; x out
+    "mg_restrict": lambda Np, F, C, V, A: 24.5 * A,         # b, diag, 3 couplings, x in; coarse b out
+    "mg_prolong": lambda Np, F, C, V, A: 12.5 * A,          # x, 1/diag, coarse x in; x out
+    "mg_post": lambda Np, F, C, V, A: 28 * A,               # 3 couplings, b, 1/diag, x in; x out
+    "extrap": lambda Np, F, C, V, A: 5 * F / 3,             # one layer over one component: value + flag
+}
 
 
-def cpu_baseline(n, threads, steps=3):
-    """The oracle in FP32 (reference algorithm incl. serial IC(0)-PCG, threaded particle/grid loops)
-    on the n^3 version of the workload, one substep; returns (particle-steps/s, seconds, particles)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_step_timed(o, dt, stage_s):
+    """One substep of the oracle, stage by stage as in its step() (PIC_Solver.cpp:292-294,
+    FLIP_Solver.cpp:42-53, PIC_Solver.cpp:533-539), with the wall time of each stage added to
+    stage_s (stage names after the reference's printRunTime calls); returns the solve's seconds."""
+    seq = [("move", lambda: o.move_particles(dt)), ("p2g", o.map_particles_to_grid),
+           ("extrapolate", o.extrapolate_velocity), ("constrain", o.constrain_grid_velocity),
+           ("backup+gravity", lambda: (o.backup_grid_velocity(), o.add_gravity(dt))),
+           ("weights", o.compute_cell_weights), ("phi", o.compute_fluid_sdf), ("assembly", lambda: o.compute_system(dt)),
+           ("pcg", o.solve_system), ("update", lambda: o.update_projected_velocity(dt)),
+           ("extrapolate", o.extrapolate_velocity), ("constrain", o.constrain_grid_velocity),
+           ("g2p", lambda: (o.compute_changes_grid_velocity(), o.map_grid_to_particles()))]
+    pcg_s = 0.0
+    for name, fn in seq:
+        t = time.perf_counter()
+        fn()
+        e = time.perf_counter() - t
+        stage_s[name] = stage_s.get(name, 0.0) + e
+        if name == "pcg":
+            pcg_s += e
+    return pcg_s
+
+
+def cpu_run(n, sphere, walls, steps, threads, seed=2024):
+    """The oracle in FP32 (the reference's algorithm: CAS P2G, IC(0)-PCG with the infinity-norm stop at
+    rel-tol 1e-6) on an n^3 dam break; per-run totals."""
     import numpy as np
 
     import banana_amd
     from banana_amd import scenes
     import oracle_lib
 
-    if not os.path.exists(oracle_lib.ORACLE_SO):
-        oracle_lib.build()
-    params, spec = scenes.dam_break(n, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
-    pos, vel = banana_amd.generate_scene_host(params, spec, 2024)
+    params, spec = scenes.dam_break(n, sphere=sphere, boundary_sdf_at_init=walls, cg_tol=1e-6)
+    pos, vel = banana_amd.generate_scene_host(params, spec, seed)
     o = oracle_lib.Oracle(params, precision=32, threads=threads)
     o.set_particles(pos, vel)
+    dt = np.float32(1.0 / 600.0)
+    stage_s, iters, pcg_s, res = {}, 0, 0.0, 0.0
     t0 = time.perf_counter()
-    iters = 0
-    for _ in range(steps):
-        o.step(np.float32(1.0 / 600.0))
-        iters += o.pcg_stats()[0]
-    dt = time.perf_counter() - t0
-    return pos.shape[0] * steps / dt, dt, pos.shape[0], iters / steps
+    first = None
+    for k in range(steps):
+        pcg_s += oracle_step_timed(o, dt, stage_s)
+        it, res, _ = o.pcg_stats()
+        iters += it
+        if k == 0:
+            first = time.perf_counter() - t0
+    secs = time.perf_counter() - t0
+    return {"workload": f"dam break {n}^3{' + sphere' if sphere else ''}, {'walls (boundary SDF at init)' if walls else 'reference default boundary'}",
+            "particles": int(pos.shape[0]), "steps": steps, "threads": threads, "seconds": round(secs, 3),
+            "first_step_s": round(first, 3), "particle_steps_per_s": round(pos.shape[0] * steps / secs, 1),
+            "pcg_iterations": iters, "pcg_iterations_per_s": round(iters / pcg_s, 1) if pcg_s > 0 else None,
+            "final_residual_inf": res, "stage_s": {k: round(v, 3) for k, v in stage_s.items()}}
+
+
+def cpu_baseline(args, log):
+    """Bounded CPU samples on the host: the 128^3 version of the headline scene (the `value`), and the
+    dam-break configurations of the reference's CPU plan (64^3: 1 and 10 steps, both boundary modes,
+    all threads and one thread; 128^3: 1 step)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    if not os.path.exists(oracle_lib.ORACLE_SO):
+        oracle_lib.build()
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = nproc
+    # the GPU box grants 16 host threads to one GPU's job, whatever nproc reports
+    threads = max(1, min(aff, 16))
+    model = cpu_model()
+    log(f"cpu baseline: {model}, nproc {nproc}, affinity {aff}, using {threads} threads")
+    head = cpu_run(args.cpu_n, True, True, args.cpu_steps, threads)
+    log(f"cpu baseline {head['workload']}: {head}")
+    runs = []
+    if args.cpu_configs:
+        for walls in (False, True):
+            runs.append(cpu_run(64, False, walls, 10, threads))
+            runs.append(cpu_run(64, False, walls, 1, 1))
+        runs.append(cpu_run(128, False, True, 1, threads))
+        for r in runs:
+            log(f"cpu baseline run: {r}")
+    return {"value": head["particle_steps_per_s"], "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{head['steps']} substeps of the {args.cpu_n}^3 version of this scene ({head['particles']} particles), "
+                      f"FP32 oracle with the reference's IC(0)-PCG at rel-tol 1e-6 on {threads} threads, {head['seconds']} s",
+            "cpu_model": model, "nproc": nproc, "affinity": aff, "headline_run": head, "runs": runs}
 
 
 def batched_leg(args, log):
@@ -165,10 +257,10 @@ def served_leg(args, log, batching):
     return out
 
 
-def measured_traffic(grid):
-    """HBM bytes per k_pcg_spmv launch (FETCH_SIZE + WRITE_SIZE counter runs, scaled by a 1 GiB copy)
-    as recorded in profiles/ for this workload's grid, else None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_pcg_spmv_dambreak256_traffic.json")
+def measured_traffic(kernel, grid):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE counter runs, scaled by a copy of known
+    size) as recorded in profiles/ for this workload's grid, else None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"k_{kernel}_dambreak256_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -206,6 +298,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge in cells (256 = the headline workload)")
     ap.add_argument("--pressure", choices=["refined", "fp32"], default="refined")
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="relative tolerance of each inner FP32 solve under the FP64 correction (library default when omitted)")
     ap.add_argument("--mg-sweeps", type=int, default=None, help="smoothing sweeps per multigrid level (library default when omitted)")
     ap.add_argument("--mg-coarsest-sweeps", type=int, default=None,
                     help="symmetric sweeps of the coarsest multigrid level (library default when omitted)")
@@ -215,7 +309,9 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--cpu-n", type=int, default=128, help="grid edge of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3, help="substeps of the CPU baseline sample")
-    ap.add_argument("--timing", type=int, default=2, help="0: none, 1: stage events, 2: + per-kernel events")
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="extra steps after the timed ones with HIP events around every stage and main kernel launch")
+    ap.add_argument("--cpu-configs", type=int, default=1, help="also time the 64^3 and 128^3 dam-break CPU samples")
     ap.add_argument("--batch-scenes", type=int, default=512, help="scenes of the batched leg (0: skip the leg)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson request stream (0: skip)")
@@ -234,8 +330,10 @@ def main():
     from banana_amd import scenes
 
     params, spec = scenes.dam_break(args.n, sphere=True, boundary_sdf_at_init=True, cg_tol=1e-6)
-    params.update(pressure_mode=1 if args.pressure == "refined" else 0, refine_tolerance=1e-10, timing=args.timing,
+    params.update(pressure_mode=1 if args.pressure == "refined" else 0, refine_tolerance=1e-10, timing=0,
                   preconditioner=1 if args.precond == "mg" else 0)
+    if args.inner_tol is not None:
+        params["refine_inner_tolerance"] = args.inner_tol
     if args.mg_sweeps is not None:
         params["mg_sweeps"] = args.mg_sweeps
     if args.mg_coarsest_sweeps is not None:
@@ -260,34 +358,71 @@ def main():
             f"(outer {st['outer_iterations']}), active cells {st['num_active_cells']}")
     sim.synchronize()
 
-    spmv_ms = 0.0
-    spmv_launches = 0
-    spmv_bytes = 0.0
+    # timed steps: no events in the stream
     pcg_iters = 0
     per_step = []
-    stage_ms = {}
-    stage_bytes = {}
+    rel_res = []
     t_start = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
         sim.step(dt)
-        st = sim.stats()  # the step has synchronized (end of step collects its events)
+        st = sim.stats()  # the step has synchronized
         per_step.append(time.perf_counter() - ts)
-        it = st["pcg_iterations"]
-        pcg_iters += it
-        spmv_ms += st["kernel_ms"]["pcg_spmv"]
-        spmv_launches += st["kernel_launches"]["pcg_spmv"]
-        spmv_bytes += spmv_bytes_per_cell() * st["num_active_cells"] * it
-        for k, v in st["stage_ms"].items():
-            stage_ms[k] = stage_ms.get(k, 0.0) + v
-        for k, v in st["stage_bytes"].items():
-            stage_bytes[k] = stage_bytes.get(k, 0.0) + v
-        log(f"step {i + 1}/{args.steps}: {1e3 * per_step[-1]:.1f} ms, pcg {it} (outer {st['outer_iterations']}), "
+        pcg_iters += st["pcg_iterations"]
+        rel_res.append(st["pcg_residual"] / st["rhs_max"] if st["rhs_max"] > 0 else 0.0)
+        log(f"step {i + 1}/{args.steps}: {1e3 * per_step[-1]:.1f} ms, pcg {st['pcg_iterations']} (outer {st['outer_iterations']}), "
             f"residual {st['pcg_residual']:.2e}/{st['rhs_max']:.2e}, active cells {st['num_active_cells']}")
     sim.synchronize()
     elapsed = time.perf_counter() - t_start
     np_ = sim.num_particles()
     value = np_ * args.steps / elapsed
+
+    # instrumented steps: HIP events around every stage and every launch of the main kernels
+    kernels, stage_ms, stage_bytes, prof_iters = {}, {}, {}, 0
+    if args.profile_steps > 0:
+        sim.set_timing(2)
+        acc_ms, acc_n = {}, {}
+        work = {}
+        for i in range(args.profile_steps):
+            sim.step(dt)
+            st = sim.stats()
+            it, outer = st["pcg_iterations"], st["outer_iterations"]
+            prof_iters += it
+            Np, F = st["num_particles"], sum(int(np.prod(sim.field_shape(c))) for c in ("u", "v", "w"))
+            C = st["num_cells"]
+            V = (sim.n[0] + 1) * (sim.n[1] + 1) * (sim.n[2] + 1)
+            A = st["num_active_cells"]
+            for k, v in st["kernel_ms"].items():
+                n = st["kernel_launches"][k]
+                if not n:
+                    continue
+                acc_ms[k] = acc_ms.get(k, 0.0) + v
+                acc_n[k] = acc_n.get(k, 0) + n
+                # launches that did the work: PCG kernels past the stop flag return at once
+                wl = it if k in ("pcg_spmv", "pcg_update") else (it + outer if k.startswith("mg_") and k != "mg_coarse" else n)
+                wl = min(wl, n)
+                b = KERNEL_BYTES[k](Np, F, C, V, A) if k in KERNEL_BYTES else None
+                w = work.setdefault(k, [0, 0.0, b is not None])
+                w[0] += wl
+                if b is not None:
+                    w[1] += b * wl
+            for k, v in st["stage_ms"].items():
+                stage_ms[k] = stage_ms.get(k, 0.0) + v
+            for k, v in st["stage_bytes"].items():
+                stage_bytes[k] = stage_bytes.get(k, 0.0) + v
+            log(f"profiled step {i + 1}/{args.profile_steps}: pcg {it} (outer {outer})")
+        sim.set_timing(0)
+        for k in acc_ms:
+            n, ms = acc_n[k], acc_ms[k]
+            wl, bytes_, has_b = work[k]
+            e = {"launches": n, "working_launches": wl, "avg_us": round(1e3 * ms / n, 2),
+                 "ms_per_step": round(ms / args.profile_steps, 3)}
+            if has_b and ms > 0:
+                gbs = bytes_ / (ms * 1e-3) / 1e9
+                e.update({"bytes_per_working_launch": round(bytes_ / wl), "achieved_gbs": round(gbs, 1),
+                          "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            kernels[k] = e
+    kernels = dict(sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"]))
 
     zsort_ms = []
     for _ in range(2):  # the first call also allocates its histogram buffer
@@ -300,28 +435,15 @@ def main():
         log(f"copy calibration: {args.copy_calibration} bytes device-to-device")
 
     roofline = None
-    if spmv_launches and spmv_ms > 0:
-        avg_ms = spmv_ms / spmv_launches
-        achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-        roofline = {"kernel": "k_pcg_spmv", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(list(sim.n)),
-                    "avg_launch_us": round(avg_ms * 1e3, 2), "launches": spmv_launches,
-                    "bytes_per_launch": spmv_bytes / max(pcg_iters, 1)}
+    dom = next((k for k, e in kernels.items() if "frac" in e), None)
+    if dom is not None:
+        e = kernels[dom]
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": e["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": e["frac"], "traffic": measured_traffic(dom, list(sim.n)), "avg_launch_us": e["avg_us"],
+                    "launches": e["launches"], "working_launches": e["working_launches"],
+                    "bytes_per_launch": e["bytes_per_working_launch"]}
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = os.cpu_count() or 1
-        try:
-            aff = len(os.sched_getaffinity(0))
-            threads = min(threads, aff)
-        except Exception:
-            pass
-        threads = min(threads, 16)
-        log(f"cpu baseline: oracle FP32 on {threads} threads, {args.cpu_n}^3 ...")
-        v, secs, npc, iters = cpu_baseline(args.cpu_n, threads, args.cpu_steps)
-        cpu = {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"first {args.cpu_steps} substeps of the {args.cpu_n}^3 version of this scene ({npc} particles), "
-                         f"FP32 oracle with the reference's IC(0)-PCG at rel-tol 1e-6 ({iters:.0f} iterations/step), {secs:.1f} s"}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, log)
 
     batched = batched_leg(args, log) if args.batch_scenes > 0 else None
     served = None
@@ -345,14 +467,17 @@ def main():
         "config": {"workload": f"FLIP dam break {args.n}^3 + falling sphere, 8 particles/cell, solid walls, dt=1/600, "
                                f"pressure {'FP32 PCG under FP64 defect correction to 1e-10' if args.pressure == 'refined' else 'FP32 PCG rel-tol 1e-6'}",
                    "grid": list(sim.n), "particles": np_, "pcg_iterations_per_step": pcg_iters / args.steps,
-                   "preconditioner": args.precond, "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()}},
+                   "pcg_iterations_per_s": round(pcg_iters / elapsed, 1),
+                   "fp64_relative_residual_max": max(rel_res) if rel_res else None,
+                   "preconditioner": args.precond,
+                   "stage_ms_per_step": {k: round(v / args.profile_steps, 3) for k, v in stage_ms.items()} if args.profile_steps else None},
         "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
         "zsort": {"ms_per_call": round(zsort_ms[-1], 3), "particles": np_},
-        # per stage: algorithmic bytes (the library's byte model; PCG counts 64 B per fluid cell and
-        # inner iteration, the Jacobi form, so the multigrid's extra traffic is not in it) over the
+        # per stage (instrumented steps): the library's algorithmic bytes (stats.stage_bytes) over the
         # stage's event time
         "stage_roofline": {k: {"gbs": round(stage_bytes[k] / (stage_ms[k] * 1e6), 1),
                                "frac": round(stage_bytes[k] / (stage_ms[k] * 1e6) / HBM_PEAK_GBS, 3)}

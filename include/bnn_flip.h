@@ -46,9 +46,23 @@ typedef enum bnn_pressure_mode {
                                  on max|b - A x| <= refine_tolerance * max|b| (FP64 residual) */
 } bnn_pressure_mode;
 
+/* Particle-grid transfer scheme (the reference's solver class: FLIP_3DSolver, APIC_3DSolver,
+ * AFLIP_3DSolver; FLIP_Solver.cpp:57-133, APIC_Solver.cpp:42-125, AFLIP_Solver.cpp:43-140). */
+typedef enum bnn_transfer_scheme {
+    BNN_TRANSFER_FLIP  = 0, /* FLIP blend of PIC and the grid change */
+    BNN_TRANSFER_APIC  = 1, /* affine PIC: each particle carries C (the velocity gradient), splats
+                               w (v + C (x_node - x_p)), gathers v and C; no FLIP blend and no
+                               extrapolation / constrain before the solve */
+    BNN_TRANSFER_AFLIP = 2  /* APIC's transfers with FLIP's step and blend, for v and C */
+} bnn_transfer_scheme;
+
 typedef enum bnn_preconditioner {
     BNN_PRECOND_JACOBI = 0, /* PCGSolver::JACOBI (PCGSolver.cpp:301-313) */
-    BNN_PRECOND_MG     = 1  /* geometric multigrid V-cycle (extension, see the design notes) */
+    BNN_PRECOND_MG     = 1, /* geometric multigrid V-cycle (extension, see the design notes) */
+    BNN_PRECOND_IC0    = 2  /* PCGSolver::MICCL0_SYMMETRIC, the reference's default: natural-order
+                               IC(0) with the 0.25 pivot safeguard (PCGSolver.cpp:398-466, 243-269),
+                               level-scheduled over the 3n-2 hyperplanes i+j+k; latency-bound, for
+                               comparing iterates with the reference, not for speed */
 } bnn_preconditioner;
 
 /* Parameters: field names follow the reference JSON keys (ParticleSolverData.cpp:33-78, 148-211,
@@ -75,7 +89,8 @@ typedef struct bnn_flip_params {
     int32_t  boundary_sdf_at_init;          /* extension: compute the boundary SDF once at scene
                                                creation (false = reference behaviour, walls act
                                                only through particle projection) */
-    int32_t  correct_position;              /* CorrectPosition (false; true -> BNN_ERR_UNSUPPORTED) */
+    int32_t  correct_position;              /* CorrectPosition (false): correctParticlePositions after
+                                               each move (PIC_Solver.cpp:377-485) */
     int32_t  exit_if_pressure_projection_failed; /* ExitIfPressureProjectionFailed: when set, a
                                                failed solve makes step() return
                                                BNN_ERR_NOT_CONVERGED instead of BNN_OK */
@@ -94,6 +109,14 @@ typedef struct bnn_flip_params {
                                                piecewise-constant coarse space under-corrects) */
     int32_t  mg_coarsest_sweeps;            /* MG: symmetric sweeps on the coarsest level (8) */
     uint32_t start_frame, final_frame;      /* GlobalParameters StartFrame / FinalFrame (1, 1) */
+    float    repulsive_force_stiffness;     /* RepulsiveForceStiffness (50) */
+    uint64_t random_seed;                   /* extension: seed of correctParticlePositions' random
+                                               push for coincident pairs (the reference's generator is
+                                               unseeded); step s uses a hash of (seed, s) */
+    float    refine_inner_tolerance;        /* extension, REFINED: relative residual each inner FP32
+                                               PCG solve reaches before the next FP64 correction */
+    int32_t  transfer_scheme;               /* bnn_transfer_scheme (FLIP); set from GlobalParameters
+                                               Solver by bnn_scene_from_json */
 } bnn_flip_params;
 
 /* Seeded scene generator input (replaces ParticleGenerator, ParticleGenerator.hpp:87-136): one
@@ -139,13 +162,22 @@ typedef enum bnn_stage {
 } bnn_stage;
 
 /* per-kernel timers (timing >= 2): a HIP event pair on the solver stream around every launch */
-#define BNN_NUM_KERNEL_TIMERS 4
+#define BNN_NUM_KERNEL_TIMERS 13
 typedef enum bnn_kernel_timer {
-    BNN_KT_PCG_SPMV   = 0, /* s = z + beta s; q = A s; s.q */
-    BNN_KT_PCG_UPDATE = 1, /* x, r updates; max|r| (Jacobi: also z = M^-1 r and z.r; with the
-                              multigrid preconditioner z.r comes from the V-cycle) */
-    BNN_KT_P2G        = 2,
-    BNN_KT_FLUID_SDF  = 3
+    BNN_KT_PCG_SPMV    = 0,  /* s = z + beta s; q = A s; s.q */
+    BNN_KT_PCG_UPDATE  = 1,  /* x, r updates; max|r| (Jacobi: also z = M^-1 r and z.r; with the
+                                multigrid preconditioner z.r comes from the V-cycle) */
+    BNN_KT_P2G         = 2,
+    BNN_KT_FLUID_SDF   = 3,
+    BNN_KT_G2P         = 4,
+    BNN_KT_MOVE        = 5,
+    BNN_KT_MG_PRE      = 6,  /* multigrid pre-smoother, finest level */
+    BNN_KT_MG_RESTRICT = 7,  /* residual + restriction, finest level */
+    BNN_KT_MG_PROLONG  = 8,  /* coarse-grid correction, finest level */
+    BNN_KT_MG_POST     = 9,  /* post-smoother (and z.r partials), finest level */
+    BNN_KT_MG_COARSE   = 10, /* every V-cycle launch on the coarser levels */
+    BNN_KT_EXTRAP      = 11, /* extrapolation layer passes */
+    BNN_KT_PERMUTE     = 12  /* binning payload permute */
 } bnn_kernel_timer;
 
 typedef struct bnn_flip_stats {
@@ -209,6 +241,13 @@ bnn_status  bnn_flip_timestep_cfl(bnn_flip* h, float* dt);
 
 /* --- override points of the reference (PIC_Solver.h:64-70, FLIP_Solver.h:49-52) ------------- */
 bnn_status  bnn_flip_move_particles(bnn_flip* h, float dt);          /* PIC_Solver.cpp:353-373 */
+/* PIC_Solver::correctParticlePositions (PIC_Solver.cpp:377-485) with the handle's seed for this step
+ * (random_seed and the step count); runs only when CorrectPosition is set, like the reference */
+bnn_status  bnn_flip_correct_particle_positions(bnn_flip* h, float dt);
+/* APIC / AFLIP: the particles' affine matrices C (ParticleData::C, APIC_Data.h), 9 floats per particle
+ * in the caller's order, row d = the gradient of velocity component d; new particles start at 0 */
+bnn_status  bnn_flip_get_affine(bnn_flip* h, float* c);
+bnn_status  bnn_flip_set_affine(bnn_flip* h, const float* c);
 /* PIC_Solver::sortParticles (PIC_Solver.cpp:315-328): with EnableSortParticle, at finishedFrame 0
  * and whenever (finishedFrame + 1) % SortFrequency == 0, reorders the particles' user order by the
  * Morton key of NeighborSearch::z_sort (NeighborSearch.cpp:45-86), equal keys keeping their previous
@@ -225,7 +264,7 @@ bnn_status  bnn_flip_sort_particles(bnn_flip* h);
  * threads) race: serialise them. */
 #define BNN_NOALLOC_SCRATCH_BYTES (32u << 20)
 bnn_status  bnn_flip_sort_particles_noalloc(bnn_flip* h);
-bnn_status  bnn_flip_advance_velocity(bnn_flip* h, float dt);        /* FLIP_Solver.cpp:42-53 */
+bnn_status  bnn_flip_advance_velocity(bnn_flip* h, float dt);      /* FLIP_Solver.cpp:42-53 */
 bnn_status  bnn_flip_map_particles_to_grid(bnn_flip* h);              /* FLIP_Solver.cpp:57-95 */
 bnn_status  bnn_flip_map_grid_to_particles(bnn_flip* h);              /* FLIP_Solver.cpp:99-111 */
 bnn_status  bnn_flip_extrapolate_velocity(bnn_flip* h);               /* PIC_Solver.cpp:985-1090 */
@@ -244,6 +283,13 @@ bnn_status  bnn_flip_get_grid(bnn_flip* h, int32_t field, void* dst);
 bnn_status  bnn_flip_get_pressure(bnn_flip* h, double* dst, uint64_t* n_active);
 /* right-hand side of the last assembled system, same order */
 bnn_status  bnn_flip_get_rhs(bnn_flip* h, double* dst, uint64_t* n_active);
+/* z = M^-1 r with the handle's preconditioner on the last assembled system (PCGSolver.cpp:221-269),
+ * r and z in the order of bnn_flip_get_pressure (n_active entries, rounded to FP32 on the way in).
+ * Uses the solver's work vectors; the pressure is left as it is. */
+bnn_status  bnn_flip_apply_preconditioner(bnn_flip* h, const double* r, double* z);
+/* BNN_PRECOND_IC0: 1/sqrt of the modified pivots of the last factorisation (the reference's
+ * icInvDiag after formPreconditioner), same order */
+bnn_status  bnn_flip_get_ic0_factor(bnn_flip* h, double* dst, uint64_t* n_active);
 /* last binning: perm[i] = index (in the array order before the sort) of sorted slot i,
  * cell_start[c] = first sorted slot of cell c (ncells+1 entries). Either may be NULL. */
 bnn_status  bnn_flip_get_bins(bnn_flip* h, uint32_t* perm, uint32_t* cell_start);
@@ -254,6 +300,45 @@ bnn_status  bnn_flip_get_stats(bnn_flip* h, bnn_flip_stats* s);
 bnn_status  bnn_flip_set_grid(bnn_flip* h, int32_t field, const void* src);
 /* wait for all queued device work of this handle */
 bnn_status  bnn_flip_synchronize(bnn_flip* h);
+/* --- frame output and memory states (BananaParticleData files, ParticleSerialization.cpp:533-830) --
+ * bnn_data_file_path: DataIO::getFilePath (DataIO.h:119-124), "<data_path>/<folder>/frame.%04d.bnn".
+ * bnn_flip_save_frame_data: PIC_Solver::saveFrameData (PIC_Solver.cpp:232-262): particle_radius and
+ *   the positions compressed to uint16 per component (ParticleHelpers.cpp:47-79); with
+ *   save_velocity also the compressed velocities, with save_object_index NObjects and object_index
+ *   (the "ParticleVelocity" / "ObjectIndex" entries of OptionalSavingData). Missing folders are created.
+ * bnn_flip_save_memory_state / bnn_flip_load_memory_state: PIC_Solver::saveMemoryState /
+ *   loadMemoryState (PIC_Solver.cpp:158-229): grid resolution, the u/v/w grids, particle radius,
+ *   object count, and per particle the object index, position and velocity, uncompressed. Loading
+ *   checks the grid resolution and particle radius against the scene, replaces particles and face
+ *   velocities, and sets the finished frame to `frame`.
+ * bnn_latest_state_index: the state index loadMemoryState picks: start_frame - 1 when that file
+ *   exists and start_frame - 1 > 1, else the highest index in [1, final_frame] with a file, else -1. */
+bnn_status  bnn_data_file_path(const char* data_path, const char* folder, int32_t frame, char* out, size_t cap);
+int32_t     bnn_latest_state_index(const char* data_path, const char* folder, uint32_t start_frame, uint32_t final_frame);
+bnn_status  bnn_flip_save_frame_data(bnn_flip* h, const char* path, int32_t save_velocity, int32_t save_object_index);
+bnn_status  bnn_flip_save_memory_state(bnn_flip* h, const char* path);
+bnn_status  bnn_flip_load_memory_state(bnn_flip* h, const char* path, uint32_t frame);
+bnn_status  bnn_flip_get_object_index(bnn_flip* h, uint16_t* dst, uint32_t* n_objects);
+
+/* File IO keys of GlobalParameters (ParticleSolverData.h:56-64, ParticleSolverData.cpp:48-56) with
+ * the reference's defaults; bnn_io_from_json reads them from a scene. */
+typedef struct bnn_io_params {
+    char     data_path[512];              /* DataPath ("./SimData") */
+    char     memory_state_folder[128];    /* MemoryStateDataFolder ("MemoryState") */
+    char     frame_data_folder[128];      /* FrameDataFolder ("FrameData") */
+    int32_t  load_memory_state;           /* LoadMemoryState (true) */
+    int32_t  save_memory_state;           /* SaveMemoryState (false) */
+    int32_t  save_frame_data;             /* SaveFrameData (false) */
+    uint32_t frame_per_state;             /* FramePerState (1) */
+    int32_t  save_velocity;               /* OptionalSavingData has "ParticleVelocity" */
+    int32_t  save_object_index;           /* OptionalSavingData has "ObjectIndex" */
+} bnn_io_params;
+void        bnn_io_default_params(bnn_io_params* io);
+bnn_status  bnn_io_from_json(const char* json_text, bnn_io_params* io);
+
+/* changes bnn_flip_params.timing of a live handle (0: no events, 1: stage events, 2: also per-kernel
+ * events); takes effect at the next step */
+bnn_status  bnn_flip_set_timing(bnn_flip* h, int32_t timing);
 /* device bytes a scene with these parameters and n_particles keeps resident in HBM (what
  * bnn_flip_create + bnn_flip_set_particles allocate; stepping allocates nothing per scene) */
 bnn_status  bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, uint64_t* bytes);

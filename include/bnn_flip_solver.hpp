@@ -52,13 +52,29 @@ public:
         ss << f.rdbuf();
         loadSceneFromString(ss.str(), seed);
     }
+    // PIC_Solver::generateParticles (PIC_Solver.cpp:40-66): the latest memory state is loaded when
+    // LoadMemoryState is set and one exists; otherwise the particles are generated (and sorted) and
+    // frame 0 is saved when SaveFrameData is set
     void loadSceneFromString(const std::string& json, uint64_t seed = 0) {
         reset();
         bnn_flip*  h  = nullptr;
         bnn_status st = bnn_flip_create_from_json(json.c_str(), seed, &h);
         if(st != BNN_OK) throw flip_error(st, bnn_flip_last_error(nullptr));
         m_h = h;
+        check(bnn_io_from_json(json.c_str(), &m_io));
+        const bnn_flip_params p = params();
+        const int32_t idx = m_io.load_memory_state ? bnn_latest_state_index(m_io.data_path, m_io.memory_state_folder, p.start_frame, p.final_frame)
+                                                   : -1;
+        if(idx >= 0) {
+            loadMemoryState(filePath(m_io.memory_state_folder, idx), (uint32_t)idx);
+            m_loadedState = idx;
+        } else {
+            saveFrameData();
+        }
     }
+    const bnn_io_params& ioParams() const { return m_io; }
+    void                 setIoParams(const bnn_io_params& io) { m_io = io; }
+    int32_t              loadedStateIndex() const { return m_loadedState; }
     void create(const bnn_flip_params& p) {
         reset();
         bnn_flip*  h  = nullptr;
@@ -74,8 +90,29 @@ public:
         const uint32_t        start = p.start_frame <= 1 ? done + 1 : std::min<uint32_t>(p.start_frame, done + 1);
         for(uint32_t frame = start; frame <= p.final_frame; ++frame) doFrameSimulation(frame);
     }
-    // ParticleSolver::doFrameSimulation (ParticleSolver.cpp:200-219): sortParticles + advanceFrame
-    void doFrameSimulation(uint32_t frame) { check(bnn_flip_do_frame_simulation(h(), frame)); }
+    // ParticleSolver::doFrameSimulation (ParticleSolver.cpp:200-219): sortParticles + advanceFrame,
+    // which ends with saveFrameData and saveMemoryState (PIC_Solver.cpp:307-310)
+    void doFrameSimulation(uint32_t frame) {
+        check(bnn_flip_do_frame_simulation(h(), frame));
+        saveFrameData();
+        saveMemoryState();
+    }
+
+    // PIC_Solver::saveFrameData / saveMemoryState / loadMemoryState (PIC_Solver.cpp:158-262), files
+    // named by DataIO::getFilePath; each returns the frame index written, or -1 when switched off
+    int32_t saveFrameData() {
+        if(!m_io.save_frame_data) return -1;
+        const uint32_t f = finishedFrame();
+        check(bnn_flip_save_frame_data(h(), filePath(m_io.frame_data_folder, (int32_t)f).c_str(), m_io.save_velocity, m_io.save_object_index));
+        return (int32_t)f;
+    }
+    int32_t saveMemoryState() {
+        const uint32_t f = finishedFrame();
+        if(!m_io.save_memory_state || (m_io.frame_per_state && f % m_io.frame_per_state != 0)) return -1;
+        check(bnn_flip_save_memory_state(h(), filePath(m_io.memory_state_folder, (int32_t)f).c_str()));
+        return (int32_t)f;
+    }
+    void loadMemoryState(const std::string& path, uint32_t frame) { check(bnn_flip_load_memory_state(h(), path.c_str(), frame)); }
 
     // PIC_Solver override points (PIC_Solver.h:64-70, FLIP_Solver.h:49-52)
     void  advanceFrame() { check(bnn_flip_advance_frame(h())); }
@@ -125,7 +162,21 @@ public:
     bnn_flip* handle() const { return m_h; }
 
 private:
-    bnn_flip* m_h = nullptr;
+    bnn_flip*     m_h = nullptr;
+    bnn_io_params m_io = default_io();
+    int32_t       m_loadedState = -1;
+    static bnn_io_params default_io() {
+        bnn_io_params io;
+        bnn_io_default_params(&io);
+        io.load_memory_state = 0;  // set from the scene by loadScene (the reference's default there is true)
+        return io;
+    }
+    std::string filePath(const char* folder, int32_t frame) const {
+        char buf[1024];
+        if(bnn_data_file_path(m_io.data_path, folder, frame, buf, sizeof(buf)) != BNN_OK)
+            throw flip_error(BNN_ERR_INVALID_ARG, "data path too long");
+        return buf;
+    }
     bnn_flip* h() const {
         if(!m_h) throw flip_error(BNN_ERR_STATE, "solver not created (call loadScene or create first)");
         return m_h;

@@ -293,6 +293,19 @@ struct orc_params {
 
 namespace orc {
 
+// uniform in [-1, 1) from (seed, p, q, component): 24 bits of a splitmix64 hash, so the value is
+// exact in float and the same on the device
+static inline uint64_t mix64o(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static inline float rand11(uint64_t seed, uint32_t p, uint32_t q, int comp) {
+    const uint64_t h = mix64o(seed ^ mix64o(((uint64_t)p << 32 | q) * 4u + (uint64_t)comp));
+    return (float)(uint32_t)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;
+}
+
 enum Field {
     F_U = 0, F_V, F_W,
     F_WEIGHT_U, F_WEIGHT_V, F_WEIGHT_W,
@@ -320,6 +333,10 @@ public:
     BoxSDF<R> domain;
     // particles
     std::vector<Vec> pos, vel;
+    // transfer scheme: 0 FLIP (FLIP_Solver), 1 APIC (APIC_Solver), 2 AFLIP (AFLIP_Solver); the
+    // affine matrix of each particle, C[p][d] = the gradient of velocity component d (APIC_Data.h)
+    int              scheme = 0;
+    std::vector<std::array<Vec, 3>> C;
     // grid (PIC_Data.cpp:101-117, FLIP_Data.h:37-46)
     Arr3<R>    vels[3], weights[3], tmpVels[3], oldVels[3], dVels[3];
     Arr3<char> valids[3], extrapolates[3], tmpValids[3];
@@ -349,17 +366,20 @@ public:
     void makeReady() {
         dx             = R(prm.cell_size);
         particleRadius = dx / R(prm.ratio_cell_size_pradius);
+        // the grid (origin, extent, cell counts) is the reference's, computed in its float
+        // arithmetic, in both precisions: the FP64 oracle solves the same discretisation
         for(int d = 0; d < 3; ++d) {
-            R ex    = dx * R(prm.n_expand_cells);
-            bmin[d] = R(prm.box_min[d]) - ex;
-            bmax[d] = R(prm.box_max[d]) + ex;
+            const float exf = prm.cell_size * float(prm.n_expand_cells);
+            const float lo = prm.box_min[d] - exf, hi = prm.box_max[d] + exf;
+            bmin[d]      = R(lo);
+            bmax[d]      = R(hi);
+            n[d]         = static_cast<int>(std::ceil((hi - lo) / prm.cell_size));
             domain.lo[d] = R(prm.box_min[d]);
             domain.hi[d] = R(prm.box_max[d]);
         }
         sdfRadius = dx * R(1.01 * std::sqrt(3) / 2.0);
         invdx     = R(1.0) / dx;
         halfdx    = R(0.5) * dx;
-        for(int d = 0; d < 3; ++d) n[d] = static_cast<int>(std::ceil((bmax[d] - bmin[d]) / dx));
         // gravity (ParticleSolverData.cpp:256-271)
         if(prm.gravity_type == 1) {
             R gx = R(prm.gravity_direction[0]), gy = R(prm.gravity_direction[1]), gz = R(prm.gravity_direction[2]);
@@ -466,6 +486,91 @@ public:
         });
     }
 
+    // BoundaryObject::constrainToBoundary(ppos) (BoundaryObject.cpp:55-67): position only
+    void constrainPosition(Vec& p) const {
+        Vec v{0, 0, 0};
+        const int32_t keep = prm.reflect_velocity_at_boundary;
+        const_cast<orc_params&>(prm).reflect_velocity_at_boundary = 0;
+        constrainToBoundary(p, v);
+        const_cast<orc_params&>(prm).reflect_velocity_at_boundary = keep;
+    }
+
+    // PIC_Solver::correctParticlePositions (PIC_Solver.cpp:377-485), 3-D branch: a repulsion from
+    // every other particle of the 27 cells around the particle's cell, w = smooth_kernel(d^2, r^2)
+    // (MathHelpers.h:661-664) with r = 2 particleRadius / sqrt(3), spring += w (p - q) / d; for a
+    // pair closer than 0.01 r a random push (threshold / dt) * u, u in [-1, 1)^3; then advectionSteps
+    // times pos += spring * (r * dt / advectionSteps * stiffness), each followed by the boundary
+    // projection. Neighbours in ascending cell index (k, j, i) and, in a cell, ascending particle
+    // index (the reference's cell lists are unordered).
+    // Parity unpinned against the running reference: its random push comes from an unseeded
+    // generator (NumberHelpers.h:146,211), so here u is a hash of (seed, p, q, component); and
+    // smooth_kernel's pow(1 - x, 3) is written (t * t) * t.
+    void correctParticlePositions(R dt, R stiffness, uint64_t seed) {
+        const int nx = n[0], ny = n[1], nz = n[2];
+        const size_t np_ = pos.size();
+        std::vector<int64_t> cellOf(np_);
+        std::vector<uint32_t> start((size_t)nx * ny * nz + 1, 0);
+        auto cellIdx = [&](const Vec& p, int d) { return (int)((d == 0 ? p.x - bmin[0] : d == 1 ? p.y - bmin[1] : p.z - bmin[2]) / dx); };
+        for(size_t p = 0; p < np_; ++p) {
+            int c[3];
+            for(int d = 0; d < 3; ++d) c[d] = std::min(std::max(cellIdx(pos[p], d), 0), n[d] - 1);
+            cellOf[p] = ((int64_t)c[2] * ny + c[1]) * nx + c[0];
+            ++start[cellOf[p] + 1];
+        }
+        for(size_t c = 0; c < (size_t)nx * ny * nz; ++c) start[c + 1] += start[c];
+        std::vector<uint32_t> list(np_), fill(start.begin(), start.end() - 1);
+        for(size_t p = 0; p < np_; ++p) list[fill[cellOf[p]]++] = (uint32_t)p;
+        const R pr        = R(prm.cell_size) / R(prm.ratio_cell_size_pradius);
+        const R radius    = R(2.0) * pr / R(std::sqrt(3));
+        const R radius2   = radius * radius;
+        const R threshold = R(0.01) * radius, threshold2 = threshold * threshold;
+        const R substep   = dt / R(prm.advection_steps);
+        const R K         = radius * substep * stiffness;
+        std::vector<Vec> out(np_);
+        pool.parallel_for(np_, [&](size_t p) {
+            const Vec pp = pos[p];
+            int       pc[3];
+            for(int d = 0; d < 3; ++d) pc[d] = cellIdx(pp, d);  // getCellIdx<Int>: not clamped
+            Vec spring{0, 0, 0};
+            for(int k = -1; k <= 1; ++k)
+                for(int j = -1; j <= 1; ++j)
+                    for(int i = -1; i <= 1; ++i) {
+                        const int ci = pc[0] + i, cj = pc[1] + j, ck = pc[2] + k;
+                        if(ci < 0 || cj < 0 || ck < 0 || ci >= nx || cj >= ny || ck >= nz) continue;  // Grid::isValidCell
+                        const size_t c = ((size_t)ck * ny + cj) * nx + ci;
+                        for(uint32_t e = start[c]; e < start[c + 1]; ++e) {
+                            const uint32_t q = list[e];
+                            if(q == p) continue;
+                            const Vec xpq{pp.x - pos[q].x, pp.y - pos[q].y, pp.z - pos[q].z};
+                            const R   d2 = xpq.x * xpq.x + xpq.y * xpq.y + xpq.z * xpq.z;
+                            const R   x2 = d2 / radius2;
+                            const R   t  = R(1.0) - x2;
+                            const R   w  = x2 > R(1.0) ? R(0) : (t * t) * t;
+                            if(d2 > threshold2) {
+                                const R s = std::sqrt(d2);
+                                spring.x += w * xpq.x / s;
+                                spring.y += w * xpq.y / s;
+                                spring.z += w * xpq.z / s;
+                            } else {
+                                const R a = threshold / dt;
+                                spring.x += a * R(rand11(seed, (uint32_t)p, q, 0));
+                                spring.y += a * R(rand11(seed, (uint32_t)p, q, 1));
+                                spring.z += a * R(rand11(seed, (uint32_t)p, q, 2));
+                            }
+                        }
+                    }
+            Vec np2 = pp;
+            for(uint32_t it = 0; it < prm.advection_steps; ++it) {
+                np2.x += spring.x * K;
+                np2.y += spring.y * K;
+                np2.z += spring.z * K;
+                constrainPosition(np2);
+            }
+            out[p] = np2;
+        });
+        pos = out;
+    }
+
     // float/double CAS add (AtomicOperations.h:31-49), used only when threads > 1
     static void atomicAddR(R& dst, R v) {
         std::atomic_ref<R> a(dst);
@@ -497,7 +602,16 @@ public:
                           (R(1.0) - fi) * fj * fk,                       fi * fj * fk};
                 for(int c = 0; c < 8; ++c) {
                     int    i = ii[0] + (c & 1), j = ii[1] + ((c >> 1) & 1), k = ii[2] + ((c >> 2) & 1);
-                    R      mom = w[c] * vel[p][d];
+                    R      vd  = vel[p][d];
+                    if(scheme != 0) {
+                        // APIC_Solver.cpp:73-76: w (v_d + dot(C_d, gpos - ppos)), gpos =
+                        // getWorldCoordinate(index + extra) = (index + extra) * dx + bMin
+                        const R ex[3] = {d == 0 ? R(0) : R(0.5), d == 1 ? R(0) : R(0.5), d == 2 ? R(0) : R(0.5)};
+                        const R gx = (R(i) + ex[0]) * dx + bmin[0], gy = (R(j) + ex[1]) * dx + bmin[1], gz = (R(k) + ex[2]) * dx + bmin[2];
+                        const Vec& cd = C[p][d];
+                        vd = vd + (cd.x * (gx - pos[p].x) + cd.y * (gy - pos[p].y) + cd.z * (gz - pos[p].z));
+                    }
+                    R      mom = w[c] * vd;
                     size_t id  = vels[d].idx(i, j, k);
                     if(par) {
                         atomicAddR(vels[d].d[id], mom);
@@ -961,9 +1075,62 @@ public:
             for(size_t i = 0; i < vels[d].size(); ++i) dVels[d].d[i] = vels[d].d[i] - oldVels[d].d[i];
     }
 
-    // FLIP_Solver.cpp:99-111, 125-133
+    // ArrayHelpers.cpp:275-302 (interpolateGradientValue, 3-D): the gradient of the trilinear
+    // interpolant of g over the cell of q, divided by the cell size; the eight corner terms are
+    // added in the reference's order
+    Vec interpGrad(const Arr3<R>& g, R qx, R qy, R qz) const {
+        int i, j, k;
+        R   fi, fj, fk;
+        get_barycentric(qx, i, fi, 0, g.nx);
+        get_barycentric(qy, j, fj, 0, g.ny);
+        get_barycentric(qz, k, fk, 0, g.nz);
+        const R v000 = g(i, j, k), v001 = g(i, j, k + 1), v010 = g(i, j + 1, k), v011 = g(i, j + 1, k + 1);
+        const R v100 = g(i + 1, j, k), v101 = g(i + 1, j, k + 1), v110 = g(i + 1, j + 1, k), v111 = g(i + 1, j + 1, k + 1);
+        const R a = R(1.0) - fi, b = R(1.0) - fj, c = R(1.0) - fk;
+        Vec     gr;
+        gr.x = v000 * (-b * c) + v001 * (-b * fk) + v010 * (-fj * c) + v011 * (-fj * fk) + v100 * (b * c) + v101 * (b * fk) +
+               v110 * (fj * c) + v111 * (fj * fk);
+        gr.y = v000 * (-a * c) + v001 * (-a * fk) + v010 * (a * c) + v011 * (a * fk) + v100 * (-fi * c) + v101 * (-fi * fk) +
+               v110 * (fi * c) + v111 * (fi * fk);
+        gr.z = v000 * (-a * b) + v001 * (a * b) + v010 * (-a * fj) + v011 * (a * fj) + v100 * (-fi * b) + v101 * (fi * b) +
+               v110 * (-fi * fj) + v111 * (fi * fj);
+        gr.x /= dx;
+        gr.y /= dx;
+        gr.z /= dx;
+        return gr;
+    }
+    // APIC_Solver.cpp:106-115: row d of C is the gradient of velocity component d at gridPos - extra
+    std::array<Vec, 3> affineFromGrid(const Arr3<R>* g, const Vec& gp) const {
+        return {interpGrad(g[0], gp.x, gp.y - R(0.5), gp.z - R(0.5)), interpGrad(g[1], gp.x - R(0.5), gp.y, gp.z - R(0.5)),
+                interpGrad(g[2], gp.x - R(0.5), gp.y - R(0.5), gp.z)};
+    }
+
+    // FLIP_Solver.cpp:99-111, 125-133; APIC_Solver.cpp:94-102; AFLIP_Solver.cpp:94-113
     void mapGrid2Particles() {
         const R r = R(prm.pic_flip_ratio);
+        if(scheme == 1) {  // APIC: PIC velocity and the affine matrix from the grid
+            pool.parallel_for(pos.size(), [&](size_t p) {
+                Vec gp = gridCoord(pos[p]);
+                vel[p] = velFromGrid(vels, gp);
+                C[p]   = affineFromGrid(vels, gp);
+            });
+            return;
+        }
+        if(scheme == 2) {  // AFLIP: the FLIP blend for both the velocity and the affine matrix
+            pool.parallel_for(pos.size(), [&](size_t p) {
+                Vec                gp  = gridCoord(pos[p]);
+                Vec                gv  = velFromGrid(vels, gp);
+                Vec                dgv = velFromGrid(dVels, gp);
+                std::array<Vec, 3> gc  = affineFromGrid(vels, gp);
+                std::array<Vec, 3> dgc = affineFromGrid(dVels, gp);
+                Vec                pv  = vel[p];
+                std::array<Vec, 3> pc  = C[p];
+                for(int d = 0; d < 3; ++d) vel[p][d] = (R(1.0) - r) * gv[d] + r * (pv[d] + dgv[d]);
+                for(int d = 0; d < 3; ++d)
+                    for(int e = 0; e < 3; ++e) C[p][d][e] = (R(1.0) - r) * gc[d][e] + r * (pc[d][e] + dgc[d][e]);
+            });
+            return;
+        }
         pool.parallel_for(pos.size(), [&](size_t p) {
             Vec gp  = gridCoord(pos[p]);
             Vec gv  = velFromGrid(vels, gp);
@@ -973,8 +1140,17 @@ public:
         });
     }
 
-    // FLIP_Solver.cpp:42-53
+    // FLIP_Solver.cpp:42-53 (FLIP and AFLIP, AFLIP_Solver.cpp:43-45); APIC_Solver.cpp:41-48
     bool advanceVelocity(R dt) {
+        if(scheme == 1) {
+            mapParticles2Grid();
+            addGravity(dt);
+            bool ok = pressureProjection(dt);
+            extrapolateVelocity();
+            constrainGridVelocity();
+            mapGrid2Particles();
+            return ok;
+        }
         mapParticles2Grid();
         extrapolateVelocity();
         constrainGridVelocity();
@@ -1242,11 +1418,37 @@ void orc_set_particles(void* h, const float* pos, const float* vel, uint64_t n) 
     dispatch(h, [&](auto& s) {
         s.pos.resize(n);
         s.vel.resize(n);
+        s.C.assign(n, {});
         for(uint64_t i = 0; i < n; ++i)
             for(int d = 0; d < 3; ++d) {
                 s.pos[i][d] = pos[3 * i + d];
                 s.vel[i][d] = vel[3 * i + d];
             }
+        return 0;
+    });
+}
+// transfer scheme: 0 FLIP, 1 APIC, 2 AFLIP
+void orc_set_scheme(void* h, int32_t scheme) {
+    dispatch(h, [&](auto& s) {
+        s.scheme = scheme;
+        return 0;
+    });
+}
+// affine matrices, 9 floats per particle, row d = C_d (the gradient of velocity component d)
+void orc_set_affine(void* h, const float* c) {
+    dispatch(h, [&](auto& s) {
+        using R = std::decay_t<decltype(s.dx)>;
+        for(size_t p = 0; p < s.C.size(); ++p)
+            for(int d = 0; d < 3; ++d)
+                for(int e = 0; e < 3; ++e) s.C[p][d][e] = (R)c[9 * p + 3 * d + e];
+        return 0;
+    });
+}
+void orc_get_affine(void* h, float* c) {
+    dispatch(h, [&](auto& s) {
+        for(size_t p = 0; p < s.C.size(); ++p)
+            for(int d = 0; d < 3; ++d)
+                for(int e = 0; e < 3; ++e) c[9 * p + 3 * d + e] = (float)s.C[p][d][e];
         return 0;
     });
 }
@@ -1316,6 +1518,29 @@ void orc_get_matrix(void* h, int32_t* cols, double* vals) {
         return 0;
     });
 }
+// IC(0) of the assembled system (PCGSolver.cpp:398-466): the inverse square roots of the modified
+// diagonal, one per active cell
+void orc_ic0_factor(void* h, double* inv_diag) {
+    dispatch(h, [&](auto& s) {
+        s.formIC0();
+        for(uint32_t i = 0; i < s.nActive; ++i) inv_diag[i] = (double)s.icInvDiag[i];
+        return 0;
+    });
+}
+// z = M^-1 r with the selected preconditioner (PCGSolver.cpp:221-269), r and z in active-cell order
+// (rounded to the oracle's precision on the way in)
+void orc_apply_precond(void* h, const double* r, double* z) {
+    dispatch(h, [&](auto& s) {
+        using R = std::decay_t<decltype(s.dx)>;
+        if(s.precond == 2) s.formIC0();
+        if(s.precond == 1) s.formJacobi();
+        std::vector<R> in(s.nActive), out;
+        for(uint32_t i = 0; i < s.nActive; ++i) in[i] = (R)r[i];
+        s.applyPrecond(in, out);
+        for(uint32_t i = 0; i < s.nActive; ++i) z[i] = (double)out[i];
+        return 0;
+    });
+}
 void orc_pcg_stats(void* h, uint32_t* iters, double* residual, int32_t* success) {
     dispatch(h, [&](auto& s) {
         *iters    = s.outIters;
@@ -1327,6 +1552,12 @@ void orc_pcg_stats(void* h, uint32_t* iters, double* residual, int32_t* success)
 int32_t orc_step(void* h, float dt) {
     return dispatch(h, [&](auto& s) { return s.step(dt) ? 0 : 1; });
 }
+int32_t orc_advance_velocity(void* h, float dt) {
+    return dispatch(h, [&](auto& s) {
+        using R = std::decay_t<decltype(s.dx)>;
+        return s.advanceVelocity((R)dt) ? 0 : 1;
+    });
+}
 int32_t orc_advance_frame(void* h, float* last_substep, uint32_t* substeps) {
     return dispatch(h, [&](auto& s) {
         int r = s.advanceFrame() ? 0 : 1;
@@ -1336,6 +1567,13 @@ int32_t orc_advance_frame(void* h, float* last_substep, uint32_t* substeps) {
     });
 }
 // individual stages, named after the reference methods
+void orc_correct_particle_positions(void* h, float dt, float stiffness, uint64_t seed) {
+    dispatch(h, [&](auto& s) {
+        using R = std::decay_t<decltype(s.dx)>;
+        s.correctParticlePositions((R)dt, (R)stiffness, seed);
+        return 0;
+    });
+}
 void orc_move_particles(void* h, float dt) { dispatch(h, [&](auto& s) { s.moveParticles(dt); return 0; }); }
 void orc_map_particles_to_grid(void* h) { dispatch(h, [&](auto& s) { s.mapParticles2Grid(); return 0; }); }
 void orc_extrapolate_velocity(void* h) { dispatch(h, [&](auto& s) { s.extrapolateVelocity(); return 0; }); }

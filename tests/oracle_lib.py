@@ -81,6 +81,14 @@ def lib():
         L.orc_get_system.argtypes = [vp, vp, vp]
         L.orc_get_matrix.argtypes = [vp, vp, vp]
         L.orc_pcg_stats.argtypes = [vp, P(u32), P(C.c_double), P(i32)]
+        L.orc_ic0_factor.argtypes = [vp, vp]
+        L.orc_correct_particle_positions.argtypes = [vp, f32, f32, u64]
+        L.orc_set_scheme.argtypes = [vp, i32]
+        L.orc_set_affine.argtypes = [vp, vp]
+        L.orc_get_affine.argtypes = [vp, vp]
+        L.orc_advance_velocity.restype = i32
+        L.orc_advance_velocity.argtypes = [vp, f32]
+        L.orc_apply_precond.argtypes = [vp, vp, vp]
         L.orc_step.restype = i32
         L.orc_step.argtypes = [vp, f32]
         L.orc_advance_frame.restype = i32
@@ -229,6 +237,34 @@ class Oracle:
         vals = np.zeros((na, 7), np.float64)
         self.L.orc_get_matrix(self.h, ptr(cols), ptr(vals))
         return cols, vals
+
+    def num_particles(self):
+        return int(self.L.orc_num_particles(self.h))
+
+    def set_scheme(self, scheme):
+        """0 FLIP, 1 APIC, 2 AFLIP"""
+        self.L.orc_set_scheme(self.h, scheme)
+
+    def affine(self):
+        c = np.zeros((self.num_particles(), 3, 3), np.float32)
+        self.L.orc_get_affine(self.h, ptr(c))
+        return c
+
+    def set_affine(self, c):
+        c = np.ascontiguousarray(c, np.float32)
+        assert c.shape == (self.num_particles(), 3, 3)
+        self.L.orc_set_affine(self.h, ptr(c))
+
+    def ic0_factor(self):
+        inv = np.zeros(self.num_active(), np.float64)
+        self.L.orc_ic0_factor(self.h, ptr(inv))
+        return inv
+
+    def apply_precond(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        self.L.orc_apply_precond(self.h, ptr(r), ptr(z))
+        return z
 
     def pcg_stats(self):
         it, res, ok = C.c_uint32(), C.c_double(), C.c_int32()

@@ -57,6 +57,7 @@ def test_default_params_are_reference_defaults(bnn):
     assert np.isclose(p.cg_relative_tolerance, 1e-15) and p.max_cg_iteration == 5000
     assert np.isclose(p.pic_flip_ratio, 0.97) and p.advection_steps == 1 and p.apply_gravity == 1
     assert p.boundary_sdf_at_init == 0 and p.correct_position == 0 and p.sort_frequency == 10
+    assert p.repulsive_force_stiffness == 50.0  # ParticleSolverData.h:143
 
 
 def test_host_generator_matches_oracle(bnn, oracle):
@@ -76,8 +77,9 @@ def test_create_reports_errors_without_exiting(bnn):
     assert st == 1 and not h.value
     assert b"CellSize" in L.bnn_flip_last_error(None)
     p = banana_amd.make_params(scenes.base_params(16))
-    p.correct_position = 1
-    assert L.bnn_flip_create(C.byref(p), C.byref(h)) == 5  # BNN_ERR_UNSUPPORTED
+    p.preconditioner = 7
+    assert L.bnn_flip_create(C.byref(p), C.byref(h)) == 1  # BNN_ERR_INVALID_ARG
+    assert b"preconditioner" in L.bnn_flip_last_error(None)
     # null arguments
     assert L.bnn_flip_step(None, 0.01) == 1
     assert L.bnn_flip_destroy(None) == 1

@@ -334,3 +334,61 @@ def test_mg_preconditioner_cuts_iterations(oracle, bnn, n):
     assert res[1][0] * 3 < res[0][0]
     assert rel_l2(res[1][1], res[0][1]) <= 1e-8
     assert rel_l2(res[1][2], res[0][2]) <= 1e-6
+
+
+def _mix64(x):
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def test_correct_positions_bit_exact(oracle, bnn):
+    """correctParticlePositions (PIC_Solver.cpp:377-485) on the device equals the oracle bit for bit
+    (same neighbour order: cells ascending, particles by index inside a cell; same seeded push),
+    including coincident pairs and particles near the walls."""
+    params, spec = scenes.dam_break(24, sphere=True)
+    pos, vel = bnn.generate_scene_host(params, spec, 13)
+    pos[100:150] = pos[200:250]  # coincident pairs take the random push
+    params.update(correct_position=1, random_seed=99)
+    g = bnn.FLIPSolver(params)
+    g.set_particles(pos, vel)
+    g.correct_particle_positions(DT)
+    o = oracle.Oracle(params)
+    o.set_particles(pos, vel)
+    seed = _mix64(99 ^ _mix64(1))  # the handle's seed for its first step
+    o.correct_particle_positions(DT, np.float32(params.get("repulsive_force_stiffness", 50.0)), seed)
+    pg, _ = g.particles()
+    po, _ = o.particles()
+    assert not np.array_equal(pg, pos)
+    assert np.array_equal(pg, po), np.abs(pg - po).max()
+
+
+def test_step_with_correct_position(oracle, bnn):
+    """Whole substeps with CorrectPosition on (move, correct, advance velocity), set up like
+    test_step_parity: the device in REFINED mode against the FP32 oracle (IC(0)-PCG at its floor) and
+    the FP64 oracle, with the same seeded push."""
+    params, spec = scenes.dam_break(32, sphere=True, boundary_sdf_at_init=True)
+    params.update(correct_position=1, random_seed=5, cg_relative_tolerance=1e-7)
+    pos, vel = bnn.generate_scene_host(params, spec, 11)
+    g = bnn.FLIPSolver(params, pressure_mode=1, refine_tolerance=1e-10)
+    g.set_particles(pos, vel)
+    g.step(DT)
+    assert g.stats()["pcg_success"] == 1
+    out = {}
+    for prec in (32, 64):
+        o = oracle.Oracle(params, precision=prec)
+        o.set_particles(pos, vel)
+        if prec == 64:
+            o.set_cg(1e-14, 20000)
+        o.move_particles(DT)
+        o.correct_particle_positions(DT, np.float32(50.0), _mix64(5 ^ _mix64(1)))
+        assert o.advance_velocity(DT) == 0
+        out[prec] = o.particles()
+    pg, vg = g.particles()
+    assert np.abs(pg - pos).max() > 1e-3  # the correction moved particles
+    for prec in (32, 64):
+        po, vo = out[prec]
+        assert np.abs(pg - po).max() <= 1e-5, prec
+        assert rel_l2(vg, vo) <= 1e-5, prec

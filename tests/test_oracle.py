@@ -348,6 +348,65 @@ def test_pcg_matches_dense_solve(oracle):
         assert np.linalg.norm(p - ref) <= 1e-11 * np.linalg.norm(ref)
 
 
+def _ic0_lower(o, A, inv):
+    """The IC(0) factor as a dense lower-triangular L (PCGSolver.h:35-50): L_ii = 1/inv_i and, below the
+    diagonal, L_ji = A_ji * inv_i on A's pattern."""
+    na = A.shape[0]
+    L = np.zeros_like(A)
+    for i in range(na):
+        if inv[i] == 0:
+            continue
+        L[i, i] = 1.0 / inv[i]
+        below = np.nonzero(A[i + 1:, i])[0] + i + 1
+        L[below, i] = A[below, i] * inv[i]
+    return L
+
+
+def test_ic0_factor_reproduces_the_matrix(oracle):
+    """IC(0) keeps A's pattern and drops the fill, so (L L^T)_ij = A_ij on the pattern wherever the
+    0.25 safeguard (PCGSolver.h:88-89) did not replace the pivot. The diagonal recurrence is the
+    closed form d_i = a_ii - sum over lower neighbours k of a_ik^2 / d_k (on the 7-point stencil no
+    two lower neighbours of a cell are coupled, so no dropped fill enters it)."""
+    params, o = _assembled(oracle, n=12)
+    A = _dense(o)
+    inv = o.ic0_factor()
+    nz = np.diag(A) > 0
+    assert np.all(inv[~nz] == 0)
+    # closed-form recurrence, natural order
+    d = np.zeros(A.shape[0])
+    guarded = np.zeros(A.shape[0], bool)
+    for i in range(A.shape[0]):
+        if not nz[i]:
+            continue
+        lower = [k for k in np.nonzero(A[i, :i])[0] if nz[k]]
+        di = A[i, i] - sum(A[i, k] ** 2 / d[k] for k in lower)
+        guarded[i] = di < 0.25 * A[i, i]
+        d[i] = A[i, i] if guarded[i] else di
+    assert np.allclose(inv[nz], 1.0 / np.sqrt(d[nz]), rtol=1e-12, atol=0)
+    L = _ic0_lower(o, A, inv)
+    M = L @ L.T
+    rows = nz & ~guarded
+    pat = (A != 0)
+    assert np.allclose(M[rows][:, nz][pat[rows][:, nz]], A[rows][:, nz][pat[rows][:, nz]], rtol=1e-12, atol=1e-18)
+
+
+def test_ic0_apply_inverts_the_factor(oracle):
+    """The forward (column-oriented) and backward (row-oriented) solves of PCGSolver.cpp:243-269
+    give z = (L L^T)^-1 r."""
+    params, o = _assembled(oracle, n=12)
+    A = _dense(o)
+    rhs, _ = o.system()
+    o.set_precond(2)
+    inv = o.ic0_factor()
+    z = o.apply_precond(rhs)
+    nz = np.diag(A) > 0
+    L = _ic0_lower(o, A, inv)
+    ref = np.zeros_like(rhs)
+    ref[nz] = np.linalg.solve((L @ L.T)[np.ix_(nz, nz)], rhs[nz])
+    assert np.allclose(z, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    assert np.all(z[~nz] == 0)
+
+
 def test_ic0_beats_jacobi(oracle):
     params, o = _assembled(oracle, n=16)
     o.set_cg(1e-10, 10000)

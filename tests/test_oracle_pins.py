@@ -257,3 +257,150 @@ def test_gravity_to_and_from_center(oracle):
             assert np.all(got[~reached] == 0), (gtype, d)
             if d == 1:
                 assert (~reached).sum() == nx * nz  # the top v layer
+
+
+# --------------------------------------------------------------------------------------------
+# f3: correctParticlePositions (PIC_Solver.cpp:377-485)
+# --------------------------------------------------------------------------------------------
+def _correct_setup(oracle, pos, cell=0.1):
+    params = scenes.base_params(10)
+    params.update(box_min=(0.0, 0.0, 0.0), box_max=(1.0, 1.0, 1.0), cell_size=cell)
+    o = oracle.Oracle(params, precision=64)
+    pos = np.asarray(pos, np.float32)
+    o.set_particles(pos, np.zeros_like(pos))
+    return params, o
+
+
+def test_correct_positions_pair_closed_form(oracle):
+    """Two particles closer than r = 2 (dx/2) / sqrt(3) push apart along their axis by
+    K (1 - d^2/r^2)^3 each, K = r * dt * stiffness; a particle with no neighbour within r stays."""
+    dx, dt, k = 0.1, 1.0 / 600.0, 50.0
+    r = 2.0 * (dx / 2.0) / np.sqrt(3.0)
+    d = 0.4 * r
+    pos = [[0.5, 0.5, 0.5], [0.5 + d, 0.5, 0.5], [0.2, 0.2, 0.2]]
+    params, o = _correct_setup(oracle, pos)
+    o.correct_particle_positions(np.float32(dt), np.float32(k), 7)
+    out, _ = o.particles()
+    push = r * dt * k * (1.0 - (np.float64(np.float32(d)) / r) ** 2) ** 3
+    assert np.allclose(out[0], [0.5 - push, 0.5, 0.5], rtol=0, atol=1e-6)
+    assert np.allclose(out[1], [0.5 + np.float32(d) + push, 0.5, 0.5], rtol=0, atol=1e-6)
+    assert np.array_equal(out[2], np.float32(pos[2]))
+
+
+def test_correct_positions_pairwise_antisymmetric(oracle):
+    """Away from walls and coincident pairs the pushes are equal and opposite per pair: the mean
+    position is unchanged (in FP64), while individual particles move."""
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(0.3, 0.7, (400, 3))
+    params, o = _correct_setup(oracle, pos)
+    p0, _ = o.particles()
+    o.correct_particle_positions(np.float32(1.0 / 600.0), np.float32(50.0), 1)
+    p1, _ = o.particles()
+    assert np.abs(p1 - p0).max() > 1e-4
+    assert np.abs(p1.astype(np.float64).mean(0) - p0.astype(np.float64).mean(0)).max() < 1e-6
+
+
+def test_correct_positions_coincident_pair_is_seeded(oracle):
+    """A coincident pair gets the random push (threshold / dt) u, |u_i| <= 1, scaled by K; the same
+    seed gives the same push, another seed another one."""
+    dx, dt, k = 0.1, 1.0 / 600.0, 50.0
+    r = 2.0 * (dx / 2.0) / np.sqrt(3.0)
+    pos = [[0.5, 0.5, 0.5], [0.5, 0.5, 0.5]]
+    outs = []
+    for seed in (3, 3, 4):
+        params, o = _correct_setup(oracle, pos)
+        o.correct_particle_positions(np.float32(dt), np.float32(k), seed)
+        outs.append(o.particles()[0])
+    assert np.array_equal(outs[0], outs[1]) and not np.array_equal(outs[0], outs[2])
+    bound = (0.01 * r / dt) * (r * dt * k)
+    assert np.all(np.abs(outs[0] - 0.5) <= bound * 1.0001)
+    assert np.abs(outs[0] - 0.5).max() > 0
+
+
+# --------------------------------------------------------------------------------------------
+# f4: APIC / AFLIP transfers (APIC_Solver.cpp:42-125, AFLIP_Solver.cpp:43-140)
+# --------------------------------------------------------------------------------------------
+def _affine_scene(oracle, n=12, precision=64, scheme=1, seed=2):
+    import banana_amd
+
+    params, spec = scenes.dam_break(n, sphere=True)
+    spec["vel_sigma"] = 0.5
+    pos, vel = banana_amd.generate_scene_host(params, spec, seed)
+    o = oracle.Oracle(params, precision=precision)
+    o.set_particles(pos, vel)
+    o.set_scheme(scheme)
+    return params, pos, vel, o
+
+
+def test_apic_p2g_with_zero_affine_is_flip_p2g(oracle):
+    """C = 0 makes the affine term vanish: APIC's splat equals FLIP's bit for bit."""
+    params, pos, vel, a = _affine_scene(oracle, precision=32, scheme=1)
+    _, _, _, f = _affine_scene(oracle, precision=32, scheme=0)
+    a.map_particles_to_grid()
+    f.map_particles_to_grid()
+    for fld in (oracle.F_U, oracle.F_V, oracle.F_W):
+        assert np.array_equal(a.field(fld), f.field(fld))
+
+
+def test_apic_transfers_reproduce_a_linear_field(oracle):
+    """Affine consistency (Jiang et al. 2015): with v_p = A x_p + b and C_p = A, the splat gives
+    every node with particle support exactly A x_node + b, and the gather returns v = A x + b and
+    C = A for particles whose stencil lies inside that support (FP64)."""
+    import banana_amd
+
+    params, spec = scenes.dam_break(12)
+    pos, _ = banana_amd.generate_scene_host(params, spec, 6)
+    A = np.array([[0.3, -0.2, 0.1], [0.05, 0.4, -0.3], [-0.1, 0.2, 0.25]])
+    b = np.array([0.5, -1.0, 0.25])
+    vel = (pos.astype(np.float64) @ A.T + b).astype(np.float32)
+    o = oracle.Oracle(params, precision=64)
+    o.set_particles(pos, vel)
+    o.set_scheme(1)
+    o.set_affine(np.repeat(A[None].astype(np.float32), pos.shape[0], 0))
+    o.map_particles_to_grid()
+    n, bmin, dx = scenes.grid_dims(params)
+    Af = A.astype(np.float32).astype(np.float64)
+    for d, fld in enumerate((oracle.F_U, oracle.F_V, oracle.F_W)):
+        g = o.field(fld)
+        valid = o.field((oracle.F_VALID_U, oracle.F_VALID_V, oracle.F_VALID_W)[d]) == 1
+        k, j, i = np.nonzero(valid)
+        off = [0.5, 0.5, 0.5]
+        off[d] = 0.0
+        x = np.stack([(i + off[0]) * dx + bmin[0], (j + off[1]) * dx + bmin[1], (k + off[2]) * dx + bmin[2]], 1)
+        # the particle velocities were rounded to float: the field is linear up to that rounding
+        expect = x @ Af[d] + b[d]
+        assert np.allclose(g[valid], expect, rtol=0, atol=2e-6), d
+    # gather back: set the grid to the exact linear field everywhere, particles get v and C exactly
+    for d, fld in enumerate((oracle.F_U, oracle.F_V, oracle.F_W)):
+        shape = o.field(fld).shape
+        kk, jj, ii = np.meshgrid(*[np.arange(s_) for s_ in shape], indexing="ij")
+        off = [0.5, 0.5, 0.5]
+        off[d] = 0.0
+        x = np.stack([(ii + off[0]) * dx + bmin[0], (jj + off[1]) * dx + bmin[1], (kk + off[2]) * dx + bmin[2]], -1)
+        o.set_field(fld, x @ Af[d] + b[d])
+    o.map_grid_to_particles()
+    pv, _ = o.particles()
+    C = o.affine()
+    _, vel2 = o.particles()
+    exp_v = pos.astype(np.float64) @ Af.T + b
+    assert np.allclose(vel2, exp_v, rtol=0, atol=1e-5)
+    assert np.allclose(C, np.repeat(Af[None], pos.shape[0], 0), rtol=0, atol=1e-5)
+
+
+def test_aflip_with_pic_ratio_zero_is_apic(oracle):
+    """MathHelpers::lerp(grid, particle + change, r) with r = 0 is the grid value: AFLIP's gather
+    then equals APIC's."""
+    params, pos, vel, a = _affine_scene(oracle, precision=32, scheme=1)
+    params2 = dict(params, pic_flip_ratio=0.0)
+    b = oracle.Oracle(params2, precision=32)
+    b.set_particles(pos, vel)
+    b.set_scheme(2)
+    for o in (a, b):
+        o.map_particles_to_grid()
+        o.extrapolate_velocity()
+        o.backup_grid_velocity()
+        o.add_gravity(np.float32(1.0 / 600.0))
+        o.compute_changes_grid_velocity()
+        o.map_grid_to_particles()
+    assert np.array_equal(a.particles()[1], b.particles()[1])
+    assert np.array_equal(a.affine(), b.affine())

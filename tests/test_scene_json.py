@@ -96,3 +96,17 @@ def test_cpp_adapter_and_cli_build(bnn, tmp_path):
     src.write_text('#include "bnn_flip_solver.hpp"\nint main(){ bnn::FLIP_Solver3D s; return s.getSolverName() == "FLIP_3DSolver" ? 0 : 1; }\n')
     subprocess.check_call(["g++", "-std=c++17", "-fsyntax-only", "-I" + os.path.join(root, "include"), str(src)])
     assert os.path.exists(os.path.join(root, "banana_amd", "bnn_flip_cli"))
+
+
+def test_solver_names_select_the_transfer_scheme(bnn):
+    """ParticleSolverFactory names of the PIC family (FLIP_Solver.h:37, APIC_Solver.h:37,
+    AFLIP_Solver.h:36) select the transfer scheme; RepulsiveForceStiffness and CorrectPosition are read."""
+    import copy
+
+    for name, scheme in (("FLIP_3DSolver", 0), ("APIC_3DSolver", 1), ("AFLIP_3DSolver", 2)):
+        d = copy.deepcopy(DAM)
+        d["GlobalParameters"]["Solver"] = name
+        d["SimulationParameters"].update(CorrectPosition=True, RepulsiveForceStiffness=20.0)
+        st, p, s, warn = parse(d)
+        assert st == 0 and p.transfer_scheme == scheme, name
+        assert p.correct_position == 1 and p.repulsive_force_stiffness == 20.0
