@@ -260,13 +260,13 @@ def served_leg(args, log, batching):
 def measured_traffic(kernel, grid):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE counter runs, scaled by a copy of known
     size) as recorded in profiles/ for this workload's grid, else None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"k_{kernel}_dambreak256_traffic.json")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "flip_step_dambreak256_traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f)
+            rec = json.load(f).get(kernel)
     except (OSError, ValueError):
         return None
-    if rec.get("grid") != grid:
+    if not rec or rec.get("grid") != grid:
         return None
     return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]
 
