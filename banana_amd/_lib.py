@@ -71,6 +71,15 @@ class IoParams(C.Structure):
                 ("frame_per_state", C.c_uint32), ("save_velocity", C.c_int32), ("save_object_index", C.c_int32)]
 
 
+class BatchShape(C.Structure):
+    _fields_ = [("p2g_segment", C.c_int32), ("pcg_chunk_first", C.c_uint32), ("pcg_chunk_max", C.c_uint32), ("graphs", C.c_int32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("shape", BatchShape), ("flushes", C.c_uint64), ("launches", C.c_uint64), ("scene_launches", C.c_uint64),
+                ("graph_replays", C.c_uint64), ("graph_captures", C.c_uint64), ("mean_scenes_per_flush", C.c_double)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("workers", C.c_uint32),
                 ("batching", C.c_int32), ("reserved", C.c_int32 * 5)]
@@ -155,6 +164,8 @@ SIGNATURES = {
     "bnn_flip_batch_step": (_i32, [_vp, _P(_vp), _P(_f32), _u32, _P(_i32)]),
     "bnn_flip_batch_advance_frame": (_i32, [_vp, _P(_vp), _u32, _P(_i32)]),
     "bnn_flip_batch_pressure_projection": (_i32, [_vp, _P(_vp), _P(_f32), _u32, _P(_i32)]),
+    "bnn_flip_batch_set_shape": (_i32, [_vp, _vp]),
+    "bnn_flip_batch_get_info": (_i32, [_vp, _vp]),
     "bnn_server_create": (_i32, [_P(ServerConfig), _P(_vp)]),
     "bnn_server_destroy": (_i32, [_vp]),
     "bnn_server_last_error": (C.c_char_p, [_vp]),

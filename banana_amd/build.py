@@ -11,7 +11,7 @@ DEPS = [os.path.join(HERE, "csrc", f) for f in ("flip_solver.hip", "flip_kernels
     os.path.join(os.path.dirname(HERE), "include", "bnn_flip.h")]
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
     # exact-order float arithmetic (bit-exact cell indices and fluid SDF)
     "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
     # no packed FP32 instructions

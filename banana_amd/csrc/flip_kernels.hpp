@@ -1,4 +1,5 @@
 // flip_kernels.hpp -- gfx950 kernels of the FLIP substep (included once, by flip_solver.hip).
+// Each kernel is a functor launched through k_one / k_many (flip_launch.hpp).
 //
 // Layout in HBM (one scene):
 //   particles   SoA float px..vz, grid coordinates gx..gz, uint32 id; kept cell-sorted after binning
@@ -10,6 +11,7 @@
 #pragma once
 
 #include "flip_device.hpp"
+#include "flip_launch.hpp"
 
 namespace bnn {
 
@@ -35,57 +37,68 @@ __device__ __forceinline__ void vel_from_grid(const VelGrid& G, const GridGeom& 
     uz = interp3(G.w, g.nx, g.ny, g.nz + 1, gpx - 0.5f, gpy - 0.5f, gpz);
 }
 
-__global__ void __launch_bounds__(kBlock) k_move(Particles P, uint32_t n, VelGrid G, GridGeom g, BoxDev dom, float dt,
-                                                 uint32_t advSteps, int reflect, float reflMult) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    float x = P.px[p], y = P.py[p], z = P.pz[p];
-    float vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
-    bool  changed = false;
-    const float sub = dt / (float)advSteps;
-    for(uint32_t s = 0; s < advSteps; ++s) {
-        // trace_rk2
-        float ux, uy, uz;
-        vel_from_grid(G, g, (x - g.bminx) / g.dx, (y - g.bminy) / g.dx, (z - g.bminz) / g.dx, ux, uy, uz);
-        const float h  = 0.5f * sub;
-        const float mx = x + h * ux, my = y + h * uy, mz = z + h * uz;
-        vel_from_grid(G, g, (mx - g.bminx) / g.dx, (my - g.bminy) / g.dx, (mz - g.bminz) / g.dx, ux, uy, uz);
-        x = x + sub * ux;
-        y = y + sub * uy;
-        z = z + sub * uz;
-        // constrainToBoundary(ppos, pvel)
-        const float phi = box_sd_out(dom, x, y, z);
-        if(phi < 0) {
-            float gx, gy, gz;
-            box_grad_out(dom, x, y, z, gx, gy, gz);
-            const float m2 = gx * gx + gy * gy + gz * gz;
-            if(m2 > kFltMin) {
-                const float s = sqrtf(m2);
-                gx /= s;
-                gy /= s;
-                gz /= s;
-                x -= phi * gx;
-                y -= phi * gy;
-                z -= phi * gz;
-                if(reflect) {
-                    const float dn = gx * vx + gy * vy + gz * vz;
-                    vx             = (vx - 2.f * dn * gx) * reflMult;
-                    vy             = (vy - 2.f * dn * gy) * reflMult;
-                    vz             = (vz - 2.f * dn * gz) * reflMult;
-                    changed        = true;
+struct k_move {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    uint32_t n;
+    VelGrid G;
+    GridGeom g;
+    BoxDev dom;
+    float dt;
+    uint32_t advSteps;
+    int reflect;
+    float reflMult;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t p = blk.id * blockDim.x + threadIdx.x;
+        if(p >= n) return;
+        float x = P.px[p], y = P.py[p], z = P.pz[p];
+        float vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
+        bool  changed = false;
+        const float sub = dt / (float)advSteps;
+        for(uint32_t s = 0; s < advSteps; ++s) {
+            // trace_rk2
+            float ux, uy, uz;
+            vel_from_grid(G, g, (x - g.bminx) / g.dx, (y - g.bminy) / g.dx, (z - g.bminz) / g.dx, ux, uy, uz);
+            const float h  = 0.5f * sub;
+            const float mx = x + h * ux, my = y + h * uy, mz = z + h * uz;
+            vel_from_grid(G, g, (mx - g.bminx) / g.dx, (my - g.bminy) / g.dx, (mz - g.bminz) / g.dx, ux, uy, uz);
+            x = x + sub * ux;
+            y = y + sub * uy;
+            z = z + sub * uz;
+            // constrainToBoundary(ppos, pvel)
+            const float phi = box_sd_out(dom, x, y, z);
+            if(phi < 0) {
+                float gx, gy, gz;
+                box_grad_out(dom, x, y, z, gx, gy, gz);
+                const float m2 = gx * gx + gy * gy + gz * gz;
+                if(m2 > kFltMin) {
+                    const float s = sqrtf(m2);
+                    gx /= s;
+                    gy /= s;
+                    gz /= s;
+                    x -= phi * gx;
+                    y -= phi * gy;
+                    z -= phi * gz;
+                    if(reflect) {
+                        const float dn = gx * vx + gy * vy + gz * vz;
+                        vx             = (vx - 2.f * dn * gx) * reflMult;
+                        vy             = (vy - 2.f * dn * gy) * reflMult;
+                        vz             = (vz - 2.f * dn * gz) * reflMult;
+                        changed        = true;
+                    }
                 }
             }
         }
+        P.px[p] = x;
+        P.py[p] = y;
+        P.pz[p] = z;
+        if(changed) {
+            P.vx[p] = vx;
+            P.vy[p] = vy;
+            P.vz[p] = vz;
+        }
     }
-    P.px[p] = x;
-    P.py[p] = y;
-    P.pz[p] = z;
-    if(changed) {
-        P.vx[p] = vx;
-        P.vy[p] = vy;
-        P.vz[p] = vz;
-    }
-}
+};
 
 // (i, j, k) of the linear index t of an sx x sy x sz array (x fastest), in 32-bit arithmetic: every
 // grid array here has fewer than 2^32 entries, and 64-bit division is many instructions on the GPU
@@ -121,76 +134,102 @@ __device__ __forceinline__ uint32_t key_run(uint32_t c, bool active, int& leader
     return (uint32_t)(end - lane);
 }
 
-__global__ void __launch_bounds__(kBlock) k_cell_key(const float* __restrict__ px, const float* __restrict__ py,
-                                                     const float* __restrict__ pz, uint32_t n, GridGeom g,
-                                                     uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
-    const uint32_t p      = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool     active = p < n;
-    uint32_t       c      = 0xFFFFFFFFu;
-    if(active) {
-        const int i = cell_axis(px[p], g.bminx, g.dx, g.nx);
-        const int j = cell_axis(py[p], g.bminy, g.dx, g.ny);
-        const int k = cell_axis(pz[p], g.bminz, g.dx, g.nz);
-        c           = (uint32_t)lin3(i, j, k, g.nx, g.ny);
-        key[p]      = c;
+struct k_cell_key {
+    static constexpr int kThreads = kBlock;
+    const float* __restrict__ px;
+    const float* __restrict__ py;
+    const float* __restrict__ pz;
+    uint32_t n;
+    GridGeom g;
+    uint32_t* __restrict__ key;
+    uint32_t* __restrict__ count;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t p      = blk.id * blockDim.x + threadIdx.x;
+        const bool     active = p < n;
+        uint32_t       c      = 0xFFFFFFFFu;
+        if(active) {
+            const int i = cell_axis(px[p], g.bminx, g.dx, g.nx);
+            const int j = cell_axis(py[p], g.bminy, g.dx, g.ny);
+            const int k = cell_axis(pz[p], g.bminz, g.dx, g.nz);
+            c           = (uint32_t)lin3(i, j, k, g.nx, g.ny);
+            key[p]      = c;
+        }
+        int            leader;
+        const uint32_t len = key_run(c, active, leader);
+        if(len) atomicAdd(&count[c], len);
     }
-    int            leader;
-    const uint32_t len = key_run(c, active, leader);
-    if(len) atomicAdd(&count[c], len);
-}
+};
 
 // counts return to zero here (each particle takes one slot), so the count array needs no clearing;
 // a run's lanes take consecutive slots in lane order
-__global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t* __restrict__ key, uint32_t n,
-                                                    const uint32_t* __restrict__ cell_start, uint32_t* __restrict__ count,
-                                                    uint32_t* __restrict__ perm) {
-    const uint32_t p      = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool     active = p < n;
-    const uint32_t c      = active ? key[p] : 0xFFFFFFFFu;
-    int            leader;
-    const uint32_t len  = key_run(c, active, leader);
-    uint32_t       left = 0;  // count before the run's subtraction
-    if(len) left = atomicSub(&count[c], len);
-    left = __shfl(left, leader, 64);
-    const uint32_t runLen = __shfl(len, leader, 64);
-    if(active) perm[cell_start[c] + left - runLen + (uint32_t)((int)(threadIdx.x & 63) - leader)] = p;
-}
+struct k_scatter {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ key;
+    uint32_t n;
+    const uint32_t* __restrict__ cell_start;
+    uint32_t* __restrict__ count;
+    uint32_t* __restrict__ perm;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t p      = blk.id * blockDim.x + threadIdx.x;
+        const bool     active = p < n;
+        const uint32_t c      = active ? key[p] : 0xFFFFFFFFu;
+        int            leader;
+        const uint32_t len  = key_run(c, active, leader);
+        uint32_t       left = 0;  // count before the run's subtraction
+        if(len) left = atomicSub(&count[c], len);
+        left = __shfl(left, leader, 64);
+        const uint32_t runLen = __shfl(len, leader, 64);
+        if(active) perm[cell_start[c] + left - runLen + (uint32_t)((int)(threadIdx.x & 63) - leader)] = p;
+    }
+};
 
 // make each cell's segment ascending: the counting sort becomes stable (ties by particle index)
-__global__ void __launch_bounds__(kBlock) k_segsort(const uint32_t* __restrict__ cell_start, uint32_t ncells,
-                                                    uint32_t* __restrict__ perm) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if(c >= ncells) return;
-    const uint32_t b = cell_start[c], e = cell_start[c + 1];
-    if(e - b < 2) return;
-    for(uint32_t i = b + 1; i < e; ++i) {
-        uint32_t v = perm[i];
-        uint32_t j = i;
-        while(j > b && perm[j - 1] > v) {
-            perm[j] = perm[j - 1];
-            --j;
+struct k_segsort {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ cell_start;
+    uint32_t ncells;
+    uint32_t* __restrict__ perm;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t c = blk.id * blockDim.x + threadIdx.x;
+        if(c >= ncells) return;
+        const uint32_t b = cell_start[c], e = cell_start[c + 1];
+        if(e - b < 2) return;
+        for(uint32_t i = b + 1; i < e; ++i) {
+            uint32_t v = perm[i];
+            uint32_t j = i;
+            while(j > b && perm[j - 1] > v) {
+                perm[j] = perm[j - 1];
+                --j;
+            }
+            perm[j] = v;
         }
-        perm[j] = v;
     }
-}
+};
 
-__global__ void __launch_bounds__(kBlock) k_permute(Particles in, Particles out, const uint32_t* __restrict__ perm, uint32_t n,
-                                                    GridGeom g) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    const uint32_t s = perm[i];
-    const float    x = in.px[s], y = in.py[s], z = in.pz[s];
-    out.px[i]        = x;
-    out.py[i]        = y;
-    out.pz[i]        = z;
-    out.vx[i]        = in.vx[s];
-    out.vy[i]        = in.vy[s];
-    out.vz[i]        = in.vz[s];
-    out.id[i]        = in.id[s];
-    out.gx[i]        = (x - g.bminx) / g.dx;
-    out.gy[i]        = (y - g.bminy) / g.dx;
-    out.gz[i]        = (z - g.bminz) / g.dx;
-}
+struct k_permute {
+    static constexpr int kThreads = kBlock;
+    Particles in;
+    Particles out;
+    const uint32_t* __restrict__ perm;
+    uint32_t n;
+    GridGeom g;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t i = blk.id * blockDim.x + threadIdx.x;
+        if(i >= n) return;
+        const uint32_t s = perm[i];
+        const float    x = in.px[s], y = in.py[s], z = in.pz[s];
+        out.px[i]        = x;
+        out.py[i]        = y;
+        out.pz[i]        = z;
+        out.vx[i]        = in.vx[s];
+        out.vy[i]        = in.vy[s];
+        out.vz[i]        = in.vz[s];
+        out.id[i]        = in.id[s];
+        out.gx[i]        = (x - g.bminx) / g.dx;
+        out.gy[i]        = (y - g.bminy) / g.dx;
+        out.gz[i]        = (z - g.bminz) / g.dx;
+    }
+};
 
 //==================================================================================================
 // Z-order sort of the particles (PIC_Solver.cpp:315-328 -> NeighborSearch::z_sort,
@@ -223,116 +262,147 @@ __device__ __forceinline__ uint64_t zsort_key(float x, float y, float z, float i
 }
 
 // keys in external order, split in 32-bit halves; values = external index
-__global__ void __launch_bounds__(kBlock) k_zsort_keys(Particles P, uint32_t n, float inv, uint32_t* __restrict__ klo,
-                                                       uint32_t* __restrict__ khi, uint32_t* __restrict__ val) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    const uint64_t k = zsort_key(P.px[p], P.py[p], P.pz[p], inv);
-    const uint32_t e = P.id[p];
-    klo[e]           = (uint32_t)k;
-    khi[e]           = (uint32_t)(k >> 32);
-    val[e]           = e;
-}
+struct k_zsort_keys {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    uint32_t n;
+    float inv;
+    uint32_t* __restrict__ klo;
+    uint32_t* __restrict__ khi;
+    uint32_t* __restrict__ val;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t p = blk.id * blockDim.x + threadIdx.x;
+        if(p >= n) return;
+        const uint64_t k = zsort_key(P.px[p], P.py[p], P.pz[p], inv);
+        const uint32_t e = P.id[p];
+        klo[e]           = (uint32_t)k;
+        khi[e]           = (uint32_t)(k >> 32);
+        val[e]           = e;
+    }
+};
 
 // AND and OR of all keys (digits on which every key agrees need no pass); acc = {and, or}, preset
 // to {~0, 0}
-__global__ void __launch_bounds__(kBlock) k_zsort_andor(const uint32_t* __restrict__ klo, const uint32_t* __restrict__ khi,
-                                                        uint32_t n, unsigned long long* __restrict__ acc) {
-    __shared__ unsigned long long sa[kBlock / 64], so[kBlock / 64];
-    unsigned long long a = ~0ull, o = 0ull;
-    for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const unsigned long long k = (unsigned long long)khi[i] << 32 | klo[i];
-        a &= k;
-        o |= k;
-    }
-    for(int off = 32; off > 0; off >>= 1) {
-        a &= __shfl_xor(a, off, 64);
-        o |= __shfl_xor(o, off, 64);
-    }
-    if((threadIdx.x & 63) == 0) {
-        sa[threadIdx.x >> 6] = a;
-        so[threadIdx.x >> 6] = o;
-    }
-    __syncthreads();
-    if(threadIdx.x == 0) {
-        for(int w = 1; w < kBlock / 64; ++w) {
-            a &= sa[w];
-            o |= so[w];
+struct k_zsort_andor {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ klo;
+    const uint32_t* __restrict__ khi;
+    uint32_t n;
+    unsigned long long* __restrict__ acc;
+    __device__ void operator()(Blk blk) const {
+        __shared__ unsigned long long sa[kBlock / 64], so[kBlock / 64];
+        unsigned long long a = ~0ull, o = 0ull;
+        for(uint32_t i = blk.id * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
+            const unsigned long long k = (unsigned long long)khi[i] << 32 | klo[i];
+            a &= k;
+            o |= k;
         }
-        atomicAnd(&acc[0], a);
-        atomicOr(&acc[1], o);
+        for(int off = 32; off > 0; off >>= 1) {
+            a &= __shfl_xor(a, off, 64);
+            o |= __shfl_xor(o, off, 64);
+        }
+        if((threadIdx.x & 63) == 0) {
+            sa[threadIdx.x >> 6] = a;
+            so[threadIdx.x >> 6] = o;
+        }
+        __syncthreads();
+        if(threadIdx.x == 0) {
+            for(int w = 1; w < kBlock / 64; ++w) {
+                a &= sa[w];
+                o |= so[w];
+            }
+            atomicAnd(&acc[0], a);
+            atomicOr(&acc[1], o);
+        }
     }
-}
+};
 
 constexpr int kRsItems = 16;                  // keys per thread per tile
 constexpr int kRsTile  = kBlock * kRsItems;   // keys per tile (one workgroup)
 static_assert(kBlock == 256, "the radix passes use one thread per digit value");
 
 // digit histogram of one tile, stored digit-major: hist[d * ntiles + tile]
-__global__ void __launch_bounds__(kBlock) k_rs_hist(const uint32_t* __restrict__ kw, uint32_t n, int shift, uint32_t ntiles,
-                                                    uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kRsTile;
-    for(int c = 0; c < kRsItems; ++c) {
-        const size_t i = base + (size_t)c * kBlock + threadIdx.x;
-        if(i < n) atomicAdd(&h[(kw[i] >> shift) & 255u], 1u);
+struct k_rs_hist {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ kw;
+    uint32_t n;
+    int shift;
+    uint32_t ntiles;
+    uint32_t* __restrict__ hist;
+    __device__ void operator()(Blk blk) const {
+        __shared__ uint32_t h[256];
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        const size_t base = (size_t)blk.id * kRsTile;
+        for(int c = 0; c < kRsItems; ++c) {
+            const size_t i = base + (size_t)c * kBlock + threadIdx.x;
+            if(i < n) atomicAdd(&h[(kw[i] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        hist[(size_t)threadIdx.x * ntiles + blk.id] = h[threadIdx.x];
     }
-    __syncthreads();
-    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-}
+};
 
 // stable scatter of one tile: offs = exclusive scan of hist; inside the tile, keys keep their order
 // (wave ballots give the rank among equal digits, earlier waves and chunks come first)
-__global__ void __launch_bounds__(kBlock) k_rs_scatter(const uint32_t* __restrict__ klo, const uint32_t* __restrict__ khi,
-                                                       const uint32_t* __restrict__ val, uint32_t n, int word, int shift,
-                                                       uint32_t ntiles, const uint32_t* __restrict__ offs,
-                                                       uint32_t* __restrict__ olo, uint32_t* __restrict__ ohi,
-                                                       uint32_t* __restrict__ oval) {
-    constexpr int       kWaves = kBlock / 64;
-    __shared__ uint32_t next[256];
-    __shared__ uint32_t wcnt[kWaves][256];
-    const int           t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    next[t] = offs[(size_t)t * ntiles + blockIdx.x];
-    for(int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
-    for(int c = 0; c < kRsItems; ++c) {
-        const size_t i   = (size_t)blockIdx.x * kRsTile + (size_t)c * kBlock + t;
-        const bool   act = i < n;
-        uint32_t     lo = 0, hi = 0, v = 0, d = 0;
-        if(act) {
-            lo = klo[i];
-            hi = khi[i];
-            v  = val[i];
-            d  = ((word ? hi : lo) >> shift) & 255u;
-        }
-        uint64_t peers = __ballot(act);
-        for(int b = 0; b < 8; ++b) {
-            const uint64_t m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
-        }
-        const uint32_t rank = (uint32_t)__popcll(peers & below);
-        if(act && rank == 0) wcnt[wv][d] = (uint32_t)__popcll(peers);
+struct k_rs_scatter {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ klo;
+    const uint32_t* __restrict__ khi;
+    const uint32_t* __restrict__ val;
+    uint32_t n;
+    int word;
+    int shift;
+    uint32_t ntiles;
+    const uint32_t* __restrict__ offs;
+    uint32_t* __restrict__ olo;
+    uint32_t* __restrict__ ohi;
+    uint32_t* __restrict__ oval;
+    __device__ void operator()(Blk blk) const {
+        constexpr int       kWaves = kBlock / 64;
+        __shared__ uint32_t next[256];
+        __shared__ uint32_t wcnt[kWaves][256];
+        const int           t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        next[t] = offs[(size_t)t * ntiles + blk.id];
+        for(int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
         __syncthreads();
-        if(act) {
-            uint32_t pos = next[d] + rank;
-            for(int w = 0; w < wv; ++w) pos += wcnt[w][d];
-            olo[pos]  = lo;
-            ohi[pos]  = hi;
-            oval[pos] = v;
+        const uint64_t below = (1ull << lane) - 1ull;
+        for(int c = 0; c < kRsItems; ++c) {
+            const size_t i   = (size_t)blk.id * kRsTile + (size_t)c * kBlock + t;
+            const bool   act = i < n;
+            uint32_t     lo = 0, hi = 0, v = 0, d = 0;
+            if(act) {
+                lo = klo[i];
+                hi = khi[i];
+                v  = val[i];
+                d  = ((word ? hi : lo) >> shift) & 255u;
+            }
+            uint64_t peers = __ballot(act);
+            for(int b = 0; b < 8; ++b) {
+                const uint64_t m = __ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? m : ~m;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & below);
+            if(act && rank == 0) wcnt[wv][d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if(act) {
+                uint32_t pos = next[d] + rank;
+                for(int w = 0; w < wv; ++w) pos += wcnt[w][d];
+                olo[pos]  = lo;
+                ohi[pos]  = hi;
+                oval[pos] = v;
+            }
+            __syncthreads();
+            uint32_t tot = 0;
+            for(int w = 0; w < kWaves; ++w) {
+                tot += wcnt[w][t];
+                wcnt[w][t] = 0;
+            }
+            next[t] += tot;
+            __syncthreads();
         }
-        __syncthreads();
-        uint32_t tot = 0;
-        for(int w = 0; w < kWaves; ++w) {
-            tot += wcnt[w][t];
-            wcnt[w][t] = 0;
-        }
-        next[t] += tot;
-        __syncthreads();
     }
-}
+};
 
 //==================================================================================================
 // PIC_Solver::correctParticlePositions (PIC_Solver.cpp:377-485), 3-D: each particle is pushed away
@@ -362,89 +432,115 @@ struct CorrectParams {
     uint64_t seed;
 };
 
-__global__ void __launch_bounds__(kBlock) k_correct_positions(Particles P, const uint32_t* __restrict__ cell_start, uint32_t n,
-                                                              GridGeom g, BoxDev dom, CorrectParams c, float* __restrict__ ox,
-                                                              float* __restrict__ oy, float* __restrict__ oz) {
-#pragma clang fp contract(off)
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    const float    x = P.px[p], y = P.py[p], z = P.pz[p];
-    const uint32_t pid = P.id[p];
-    const int      ci = (int)((x - g.bminx) / g.dx), cj = (int)((y - g.bminy) / g.dx), ck = (int)((z - g.bminz) / g.dx);
-    float          sx = 0.f, sy = 0.f, sz = 0.f;
-    for(int k = ck - 1; k <= ck + 1; ++k) {
-        if(k < 0 || k >= g.nz) continue;
-        for(int j = cj - 1; j <= cj + 1; ++j) {
-            if(j < 0 || j >= g.ny) continue;
-            for(int i = ci - 1; i <= ci + 1; ++i) {
-                if(i < 0 || i >= g.nx) continue;
-                const size_t   cell = lin3(i, j, k, g.nx, g.ny);
-                const uint32_t qb = cell_start[cell], qe = cell_start[cell + 1];
-                for(uint32_t q = qb; q < qe; ++q) {
-                    if(q == p) continue;
-                    const float dx = x - P.px[q], dy = y - P.py[q], dz = z - P.pz[q];
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    const float x2 = d2 / c.radius2;
-                    const float t  = 1.0f - x2;
-                    const float w  = x2 > 1.0f ? 0.f : (t * t) * t;
-                    if(d2 > c.threshold2) {
-                        const float s = sqrtf(d2);
-                        sx += w * dx / s;
-                        sy += w * dy / s;
-                        sz += w * dz / s;
-                    } else {
-                        const uint32_t qid = P.id[q];
-                        sx += c.push * rand11(c.seed, pid, qid, 0);
-                        sy += c.push * rand11(c.seed, pid, qid, 1);
-                        sz += c.push * rand11(c.seed, pid, qid, 2);
+struct k_correct_positions {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    const uint32_t* __restrict__ cell_start;
+    uint32_t n;
+    GridGeom g;
+    BoxDev dom;
+    CorrectParams c;
+    float* __restrict__ ox;
+    float* __restrict__ oy;
+    float* __restrict__ oz;
+    __device__ void operator()(Blk blk) const {
+    #pragma clang fp contract(off)
+        const uint32_t p = blk.id * blockDim.x + threadIdx.x;
+        if(p >= n) return;
+        const float    x = P.px[p], y = P.py[p], z = P.pz[p];
+        const uint32_t pid = P.id[p];
+        const int      ci = (int)((x - g.bminx) / g.dx), cj = (int)((y - g.bminy) / g.dx), ck = (int)((z - g.bminz) / g.dx);
+        float          sx = 0.f, sy = 0.f, sz = 0.f;
+        for(int k = ck - 1; k <= ck + 1; ++k) {
+            if(k < 0 || k >= g.nz) continue;
+            for(int j = cj - 1; j <= cj + 1; ++j) {
+                if(j < 0 || j >= g.ny) continue;
+                for(int i = ci - 1; i <= ci + 1; ++i) {
+                    if(i < 0 || i >= g.nx) continue;
+                    const size_t   cell = lin3(i, j, k, g.nx, g.ny);
+                    const uint32_t qb = cell_start[cell], qe = cell_start[cell + 1];
+                    for(uint32_t q = qb; q < qe; ++q) {
+                        if(q == p) continue;
+                        const float dx = x - P.px[q], dy = y - P.py[q], dz = z - P.pz[q];
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const float x2 = d2 / c.radius2;
+                        const float t  = 1.0f - x2;
+                        const float w  = x2 > 1.0f ? 0.f : (t * t) * t;
+                        if(d2 > c.threshold2) {
+                            const float s = sqrtf(d2);
+                            sx += w * dx / s;
+                            sy += w * dy / s;
+                            sz += w * dz / s;
+                        } else {
+                            const uint32_t qid = P.id[q];
+                            sx += c.push * rand11(c.seed, pid, qid, 0);
+                            sy += c.push * rand11(c.seed, pid, qid, 1);
+                            sz += c.push * rand11(c.seed, pid, qid, 2);
+                        }
                     }
                 }
             }
         }
-    }
-    float nx_ = x, ny_ = y, nz_ = z;
-    for(uint32_t it = 0; it < c.advSteps; ++it) {
-        nx_ += sx * c.K;
-        ny_ += sy * c.K;
-        nz_ += sz * c.K;
-        const float phi = box_sd_out(dom, nx_, ny_, nz_);
-        if(phi < 0) {
-            float gx, gy, gz;
-            box_grad_out(dom, nx_, ny_, nz_, gx, gy, gz);
-            const float m2 = gx * gx + gy * gy + gz * gz;
-            if(m2 > kFltMin) {
-                const float s = sqrtf(m2);
-                gx /= s;
-                gy /= s;
-                gz /= s;
-                nx_ -= phi * gx;
-                ny_ -= phi * gy;
-                nz_ -= phi * gz;
+        float nx_ = x, ny_ = y, nz_ = z;
+        for(uint32_t it = 0; it < c.advSteps; ++it) {
+            nx_ += sx * c.K;
+            ny_ += sy * c.K;
+            nz_ += sz * c.K;
+            const float phi = box_sd_out(dom, nx_, ny_, nz_);
+            if(phi < 0) {
+                float gx, gy, gz;
+                box_grad_out(dom, nx_, ny_, nz_, gx, gy, gz);
+                const float m2 = gx * gx + gy * gy + gz * gz;
+                if(m2 > kFltMin) {
+                    const float s = sqrtf(m2);
+                    gx /= s;
+                    gy /= s;
+                    gz /= s;
+                    nx_ -= phi * gx;
+                    ny_ -= phi * gy;
+                    nz_ -= phi * gz;
+                }
             }
         }
+        ox[p] = nx_;
+        oy[p] = ny_;
+        oz[p] = nz_;
     }
-    ox[p] = nx_;
-    oy[p] = ny_;
-    oz[p] = nz_;
-}
+};
 
 // identity elements of the AND / OR key accumulators
-__global__ void k_zsort_acc_init(unsigned long long* __restrict__ acc) {
-    if(threadIdx.x == 0) {
-        acc[0] = ~0ull;
-        acc[1] = 0ull;
+struct k_zsort_acc_init {
+    static constexpr int kThreads = 64;
+    unsigned long long* __restrict__ acc;
+    __device__ void operator()(Blk blk) const {
+        if(threadIdx.x == 0) {
+            acc[0] = ~0ull;
+            acc[1] = 0ull;
+        }
     }
-}
+};
 
 // rank[perm[q]] = q, then every particle's external id becomes its rank
-__global__ void __launch_bounds__(kBlock) k_zsort_rank(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ rank) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if(q < n) rank[perm[q]] = q;
-}
-__global__ void __launch_bounds__(kBlock) k_zsort_rename(uint32_t* __restrict__ id, uint32_t n, const uint32_t* __restrict__ rank) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p < n) id[p] = rank[id[p]];
-}
+struct k_zsort_rank {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ perm;
+    uint32_t n;
+    uint32_t* __restrict__ rank;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t q = blk.id * blockDim.x + threadIdx.x;
+        if(q < n) rank[perm[q]] = q;
+    }
+};
+struct k_zsort_rename {
+    static constexpr int kThreads = kBlock;
+    uint32_t* __restrict__ id;
+    uint32_t n;
+    const uint32_t* __restrict__ rank;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t p = blk.id * blockDim.x + threadIdx.x;
+        if(p < n) id[p] = rank[id[p]];
+    }
+};
 
 //--------------------------------------------------------------------------------------------------
 // device-wide exclusive scan of uint32 (3 phases); tile = 256 threads x 8 items
@@ -472,52 +568,70 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return off + x - v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restrict__ in, size_t n, uint32_t* __restrict__ partial) {
-    __shared__ uint32_t sh[kBlock / 64];
-    const size_t        base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    uint32_t            s    = 0;
-#pragma unroll
-    for(int i = 0; i < kScanItems; ++i)
-        if(base + i < n) s += in[base + i];
-    uint32_t total;
-    block_exclusive_scan(s, sh, total);
-    if(threadIdx.x == 0) partial[blockIdx.x] = total;
-}
+struct k_scan_reduce {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ in;
+    size_t n;
+    uint32_t* __restrict__ partial;
+    __device__ void operator()(Blk blk) const {
+        __shared__ uint32_t sh[kBlock / 64];
+        const size_t        base = (size_t)blk.id * kScanTile + (size_t)threadIdx.x * kScanItems;
+        uint32_t            s    = 0;
+    #pragma unroll
+        for(int i = 0; i < kScanItems; ++i)
+            if(base + i < n) s += in[base + i];
+        uint32_t total;
+        block_exclusive_scan(s, sh, total);
+        if(threadIdx.x == 0) partial[blk.id] = total;
+    }
+};
 
 // single block: exclusive scan of the per-tile partials, in chunks of 256
-__global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t* partial, uint32_t np, uint32_t* total_out) {
-    __shared__ uint32_t sh[kBlock / 64];
-    uint32_t            carry = 0;
-    for(uint32_t b = 0; b < np; b += kBlock) {
-        const uint32_t i = b + threadIdx.x;
-        const uint32_t v = i < np ? partial[i] : 0u;
-        uint32_t       tot;
-        const uint32_t ex = block_exclusive_scan(v, sh, tot);
-        if(i < np) partial[i] = carry + ex;
-        carry += tot;
+struct k_scan_partials {
+    static constexpr int kThreads = kBlock;
+    uint32_t* partial;
+    uint32_t np;
+    uint32_t* total_out;
+    __device__ void operator()(Blk blk) const {
+        __shared__ uint32_t sh[kBlock / 64];
+        uint32_t            carry = 0;
+        for(uint32_t b = 0; b < np; b += kBlock) {
+            const uint32_t i = b + threadIdx.x;
+            const uint32_t v = i < np ? partial[i] : 0u;
+            uint32_t       tot;
+            const uint32_t ex = block_exclusive_scan(v, sh, tot);
+            if(i < np) partial[i] = carry + ex;
+            carry += tot;
+        }
+        if(threadIdx.x == 0 && total_out) *total_out = carry;
     }
-    if(threadIdx.x == 0 && total_out) *total_out = carry;
-}
+};
 
-__global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* __restrict__ in, size_t n, const uint32_t* __restrict__ partial,
-                                                      uint32_t* __restrict__ out) {
-    __shared__ uint32_t sh[kBlock / 64];
-    const size_t        base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    uint32_t            v[kScanItems];
-    uint32_t            s = 0;
-#pragma unroll
-    for(int i = 0; i < kScanItems; ++i) {
-        v[i] = base + i < n ? in[base + i] : 0u;
-        s += v[i];
+struct k_scan_down {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ in;
+    size_t n;
+    const uint32_t* __restrict__ partial;
+    uint32_t* __restrict__ out;
+    __device__ void operator()(Blk blk) const {
+        __shared__ uint32_t sh[kBlock / 64];
+        const size_t        base = (size_t)blk.id * kScanTile + (size_t)threadIdx.x * kScanItems;
+        uint32_t            v[kScanItems];
+        uint32_t            s = 0;
+    #pragma unroll
+        for(int i = 0; i < kScanItems; ++i) {
+            v[i] = base + i < n ? in[base + i] : 0u;
+            s += v[i];
+        }
+        uint32_t total;
+        uint32_t run = block_exclusive_scan(s, sh, total) + partial[blk.id];
+    #pragma unroll
+        for(int i = 0; i < kScanItems; ++i) {
+            if(base + i < n) out[base + i] = run;
+            run += v[i];
+        }
     }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(s, sh, total) + partial[blockIdx.x];
-#pragma unroll
-    for(int i = 0; i < kScanItems; ++i) {
-        if(base + i < n) out[base + i] = run;
-        run += v[i];
-    }
-}
+};
 
 //==================================================================================================
 // mapParticles2Grid as a gather over cell-sorted particles (FLIP_Solver.cpp:57-95,
@@ -546,15 +660,16 @@ __device__ __forceinline__ bool stencil_weight(float q, int i_high, int target, 
 }
 
 //--------------------------------------------------------------------------------------------------
-// P2G as a gather that streams along z. A thread owns the nodes (i, j, k0 .. k0+kP2gSeg-1) of all
-// three face grids and reads each particle row z = k0-1 .. k0+kP2gSeg once: a particle binned to
+// P2G as a gather that streams along z. A thread owns the nodes (i, j, k0 .. k0+kSeg-1) of all
+// three face grids and reads each particle row z = k0-1 .. k0+kSeg once: a particle binned to
 // row z reaches only nodes z-1 .. z+1 along z (its z base is z-1 or z for the half-shifted stencil,
 // z for the node-aligned one, clamps included), which a three-node register window holds. Node k
 // still receives rows k-1, k, k+1 in that order, each row's y rows in order and each y row's
 // particles in cell-sorted order, with the weight (wx*wy)*wz and the momentum w*v rounded before
 // each add (no contraction): the per-node sums are those of a per-node gather over its 27 cells.
 //--------------------------------------------------------------------------------------------------
-constexpr int kP2gSeg = 8;
+// The segment length kSeg (the z extent of a thread's tile: 4, 8 or 16) changes which thread sums a
+// node, never the order of its sum, so every length gives the same bits; the batch autotuner picks it.
 
 struct P2gAcc {
     float mu, wu, mv, wv, mw, ww;
@@ -704,23 +819,30 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
     if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
 }
 
-template<bool kAffine>
-__global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, Affine A, const uint32_t* __restrict__ cell_start, GridGeom g,
-                                                    FaceOut out) {
-    const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
-    const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
-    const uint32_t nseg = (uint32_t)(ez + kP2gSeg - 1) / kP2gSeg;
-    // consecutive blocks (neighbouring j rows, which read the same particle rows) stay on one XCD
-    const uint32_t nblk = gridDim.x, b = blockIdx.x, xcd = b % 8u, q = nblk / 8u, rr = nblk % 8u;
-    const uint32_t t    = (xcd * q + min(xcd, rr) + b / 8u) * blockDim.x + threadIdx.x;
-    if(t >= cols * nseg) return;
-    const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
-    const int  k0 = seg * kP2gSeg, k1 = min(k0 + kP2gSeg, ez);  // nodes [k0, k1)
-    // cells read: x in [i-1, i+1], y in [j-1, j+1], z in [k0-1, k1]
-    const bool interior = i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k0 >= 2 && k1 <= g.nz - 2;
-    if(interior) p2g_column<true, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
-    else p2g_column<false, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
-}
+template<bool kAffine, int kSeg>
+struct k_p2g_col {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    Affine A;
+    const uint32_t* __restrict__ cell_start;
+    GridGeom g;
+    FaceOut out;
+    __device__ void operator()(Blk blk) const {
+        const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
+        const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
+        const uint32_t nseg = (uint32_t)(ez + kSeg - 1) / kSeg;
+        // consecutive blocks (neighbouring j rows, which read the same particle rows) stay on one XCD
+        const uint32_t nblk = blk.n, b = blk.id, xcd = b % 8u, q = nblk / 8u, rr = nblk % 8u;
+        const uint32_t t    = (xcd * q + min(xcd, rr) + b / 8u) * blockDim.x + threadIdx.x;
+        if(t >= cols * nseg) return;
+        const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
+        const int  k0 = seg * kSeg, k1 = min(k0 + kSeg, ez);  // nodes [k0, k1)
+        // cells read: x in [i-1, i+1], y in [j-1, j+1], z in [k0-1, k1]
+        const bool interior = i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k0 >= 2 && k1 <= g.nz - 2;
+        if(interior) p2g_column<true, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
+        else p2g_column<false, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
+    }
+};
 
 //==================================================================================================
 // extrapolateVelocity: one Jacobi layer of one component (PIC_Solver.cpp:992-1090), in place.
@@ -735,76 +857,107 @@ __global__ void __launch_bounds__(kBlock) k_p2g_col(Particles P, Affine A, const
 // Faces that no layer can reach: `near` marks the 8^3 face bricks that hold a valid face or touch
 // one that does (26 neighbours). A face outside them is at least 9 face steps from every valid face,
 // so four layers cannot fill it: it is skipped (layer 0 still clears its extrapolated flag).
-__global__ void __launch_bounds__(256) k_extrap_brick_valid(const uint8_t* __restrict__ val, int sx, int sy, int sz,
-                                                            uint8_t* __restrict__ has) {
-    const int nbx = (sx + 7) / 8, nby = (sy + 7) / 8;
-    const int b = blockIdx.x, bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    int       any = 0;
-    for(int l = threadIdx.x; l < 512; l += 256) {
-        const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
-        if(i < sx && j < sy && k < sz && val[lin3(i, j, k, sx, sy)] != 0) any = 1;
+struct k_extrap_brick_valid {
+    static constexpr int kThreads = 256;
+    const uint8_t* __restrict__ val;
+    int sx;
+    int sy;
+    int sz;
+    uint8_t* __restrict__ has;
+    __device__ void operator()(Blk blk) const {
+        const int nbx = (sx + 7) / 8, nby = (sy + 7) / 8;
+        const int b = blk.id, bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+        int       any = 0;
+        for(int l = threadIdx.x; l < 512; l += 256) {
+            const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+            if(i < sx && j < sy && k < sz && val[lin3(i, j, k, sx, sy)] != 0) any = 1;
+        }
+        any = __syncthreads_or(any);
+        if(threadIdx.x == 0) has[b] = (uint8_t)any;
     }
-    any = __syncthreads_or(any);
-    if(threadIdx.x == 0) has[b] = (uint8_t)any;
-}
-__global__ void __launch_bounds__(kBlock) k_extrap_brick_near(const uint8_t* __restrict__ has, int nbx, int nby, int nbz,
-                                                              uint8_t* __restrict__ near) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if(b >= nbx * nby * nbz) return;
-    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    uint8_t   v = 0;
-    for(int dz = -1; dz <= 1; ++dz)
-        for(int dy = -1; dy <= 1; ++dy)
-            for(int dx = -1; dx <= 1; ++dx) {
-                const int x = bx + dx, y = by + dy, z = bz + dz;
-                if(x >= 0 && y >= 0 && z >= 0 && x < nbx && y < nby && z < nbz) v |= has[(z * nby + y) * nbx + x];
-            }
-    near[b] = v;
-}
+};
+struct k_extrap_brick_near {
+    static constexpr int kThreads = kBlock;
+    const uint8_t* __restrict__ has;
+    int nbx;
+    int nby;
+    int nbz;
+    uint8_t* __restrict__ near;
+    __device__ void operator()(Blk blk) const {
+        const int b = blk.id * blockDim.x + threadIdx.x;
+        if(b >= nbx * nby * nbz) return;
+        const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+        uint8_t   v = 0;
+        for(int dz = -1; dz <= 1; ++dz)
+            for(int dy = -1; dy <= 1; ++dy)
+                for(int dx = -1; dx <= 1; ++dx) {
+                    const int x = bx + dx, y = by + dy, z = bz + dz;
+                    if(x >= 0 && y >= 0 && z >= 0 && x < nbx && y < nby && z < nbz) v |= has[(z * nby + y) * nbx + x];
+                }
+        near[b] = v;
+    }
+};
 
-// launched as (ceil(sx / 64), ceil(sy / 4), sz) blocks of 64 x 4 threads: (i, j, k) without divisions
-__global__ void __launch_bounds__(256) k_extrap_stamp(float* __restrict__ a, uint8_t* __restrict__ val, uint8_t* __restrict__ ext,
-                                                      const uint8_t* __restrict__ near, int sx, int sy, int sz, int layer) {
-    const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
-    if(i >= sx || j >= sy) return;
-    const size_t t = lin3(i, j, k, sx, sy);
-    if(!near[((k >> 3) * ((sy + 7) / 8) + (j >> 3)) * ((sx + 7) / 8) + (i >> 3)]) {
-        if(layer == 0) ext[t] = 0;
-        return;
-    }
-    bool filled = false;
-    if(val[t] == 0) {
-        if(i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2) {
-            const size_t  sxy = (size_t)sx * sy;
-            const uint8_t lim = (uint8_t)(layer + 1);
-            auto          ok  = [&](size_t n) {
-                const uint8_t s = val[n];
-                return s >= 1 && s <= lim;
-            };
-            float    sum   = 0.f;
-            uint32_t count = 0;
-            if(ok(t + 1)) { sum += a[t + 1]; ++count; }
-            if(ok(t - 1)) { sum += a[t - 1]; ++count; }
-            if(ok(t + sx)) { sum += a[t + sx]; ++count; }
-            if(ok(t - sx)) { sum += a[t - sx]; ++count; }
-            if(ok(t + sxy)) { sum += a[t + sxy]; ++count; }
-            if(ok(t - sxy)) { sum += a[t - sxy]; ++count; }
-            if(count > 0) {
-                a[t]   = sum / (float)count;
-                val[t] = (uint8_t)(layer + 2);
-                ext[t] = 1;
-                filled = true;
+// one block per 64 x 4 x 1 run of faces: blocks numbered x-fastest over (ceil(sx / 64), ceil(sy / 4), sz)
+struct k_extrap_stamp {
+    static constexpr int kThreads = 256;
+    float* __restrict__ a;
+    uint8_t* __restrict__ val;
+    uint8_t* __restrict__ ext;
+    const uint8_t* __restrict__ near;
+    int sx;
+    int sy;
+    int sz;
+    int layer;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t gx = (uint32_t)(sx + 63) / 64u, gy = (uint32_t)(sy + 3) / 4u, r = blk.id / gx;
+        const int      i = (int)((blk.id - r * gx) * 64u + (threadIdx.x & 63u)), j = (int)((r % gy) * 4u + (threadIdx.x >> 6));
+        const int      k = (int)(r / gy);
+        if(i >= sx || j >= sy) return;
+        const size_t t = lin3(i, j, k, sx, sy);
+        if(!near[((k >> 3) * ((sy + 7) / 8) + (j >> 3)) * ((sx + 7) / 8) + (i >> 3)]) {
+            if(layer == 0) ext[t] = 0;
+            return;
+        }
+        bool filled = false;
+        if(val[t] == 0) {
+            if(i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2) {
+                const size_t  sxy = (size_t)sx * sy;
+                const uint8_t lim = (uint8_t)(layer + 1);
+                auto          ok  = [&](size_t n) {
+                    const uint8_t s = val[n];
+                    return s >= 1 && s <= lim;
+                };
+                float    sum   = 0.f;
+                uint32_t count = 0;
+                if(ok(t + 1)) { sum += a[t + 1]; ++count; }
+                if(ok(t - 1)) { sum += a[t - 1]; ++count; }
+                if(ok(t + sx)) { sum += a[t + sx]; ++count; }
+                if(ok(t - sx)) { sum += a[t - sx]; ++count; }
+                if(ok(t + sxy)) { sum += a[t + sxy]; ++count; }
+                if(ok(t - sxy)) { sum += a[t - sxy]; ++count; }
+                if(count > 0) {
+                    a[t]   = sum / (float)count;
+                    val[t] = (uint8_t)(layer + 2);
+                    ext[t] = 1;
+                    filled = true;
+                }
             }
         }
+        if(layer == 0 && !filled) ext[t] = 0;
     }
-    if(layer == 0 && !filled) ext[t] = 0;
-}
+};
 
 // after the last layer: stamps back to valid flags
-__global__ void __launch_bounds__(kBlock) k_extrap_finish(uint8_t* __restrict__ val, size_t n) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t < n && val[t] > 1) val[t] = 1;
-}
+struct k_extrap_finish {
+    static constexpr int kThreads = kBlock;
+    uint8_t* __restrict__ val;
+    size_t n;
+    __device__ void operator()(Blk blk) const {
+        const size_t t = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t < n && val[t] > 1) val[t] = 1;
+    }
+};
 
 //==================================================================================================
 // constrainGridVelocity, component d (PIC_Solver.cpp:1094-1115, ArrayHelpers.cpp:214-251)
@@ -824,117 +977,162 @@ __device__ __forceinline__ void interp_gradient(const float* __restrict__ b, int
     gz = bilerpf(v001 - v000, v101 - v100, v011 - v010, v111 - v110, fi, fj);
 }
 
-__global__ void __launch_bounds__(kBlock) k_constrain(VelGrid G, GridGeom g, int d, const uint8_t* __restrict__ ext,
-                                                      const float* __restrict__ weight, const float* __restrict__ bsdf,
-                                                      float* __restrict__ out) {
-    const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
-    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t N  = (size_t)sx * sy * sz;
-    if(t >= N) return;
-    const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
-    float        val = cur[t];
-    if(ext[t] == 1 && weight[t] < kFltMin) {
-        int i, j, k;
-    unlin3((uint32_t)t, sx, sy, i, j, k);
-        float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
-        gp[d]           = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
-        float vx, vy, vz, nx, ny, nz;
-        vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
-        interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
-        const float m2 = nx * nx + ny * ny + nz * nz;
-        if(m2 > kFltMin) {
-            const float s = sqrtf(m2);
-            nx /= s;
-            ny /= s;
-            nz /= s;
-            const float dt = vx * nx + vy * ny + vz * nz;
-            vx -= dt * nx;
-            vy -= dt * ny;
-            vz -= dt * nz;
-            val = d == 0 ? vx : (d == 1 ? vy : vz);
+struct k_constrain {
+    static constexpr int kThreads = kBlock;
+    VelGrid G;
+    GridGeom g;
+    int d;
+    const uint8_t* __restrict__ ext;
+    const float* __restrict__ weight;
+    const float* __restrict__ bsdf;
+    float* __restrict__ out;
+    __device__ void operator()(Blk blk) const {
+        const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
+        const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
+        const size_t N  = (size_t)sx * sy * sz;
+        if(t >= N) return;
+        const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
+        float        val = cur[t];
+        if(ext[t] == 1 && weight[t] < kFltMin) {
+            int i, j, k;
+        unlin3((uint32_t)t, sx, sy, i, j, k);
+            float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+            gp[d]           = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+            float vx, vy, vz, nx, ny, nz;
+            vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
+            interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
+            const float m2 = nx * nx + ny * ny + nz * nz;
+            if(m2 > kFltMin) {
+                const float s = sqrtf(m2);
+                nx /= s;
+                ny /= s;
+                nz /= s;
+                const float dt = vx * nx + vy * ny + vz * nz;
+                vx -= dt * nx;
+                vy -= dt * ny;
+                vz -= dt * nz;
+                val = d == 0 ? vx : (d == 1 ? vy : vz);
+            }
         }
+        out[t] = val;
     }
-    out[t] = val;
-}
+};
 
 //==================================================================================================
 // addGravity (PIC_Solver.cpp:505-529): Earth / Directional add a constant per component
 //==================================================================================================
 // c = a * b elementwise
-__global__ void __launch_bounds__(kBlock) k_mul_arrays(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ c,
-                                                       size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(i < n) c[i] = a[i] * b[i];
-}
+struct k_mul_arrays {
+    static constexpr int kThreads = kBlock;
+    const float* __restrict__ a;
+    const float* __restrict__ b;
+    float* __restrict__ c;
+    size_t n;
+    __device__ void operator()(Blk blk) const {
+        const size_t i = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(i < n) c[i] = a[i] * b[i];
+    }
+};
 
-__global__ void __launch_bounds__(kBlock) k_add_const(float* __restrict__ a, size_t n, float add, int subtract) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= n) return;
-    if(subtract) a[t] -= add;
-    else a[t] += add;
-}
+struct k_add_const {
+    static constexpr int kThreads = kBlock;
+    float* __restrict__ a;
+    size_t n;
+    float add;
+    int subtract;
+    __device__ void operator()(Blk blk) const {
+        const size_t t = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t >= n) return;
+        if(subtract) a[t] -= add;
+        else a[t] += add;
+    }
+};
 
 // ToCenter / FromCenter: the reference walks velocities[0]'s index range for every component
 // (PIC_Solver.cpp:521); indices outside component d's array are skipped here.
-__global__ void __launch_bounds__(kBlock) k_gravity_center(float* __restrict__ a, GridGeom g, int d, float cx, float cy, float cz,
-                                                           int toCenter, float dt) {
-    const int    sx = g.nx + 1, sy = g.ny, sz = g.nz;
-    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= (size_t)sx * sy * sz) return;
-    int i, j, k;
-    unlin3((uint32_t)t, sx, sy, i, j, k);
-    const int ax = g.nx + (d == 0), ay = g.ny + (d == 1), az = g.nz + (d == 2);
-    if(i >= ax || j >= ay || k >= az) return;
-    float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
-    gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
-    const float wx = gp[0] * g.dx + g.bminx, wy = gp[1] * g.dx + g.bminy, wz = gp[2] * g.dx + g.bminz;
-    float       dv[3];
-    dv[0]          = toCenter ? cx - wx : wx - cx;
-    dv[1]          = toCenter ? cy - wy : wy - cy;
-    dv[2]          = toCenter ? cz - wz : wz - cz;
-    const float is = 1.0f / sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-    a[lin3(i, j, k, ax, ay)] += (9.81f * (dv[d] * is)) * dt;
-}
+struct k_gravity_center {
+    static constexpr int kThreads = kBlock;
+    float* __restrict__ a;
+    GridGeom g;
+    int d;
+    float cx;
+    float cy;
+    float cz;
+    int toCenter;
+    float dt;
+    __device__ void operator()(Blk blk) const {
+        const int    sx = g.nx + 1, sy = g.ny, sz = g.nz;
+        const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t >= (size_t)sx * sy * sz) return;
+        int i, j, k;
+        unlin3((uint32_t)t, sx, sy, i, j, k);
+        const int ax = g.nx + (d == 0), ay = g.ny + (d == 1), az = g.nz + (d == 2);
+        if(i >= ax || j >= ay || k >= az) return;
+        float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+        gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+        const float wx = gp[0] * g.dx + g.bminx, wy = gp[1] * g.dx + g.bminy, wz = gp[2] * g.dx + g.bminz;
+        float       dv[3];
+        dv[0]          = toCenter ? cx - wx : wx - cx;
+        dv[1]          = toCenter ? cy - wy : wy - cy;
+        dv[2]          = toCenter ? cz - wz : wz - cz;
+        const float is = 1.0f / sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        a[lin3(i, j, k, ax, ay)] += (9.81f * (dv[d] * is)) * dt;
+    }
+};
 
 //==================================================================================================
 // computeCellWeights (PIC_Solver.cpp:543-597); counts faces with weight < FLT_MIN (solid faces)
 //==================================================================================================
-__global__ void __launch_bounds__(kBlock) k_weights(const float* __restrict__ b, GridGeom g, float* __restrict__ wu,
-                                                    float* __restrict__ wv, float* __restrict__ ww, uint32_t* solidFaces) {
-    const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
-    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= (size_t)sx * sy * sz) return;
-    int i, j, k;
-    unlin3((uint32_t)t, sx, sy, i, j, k);
-    auto      B = [&](int a, int c, int e) { return b[lin3(a, c, e, sx, sy)]; };
-    uint32_t  solid = 0;
-    if(j < g.ny && k < g.nz) {
-        float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j + 1, k), B(i, j, k + 1), B(i, j + 1, k + 1)));
-        wu[lin3(i, j, k, g.nx + 1, g.ny)] = w;
-        solid += w < kFltMin;
+struct k_weights {
+    static constexpr int kThreads = kBlock;
+    const float* __restrict__ b;
+    GridGeom g;
+    float* __restrict__ wu;
+    float* __restrict__ wv;
+    float* __restrict__ ww;
+    uint32_t* solidFaces;
+    __device__ void operator()(Blk blk) const {
+        const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
+        const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t >= (size_t)sx * sy * sz) return;
+        int i, j, k;
+        unlin3((uint32_t)t, sx, sy, i, j, k);
+        auto      B = [&](int a, int c, int e) { return b[lin3(a, c, e, sx, sy)]; };
+        uint32_t  solid = 0;
+        if(j < g.ny && k < g.nz) {
+            float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j + 1, k), B(i, j, k + 1), B(i, j + 1, k + 1)));
+            wu[lin3(i, j, k, g.nx + 1, g.ny)] = w;
+            solid += w < kFltMin;
+        }
+        if(i < g.nx && k < g.nz) {
+            float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j, k + 1), B(i + 1, j, k), B(i + 1, j, k + 1)));
+            wv[lin3(i, j, k, g.nx, g.ny + 1)] = w;
+            solid += w < kFltMin;
+        }
+        if(i < g.nx && j < g.ny) {
+            float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j + 1, k), B(i + 1, j, k), B(i + 1, j + 1, k)));
+            ww[lin3(i, j, k, g.nx, g.ny)] = w;
+            solid += w < kFltMin;
+        }
+        if(solid) atomicAdd(solidFaces, solid);
     }
-    if(i < g.nx && k < g.nz) {
-        float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j, k + 1), B(i + 1, j, k), B(i + 1, j, k + 1)));
-        wv[lin3(i, j, k, g.nx, g.ny + 1)] = w;
-        solid += w < kFltMin;
-    }
-    if(i < g.nx && j < g.ny) {
-        float w = clamp01f(1.0f - fraction_inside4(B(i, j, k), B(i, j + 1, k), B(i + 1, j, k), B(i + 1, j + 1, k)));
-        ww[lin3(i, j, k, g.nx, g.ny)] = w;
-        solid += w < kFltMin;
-    }
-    if(solid) atomicAdd(solidFaces, solid);
-}
+};
 
 // computeBoundarySDF with the domain box as the boundary (PIC_Solver.cpp:1158-1168)
-__global__ void __launch_bounds__(kBlock) k_boundary_sdf(float* __restrict__ b, GridGeom g, BoxDev dom) {
-    const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
-    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= (size_t)sx * sy * sz) return;
-    int i, j, k;
-    unlin3((uint32_t)t, sx, sy, i, j, k);
-    b[t]        = box_sd_out(dom, (float)i * g.dx + g.bminx, (float)j * g.dx + g.bminy, (float)k * g.dx + g.bminz);
-}
+struct k_boundary_sdf {
+    static constexpr int kThreads = kBlock;
+    float* __restrict__ b;
+    GridGeom g;
+    BoxDev dom;
+    __device__ void operator()(Blk blk) const {
+        const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
+        const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t >= (size_t)sx * sy * sz) return;
+        int i, j, k;
+        unlin3((uint32_t)t, sx, sy, i, j, k);
+        b[t]        = box_sd_out(dom, (float)i * g.dx + g.bminx, (float)j * g.dx + g.bminy, (float)k * g.dx + g.bminz);
+    }
+};
 
 //==================================================================================================
 // computeFluidSDF as a gather (PIC_Solver.cpp:601-689): phi(c) = min over particles whose clamped
@@ -948,118 +1146,145 @@ __global__ void __launch_bounds__(kBlock) k_boundary_sdf(float* __restrict__ b, 
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
 constexpr int kTileThreads = kTileX * kTileY * kTileZ;  // 256
 
-__device__ __forceinline__ void tile_of_block(int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
-    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blockIdx.x, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
+__device__ __forceinline__ void tile_of_block(Blk blk, int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
+    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blk.id, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
     const uint32_t t = xcd * q + min(xcd, r) + b / 8u;
     tx = (int)(t % (uint32_t)ntx);
     ty = (int)((t / (uint32_t)ntx) % (uint32_t)nty);
     tz = (int)(t / (uint32_t)(ntx * nty));
 }
 
-__global__ void __launch_bounds__(kTileThreads) k_fluid_sdf(Particles P, const uint32_t* __restrict__ cell_start, GridGeom g,
-                                                            const float* __restrict__ bsdf, float* __restrict__ phi) {
-    const int ntx = (g.nx + kTileX - 1) / kTileX, nty = (g.ny + kTileY - 1) / kTileY, ntz = (g.nz + kTileZ - 1) / kTileZ;
-    int       tx, ty, tz;
-    tile_of_block(ntx, nty, ntz, tx, ty, tz);
-    const int  x0 = tx * kTileX, y0 = ty * kTileY, z0 = tz * kTileZ;
-    const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
-    if(i >= g.nx || j >= g.ny || k >= g.nz) return;
-    const float sx      = ((float)i + 0.5f) * g.dx + g.bminx;
-    const float sy      = ((float)j + 0.5f) * g.dx + g.bminy;
-    const float sz      = ((float)k + 0.5f) * g.dx + g.bminz;
-    float       best    = g.dx * 3.0f;
-    float       bestLen = 3.40282347e+38f;  // smallest |centre - p| so far
-    // The own cell first, then its 6 face, 12 edge and 8 corner neighbours: no particle of a class-c
-    // neighbour is nearer to the centre than 0, 0.5, 0.7071 or 0.8660 dx (less 1e-3 dx for the
-    // rounding of particle-to-cell assignment), so once the nearest particle found beats a class's
-    // bound, that class and all farther ones cannot lower the minimum. min() is exact, so visiting
-    // fewer cells gives the same bits.
-    const float bound[4] = {0.f, 0.499f * g.dx, 0.7061068f * g.dx, 0.8650254f * g.dx};
-    for(int cls = 0; cls < 4; ++cls) {
-        if(bestLen < bound[cls]) break;
-        for(int dz = -1; dz <= 1; ++dz)
-            for(int dy = -1; dy <= 1; ++dy)
-                for(int dxo = -1; dxo <= 1; ++dxo) {
-                    if(abs(dz) + abs(dy) + abs(dxo) != cls) continue;
-                    const int x = i + dxo, y = j + dy, z = k + dz;
-                    if(x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
-                    const bool boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
-                    auto       visit    = [&](float ppx, float ppy, float ppz) {
-                        if(boundary) {
-                            // particles clamped into a boundary cell: use their true (unclamped) window
-                            const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
-                                      ck = (int)((ppz - g.bminz) / g.dx);
-                            if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) return;
-                        }
-                        const float len = len3f(sx - ppx, sy - ppy, sz - ppz);
-                        const float v   = len - g.sdfRadius;
-                        if(v < best) best = v;
-                        if(len < bestLen) bestLen = len;
-                    };
-                    const size_t   c  = lin3(x, y, z, g.nx, g.ny);
-                    const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
-                    for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
-                }
+struct k_fluid_sdf {
+    static constexpr int kThreads = kTileThreads;
+    Particles P;
+    const uint32_t* __restrict__ cell_start;
+    GridGeom g;
+    const float* __restrict__ bsdf;
+    float* __restrict__ phi;
+    __device__ void operator()(Blk blk) const {
+        const int ntx = (g.nx + kTileX - 1) / kTileX, nty = (g.ny + kTileY - 1) / kTileY, ntz = (g.nz + kTileZ - 1) / kTileZ;
+        int       tx, ty, tz;
+        tile_of_block(blk, ntx, nty, ntz, tx, ty, tz);
+        const int  x0 = tx * kTileX, y0 = ty * kTileY, z0 = tz * kTileZ;
+        const int  i = x0 + (int)threadIdx.x % kTileX, j = y0 + ((int)threadIdx.x / kTileX) % kTileY, k = z0 + (int)threadIdx.x / (kTileX * kTileY);
+        if(i >= g.nx || j >= g.ny || k >= g.nz) return;
+        const float sx      = ((float)i + 0.5f) * g.dx + g.bminx;
+        const float sy      = ((float)j + 0.5f) * g.dx + g.bminy;
+        const float sz      = ((float)k + 0.5f) * g.dx + g.bminz;
+        float       best    = g.dx * 3.0f;
+        float       bestLen = 3.40282347e+38f;  // smallest |centre - p| so far
+        // The own cell first, then its 6 face, 12 edge and 8 corner neighbours: no particle of a class-c
+        // neighbour is nearer to the centre than 0, 0.5, 0.7071 or 0.8660 dx (less 1e-3 dx for the
+        // rounding of particle-to-cell assignment), so once the nearest particle found beats a class's
+        // bound, that class and all farther ones cannot lower the minimum. min() is exact, so visiting
+        // fewer cells gives the same bits.
+        const float bound[4] = {0.f, 0.499f * g.dx, 0.7061068f * g.dx, 0.8650254f * g.dx};
+        for(int cls = 0; cls < 4; ++cls) {
+            if(bestLen < bound[cls]) break;
+            for(int dz = -1; dz <= 1; ++dz)
+                for(int dy = -1; dy <= 1; ++dy)
+                    for(int dxo = -1; dxo <= 1; ++dxo) {
+                        if(abs(dz) + abs(dy) + abs(dxo) != cls) continue;
+                        const int x = i + dxo, y = j + dy, z = k + dz;
+                        if(x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                        const bool boundary = x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1;
+                        auto       visit    = [&](float ppx, float ppy, float ppz) {
+                            if(boundary) {
+                                // particles clamped into a boundary cell: use their true (unclamped) window
+                                const int ci = (int)((ppx - g.bminx) / g.dx), cj = (int)((ppy - g.bminy) / g.dx),
+                                          ck = (int)((ppz - g.bminz) / g.dx);
+                                if(i < ci - 1 || i > ci + 1 || j < cj - 1 || j > cj + 1 || k < ck - 1 || k > ck + 1) return;
+                            }
+                            const float len = len3f(sx - ppx, sy - ppy, sz - ppz);
+                            const float v   = len - g.sdfRadius;
+                            if(v < best) best = v;
+                            if(len < bestLen) bestLen = len;
+                        };
+                        const size_t   c  = lin3(x, y, z, g.nx, g.ny);
+                        const uint32_t pb = cell_start[c], pe = cell_start[c + 1];
+                        for(uint32_t p = pb; p < pe; ++p) visit(P.px[p], P.py[p], P.pz[p]);
+                    }
+        }
+        if(best < g.halfdx) {
+            const int   sxn = g.nx + 1, syn = g.ny + 1;
+            auto        B   = [&](int a, int c, int e) { return bsdf[lin3(a, c, e, sxn, syn)]; };
+            const float s   = 0.125f * (B(i, j, k) + B(i + 1, j, k) + B(i, j + 1, k) + B(i + 1, j + 1, k) + B(i, j, k + 1) +
+                                      B(i + 1, j, k + 1) + B(i, j + 1, k + 1) + B(i + 1, j + 1, k + 1));
+            if(s < 0) best = -g.halfdx;
+        }
+        phi[lin3(i, j, k, g.nx, g.ny)] = best;
     }
-    if(best < g.halfdx) {
-        const int   sxn = g.nx + 1, syn = g.ny + 1;
-        auto        B   = [&](int a, int c, int e) { return bsdf[lin3(a, c, e, sxn, syn)]; };
-        const float s   = 0.125f * (B(i, j, k) + B(i + 1, j, k) + B(i, j + 1, k) + B(i + 1, j + 1, k) + B(i, j, k + 1) +
-                                  B(i + 1, j, k + 1) + B(i, j + 1, k + 1) + B(i + 1, j + 1, k + 1));
-        if(s < 0) best = -g.halfdx;
-    }
-    phi[lin3(i, j, k, g.nx, g.ny)] = best;
-}
+};
 
 //==================================================================================================
 // Pressure bricks
 //==================================================================================================
 // one block (512 threads) per brick: flag bricks holding a fluid cell, count fluid cells
-__global__ void __launch_bounds__(512) k_brick_flags(const float* __restrict__ phi, GridGeom g, uint32_t* __restrict__ flag,
-                                                     uint32_t* __restrict__ nFluid) {
-    const int b  = blockIdx.x;
-    const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
-    const int l  = threadIdx.x;
-    const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
-    int fluid = 0;
-    if(i < g.nx && j < g.ny && k < g.nz) fluid = phi[lin3(i, j, k, g.nx, g.ny)] < 0.f;
-    const int cnt = __syncthreads_count(fluid);
-    if(l == 0) {
-        flag[b] = cnt > 0 ? 1u : 0u;
-        if(cnt) atomicAdd(nFluid, (uint32_t)cnt);
+struct k_brick_flags {
+    static constexpr int kThreads = 512;
+    const float* __restrict__ phi;
+    GridGeom g;
+    uint32_t* __restrict__ flag;
+    uint32_t* __restrict__ nFluid;
+    __device__ void operator()(Blk blk) const {
+        const int b  = blk.id;
+        const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+        const int l  = threadIdx.x;
+        const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+        int fluid = 0;
+        if(i < g.nx && j < g.ny && k < g.nz) fluid = phi[lin3(i, j, k, g.nx, g.ny)] < 0.f;
+        const int cnt = __syncthreads_count(fluid);
+        if(l == 0) {
+            flag[b] = cnt > 0 ? 1u : 0u;
+            if(cnt) atomicAdd(nFluid, (uint32_t)cnt);
+        }
     }
-}
+};
 
-__global__ void __launch_bounds__(kBlock) k_brick_slots(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
-                                                        uint32_t nb, int32_t* __restrict__ brickSlot, int32_t* __restrict__ slotBrick) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if(b >= nb) return;
-    if(flag[b]) {
-        brickSlot[b]        = (int32_t)scan[b];
-        slotBrick[scan[b]] = (int32_t)b;
-    } else {
-        brickSlot[b] = -1;
+struct k_brick_slots {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* __restrict__ flag;
+    const uint32_t* __restrict__ scan;
+    uint32_t nb;
+    int32_t* __restrict__ brickSlot;
+    int32_t* __restrict__ slotBrick;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t b = blk.id * blockDim.x + threadIdx.x;
+        if(b >= nb) return;
+        if(flag[b]) {
+            brickSlot[b]        = (int32_t)scan[b];
+            slotBrick[scan[b]] = (int32_t)b;
+        } else {
+            brickSlot[b] = -1;
+        }
     }
-}
+};
 
 // neighbour slots per active brick: -x, +x, -y, +y, -z, +z
-__global__ void __launch_bounds__(kBlock) k_brick_nbrs(const int32_t* __restrict__ slotBrick, uint32_t nSlots,
-                                                       const int32_t* __restrict__ brickSlot, GridGeom g, int32_t* __restrict__ nbr) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if(s >= nSlots) return;
-    const int b  = slotBrick[s];
-    const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
-    auto      at = [&](int x, int y, int z) -> int32_t {
-        if(x < 0 || y < 0 || z < 0 || x >= g.nbx || y >= g.nby || z >= g.nbz) return -1;
-        return brickSlot[(z * g.nby + y) * g.nbx + x];
-    };
-    nbr[6 * s + 0] = at(bx - 1, by, bz);
-    nbr[6 * s + 1] = at(bx + 1, by, bz);
-    nbr[6 * s + 2] = at(bx, by - 1, bz);
-    nbr[6 * s + 3] = at(bx, by + 1, bz);
-    nbr[6 * s + 4] = at(bx, by, bz - 1);
-    nbr[6 * s + 5] = at(bx, by, bz + 1);
-}
+struct k_brick_nbrs {
+    static constexpr int kThreads = kBlock;
+    const int32_t* __restrict__ slotBrick;
+    uint32_t nSlots;
+    const int32_t* __restrict__ brickSlot;
+    GridGeom g;
+    int32_t* __restrict__ nbr;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t s = blk.id * blockDim.x + threadIdx.x;
+        if(s >= nSlots) return;
+        const int b  = slotBrick[s];
+        const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+        auto      at = [&](int x, int y, int z) -> int32_t {
+            if(x < 0 || y < 0 || z < 0 || x >= g.nbx || y >= g.nby || z >= g.nbz) return -1;
+            return brickSlot[(z * g.nby + y) * g.nbx + x];
+        };
+        nbr[6 * s + 0] = at(bx - 1, by, bz);
+        nbr[6 * s + 1] = at(bx + 1, by, bz);
+        nbr[6 * s + 2] = at(bx, by - 1, bz);
+        nbr[6 * s + 3] = at(bx, by + 1, bz);
+        nbr[6 * s + 4] = at(bx, by, bz - 1);
+        nbr[6 * s + 5] = at(bx, by, bz + 1);
+    }
+};
 
 // vector index of the neighbour of local cell l in direction dir (-1 when absent)
 __device__ __forceinline__ int32_t nbr_index(const int32_t* __restrict__ nbr, int32_t s, int l, int dir) {
@@ -1084,69 +1309,79 @@ struct SystemOut {
     float * rhs32, *diag32, *cx32, *cy32, *cz32, *invdiag32;
 };
 
-__global__ void __launch_bounds__(512) k_assemble(const int32_t* __restrict__ slotBrick, GridGeom g, const float* __restrict__ phi,
-                                                  VelGrid G, const float* __restrict__ wu, const float* __restrict__ wv,
-                                                  const float* __restrict__ ww, float dt, SystemOut S) {
-    const int s = blockIdx.x, l = threadIdx.x;
-    const int b = slotBrick[s];
-    const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
-    const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
-    const size_t o = (size_t)s * 512 + l;
-    double rhs = 0, diag = 0, cx = 0, cy = 0, cz = 0;
-    const bool interior = i >= 1 && j >= 1 && k >= 1 && i <= g.nx - 2 && j <= g.ny - 2 && k <= g.nz - 2;
-    if(interior) {
-        auto P = [&](int a, int c, int e) { return phi[lin3(a, c, e, g.nx, g.ny)]; };
-        const float pc = P(i, j, k);
-        if(pc < 0) {
-            const int    ux = g.nx + 1, vy = g.ny + 1;
-            const double wr = wu[lin3(i + 1, j, k, ux, g.ny)], wl = wu[lin3(i, j, k, ux, g.ny)];
-            const double wt = wv[lin3(i, j + 1, k, g.nx, vy)], wb = wv[lin3(i, j, k, g.nx, vy)];
-            const double wf = ww[lin3(i, j, k + 1, g.nx, g.ny)], wn = ww[lin3(i, j, k, g.nx, g.ny)];
-            rhs -= wr * (double)G.u[lin3(i + 1, j, k, ux, g.ny)];
-            rhs += wl * (double)G.u[lin3(i, j, k, ux, g.ny)];
-            rhs -= wt * (double)G.v[lin3(i, j + 1, k, g.nx, vy)];
-            rhs += wb * (double)G.v[lin3(i, j, k, g.nx, vy)];
-            rhs -= wf * (double)G.w[lin3(i, j, k + 1, g.nx, g.ny)];
-            rhs += wn * (double)G.w[lin3(i, j, k, g.nx, g.ny)];
-            const double ddt = (double)dt;
-            const double pcd = (double)pc;
-            auto side = [&](float pn, double w, double& coupling, bool upper) {
-                const double term = w * ddt;
-                if(pn < 0) {
-                    diag += term;
-                    if(upper) coupling = -term;
-                } else {
-                    const double th = fraction_inside2(pcd, (double)pn);
-                    diag += term / (th > 0.01 ? th : 0.01);
-                }
-            };
-            double dummy = 0;
-            side(P(i + 1, j, k), wr, cx, true);
-            side(P(i - 1, j, k), wl, dummy, false);
-            side(P(i, j + 1, k), wt, cy, true);
-            side(P(i, j - 1, k), wb, dummy, false);
-            side(P(i, j, k + 1), wf, cz, true);
-            side(P(i, j, k - 1), wn, dummy, false);
-            // couplings into boundary-layer cells (empty rows) are dropped
-            if(i + 1 > g.nx - 2) cx = 0;
-            if(j + 1 > g.ny - 2) cy = 0;
-            if(k + 1 > g.nz - 2) cz = 0;
+struct k_assemble {
+    static constexpr int kThreads = 512;
+    const int32_t* __restrict__ slotBrick;
+    GridGeom g;
+    const float* __restrict__ phi;
+    VelGrid G;
+    const float* __restrict__ wu;
+    const float* __restrict__ wv;
+    const float* __restrict__ ww;
+    float dt;
+    SystemOut S;
+    __device__ void operator()(Blk blk) const {
+        const int s = blk.id, l = threadIdx.x;
+        const int b = slotBrick[s];
+        const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+        const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+        const size_t o = (size_t)s * 512 + l;
+        double rhs = 0, diag = 0, cx = 0, cy = 0, cz = 0;
+        const bool interior = i >= 1 && j >= 1 && k >= 1 && i <= g.nx - 2 && j <= g.ny - 2 && k <= g.nz - 2;
+        if(interior) {
+            auto P = [&](int a, int c, int e) { return phi[lin3(a, c, e, g.nx, g.ny)]; };
+            const float pc = P(i, j, k);
+            if(pc < 0) {
+                const int    ux = g.nx + 1, vy = g.ny + 1;
+                const double wr = wu[lin3(i + 1, j, k, ux, g.ny)], wl = wu[lin3(i, j, k, ux, g.ny)];
+                const double wt = wv[lin3(i, j + 1, k, g.nx, vy)], wb = wv[lin3(i, j, k, g.nx, vy)];
+                const double wf = ww[lin3(i, j, k + 1, g.nx, g.ny)], wn = ww[lin3(i, j, k, g.nx, g.ny)];
+                rhs -= wr * (double)G.u[lin3(i + 1, j, k, ux, g.ny)];
+                rhs += wl * (double)G.u[lin3(i, j, k, ux, g.ny)];
+                rhs -= wt * (double)G.v[lin3(i, j + 1, k, g.nx, vy)];
+                rhs += wb * (double)G.v[lin3(i, j, k, g.nx, vy)];
+                rhs -= wf * (double)G.w[lin3(i, j, k + 1, g.nx, g.ny)];
+                rhs += wn * (double)G.w[lin3(i, j, k, g.nx, g.ny)];
+                const double ddt = (double)dt;
+                const double pcd = (double)pc;
+                auto side = [&](float pn, double w, double& coupling, bool upper) {
+                    const double term = w * ddt;
+                    if(pn < 0) {
+                        diag += term;
+                        if(upper) coupling = -term;
+                    } else {
+                        const double th = fraction_inside2(pcd, (double)pn);
+                        diag += term / (th > 0.01 ? th : 0.01);
+                    }
+                };
+                double dummy = 0;
+                side(P(i + 1, j, k), wr, cx, true);
+                side(P(i - 1, j, k), wl, dummy, false);
+                side(P(i, j + 1, k), wt, cy, true);
+                side(P(i, j - 1, k), wb, dummy, false);
+                side(P(i, j, k + 1), wf, cz, true);
+                side(P(i, j, k - 1), wn, dummy, false);
+                // couplings into boundary-layer cells (empty rows) are dropped
+                if(i + 1 > g.nx - 2) cx = 0;
+                if(j + 1 > g.ny - 2) cy = 0;
+                if(k + 1 > g.nz - 2) cz = 0;
+            }
         }
+        if(S.rhs64) S.rhs64[o] = rhs;
+        if(S.diag64) {
+            S.diag64[o] = diag;
+            S.cx64[o]   = cx;
+            S.cy64[o]   = cy;
+            S.cz64[o]   = cz;
+        }
+        S.rhs32[o]     = (float)rhs;
+        S.diag32[o]    = (float)diag;
+        S.cx32[o]      = (float)cx;
+        S.cy32[o]      = (float)cy;
+        S.cz32[o]      = (float)cz;
+        S.invdiag32[o] = diag > 0 ? (float)(1.0 / diag) : 0.f;
     }
-    if(S.rhs64) S.rhs64[o] = rhs;
-    if(S.diag64) {
-        S.diag64[o] = diag;
-        S.cx64[o]   = cx;
-        S.cy64[o]   = cy;
-        S.cz64[o]   = cz;
-    }
-    S.rhs32[o]     = (float)rhs;
-    S.diag32[o]    = (float)diag;
-    S.cx32[o]      = (float)cx;
-    S.cy32[o]      = (float)cy;
-    S.cz32[o]      = (float)cz;
-    S.invdiag32[o] = diag > 0 ? (float)(1.0 / diag) : 0.f;
-}
+};
 
 //==================================================================================================
 // PCG (PCGSolver.cpp:143-199) with Jacobi preconditioning, FP32 vectors and FP64 scalars.
@@ -1161,7 +1396,7 @@ struct PcgState {
 };
 
 // Reductions are two-level and deterministic: each block of a main kernel reduces a fixed set of
-// bricks (b, b + gridDim.x, ... or a contiguous chunk) and writes one partial with a plain store; a
+// bricks (b, b + the block count, ... or a contiguous chunk) and writes one partial with a plain store; a
 // one-block fold kernel then combines the partials in index order. Kernel boundaries order the two,
 // so no device-scope fences are needed inside the main kernels.
 constexpr int kMaxRedBlocks = 2048;
@@ -1195,180 +1430,234 @@ __device__ __forceinline__ double fold(const double* __restrict__ p, int n, bool
 }
 
 // init: x = 0, r = b, s = 0, z = M^-1 r; partials of z.r and max|b|
-__global__ void __launch_bounds__(512) k_pcg_init(const float* __restrict__ b, const float* __restrict__ invdiag,
-                                                  float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
-                                                  float* __restrict__ s, uint32_t nSlots, double* __restrict__ pa,
-                                                  double* __restrict__ pb, int jacobi) {
-    __shared__ double sh[8];
-    double            rz = 0, m = 0;
-    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
-        const size_t o  = (size_t)slot * 512 + threadIdx.x;
-        const float  bv = b[o];
-        x[o]            = 0.f;
-        r[o]            = bv;
-        s[o]            = 0.f;
-        if(jacobi) {
-            const float zv = invdiag[o] * bv;
-            z[o]           = zv;
-            rz += (double)zv * (double)bv;
+struct k_pcg_init {
+    static constexpr int kThreads = 512;
+    const float* __restrict__ b;
+    const float* __restrict__ invdiag;
+    float* __restrict__ x;
+    float* __restrict__ r;
+    float* __restrict__ z;
+    float* __restrict__ s;
+    uint32_t nSlots;
+    double* __restrict__ pa;
+    double* __restrict__ pb;
+    int jacobi;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[8];
+        double            rz = 0, m = 0;
+        for(uint32_t slot = blk.id; slot < nSlots; slot += blk.n) {
+            const size_t o  = (size_t)slot * 512 + threadIdx.x;
+            const float  bv = b[o];
+            x[o]            = 0.f;
+            r[o]            = bv;
+            s[o]            = 0.f;
+            if(jacobi) {
+                const float zv = invdiag[o] * bv;
+                z[o]           = zv;
+                rz += (double)zv * (double)bv;
+            }
+            m = fmax(m, (double)fabsf(bv));
         }
-        m = fmax(m, (double)fabsf(bv));
+        if(jacobi) rz = block_sum<512>(rz, sh);
+        m = block_max<512>(m, sh);
+        if(threadIdx.x == 0) {
+            if(jacobi) pa[blk.id] = rz;
+            pb[blk.id] = m;
+        }
     }
-    if(jacobi) rz = block_sum<512>(rz, sh);
-    m = block_max<512>(m, sh);
-    if(threadIdx.x == 0) {
-        if(jacobi) pa[blockIdx.x] = rz;
-        pb[blockIdx.x] = m;
-    }
-}
+};
 
 // pa: na partials of z.r, pb: nb partials of max|b|
-__global__ void __launch_bounds__(1024) k_pcg_init_fin(PcgState* st, const double* __restrict__ pa, int na,
-                                                       const double* __restrict__ pb, int nb, float relTol, uint32_t maxIter) {
-    __shared__ double sh[16];
-    const double      A = fold(pa, na, false, sh);
-    const double      B = fold(pb, nb, true, sh);
-    if(threadIdx.x != 0) return;
-    st->bmax    = (float)B;
-    st->res     = (float)B;
-    st->iter    = 0;
-    st->maxIter = maxIter;
-    st->beta    = 0;
-    st->rho     = A;
-    st->done    = 0;
-    st->success = 0;
-    st->tol     = relTol * (float)B;  // PCGSolver.cpp:160
-    if((float)B < kFltMin) {          // PCGSolver.cpp:155-158
-        st->done    = 1;
-        st->success = 1;
-    } else if(A < (double)kFltMin || isnan(A)) {  // PCGSolver.cpp:164-167
-        st->done = 1;
-    } else if(maxIter == 0) {
-        st->done = 1;
+struct k_pcg_init_fin {
+    static constexpr int kThreads = 1024;
+    PcgState* st;
+    const double* __restrict__ pa;
+    int na;
+    const double* __restrict__ pb;
+    int nb;
+    float relTol;
+    uint32_t maxIter;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[16];
+        const double      A = fold(pa, na, false, sh);
+        const double      B = fold(pb, nb, true, sh);
+        if(threadIdx.x != 0) return;
+        st->bmax    = (float)B;
+        st->res     = (float)B;
+        st->iter    = 0;
+        st->maxIter = maxIter;
+        st->beta    = 0;
+        st->rho     = A;
+        st->done    = 0;
+        st->success = 0;
+        st->tol     = relTol * (float)B;  // PCGSolver.cpp:160
+        if((float)B < kFltMin) {          // PCGSolver.cpp:155-158
+            st->done    = 1;
+            st->success = 1;
+        } else if(A < (double)kFltMin || isnan(A)) {  // PCGSolver.cpp:164-167
+            st->done = 1;
+        } else if(maxIter == 0) {
+            st->done = 1;
+        }
     }
-}
+};
 
 // s_out = z + beta s_in ; q = A s_out ; partial s_out . q
 // the slots of this block when the grid shares n slots: one contiguous chunk per block, and the
-// chunks of one XCD's blocks (equal blockIdx % 8) adjacent, so that brick faces read from
+// chunks of one XCD's blocks (equal block index mod 8) adjacent, so that brick faces read from
 // neighbouring bricks are mostly in that XCD's L2
-__device__ __forceinline__ void block_chunk(uint32_t n, uint32_t& s0, uint32_t& s1) {
-    const uint32_t G = gridDim.x, b = blockIdx.x, xcd = b % 8u, q = G / 8u, r = G % 8u;
+__device__ __forceinline__ void block_chunk(Blk blk, uint32_t n, uint32_t& s0, uint32_t& s1) {
+    const uint32_t G = blk.n, b = blk.id, xcd = b % 8u, q = G / 8u, r = G % 8u;
     const uint32_t v = xcd * q + min(xcd, r) + b / 8u;
     s0               = (uint32_t)(((uint64_t)n * v) / G);
     s1               = (uint32_t)(((uint64_t)n * (v + 1)) / G);
 }
 
-__global__ void __launch_bounds__(512) k_pcg_spmv(const int32_t* __restrict__ nbr, const float* __restrict__ z,
-                                                  const float* __restrict__ sin, float* __restrict__ sout, float* __restrict__ q,
-                                                  const float* __restrict__ diag, const float* __restrict__ cx,
-                                                  const float* __restrict__ cy, const float* __restrict__ cz, uint32_t nSlots,
-                                                  const PcgState* __restrict__ st, double* __restrict__ pa) {
-    __shared__ double sh[8];
-    if(st && st->done) return;
-    const float beta = (float)st->beta;
-    const int   l    = threadIdx.x;
-    double      d    = 0;
-    uint32_t    s0, s1;
-    block_chunk(nSlots, s0, s1);
-    for(uint32_t slot = s0; slot < s1; ++slot) {
-        const int     s   = (int)slot;
-        const size_t  o   = (size_t)s * 512 + l;
-        auto          S   = [&](int32_t idx) { return z[idx] + beta * sin[idx]; };
-        const float   sc  = z[o] + beta * sin[o];
-        const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
-        const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
-        const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
-        // a missing neighbour reads the cell itself with a zero coupling (adds exactly nothing), so
-        // all loads are unconditional
-        const int32_t oi = (int32_t)o;
-        const int32_t jxm = ixm >= 0 ? ixm : oi, jxp = ixp >= 0 ? ixp : oi, jym = iym >= 0 ? iym : oi;
-        const int32_t jyp = iyp >= 0 ? iyp : oi, jzm = izm >= 0 ? izm : oi, jzp = izp >= 0 ? izp : oi;
-        const float   czm = cz[jzm], cym = cy[jym], cxm = cx[jxm], cxp = cx[o], cyp = cy[o], czp = cz[o];
-        const float   szm = S(jzm), sym = S(jym), sxm = S(jxm), sxp = S(jxp), syp = S(jyp), szp = S(jzp);
-        float         acc = 0.f;
-        acc += (izm >= 0 ? czm : 0.f) * szm;
-        acc += (iym >= 0 ? cym : 0.f) * sym;
-        acc += (ixm >= 0 ? cxm : 0.f) * sxm;
-        acc += diag[o] * sc;
-        acc += (ixp >= 0 ? cxp : 0.f) * sxp;
-        acc += (iyp >= 0 ? cyp : 0.f) * syp;
-        acc += (izp >= 0 ? czp : 0.f) * szp;
-        sout[o] = sc;
-        q[o]    = acc;
-        d += (double)sc * (double)acc;
+struct k_pcg_spmv {
+    static constexpr int kThreads = 512;
+    const int32_t* __restrict__ nbr;
+    const float* __restrict__ z;
+    const float* __restrict__ sin;
+    float* __restrict__ sout;
+    float* __restrict__ q;
+    const float* __restrict__ diag;
+    const float* __restrict__ cx;
+    const float* __restrict__ cy;
+    const float* __restrict__ cz;
+    uint32_t nSlots;
+    const PcgState* __restrict__ st;
+    double* __restrict__ pa;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[8];
+        if(st && st->done) return;
+        const float beta = (float)st->beta;
+        const int   l    = threadIdx.x;
+        double      d    = 0;
+        uint32_t    s0, s1;
+        block_chunk(blk, nSlots, s0, s1);
+        for(uint32_t slot = s0; slot < s1; ++slot) {
+            const int     s   = (int)slot;
+            const size_t  o   = (size_t)s * 512 + l;
+            auto          S   = [&](int32_t idx) { return z[idx] + beta * sin[idx]; };
+            const float   sc  = z[o] + beta * sin[o];
+            const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
+            const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
+            const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
+            // a missing neighbour reads the cell itself with a zero coupling (adds exactly nothing), so
+            // all loads are unconditional
+            const int32_t oi = (int32_t)o;
+            const int32_t jxm = ixm >= 0 ? ixm : oi, jxp = ixp >= 0 ? ixp : oi, jym = iym >= 0 ? iym : oi;
+            const int32_t jyp = iyp >= 0 ? iyp : oi, jzm = izm >= 0 ? izm : oi, jzp = izp >= 0 ? izp : oi;
+            const float   czm = cz[jzm], cym = cy[jym], cxm = cx[jxm], cxp = cx[o], cyp = cy[o], czp = cz[o];
+            const float   szm = S(jzm), sym = S(jym), sxm = S(jxm), sxp = S(jxp), syp = S(jyp), szp = S(jzp);
+            float         acc = 0.f;
+            acc += (izm >= 0 ? czm : 0.f) * szm;
+            acc += (iym >= 0 ? cym : 0.f) * sym;
+            acc += (ixm >= 0 ? cxm : 0.f) * sxm;
+            acc += diag[o] * sc;
+            acc += (ixp >= 0 ? cxp : 0.f) * sxp;
+            acc += (iyp >= 0 ? cyp : 0.f) * syp;
+            acc += (izp >= 0 ? czp : 0.f) * szp;
+            sout[o] = sc;
+            q[o]    = acc;
+            d += (double)sc * (double)acc;
+        }
+        d = block_sum<512>(d, sh);
+        if(threadIdx.x == 0) pa[blk.id] = d;
     }
-    d = block_sum<512>(d, sh);
-    if(threadIdx.x == 0) pa[blockIdx.x] = d;
-}
+};
 
 // tmp = s.q ; alpha = rho / tmp (PCGSolver.cpp:174-180)
-__global__ void __launch_bounds__(1024) k_pcg_alpha(PcgState* st, const double* __restrict__ pa, int n) {
-    __shared__ double sh[16];
-    if(st && st->done) return;
-    const double A = fold(pa, n, false, sh);
-    if(threadIdx.x != 0) return;
-    st->tmp = A;
-    if(A < (double)kFltMin) {
-        st->done    = 1;
-        st->success = 1;
-        st->iter += 1;
-    } else {
-        st->alpha = st->rho / A;
+struct k_pcg_alpha {
+    static constexpr int kThreads = 1024;
+    PcgState* st;
+    const double* __restrict__ pa;
+    int n;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[16];
+        if(st && st->done) return;
+        const double A = fold(pa, n, false, sh);
+        if(threadIdx.x != 0) return;
+        st->tmp = A;
+        if(A < (double)kFltMin) {
+            st->done    = 1;
+            st->success = 1;
+            st->iter += 1;
+        } else {
+            st->alpha = st->rho / A;
+        }
     }
-}
+};
 
 // x += alpha s ; r -= alpha q ; z = M^-1 r ; partials z.r and max|r|
-__global__ void __launch_bounds__(512) k_pcg_update(const float* __restrict__ s, const float* __restrict__ q,
-                                                    const float* __restrict__ invdiag, float* __restrict__ x, float* __restrict__ r,
-                                                    float* __restrict__ z, uint32_t nSlots, const PcgState* __restrict__ st,
-                                                    double* __restrict__ pa, double* __restrict__ pb, int jacobi) {
-    __shared__ double sh[8];
-    if(st && st->done) return;
-    const float alpha = (float)st->alpha;
-    double      rz = 0, m = 0;
-    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
-        const size_t o  = (size_t)slot * 512 + threadIdx.x;
-        const float  xv = x[o] + alpha * s[o];
-        const float  rv = r[o] + (-alpha) * q[o];
-        x[o]            = xv;
-        r[o]            = rv;
-        if(jacobi) {  // z = M^-1 r fused here; the multigrid preconditioner runs after this kernel
-            const float zv = invdiag[o] * rv;
-            z[o]           = zv;
-            rz += (double)zv * (double)rv;
+struct k_pcg_update {
+    static constexpr int kThreads = 512;
+    const float* __restrict__ s;
+    const float* __restrict__ q;
+    const float* __restrict__ invdiag;
+    float* __restrict__ x;
+    float* __restrict__ r;
+    float* __restrict__ z;
+    uint32_t nSlots;
+    const PcgState* __restrict__ st;
+    double* __restrict__ pa;
+    double* __restrict__ pb;
+    int jacobi;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[8];
+        if(st && st->done) return;
+        const float alpha = (float)st->alpha;
+        double      rz = 0, m = 0;
+        for(uint32_t slot = blk.id; slot < nSlots; slot += blk.n) {
+            const size_t o  = (size_t)slot * 512 + threadIdx.x;
+            const float  xv = x[o] + alpha * s[o];
+            const float  rv = r[o] + (-alpha) * q[o];
+            x[o]            = xv;
+            r[o]            = rv;
+            if(jacobi) {  // z = M^-1 r fused here; the multigrid preconditioner runs after this kernel
+                const float zv = invdiag[o] * rv;
+                z[o]           = zv;
+                rz += (double)zv * (double)rv;
+            }
+            m = fmax(m, (double)fabsf(rv));
         }
-        m = fmax(m, (double)fabsf(rv));
+        if(jacobi) rz = block_sum<512>(rz, sh);
+        m = block_max<512>(m, sh);
+        if(threadIdx.x == 0) {
+            if(jacobi) pa[blk.id] = rz;
+            pb[blk.id] = m;
+        }
     }
-    if(jacobi) rz = block_sum<512>(rz, sh);
-    m = block_max<512>(m, sh);
-    if(threadIdx.x == 0) {
-        if(jacobi) pa[blockIdx.x] = rz;
-        pb[blockIdx.x] = m;
-    }
-}
+};
 
 // res = max|r| ; stop test (PCGSolver.cpp:182-187) ; beta = rz / rho (:190-194)
 // pa: na partials of z.r, pb: nb partials of max|r|
-__global__ void __launch_bounds__(1024) k_pcg_beta(PcgState* st, const double* __restrict__ pa, int na, const double* __restrict__ pb,
-                                                   int nb) {
-    __shared__ double sh[16];
-    if(st && st->done) return;
-    const double A = fold(pa, na, false, sh);
-    const double B = fold(pb, nb, true, sh);
-    if(threadIdx.x != 0) return;
-    st->iter += 1;
-    st->res = (float)B;
-    if((float)B < st->tol) {
-        st->done    = 1;
-        st->success = 1;
-    } else if(st->iter >= st->maxIter) {
-        st->done = 1;
-    } else {
-        st->beta = A / st->rho;
-        st->rho  = A;
+struct k_pcg_beta {
+    static constexpr int kThreads = 1024;
+    PcgState* st;
+    const double* __restrict__ pa;
+    int na;
+    const double* __restrict__ pb;
+    int nb;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[16];
+        if(st && st->done) return;
+        const double A = fold(pa, na, false, sh);
+        const double B = fold(pb, nb, true, sh);
+        if(threadIdx.x != 0) return;
+        st->iter += 1;
+        st->res = (float)B;
+        if((float)B < st->tol) {
+            st->done    = 1;
+            st->success = 1;
+        } else if(st->iter >= st->maxIter) {
+            st->done = 1;
+        } else {
+            st->beta = A / st->rho;
+            st->rho  = A;
+        }
     }
-}
+};
 
 //--------------------------------------------------------------------------------------------------
 // FP64 defect correction: r64 = b64 - A64 x64, max|r64|, max|b64|, r32 = (float) r64 (inner rhs)
@@ -1377,61 +1666,85 @@ struct OuterState {
     double res, bmax;
 };
 
-__global__ void __launch_bounds__(512) k_residual64(const int32_t* __restrict__ nbr, const double* __restrict__ x,
-                                                    const double* __restrict__ b, const double* __restrict__ diag,
-                                                    const double* __restrict__ cx, const double* __restrict__ cy,
-                                                    const double* __restrict__ cz, float* __restrict__ r32, uint32_t nSlots,
-                                                    double* __restrict__ pa, double* __restrict__ pb, int first) {
-    __shared__ double sh[8];
-    const int         l = threadIdx.x;
-    double            m = 0, bm = 0;
-    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
-        const int    s  = (int)slot;
-        const size_t o  = (size_t)s * 512 + l;
-        double       ax = 0.0;
-        if(!first) {
-            const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
-            const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
-            const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
-            if(izm >= 0) ax += cz[izm] * x[izm];
-            if(iym >= 0) ax += cy[iym] * x[iym];
-            if(ixm >= 0) ax += cx[ixm] * x[ixm];
-            ax += diag[o] * x[o];
-            if(ixp >= 0) ax += cx[o] * x[ixp];
-            if(iyp >= 0) ax += cy[o] * x[iyp];
-            if(izp >= 0) ax += cz[o] * x[izp];
+struct k_residual64 {
+    static constexpr int kThreads = 512;
+    const int32_t* __restrict__ nbr;
+    const double* __restrict__ x;
+    const double* __restrict__ b;
+    const double* __restrict__ diag;
+    const double* __restrict__ cx;
+    const double* __restrict__ cy;
+    const double* __restrict__ cz;
+    float* __restrict__ r32;
+    uint32_t nSlots;
+    double* __restrict__ pa;
+    double* __restrict__ pb;
+    int first;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[8];
+        const int         l = threadIdx.x;
+        double            m = 0, bm = 0;
+        for(uint32_t slot = blk.id; slot < nSlots; slot += blk.n) {
+            const int    s  = (int)slot;
+            const size_t o  = (size_t)s * 512 + l;
+            double       ax = 0.0;
+            if(!first) {
+                const int32_t ixm = nbr_index(nbr, s, l, 0), ixp = nbr_index(nbr, s, l, 1);
+                const int32_t iym = nbr_index(nbr, s, l, 2), iyp = nbr_index(nbr, s, l, 3);
+                const int32_t izm = nbr_index(nbr, s, l, 4), izp = nbr_index(nbr, s, l, 5);
+                if(izm >= 0) ax += cz[izm] * x[izm];
+                if(iym >= 0) ax += cy[iym] * x[iym];
+                if(ixm >= 0) ax += cx[ixm] * x[ixm];
+                ax += diag[o] * x[o];
+                if(ixp >= 0) ax += cx[o] * x[ixp];
+                if(iyp >= 0) ax += cy[o] * x[iyp];
+                if(izp >= 0) ax += cz[o] * x[izp];
+            }
+            const double bv = b[o];
+            const double rv = bv - ax;
+            r32[o]          = (float)rv;
+            m               = fmax(m, fabs(rv));
+            bm              = fmax(bm, fabs(bv));
         }
-        const double bv = b[o];
-        const double rv = bv - ax;
-        r32[o]          = (float)rv;
-        m               = fmax(m, fabs(rv));
-        bm              = fmax(bm, fabs(bv));
+        m  = block_max<512>(m, sh);
+        bm = block_max<512>(bm, sh);
+        if(threadIdx.x == 0) {
+            pa[blk.id] = m;
+            pb[blk.id] = bm;
+        }
     }
-    m  = block_max<512>(m, sh);
-    bm = block_max<512>(bm, sh);
-    if(threadIdx.x == 0) {
-        pa[blockIdx.x] = m;
-        pb[blockIdx.x] = bm;
-    }
-}
+};
 
-__global__ void __launch_bounds__(1024) k_residual64_fin(OuterState* ost, const double* __restrict__ pa, const double* __restrict__ pb,
-                                                         int n) {
-    __shared__ double sh[16];
-    const double      M  = fold(pa, n, true, sh);
-    const double      BM = fold(pb, n, true, sh);
-    if(threadIdx.x == 0) {
-        ost->res  = M;
-        ost->bmax = BM;
+struct k_residual64_fin {
+    static constexpr int kThreads = 1024;
+    OuterState* ost;
+    const double* __restrict__ pa;
+    const double* __restrict__ pb;
+    int n;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[16];
+        const double      M  = fold(pa, n, true, sh);
+        const double      BM = fold(pb, n, true, sh);
+        if(threadIdx.x == 0) {
+            ost->res  = M;
+            ost->bmax = BM;
+        }
     }
-}
+};
 
-__global__ void __launch_bounds__(512) k_accumulate64(double* __restrict__ x64, const float* __restrict__ d, uint32_t nSlots, int first) {
-    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
-        const size_t o = (size_t)slot * 512 + threadIdx.x;
-        x64[o]         = first ? (double)d[o] : x64[o] + (double)d[o];
+struct k_accumulate64 {
+    static constexpr int kThreads = 512;
+    double* __restrict__ x64;
+    const float* __restrict__ d;
+    uint32_t nSlots;
+    int first;
+    __device__ void operator()(Blk blk) const {
+        for(uint32_t slot = blk.id; slot < nSlots; slot += blk.n) {
+            const size_t o = (size_t)slot * 512 + threadIdx.x;
+            x64[o]         = first ? (double)d[o] : x64[o] + (double)d[o];
+        }
     }
-}
+};
 
 //==================================================================================================
 // updateProjectedVelocity (PIC_Solver.cpp:911-981), one thread per face; faces the reference does
@@ -1448,33 +1761,42 @@ template<class PT> __device__ __forceinline__ PT pressure_at(const PressView<PT>
 }
 
 template<class PT>
-__global__ void __launch_bounds__(kBlock) k_update_velocity(float* __restrict__ vel, uint8_t* __restrict__ valid,
-                                                            const float* __restrict__ weight, const float* __restrict__ phi,
-                                                            PressView<PT> pv, GridGeom g, int d, float dt) {
-    const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
-    const size_t t  = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= (size_t)sx * sy * sz) return;
-    int i, j, k;
-    unlin3((uint32_t)t, sx, sy, i, j, k);
-    const int ai = d == 0 ? i : (d == 1 ? j : k);       // index along the face normal
-    const int an = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
-    bool      upd = false;
-    if(ai > 0 && ai < an && weight[t] > 0.f) {
-        const int   li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
-        const float pc = phi[lin3(i, j, k, g.nx, g.ny)];
-        const float pl = phi[lin3(li, lj, lk, g.nx, g.ny)];
-        if(pc < 0 || pl < 0) {
-            const PT prc = pc < 0 ? pressure_at(pv, g, i, j, k) : PT(0);
-            const PT prl = pl < 0 ? pressure_at(pv, g, li, lj, lk) : PT(0);
-            PT       th  = fraction_inside2((PT)pl, (PT)pc);
-            th           = th > PT(0.01) ? th : PT(0.01);  // std::max(0.01, th)
-            vel[t]       = (float)((PT)vel[t] - (PT)dt * (prc - prl) / th);
-            upd          = true;
+struct k_update_velocity {
+    static constexpr int kThreads = kBlock;
+    float* __restrict__ vel;
+    uint8_t* __restrict__ valid;
+    const float* __restrict__ weight;
+    const float* __restrict__ phi;
+    PressView<PT> pv;
+    GridGeom g;
+    int d;
+    float dt;
+    __device__ void operator()(Blk blk) const {
+        const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
+        const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t >= (size_t)sx * sy * sz) return;
+        int i, j, k;
+        unlin3((uint32_t)t, sx, sy, i, j, k);
+        const int ai = d == 0 ? i : (d == 1 ? j : k);       // index along the face normal
+        const int an = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+        bool      upd = false;
+        if(ai > 0 && ai < an && weight[t] > 0.f) {
+            const int   li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
+            const float pc = phi[lin3(i, j, k, g.nx, g.ny)];
+            const float pl = phi[lin3(li, lj, lk, g.nx, g.ny)];
+            if(pc < 0 || pl < 0) {
+                const PT prc = pc < 0 ? pressure_at(pv, g, i, j, k) : PT(0);
+                const PT prl = pl < 0 ? pressure_at(pv, g, li, lj, lk) : PT(0);
+                PT       th  = fraction_inside2((PT)pl, (PT)pc);
+                th           = th > PT(0.01) ? th : PT(0.01);  // std::max(0.01, th)
+                vel[t]       = (float)((PT)vel[t] - (PT)dt * (prc - prl) / th);
+                upd          = true;
+            }
         }
+        if(!upd) vel[t] = 0.f;
+        valid[t] = upd ? 1 : 0;
     }
-    if(!upd) vel[t] = 0.f;
-    valid[t] = upd ? 1 : 0;
-}
+};
 
 //==================================================================================================
 // mapGrid2Particles with the FLIP blend (FLIP_Solver.cpp:99-133, MathHelpers.h:549-551)
@@ -1483,84 +1805,116 @@ struct OldGrid {
     const float *u, *v, *w;
 };
 
-__global__ void __launch_bounds__(kBlock) k_g2p(Particles P, uint32_t n, VelGrid G, OldGrid O, GridGeom g, float ratio) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
-    float       ux, uy, uz;
-    vel_from_grid(G, g, gx, gy, gz, ux, uy, uz);
-    const float dx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
-    const float dy = interp3_diff(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f);
-    const float dz = interp3_diff(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz);
-    const float a  = 1.0f - ratio;
-    P.vx[p]        = a * ux + ratio * (P.vx[p] + dx);
-    P.vy[p]        = a * uy + ratio * (P.vy[p] + dy);
-    P.vz[p]        = a * uz + ratio * (P.vz[p] + dz);
-}
+struct k_g2p {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    uint32_t n;
+    VelGrid G;
+    OldGrid O;
+    GridGeom g;
+    float ratio;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t p = blk.id * blockDim.x + threadIdx.x;
+        if(p >= n) return;
+        const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+        float       ux, uy, uz;
+        vel_from_grid(G, g, gx, gy, gz, ux, uy, uz);
+        const float dx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
+        const float dy = interp3_diff(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f);
+        const float dz = interp3_diff(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz);
+        const float a  = 1.0f - ratio;
+        P.vx[p]        = a * ux + ratio * (P.vx[p] + dx);
+        P.vy[p]        = a * uy + ratio * (P.vy[p] + dy);
+        P.vz[p]        = a * uz + ratio * (P.vz[p] + dz);
+    }
+};
 
 //==================================================================================================
 // APIC / AFLIP (APIC_Solver.cpp:42-125, AFLIP_Solver.cpp:43-140): each particle carries an affine
 // matrix C (row d = the gradient of velocity component d), stored as nine arrays in the particles'
 // storage order and moved with them by the binning permute.
 //==================================================================================================
-__global__ void __launch_bounds__(kBlock) k_permute_affine(Affine in, Affine out, const uint32_t* __restrict__ perm, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    const uint32_t s = perm[i];
-#pragma unroll
-    for(int k = 0; k < 9; ++k) out.c[k][i] = in.c[k][s];
-}
+struct k_permute_affine {
+    static constexpr int kThreads = kBlock;
+    Affine in;
+    Affine out;
+    const uint32_t* __restrict__ perm;
+    uint32_t n;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t i = blk.id * blockDim.x + threadIdx.x;
+        if(i >= n) return;
+        const uint32_t s = perm[i];
+    #pragma unroll
+        for(int k = 0; k < 9; ++k) out.c[k][i] = in.c[k][s];
+    }
+};
 
 // APIC mapGrid2Particles (APIC_Solver.cpp:94-115): v = the interpolated grid velocity, C_d = the
 // gradient of component d at gridPos - extra_d. AFLIP (AFLIP_Solver.cpp:94-140): both blended like
 // FLIP, lerp(grid, particle + change, PIC_FLIP_ratio), the changes taken from vel - backup.
 template<int kScheme>
-__global__ void __launch_bounds__(kBlock) k_g2p_affine(Particles P, Affine A, uint32_t n, VelGrid G, OldGrid O, GridGeom g, float ratio) {
-#pragma clang fp contract(off)
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= n) return;
-    const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
-    float       v[3], gc[9];
-    vel_from_grid(G, g, gx, gy, gz, v[0], v[1], v[2]);
-    interp3_grad(G.u, nullptr, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, gc[0], gc[1], gc[2]);
-    interp3_grad(G.v, nullptr, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f, g.dx, gc[3], gc[4], gc[5]);
-    interp3_grad(G.w, nullptr, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz, g.dx, gc[6], gc[7], gc[8]);
-    if(kScheme == 1) {
-        P.vx[p] = v[0];
-        P.vy[p] = v[1];
-        P.vz[p] = v[2];
-#pragma unroll
-        for(int k = 0; k < 9; ++k) A.c[k][p] = gc[k];
-        return;
+struct k_g2p_affine {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    Affine A;
+    uint32_t n;
+    VelGrid G;
+    OldGrid O;
+    GridGeom g;
+    float ratio;
+    __device__ void operator()(Blk blk) const {
+    #pragma clang fp contract(off)
+        const uint32_t p = blk.id * blockDim.x + threadIdx.x;
+        if(p >= n) return;
+        const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+        float       v[3], gc[9];
+        vel_from_grid(G, g, gx, gy, gz, v[0], v[1], v[2]);
+        interp3_grad(G.u, nullptr, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, gc[0], gc[1], gc[2]);
+        interp3_grad(G.v, nullptr, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f, g.dx, gc[3], gc[4], gc[5]);
+        interp3_grad(G.w, nullptr, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz, g.dx, gc[6], gc[7], gc[8]);
+        if(kScheme == 1) {
+            P.vx[p] = v[0];
+            P.vy[p] = v[1];
+            P.vz[p] = v[2];
+    #pragma unroll
+            for(int k = 0; k < 9; ++k) A.c[k][p] = gc[k];
+            return;
+        }
+        float dc[9];
+        interp3_grad(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, dc[0], dc[1], dc[2]);
+        interp3_grad(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f, g.dx, dc[3], dc[4], dc[5]);
+        interp3_grad(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz, g.dx, dc[6], dc[7], dc[8]);
+        const float dvx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
+        const float dvy = interp3_diff(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f);
+        const float dvz = interp3_diff(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz);
+        const float a   = 1.0f - ratio;
+        P.vx[p]         = a * v[0] + ratio * (P.vx[p] + dvx);
+        P.vy[p]         = a * v[1] + ratio * (P.vy[p] + dvy);
+        P.vz[p]         = a * v[2] + ratio * (P.vz[p] + dvz);
+    #pragma unroll
+        for(int k = 0; k < 9; ++k) A.c[k][p] = a * gc[k] + ratio * (A.c[k][p] + dc[k]);
     }
-    float dc[9];
-    interp3_grad(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, dc[0], dc[1], dc[2]);
-    interp3_grad(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f, g.dx, dc[3], dc[4], dc[5]);
-    interp3_grad(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz, g.dx, dc[6], dc[7], dc[8]);
-    const float dvx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
-    const float dvy = interp3_diff(G.v, O.v, g.nx, g.ny + 1, g.nz, gx - 0.5f, gy, gz - 0.5f);
-    const float dvz = interp3_diff(G.w, O.w, g.nx, g.ny, g.nz + 1, gx - 0.5f, gy - 0.5f, gz);
-    const float a   = 1.0f - ratio;
-    P.vx[p]         = a * v[0] + ratio * (P.vx[p] + dvx);
-    P.vy[p]         = a * v[1] + ratio * (P.vy[p] + dvy);
-    P.vz[p]         = a * v[2] + ratio * (P.vz[p] + dvz);
-#pragma unroll
-    for(int k = 0; k < 9; ++k) A.c[k][p] = a * gc[k] + ratio * (A.c[k][p] + dc[k]);
-}
+};
 
 // max |a| over an array (timestepCFL, PIC_Solver.cpp:342-349); result via atomicMax on the bits
-__global__ void __launch_bounds__(kBlock) k_maxabs(const float* __restrict__ a, size_t n, uint32_t* out) {
-    __shared__ float sh[kBlock / 64];
-    float            m = 0.f;
-    for(size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) m = fmaxf(m, fabsf(a[t]));
-    m = wave_max(m);
-    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if(threadIdx.x == 0) {
-        for(int i = 1; i < kBlock / 64; ++i) m = fmaxf(m, sh[i]);
-        atomicMax(out, __float_as_uint(m));
+struct k_maxabs {
+    static constexpr int kThreads = kBlock;
+    const float* __restrict__ a;
+    size_t n;
+    uint32_t* out;
+    __device__ void operator()(Blk blk) const {
+        __shared__ float sh[kBlock / 64];
+        float            m = 0.f;
+        for(size_t t = (size_t)blk.id * blockDim.x + threadIdx.x; t < n; t += (size_t)blk.n * blockDim.x) m = fmaxf(m, fabsf(a[t]));
+        m = wave_max(m);
+        if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if(threadIdx.x == 0) {
+            for(int i = 1; i < kBlock / 64; ++i) m = fmaxf(m, sh[i]);
+            atomicMax(out, __float_as_uint(m));
+        }
     }
-}
+};
 
 }  // namespace bnn
 
@@ -1594,43 +1948,61 @@ __device__ __forceinline__ int32_t level_cell_index(const LevelDev& L, int i, in
 }
 
 // flag the coarse bricks that hold children of active fine bricks
-__global__ void __launch_bounds__(kBlock) k_mg_flag_parents(const int32_t* __restrict__ fineSlotBrick, uint32_t nFine, int fnbx,
-                                                            int fnby, int cnbx, int cnby, uint32_t* __restrict__ cflag) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if(s >= nFine) return;
-    const int b = fineSlotBrick[s];
-    const int bx = b % fnbx, by = (b / fnbx) % fnby, bz = b / (fnbx * fnby);
-    cflag[((bz >> 1) * cnby + (by >> 1)) * cnbx + (bx >> 1)] = 1u;
-}
+struct k_mg_flag_parents {
+    static constexpr int kThreads = kBlock;
+    const int32_t* __restrict__ fineSlotBrick;
+    uint32_t nFine;
+    int fnbx;
+    int fnby;
+    int cnbx;
+    int cnby;
+    uint32_t* __restrict__ cflag;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t s = blk.id * blockDim.x + threadIdx.x;
+        if(s >= nFine) return;
+        const int b = fineSlotBrick[s];
+        const int bx = b % fnbx, by = (b / fnbx) % fnby, bz = b / (fnbx * fnby);
+        cflag[((bz >> 1) * cnby + (by >> 1)) * cnbx + (bx >> 1)] = 1u;
+    }
+};
 
 // Galerkin coefficients of the coarse level (one block per coarse brick, thread = coarse cell)
-__global__ void __launch_bounds__(512) k_mg_coarsen(LevelDev F, LevelDev C, float* __restrict__ cdiag, float* __restrict__ ccx,
-                                                    float* __restrict__ ccy, float* __restrict__ ccz, float* __restrict__ cinv) {
-    const int s = blockIdx.x, l = threadIdx.x;
-    const int b = C.slotBrick[s];
-    const int bx = b % C.nbx, by = (b / C.nbx) % C.nby, bz = b / (C.nbx * C.nby);
-    const int I = bx * 8 + (l & 7), J = by * 8 + ((l >> 3) & 7), K = bz * 8 + (l >> 6);
-    float     d = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, inner = 0.f;
-    for(int c = 0; c < 8; ++c) {
-        const int     a0 = c & 1, a1 = (c >> 1) & 1, a2 = (c >> 2) & 1;
-        const int32_t f  = level_cell_index(F, 2 * I + a0, 2 * J + a1, 2 * K + a2);
-        if(f < 0) continue;
-        d += F.diag[f];
-        if(a0) cx += F.cx[f];
-        else inner += F.cx[f];
-        if(a1) cy += F.cy[f];
-        else inner += F.cy[f];
-        if(a2) cz += F.cz[f];
-        else inner += F.cz[f];
+struct k_mg_coarsen {
+    static constexpr int kThreads = 512;
+    LevelDev F;
+    LevelDev C;
+    float* __restrict__ cdiag;
+    float* __restrict__ ccx;
+    float* __restrict__ ccy;
+    float* __restrict__ ccz;
+    float* __restrict__ cinv;
+    __device__ void operator()(Blk blk) const {
+        const int s = blk.id, l = threadIdx.x;
+        const int b = C.slotBrick[s];
+        const int bx = b % C.nbx, by = (b / C.nbx) % C.nby, bz = b / (C.nbx * C.nby);
+        const int I = bx * 8 + (l & 7), J = by * 8 + ((l >> 3) & 7), K = bz * 8 + (l >> 6);
+        float     d = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, inner = 0.f;
+        for(int c = 0; c < 8; ++c) {
+            const int     a0 = c & 1, a1 = (c >> 1) & 1, a2 = (c >> 2) & 1;
+            const int32_t f  = level_cell_index(F, 2 * I + a0, 2 * J + a1, 2 * K + a2);
+            if(f < 0) continue;
+            d += F.diag[f];
+            if(a0) cx += F.cx[f];
+            else inner += F.cx[f];
+            if(a1) cy += F.cy[f];
+            else inner += F.cy[f];
+            if(a2) cz += F.cz[f];
+            else inner += F.cz[f];
+        }
+        d += 2.f * inner;
+        const size_t o = (size_t)s * 512 + l;
+        cdiag[o]       = d;
+        ccx[o]         = cx;
+        ccy[o]         = cy;
+        ccz[o]         = cz;
+        cinv[o]        = d > 0.f ? 1.0f / d : 0.f;
     }
-    d += 2.f * inner;
-    const size_t o = (size_t)s * 512 + l;
-    cdiag[o]       = d;
-    ccx[o]         = cx;
-    ccy[o]         = cy;
-    ccz[o]         = cz;
-    cinv[o]        = d > 0.f ? 1.0f / d : 0.f;
-}
+};
 
 // sum of the six couplings times x. A missing neighbour reads the cell's own x (finite) with a zero
 // coupling, which adds exactly nothing, so every load is issued before the first use.
@@ -1670,16 +2042,22 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t b, uint32_t n) {
 // The coarse bricks a fine brick's cells and halo can have as parents: its own parent (index 6) and
 // that parent's six neighbours (-x, +x, -y, +y, -z, +z), or -1. Stored per fine slot (8 ints, the
 // last unused) when the hierarchy is built.
-__global__ void __launch_bounds__(kBlock) k_mg_parent_table(LevelDev F, LevelDev C, int32_t* __restrict__ tab) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if(s >= F.nSlots) return;
-    const int     b  = F.slotBrick[s];
-    const int     bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
-    const int32_t cs = C.brickSlot[((bz >> 1) * C.nby + (by >> 1)) * C.nbx + (bx >> 1)];
-    for(int d = 0; d < 6; ++d) tab[8 * s + d] = cs >= 0 ? C.nbr[6 * cs + d] : -1;
-    tab[8 * s + 6] = cs;
-    tab[8 * s + 7] = -1;
-}
+struct k_mg_parent_table {
+    static constexpr int kThreads = kBlock;
+    LevelDev F;
+    LevelDev C;
+    int32_t* __restrict__ tab;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t s = blk.id * blockDim.x + threadIdx.x;
+        if(s >= F.nSlots) return;
+        const int     b  = F.slotBrick[s];
+        const int     bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+        const int32_t cs = C.brickSlot[((bz >> 1) * C.nby + (by >> 1)) * C.nbx + (bx >> 1)];
+        for(int d = 0; d < 6; ++d) tab[8 * s + d] = cs >= 0 ? C.nbr[6 * cs + d] : -1;
+        tab[8 * s + 6] = cs;
+        tab[8 * s + 7] = -1;
+    }
+};
 
 // xpre + omega * x_coarse(parent) for an unknown, xpre otherwise (x + omega P x_c). cs is the coarse
 // brick holding the parent of fine cell (i, j, k) (-1: none). An unknown's parent always lies inside
@@ -1808,20 +2186,27 @@ __device__ __forceinline__ void smooth_cell_update(const SmoothCell& R, float* x
 
 // coarse-grid correction in place: L.r (the pre-smoothed x) += omega * P x_coarse on every unknown.
 // One block per brick, two cells per thread; each cell's parent lies in the brick's parent brick.
-__global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, float omega, const PcgState* __restrict__ st) {
-    if(st && st->done) return;
-    for(uint32_t v = blockIdx.x; v < F.nSlots; v += gridDim.x) {
-        const int     s      = (int)xcd_slot(v, F.nSlots);
-        const int     b      = F.slotBrick[s];
-        const int     bx     = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
-        const int32_t parent = F.ptab[8 * s + 6];
-        for(int h = 0; h < 2; ++h) {
-            const int    l = (int)threadIdx.x + 256 * h;
-            const size_t o = (size_t)s * 512 + l;
-            F.r[o]         = mg_prolonged(F, F.r, C, parent, omega, (int32_t)o, bx * 8 + (l & 7), by * 8 + ((l >> 3) & 7), bz * 8 + (l >> 6));
+struct k_mg_prolong {
+    static constexpr int kThreads = 256;
+    LevelDev F;
+    LevelDev C;
+    float omega;
+    const PcgState* __restrict__ st;
+    __device__ void operator()(Blk blk) const {
+        if(st && st->done) return;
+        for(uint32_t v = blk.id; v < F.nSlots; v += blk.n) {
+            const int     s      = (int)xcd_slot(v, F.nSlots);
+            const int     b      = F.slotBrick[s];
+            const int     bx     = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+            const int32_t parent = F.ptab[8 * s + 6];
+            for(int h = 0; h < 2; ++h) {
+                const int    l = (int)threadIdx.x + 256 * h;
+                const size_t o = (size_t)s * 512 + l;
+                F.r[o]         = mg_prolonged(F, F.r, C, parent, omega, (int32_t)o, bx * 8 + (l & 7), by * 8 + ((l >> 3) & 7), bz * 8 + (l >> 6));
+            }
         }
     }
-}
+};
 
 // Brick-hybrid Gauss-Seidel: one block per brick runs `sweeps` red-black sweeps (pre-smoother) or
 // black-red sweeps (post-smoother) on its 512 cells in LDS, with the x of neighbouring bricks held at
@@ -1837,94 +2222,114 @@ __global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, floa
 // kDot (finest level, post-smoother): also writes the brick's partial of z.r (z = the result,
 // r = L.b) to pz[slot], the PCG's next rho.
 template<int kProlong, int kDot>
-__global__ void __launch_bounds__(256) k_mg_smooth_brick(LevelDev L, LevelDev C, float omega, int sweeps, const PcgState* __restrict__ st,
-                                                         double* __restrict__ pz) {
-    __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
-    __shared__ double sh[4];
-    if(st && st->done) return;
-    const int t = threadIdx.x, ly = (t >> 2) & 7, lz = t >> 5;
-    const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
-    const int lr = ((ly + lz) & 1) ? l0 + 1 : l0, lb = ((ly + lz) & 1) ? l0 : l0 + 1;  // red (colour 0), black
-    for(uint32_t v = blockIdx.x; v < L.nSlots; v += gridDim.x) {
-        const int     s      = (int)xcd_slot(v, L.nSlots);
-        const int     b      = L.slotBrick[s];
-        const int     bx     = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
-        const int32_t parent = kProlong == 1 ? L.ptab[8 * s + 6] : -1;
-        const SmoothCell R = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lr, bx, by, bz, xs);
-        const SmoothCell K = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lb, bx, by, bz, xs);
-        smooth_halo<kProlong>(L, C, parent, omega, s, bx, by, bz, xs);
-        __syncthreads();
-        for(int it = 0; it < sweeps; ++it) {
-            smooth_cell_update(kProlong ? K : R, xs);
+struct k_mg_smooth_brick {
+    static constexpr int kThreads = 256;
+    LevelDev L;
+    LevelDev C;
+    float omega;
+    int sweeps;
+    const PcgState* __restrict__ st;
+    double* __restrict__ pz;
+    __device__ void operator()(Blk blk) const {
+        __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
+        __shared__ double sh[4];
+        if(st && st->done) return;
+        const int t = threadIdx.x, ly = (t >> 2) & 7, lz = t >> 5;
+        const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
+        const int lr = ((ly + lz) & 1) ? l0 + 1 : l0, lb = ((ly + lz) & 1) ? l0 : l0 + 1;  // red (colour 0), black
+        for(uint32_t v = blk.id; v < L.nSlots; v += blk.n) {
+            const int     s      = (int)xcd_slot(v, L.nSlots);
+            const int     b      = L.slotBrick[s];
+            const int     bx     = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
+            const int32_t parent = kProlong == 1 ? L.ptab[8 * s + 6] : -1;
+            const SmoothCell R = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lr, bx, by, bz, xs);
+            const SmoothCell K = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lb, bx, by, bz, xs);
+            smooth_halo<kProlong>(L, C, parent, omega, s, bx, by, bz, xs);
             __syncthreads();
-            smooth_cell_update(kProlong ? R : K, xs);
-            __syncthreads();
+            for(int it = 0; it < sweeps; ++it) {
+                smooth_cell_update(kProlong ? K : R, xs);
+                __syncthreads();
+                smooth_cell_update(kProlong ? R : K, xs);
+                __syncthreads();
+            }
+            float*      out             = kProlong ? L.x : L.r;
+            const float zr = xs[R.c], zb = xs[K.c];
+            out[(size_t)s * 512 + lr]   = zr;
+            out[(size_t)s * 512 + lb]   = zb;
+            if(kDot) {
+                const double d = block_sum<256>((double)zr * (double)R.bv + (double)zb * (double)K.bv, sh);
+                if(t == 0) pz[s] = d;
+            }
+            __syncthreads();  // xs is reused by the block's next brick
         }
-        float*      out             = kProlong ? L.x : L.r;
-        const float zr = xs[R.c], zb = xs[K.c];
-        out[(size_t)s * 512 + lr]   = zr;
-        out[(size_t)s * 512 + lb]   = zb;
-        if(kDot) {
-            const double d = block_sum<256>((double)zr * (double)R.bv + (double)zb * (double)K.bv, sh);
-            if(t == 0) pz[s] = d;
-        }
-        __syncthreads();  // xs is reused by the block's next brick
     }
-}
+};
 
 // r = b - A x on level F with the pre-smoothed x (held in F.r), summed over each 2x2x2 group into
 // the coarse right-hand side (P^T r). A group's eight cells lie in one fine brick, so the block of a
 // fine brick writes the 64 coarse cells it covers; r itself is never stored. Coarse cells without an
 // active child brick keep the zeros they get when the hierarchy is built. 256 threads, two cells each.
-__global__ void __launch_bounds__(256) k_mg_residual_restrict(LevelDev F, LevelDev C, const PcgState* __restrict__ st) {
-    __shared__ float rs[512];
-    if(st && st->done) return;
-    const int t = threadIdx.x;
-    for(uint32_t v = blockIdx.x; v < F.nSlots; v += gridDim.x) {
-        const int s = (int)xcd_slot(v, F.nSlots);
-        const int b = F.slotBrick[s];
-        const int bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
-        for(int h = 0; h < 2; ++h) {
-            const int    l = t + 256 * h;
-            const size_t o = (size_t)s * 512 + l;
-            rs[l]          = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
-        }
-        __syncthreads();
-        if(t < 64) {
-            const int32_t cs = F.ptab[8 * s + 6];  // the parent brick
-            const int qx = t & 3, qy = (t >> 2) & 3, qz = t >> 4;  // coarse cell inside the brick's 4x4x4
-            const int I = bx * 4 + qx, J = by * 4 + qy, K = bz * 4 + qz;
-            float     sum = 0.f;
-            for(int ch = 0; ch < 8; ++ch) {
-                const int fi = 2 * I + (ch & 1), fj = 2 * J + ((ch >> 1) & 1), fk = 2 * K + ((ch >> 2) & 1);
-                if(fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
-                sum += rs[((fk & 7) * 8 + (fj & 7)) * 8 + (fi & 7)];
+struct k_mg_residual_restrict {
+    static constexpr int kThreads = 256;
+    LevelDev F;
+    LevelDev C;
+    const PcgState* __restrict__ st;
+    __device__ void operator()(Blk blk) const {
+        __shared__ float rs[512];
+        if(st && st->done) return;
+        const int t = threadIdx.x;
+        for(uint32_t v = blk.id; v < F.nSlots; v += blk.n) {
+            const int s = (int)xcd_slot(v, F.nSlots);
+            const int b = F.slotBrick[s];
+            const int bx = b % F.nbx, by = (b / F.nbx) % F.nby, bz = b / (F.nbx * F.nby);
+            for(int h = 0; h < 2; ++h) {
+                const int    l = t + 256 * h;
+                const size_t o = (size_t)s * 512 + l;
+                rs[l]          = F.b[o] - (F.diag[o] * F.r[o] + level_offdiag_sum(F, s, l, F.r));
             }
-            if(cs >= 0 && I < C.nx && J < C.ny && K < C.nz) C.b[(size_t)cs * 512 + ((K & 7) * 8 + (J & 7)) * 8 + (I & 7)] = sum;
+            __syncthreads();
+            if(t < 64) {
+                const int32_t cs = F.ptab[8 * s + 6];  // the parent brick
+                const int qx = t & 3, qy = (t >> 2) & 3, qz = t >> 4;  // coarse cell inside the brick's 4x4x4
+                const int I = bx * 4 + qx, J = by * 4 + qy, K = bz * 4 + qz;
+                float     sum = 0.f;
+                for(int ch = 0; ch < 8; ++ch) {
+                    const int fi = 2 * I + (ch & 1), fj = 2 * J + ((ch >> 1) & 1), fk = 2 * K + ((ch >> 2) & 1);
+                    if(fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
+                    sum += rs[((fk & 7) * 8 + (fj & 7)) * 8 + (fi & 7)];
+                }
+                if(cs >= 0 && I < C.nx && J < C.ny && K < C.nz) C.b[(size_t)cs * 512 + ((K & 7) * 8 + (J & 7)) * 8 + (I & 7)] = sum;
+            }
+            __syncthreads();  // rs is reused by the block's next brick
         }
-        __syncthreads();  // rs is reused by the block's next brick
     }
-}
+};
 
 // coarsest level: x = 0, then `sweeps` red-black sweeps followed by `sweeps` black-red sweeps
 // (symmetric), all inside one block
-__global__ void __launch_bounds__(1024) k_mg_coarsest(LevelDev L, int sweeps, const PcgState* __restrict__ st) {
-    if(st && st->done) return;
-    const uint32_t n = L.nSlots * 512;
-    for(uint32_t t = threadIdx.x; t < n; t += blockDim.x) L.x[t] = 0.f;
-    __syncthreads();
-    for(int it = 0; it < 2 * sweeps; ++it) {
-        for(int half = 0; half < 2; ++half) {
-            const int color = (it < sweeps) ? half : 1 - half;
-            for(uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
-                const int s = (int)(t >> 9), l = (int)(t & 511);
-                if(cell_color(L, s, l) != color) continue;
-                L.x[t] = (L.b[t] - level_offdiag_sum(L, s, l, L.x)) * L.invdiag[t];
+struct k_mg_coarsest {
+    static constexpr int kThreads = 1024;
+    LevelDev L;
+    int sweeps;
+    const PcgState* __restrict__ st;
+    __device__ void operator()(Blk blk) const {
+        if(st && st->done) return;
+        const uint32_t n = L.nSlots * 512;
+        for(uint32_t t = threadIdx.x; t < n; t += blockDim.x) L.x[t] = 0.f;
+        __syncthreads();
+        for(int it = 0; it < 2 * sweeps; ++it) {
+            for(int half = 0; half < 2; ++half) {
+                const int color = (it < sweeps) ? half : 1 - half;
+                for(uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+                    const int s = (int)(t >> 9), l = (int)(t & 511);
+                    if(cell_color(L, s, l) != color) continue;
+                    L.x[t] = (L.b[t] - level_offdiag_sum(L, s, l, L.x)) * L.invdiag[t];
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
-}
+};
 
 // partial z.r for the PCG (after the V-cycle wrote z)
 //==================================================================================================
@@ -1952,8 +2357,8 @@ __device__ __forceinline__ int32_t sys_cell(const int32_t* __restrict__ brickSlo
 }
 
 // the cell (i, j, k = h - i - j) of plane h owned by this thread, or -1
-__device__ __forceinline__ int32_t ic0_plane_cell(const Ic0Dev& D, const GridGeom& g, int h, int& i, int& j, int& k) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ int32_t ic0_plane_cell(Blk blk, const Ic0Dev& D, const GridGeom& g, int h, int& i, int& j, int& k) {
+    const uint32_t t = blk.id * blockDim.x + threadIdx.x;
     if(t >= (uint32_t)g.nx * (uint32_t)g.ny) return -1;
     i = (int)(t % (uint32_t)g.nx);
     j = (int)(t / (uint32_t)g.nx);
@@ -1965,88 +2370,116 @@ __device__ __forceinline__ int32_t ic0_plane_cell(const Ic0Dev& D, const GridGeo
 // pivot d = a_oo - L(o,-z)^2 - L(o,-y)^2 - L(o,-x)^2 (the column updates of the lower neighbours,
 // in their processing order), inv = 1/sqrt(d), or 1/sqrt(a_oo) when d < 0.25 a_oo; then the column
 // below the diagonal is scaled: L(+x,o) = a(o,+x) * inv, and likewise +y, +z
-__global__ void __launch_bounds__(kBlock) k_ic0_factor_plane(Ic0Dev D, GridGeom g, int h) {
-#pragma clang fp contract(off)
-    int           i, j, k;
-    const int32_t o = ic0_plane_cell(D, g, h, i, j, k);
-    if(o < 0) return;
-    const float a = D.diag[o];
-    float       inv;
-    if(a < kFltMin) {
-        inv = a;  // row skipped by the factorisation (PCGSolver.cpp:419-421): its couplings are zero
-    } else {
-        const int32_t mz = sys_cell(D.brickSlot, g, i, j, k - 1), my = sys_cell(D.brickSlot, g, i, j - 1, k),
-                      mx = sys_cell(D.brickSlot, g, i - 1, j, k);
-        float d = a;
-        if(mz >= 0) {
-            const float l = D.lz[mz];
-            d -= l * l;
+struct k_ic0_factor_plane {
+    static constexpr int kThreads = kBlock;
+    Ic0Dev D;
+    GridGeom g;
+    int h;
+    __device__ void operator()(Blk blk) const {
+    #pragma clang fp contract(off)
+        int           i, j, k;
+        const int32_t o = ic0_plane_cell(blk, D, g, h, i, j, k);
+        if(o < 0) return;
+        const float a = D.diag[o];
+        float       inv;
+        if(a < kFltMin) {
+            inv = a;  // row skipped by the factorisation (PCGSolver.cpp:419-421): its couplings are zero
+        } else {
+            const int32_t mz = sys_cell(D.brickSlot, g, i, j, k - 1), my = sys_cell(D.brickSlot, g, i, j - 1, k),
+                          mx = sys_cell(D.brickSlot, g, i - 1, j, k);
+            float d = a;
+            if(mz >= 0) {
+                const float l = D.lz[mz];
+                d -= l * l;
+            }
+            if(my >= 0) {
+                const float l = D.ly[my];
+                d -= l * l;
+            }
+            if(mx >= 0) {
+                const float l = D.lx[mx];
+                d -= l * l;
+            }
+            inv = d < 0.25f * a ? 1.0f / sqrtf(a) : 1.0f / sqrtf(d);
         }
-        if(my >= 0) {
-            const float l = D.ly[my];
-            d -= l * l;
-        }
-        if(mx >= 0) {
-            const float l = D.lx[mx];
-            d -= l * l;
-        }
-        inv = d < 0.25f * a ? 1.0f / sqrtf(a) : 1.0f / sqrtf(d);
+        D.inv[o] = inv;
+        const bool skipped = a < kFltMin;  // an unscaled zero column
+        D.lx[o]  = skipped ? D.cx[o] : D.cx[o] * inv;
+        D.ly[o]  = skipped ? D.cy[o] : D.cy[o] * inv;
+        D.lz[o]  = skipped ? D.cz[o] : D.cz[o] * inv;
     }
-    D.inv[o] = inv;
-    const bool skipped = a < kFltMin;  // an unscaled zero column
-    D.lx[o]  = skipped ? D.cx[o] : D.cx[o] * inv;
-    D.ly[o]  = skipped ? D.cy[o] : D.cy[o] * inv;
-    D.lz[o]  = skipped ? D.cz[o] : D.cz[o] * inv;
-}
+};
 
 // forward solve, plane h: y_o = (((r_o - L(o,-z) y_-z) - L(o,-y) y_-y) - L(o,-x) y_-x) * inv_o
-__global__ void __launch_bounds__(kBlock) k_ic0_lower_plane(Ic0Dev D, GridGeom g, int h, const float* __restrict__ r, float* __restrict__ y,
-                                                            const PcgState* __restrict__ st) {
-#pragma clang fp contract(off)
-    if(st && st->done) return;
-    int           i, j, k;
-    const int32_t o = ic0_plane_cell(D, g, h, i, j, k);
-    if(o < 0) return;
-    const int32_t mz = sys_cell(D.brickSlot, g, i, j, k - 1), my = sys_cell(D.brickSlot, g, i, j - 1, k),
-                  mx = sys_cell(D.brickSlot, g, i - 1, j, k);
-    float v = r[o];
-    if(mz >= 0) v -= D.lz[mz] * y[mz];
-    if(my >= 0) v -= D.ly[my] * y[my];
-    if(mx >= 0) v -= D.lx[mx] * y[mx];
-    y[o] = v * D.inv[o];
-}
+struct k_ic0_lower_plane {
+    static constexpr int kThreads = kBlock;
+    Ic0Dev D;
+    GridGeom g;
+    int h;
+    const float* __restrict__ r;
+    float* __restrict__ y;
+    const PcgState* __restrict__ st;
+    __device__ void operator()(Blk blk) const {
+    #pragma clang fp contract(off)
+        if(st && st->done) return;
+        int           i, j, k;
+        const int32_t o = ic0_plane_cell(blk, D, g, h, i, j, k);
+        if(o < 0) return;
+        const int32_t mz = sys_cell(D.brickSlot, g, i, j, k - 1), my = sys_cell(D.brickSlot, g, i, j - 1, k),
+                      mx = sys_cell(D.brickSlot, g, i - 1, j, k);
+        float v = r[o];
+        if(mz >= 0) v -= D.lz[mz] * y[mz];
+        if(my >= 0) v -= D.ly[my] * y[my];
+        if(mx >= 0) v -= D.lx[mx] * y[mx];
+        y[o] = v * D.inv[o];
+    }
+};
 
 // backward solve in place, plane h (descending): z_o = (((y_o - L(+x,o) z_+x) - L(+y,o) z_+y) -
 // L(+z,o) z_+z) * inv_o
-__global__ void __launch_bounds__(kBlock) k_ic0_upper_plane(Ic0Dev D, GridGeom g, int h, float* __restrict__ z,
-                                                            const PcgState* __restrict__ st) {
-#pragma clang fp contract(off)
-    if(st && st->done) return;
-    int           i, j, k;
-    const int32_t o = ic0_plane_cell(D, g, h, i, j, k);
-    if(o < 0) return;
-    const int32_t px = i + 1 < g.nx ? sys_cell(D.brickSlot, g, i + 1, j, k) : -1;
-    const int32_t py = j + 1 < g.ny ? sys_cell(D.brickSlot, g, i, j + 1, k) : -1;
-    const int32_t pz = k + 1 < g.nz ? sys_cell(D.brickSlot, g, i, j, k + 1) : -1;
-    float         v  = z[o];
-    if(px >= 0) v -= D.lx[o] * z[px];
-    if(py >= 0) v -= D.ly[o] * z[py];
-    if(pz >= 0) v -= D.lz[o] * z[pz];
-    z[o] = v * D.inv[o];
-}
-
-__global__ void __launch_bounds__(512) k_pcg_dot_zr(const float* __restrict__ z, const float* __restrict__ r, uint32_t nSlots,
-                                                    const PcgState* __restrict__ st, double* __restrict__ pa) {
-    __shared__ double sh[8];
-    if(st && st->done) return;
-    double d = 0;
-    for(uint32_t slot = blockIdx.x; slot < nSlots; slot += gridDim.x) {
-        const size_t o = (size_t)slot * 512 + threadIdx.x;
-        d += (double)z[o] * (double)r[o];
+struct k_ic0_upper_plane {
+    static constexpr int kThreads = kBlock;
+    Ic0Dev D;
+    GridGeom g;
+    int h;
+    float* __restrict__ z;
+    const PcgState* __restrict__ st;
+    __device__ void operator()(Blk blk) const {
+    #pragma clang fp contract(off)
+        if(st && st->done) return;
+        int           i, j, k;
+        const int32_t o = ic0_plane_cell(blk, D, g, h, i, j, k);
+        if(o < 0) return;
+        const int32_t px = i + 1 < g.nx ? sys_cell(D.brickSlot, g, i + 1, j, k) : -1;
+        const int32_t py = j + 1 < g.ny ? sys_cell(D.brickSlot, g, i, j + 1, k) : -1;
+        const int32_t pz = k + 1 < g.nz ? sys_cell(D.brickSlot, g, i, j, k + 1) : -1;
+        float         v  = z[o];
+        if(px >= 0) v -= D.lx[o] * z[px];
+        if(py >= 0) v -= D.ly[o] * z[py];
+        if(pz >= 0) v -= D.lz[o] * z[pz];
+        z[o] = v * D.inv[o];
     }
-    d = block_sum<512>(d, sh);
-    if(threadIdx.x == 0) pa[blockIdx.x] = d;
-}
+};
+
+struct k_pcg_dot_zr {
+    static constexpr int kThreads = 512;
+    const float* __restrict__ z;
+    const float* __restrict__ r;
+    uint32_t nSlots;
+    const PcgState* __restrict__ st;
+    double* __restrict__ pa;
+    __device__ void operator()(Blk blk) const {
+        __shared__ double sh[8];
+        if(st && st->done) return;
+        double d = 0;
+        for(uint32_t slot = blk.id; slot < nSlots; slot += blk.n) {
+            const size_t o = (size_t)slot * 512 + threadIdx.x;
+            d += (double)z[o] * (double)r[o];
+        }
+        d = block_sum<512>(d, sh);
+        if(threadIdx.x == 0) pa[blk.id] = d;
+    }
+};
 
 }  // namespace bnn
 
@@ -2057,55 +2490,61 @@ namespace bnn {
 // four black cells (x-adjacent pairs on z = z0, z0+2, z0+4, z0+6), so a half-sweep ends with a
 // single-wave barrier. The LDS array has a zero halo, so couplings to cells outside the brick (zero)
 // are added unconditionally and change nothing.
-__global__ void __launch_bounds__(64) k_mg_coarsest1(LevelDev L, int sweeps, const PcgState* __restrict__ st) {
-    __shared__ float xs[10 * 10 * 10];
-    if(st && st->done) return;
-    const int t = threadIdx.x, px = 2 * (t & 3), ly = (t >> 2) & 7, lz0 = t >> 5;
-    const int shift = (ly + lz0) & 1;  // colour of the even-x cell of every pair of this lane
-    for(int c = t; c < 1000; c += 64) xs[c] = 0.f;
-    SmoothCell R[4], K[4];  // red (colour 0) and black cells
-#pragma unroll
-    for(int q = 0; q < 4; ++q) {
-        const int lz = lz0 + 2 * q;
-#pragma unroll
-        for(int e = 0; e < 2; ++e) {
-            const int  lx = px + e, l = (lz * 8 + ly) * 8 + lx;
-            SmoothCell C;
-            C.c   = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
-            C.czm = lz > 0 ? L.cz[l - 64] : 0.f;
-            C.cym = ly > 0 ? L.cy[l - 8] : 0.f;
-            C.cxm = lx > 0 ? L.cx[l - 1] : 0.f;
-            C.cxp = lx < 7 ? L.cx[l] : 0.f;
-            C.cyp = ly < 7 ? L.cy[l] : 0.f;
-            C.czp = lz < 7 ? L.cz[l] : 0.f;
-            C.bv  = L.b[l];
-            C.inv = L.invdiag[l];
-            if((e ^ shift) == 0) R[q] = C;
-            else K[q] = C;
-        }
-    }
-    __syncthreads();
-    for(int it = 0; it < 2 * sweeps; ++it) {
-        if(it < sweeps) {  // red-black (uniform branch)
-            smooth_cells_update4(R, xs);
-            __syncthreads();
-            smooth_cells_update4(K, xs);
-        } else {  // black-red
-            smooth_cells_update4(K, xs);
-            __syncthreads();
-            smooth_cells_update4(R, xs);
+struct k_mg_coarsest1 {
+    static constexpr int kThreads = 64;
+    LevelDev L;
+    int sweeps;
+    const PcgState* __restrict__ st;
+    __device__ void operator()(Blk blk) const {
+        __shared__ float xs[10 * 10 * 10];
+        if(st && st->done) return;
+        const int t = threadIdx.x, px = 2 * (t & 3), ly = (t >> 2) & 7, lz0 = t >> 5;
+        const int shift = (ly + lz0) & 1;  // colour of the even-x cell of every pair of this lane
+        for(int c = t; c < 1000; c += 64) xs[c] = 0.f;
+        SmoothCell R[4], K[4];  // red (colour 0) and black cells
+    #pragma unroll
+        for(int q = 0; q < 4; ++q) {
+            const int lz = lz0 + 2 * q;
+    #pragma unroll
+            for(int e = 0; e < 2; ++e) {
+                const int  lx = px + e, l = (lz * 8 + ly) * 8 + lx;
+                SmoothCell C;
+                C.c   = ((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1);
+                C.czm = lz > 0 ? L.cz[l - 64] : 0.f;
+                C.cym = ly > 0 ? L.cy[l - 8] : 0.f;
+                C.cxm = lx > 0 ? L.cx[l - 1] : 0.f;
+                C.cxp = lx < 7 ? L.cx[l] : 0.f;
+                C.cyp = ly < 7 ? L.cy[l] : 0.f;
+                C.czp = lz < 7 ? L.cz[l] : 0.f;
+                C.bv  = L.b[l];
+                C.inv = L.invdiag[l];
+                if((e ^ shift) == 0) R[q] = C;
+                else K[q] = C;
+            }
         }
         __syncthreads();
-    }
-#pragma unroll
-    for(int q = 0; q < 4; ++q) {
-        const int lz = lz0 + 2 * q;
-#pragma unroll
-        for(int e = 0; e < 2; ++e) {
-            const int lx = px + e;
-            L.x[(lz * 8 + ly) * 8 + lx] = xs[((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1)];
+        for(int it = 0; it < 2 * sweeps; ++it) {
+            if(it < sweeps) {  // red-black (uniform branch)
+                smooth_cells_update4(R, xs);
+                __syncthreads();
+                smooth_cells_update4(K, xs);
+            } else {  // black-red
+                smooth_cells_update4(K, xs);
+                __syncthreads();
+                smooth_cells_update4(R, xs);
+            }
+            __syncthreads();
+        }
+    #pragma unroll
+        for(int q = 0; q < 4; ++q) {
+            const int lz = lz0 + 2 * q;
+    #pragma unroll
+            for(int e = 0; e < 2; ++e) {
+                const int lx = px + e;
+                L.x[(lz * 8 + ly) * 8 + lx] = xs[((lz + 1) * 10 + (ly + 1)) * 10 + (lx + 1)];
+            }
         }
     }
-}
+};
 
 }  // namespace bnn

@@ -1,0 +1,90 @@
+// flip_launch.hpp -- how the substep's kernels are launched, for one scene or for many at once.
+//
+// Every kernel is a functor: its fields are the launch arguments and operator()(Blk) is the work of
+// one workgroup, which sees only its index among its scene's workgroups (Blk::id) and their count
+// (Blk::n), never blockIdx or gridDim. The same functor then runs
+//   - alone:   k_one<K>, one workgroup per scene block (the functor is the kernel argument), or
+//   - batched: k_many<K>, one launch over a flat work list of many scenes. Scene s owns the
+//              workgroups [first[s], first[s] + count[s]) and its functor is ks[s]. first[s] is a
+//              multiple of 8, so a workgroup's index mod 8 (its XCD) is the same in both forms.
+// A scene's blocks therefore do exactly the same arithmetic in either form, and its results are the
+// same bits alone and in any batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bnn {
+
+struct Blk {
+    uint32_t id, n;  // this workgroup's index among its scene's workgroups, and their count
+};
+
+template<class K> __global__ void __launch_bounds__(K::kThreads) k_one(K k) { k(Blk{blockIdx.x, gridDim.x}); }
+
+template<class K>
+__global__ void __launch_bounds__(K::kThreads) k_many(const K* __restrict__ ks, const uint32_t* __restrict__ first,
+                                                      const uint32_t* __restrict__ count, uint32_t nScenes) {
+    // the scene of this workgroup: the number of scenes s >= 1 with first[s] <= blockIdx.x
+    __shared__ uint32_t s_scene;
+    if(threadIdx.x == 0) s_scene = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    for(uint32_t i = 1 + threadIdx.x; i < nScenes; i += K::kThreads) c += first[i] <= blockIdx.x ? 1u : 0u;
+    if(c) atomicAdd(&s_scene, c);
+    __syncthreads();
+    const uint32_t s     = __builtin_amdgcn_readfirstlane(s_scene);
+    const uint32_t local = blockIdx.x - first[s];
+    const uint32_t n     = count[s];
+    if(local >= n) return;  // padding up to the next scene's multiple of 8
+    ks[s](Blk{local, n});
+}
+
+// dst[i] = v for i < n words
+struct k_fill32 {
+    static constexpr int kThreads = 256;
+    uint32_t* __restrict__ dst;
+    size_t    n;
+    uint32_t  v;
+    __device__ void operator()(Blk B) const {
+        for(size_t i = (size_t)B.id * kThreads + threadIdx.x; i < n; i += (size_t)B.n * kThreads) dst[i] = v;
+    }
+};
+
+// dst[i] = src[i] for i < n words (non-overlapping); 16-byte accesses when both ends allow
+struct k_copy32 {
+    static constexpr int kThreads = 256;
+    uint32_t* __restrict__       dst;
+    const uint32_t* __restrict__ src;
+    size_t                       n;
+    __device__ void operator()(Blk B) const {
+        const size_t stride = (size_t)B.n * kThreads, t0 = (size_t)B.id * kThreads + threadIdx.x;
+        if((((uintptr_t)dst | (uintptr_t)src) & 15u) == 0) {
+            const size_t n4 = n / 4;
+            uint4*       d4 = reinterpret_cast<uint4*>(dst);
+            const uint4* s4 = reinterpret_cast<const uint4*>(src);
+            for(size_t i = t0; i < n4; i += stride) d4[i] = s4[i];
+            for(size_t i = n4 * 4 + t0; i < n; i += stride) dst[i] = src[i];
+        } else {
+            for(size_t i = t0; i < n; i += stride) dst[i] = src[i];
+        }
+    }
+};
+
+// workgroups for a fill or copy of n words: enough to cover n in one pass up to a cap, after which
+// the workgroups stride
+static inline uint32_t fill_blocks(size_t n) {
+    const size_t b = (n + 1023) / 1024;
+    return (uint32_t)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+// readbacks of a batched flush: entry e copies words[e] 32-bit words from src[e] to the staging
+// buffer at out + offset[e] (one workgroup of 64 per entry, at most 64 words each)
+__global__ void __launch_bounds__(64) k_gather_words(const uint32_t* const* __restrict__ src, const uint32_t* __restrict__ words,
+                                                     const uint32_t* __restrict__ offset, uint32_t* __restrict__ out) {
+    const uint32_t e = blockIdx.x;
+    if(threadIdx.x < words[e]) out[offset[e] + threadIdx.x] = src[e][threadIdx.x];
+}
+
+}  // namespace bnn

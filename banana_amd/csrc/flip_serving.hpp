@@ -14,6 +14,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <optional>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -96,6 +97,248 @@ private:
         }
     }
 };
+
+//==================================================================================================
+// fused batches: many scenes' tasks as one stream of merged launches (flip_batch.hpp)
+//==================================================================================================
+class BatchRunner {
+public:
+    // a scene task in flight: the scene records into the runner until the task ends
+    struct Task {
+        FlipSolver*             s = nullptr;
+        std::optional<Co<bool>> co;
+        std::coroutine_handle<> next;
+        uint32_t                lane        = 0;
+        int32_t                 savedTiming = 0;
+        bool                    done        = false;
+        bool                    ok          = false;
+        bnn_status              status      = BNN_OK;
+        std::string             err;
+    };
+
+    // launch shapes the autotuner chooses among; none of them changes a result bit
+    struct Shape {
+        int      p2gSeg     = 8;
+        uint32_t chunkFirst = 8, chunkMax = 8;
+        bool     graphs     = true;
+    };
+
+    BatchRunner() {
+        HIP_OK(hipGetDevice(&device));
+        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        rec = std::make_unique<Recorder>(stream);
+    }
+    ~BatchRunner() {
+        rec.reset();
+        if(stream) (void)hipStreamDestroy(stream);
+    }
+    BatchRunner(const BatchRunner&)            = delete;
+    BatchRunner& operator=(const BatchRunner&) = delete;
+
+    Recorder&    recorder() { return *rec; }
+    const Shape& shape() const { return cur; }
+    void         setShape(const Shape& sh) { cur = sh; }
+
+    // starts `co` (a task of scene s); the scene must be idle (every single-scene call leaves its
+    // stream idle) and on this runner's device
+    void start(Task& t, FlipSolver* s, Co<bool>&& co) {
+        if(s->device != device) throw Error(BNN_ERR_INVALID_ARG, "a scene of the batch lives on another device");
+        if(s->rec) throw Error(BNN_ERR_STATE, "a scene is already running in a batch");
+        t.s           = s;
+        t.lane        = rec->takeLane();
+        t.savedTiming = s->prm.timing;
+        s->prm.timing = 0;  // per-scene events do not apply to merged launches
+        s->rec        = rec.get();
+        s->lane       = t.lane;
+        s->pending    = {};
+        s->p2gSeg        = cur.p2gSeg;
+        s->pcgChunkFirst = cur.chunkFirst;
+        s->pcgChunkMax   = cur.chunkMax;
+        t.co.emplace(std::move(co));
+        t.next = t.co->h;
+        t.done = false;
+    }
+
+    // resumes every task waiting at a sync point (tasks started since the last round run to their
+    // first one), then flushes; tasks that ended are then marked done and detached from the runner
+    // (their last launches were in the flush)
+    void round(const std::vector<Task*>& tasks) {
+        Recorder* const prev = tl_batch;
+        tl_batch             = rec.get();
+        // scenes at a barrier wait while any other scene can still advance
+        bool others = false;
+        for(Task* t : tasks) others |= !t->done && t->next && !t->s->atBarrier;
+        for(Task* t : tasks) {
+            if(t->done || !t->next || (others && t->s->atBarrier)) continue;
+            const std::coroutine_handle<> h = t->next;
+            t->next                         = {};
+            t->s->pending                   = {};
+            t->s->atBarrier                 = false;
+            h.resume();
+            if(!t->co->done()) {
+                t->next = t->s->pending;
+                if(!t->next) {
+                    tl_batch = prev;
+                    throw std::logic_error("a batched task stopped outside a sync point");
+                }
+            }
+        }
+        try {
+            rec->useGraphs = cur.graphs;
+            rec->flushAndSync();
+        } catch(const std::exception& e) {
+            tl_batch = prev;
+            rec->discard();
+            (void)hipStreamSynchronize(stream);
+            for(Task* t : tasks)
+                if(!t->done) fail(*t, BNN_ERR_DEVICE, e.what());
+            return;
+        }
+        tl_batch = prev;
+        for(Task* t : tasks)
+            if(!t->done && t->co->done()) collect(*t);
+    }
+
+    // runs every task to its end
+    void runAll(const std::vector<Task*>& tasks) {
+        for(;;) {
+            bool live = false;
+            for(Task* t : tasks) live |= !t->done;
+            if(!live) return;
+            round(tasks);
+        }
+    }
+
+private:
+    void detach(Task& t) {
+        t.s->rec        = nullptr;
+        t.s->pending    = {};
+        t.s->atBarrier  = false;
+        t.s->prm.timing = t.savedTiming;
+        rec->giveLane(t.lane);
+        t.co.reset();
+        t.next = {};
+        t.done = true;
+    }
+    void collect(Task& t) {
+        try {
+            t.ok     = t.co->result();
+            t.status = BNN_OK;
+        } catch(const Error& e) {
+            t.status = e.code;
+            t.err    = e.what();
+        } catch(const std::exception& e) {
+            t.status = BNN_ERR_DEVICE;
+            t.err    = e.what();
+        }
+        detach(t);
+    }
+    void fail(Task& t, bnn_status code, const std::string& msg) {
+        t.status = code;
+        t.err    = msg;
+        detach(t);
+    }
+
+    int                       device = 0;
+    hipStream_t               stream = nullptr;
+    std::unique_ptr<Recorder> rec;
+    Shape                     cur;
+};
+
+// The autotuner of a batch: the first batched calls try launch shapes, one per call: P2G segment
+// lengths 4, 8 and 16 (scored by the P2G kernel time, measured with events) together with PCG
+// stop-flag chunks of 4, 8 and 16 iterations (scored by the call's wall time less the P2G time),
+// then graphs off at the best pair. The best shape is kept for every later call with the same
+// signature (scene count and total cells, in powers of two).
+class BatchTuner {
+public:
+    struct Trial {
+        BatchRunner::Shape shape;
+        double             p2gMs = 0, restMs = 0;
+    };
+    // the shape for the next call of this signature
+    BatchRunner::Shape next(uint64_t sig) {
+        Entry& e = table[sig];
+        if(e.fixed) return e.best;
+        const size_t k = e.trials.size();
+        if(k < 3) {
+            BatchRunner::Shape sh;
+            sh.p2gSeg     = k == 0 ? 4 : (k == 1 ? 8 : 16);
+            sh.chunkFirst = sh.chunkMax = k == 0 ? 4u : (k == 1 ? 8u : 16u);
+            sh.graphs     = true;
+            return sh;
+        }
+        BatchRunner::Shape sh = bestOfThree(e);
+        sh.graphs             = false;
+        return sh;
+    }
+    // records the outcome of a call made with `sh`
+    void report(uint64_t sig, const BatchRunner::Shape& sh, double p2gMs, double wallMs) {
+        Entry& e = table[sig];
+        if(e.fixed) return;
+        e.trials.push_back(Trial{sh, p2gMs, std::max(0.0, wallMs - p2gMs)});
+        if(e.trials.size() == 4) {
+            BatchRunner::Shape b = bestOfThree(e);
+            const Trial&       g = e.trials[3];
+            double             graphRest = 1e300;
+            for(size_t i = 0; i < 3; ++i)
+                if(e.trials[i].shape.chunkFirst == b.chunkFirst) graphRest = e.trials[i].restMs;
+            b.graphs = !(g.restMs < graphRest);
+            e.best   = b;
+            e.fixed  = true;
+        }
+    }
+    bool fixed(uint64_t sig) const {
+        auto it = table.find(sig);
+        return it != table.end() && it->second.fixed;
+    }
+    std::vector<Trial> trials(uint64_t sig) const {
+        auto it = table.find(sig);
+        return it == table.end() ? std::vector<Trial>{} : it->second.trials;
+    }
+
+private:
+    struct Entry {
+        std::vector<Trial> trials;
+        BatchRunner::Shape best;
+        bool               fixed = false;
+    };
+    static BatchRunner::Shape bestOfThree(const Entry& e) {
+        size_t bp = 0, bc = 0;
+        for(size_t i = 1; i < 3; ++i) {
+            if(e.trials[i].p2gMs < e.trials[bp].p2gMs) bp = i;
+            if(e.trials[i].restMs < e.trials[bc].restMs) bc = i;
+        }
+        BatchRunner::Shape sh;
+        sh.p2gSeg     = e.trials[bp].shape.p2gSeg;
+        sh.chunkFirst = e.trials[bc].shape.chunkFirst;
+        sh.chunkMax   = e.trials[bc].shape.chunkMax;
+        sh.graphs     = true;
+        return sh;
+    }
+    std::unordered_map<uint64_t, Entry> table;
+};
+
+// the batched launch of P2G with column segment seg (4, 8 or 16), FLIP or affine transfer
+static inline Recorder::LaunchFn p2g_launch_fn(int seg, bool affine) {
+    if(affine)
+        return seg == 4 ? Recorder::fnOf<k_p2g_col<true, 4>>() : (seg == 16 ? Recorder::fnOf<k_p2g_col<true, 16>>() : Recorder::fnOf<k_p2g_col<true, 8>>());
+    return seg == 4 ? Recorder::fnOf<k_p2g_col<false, 4>>() : (seg == 16 ? Recorder::fnOf<k_p2g_col<false, 16>>() : Recorder::fnOf<k_p2g_col<false, 8>>());
+}
+
+static inline uint64_t batch_signature(const std::vector<FlipSolver*>& sc) {
+    uint64_t cells = 0;
+    for(const FlipSolver* s : sc) cells += s->nC;
+    auto lg = [](uint64_t v) {
+        uint64_t l = 0;
+        while(v > 1) {
+            v >>= 1;
+            ++l;
+        }
+        return l;
+    };
+    return lg(sc.size()) << 32 | lg(cells);
+}
 
 //==================================================================================================
 // request server
@@ -500,11 +743,11 @@ private:
         bool         ok   = true;
         try {
             if(kind == BNN_REQ_STEP) {
-                ok = sv->step(dt);
+                ok = run_now(sv->step(dt));
             } else {
                 sv->beginStats();
-                ok = sv->pressureProjection(dt);
-                sv->endStats(dt);
+                ok = run_now(sv->pressureProjection(dt));
+                run_now(sv->endStats(dt));
             }
         } catch(const Error& e) {
             code = e.code;

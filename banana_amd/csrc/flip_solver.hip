@@ -3,6 +3,7 @@
 // bnn::FlipSolver owns one scene's device state and queues the stage kernels of
 // FLIP_Solver<3,float>::advanceVelocity (FLIP_Solver.cpp:42-53) on its own HIP stream.
 #include "flip_kernels.hpp"
+#include "flip_batch.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -62,6 +63,10 @@ static inline ZsortLayout zsort_layout(uint64_t np) {
     return L;
 }
 
+// the recorder of the batch this thread is running, if any: a buffer freed meanwhile first has the
+// launches recorded so far issued and finished (they may use it)
+static thread_local Recorder* tl_batch = nullptr;
+
 // device buffer, grows on demand (contents are not preserved across growth)
 struct DevBuf {
     void*  p     = nullptr;
@@ -71,6 +76,7 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
+        if(p && tl_batch) tl_batch->flushAndSync();
         if(p) (void)hipFree(p);
         p     = nullptr;
         bytes = 0;
@@ -212,6 +218,42 @@ public:
     std::string     lastError;
     hipStream_t     stream = nullptr;
 
+    // In a batch (flip_batch.hpp) launches and readbacks go to `rec` as lane `lane`, and a sync point
+    // suspends the host coroutine, leaving its handle in `pending`.
+    Recorder*               rec  = nullptr;
+    uint32_t                lane = 0;
+    std::coroutine_handle<> pending;
+
+    template<class K> void launch(const K& k, uint32_t nblk) {
+        if(nblk == 0) return;
+        if(rec) rec->add(lane, k, nblk);
+        else hipLaunchKernelGGL(k_one<K>, dim3(nblk), dim3(K::kThreads), 0, stream, k);
+    }
+    void fill32(void* p, uint32_t v, size_t bytes) {
+        if(bytes) launch(k_fill32{static_cast<uint32_t*>(p), bytes / 4, v}, fill_blocks(bytes / 4));
+    }
+    void copy32(void* dst, const void* src, size_t bytes) {
+        if(bytes) launch(k_copy32{static_cast<uint32_t*>(dst), static_cast<const uint32_t*>(src), bytes / 4}, fill_blocks(bytes / 4));
+    }
+    void readback(void* host, const void* dev, size_t bytes) {
+        if(rec) rec->readback(lane, host, dev, bytes);
+        else HIP_OK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
+    }
+    SyncAwait syncPoint() {
+        if(rec) return SyncAwait{&pending};
+        HIP_OK(hipStreamSynchronize(stream));
+        return SyncAwait{nullptr};
+    }
+    // a phase boundary: alone nothing; in a batch the scene waits here until every scene of the
+    // batch waits at a barrier or has finished, so that the next phase's launches of all scenes
+    // merge (scenes reach the end of a solve after different numbers of iterations)
+    bool      atBarrier = false;
+    SyncAwait barrier() {
+        if(!rec) return SyncAwait{nullptr};
+        atBarrier = true;
+        return SyncAwait{&pending};
+    }
+
     // particles
     uint64_t     np = 0, cap = 0;
     ParticleBufs bufA, bufB;
@@ -249,6 +291,10 @@ public:
     std::vector<uint32_t> rankHost;
     bool        systemValid = false;
     bool        lastRefined = false;
+    // launch shapes that leave results unchanged (set by the batch autotuner): P2G column segment
+    // length, and the PCG iterations queued before the first stop-flag poll and at most per poll
+    int      p2gSeg        = 8;
+    uint32_t pcgChunkFirst = 4, pcgChunkMax = 64;
 
     // stats
     bnn_flip_stats st{};
@@ -547,13 +593,13 @@ public:
     void scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, uint32_t* part = nullptr) {
         const unsigned nt = blocks(n, kScanTile);
         if(!part) part = scanPart.as<uint32_t>();
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, part);
-        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, stream, part, nt, total);
-        hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, stream, in, n, (const uint32_t*)part, out);
+        launch(k_scan_reduce{in, n, part}, nt);
+        launch(k_scan_partials{part, nt, total}, 1);
+        launch(k_scan_down{in, n, (const uint32_t*)part, out}, nt);
     }
 
     void computeBoundarySDF() {
-        hipLaunchKernelGGL(k_boundary_sdf, dim3(blocks(nN)), dim3(kBlock), 0, stream, bsdf.as<float>(), g, dom);
+        launch(k_boundary_sdf{bsdf.as<float>(), g, dom}, blocks(nN));
         weightsDirty = true;
         bsdfNonzero  = true;
     }
@@ -563,8 +609,8 @@ public:
         if(!np) return;
         VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
         ktBegin(BNN_KT_MOVE);
-        hipLaunchKernelGGL(k_move, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, g, dom, dt,
-                           prm.advection_steps, prm.reflect_velocity_at_boundary, prm.boundary_reflection_multiplier);
+        launch(k_move{cur->view(), (uint32_t)np, G, g, dom, dt,
+                           prm.advection_steps, prm.reflect_velocity_at_boundary, prm.boundary_reflection_multiplier}, blocks(np));
         ktEnd(BNN_KT_MOVE);
         binned = false;
     }
@@ -586,35 +632,32 @@ public:
         c.K          = radius * (dt / (float)prm.advection_steps) * prm.repulsive_force_stiffness;
         c.advSteps   = prm.advection_steps;
         c.seed       = correctSeed();
-        hipLaunchKernelGGL(k_correct_positions, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
-                           (uint32_t)np, g, dom, c, alt->px.as<float>(), alt->py.as<float>(), alt->pz.as<float>());
-        HIP_OK(hipMemcpyAsync(cur->px.p, alt->px.p, np * 4, hipMemcpyDeviceToDevice, stream));
-        HIP_OK(hipMemcpyAsync(cur->py.p, alt->py.p, np * 4, hipMemcpyDeviceToDevice, stream));
-        HIP_OK(hipMemcpyAsync(cur->pz.p, alt->pz.p, np * 4, hipMemcpyDeviceToDevice, stream));
+        launch(k_correct_positions{cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
+                           (uint32_t)np, g, dom, c, alt->px.as<float>(), alt->py.as<float>(), alt->pz.as<float>()}, blocks(np));
+        copy32(cur->px.p, alt->px.p, np * 4);
+        copy32(cur->py.p, alt->py.p, np * 4);
+        copy32(cur->pz.p, alt->pz.p, np * 4);
         binned = false;
     }
 
     // cell-sorted binning: key + count, scan, scatter, per-cell ordering, permute
     void binParticles() {
         if(!np) {
-            HIP_OK(hipMemsetAsync(cellStart.p, 0, cellStart.bytes, stream));
+            fill32(cellStart.p, 0, (nC + 1) * 4);
             binned = true;
             return;
         }
         const ParticleBufs& P = *cur;
-        hipLaunchKernelGGL(k_cell_key, dim3(blocks(np)), dim3(kBlock), 0, stream, P.px.as<float>(), P.py.as<float>(),
-                           P.pz.as<float>(), (uint32_t)np, g, key.as<uint32_t>(), cellCount.as<uint32_t>());
+        launch(k_cell_key{P.px.as<float>(), P.py.as<float>(), P.pz.as<float>(), (uint32_t)np, g, key.as<uint32_t>(), cellCount.as<uint32_t>()}, blocks(np));
         scan(cellCount.as<uint32_t>(), cellStart.as<uint32_t>(), nC, cellStart.as<uint32_t>() + nC);
-        hipLaunchKernelGGL(k_scatter, dim3(blocks(np)), dim3(kBlock), 0, stream, key.as<uint32_t>(), (uint32_t)np,
-                           cellStart.as<uint32_t>(), cellCount.as<uint32_t>(), perm.as<uint32_t>());
-        hipLaunchKernelGGL(k_segsort, dim3(blocks(nC)), dim3(kBlock), 0, stream, cellStart.as<uint32_t>(), (uint32_t)nC,
-                           perm.as<uint32_t>());
+        launch(k_scatter{key.as<uint32_t>(), (uint32_t)np, cellStart.as<uint32_t>(), cellCount.as<uint32_t>(), perm.as<uint32_t>()}, blocks(np));
+        launch(k_segsort{cellStart.as<uint32_t>(), (uint32_t)nC, perm.as<uint32_t>()}, blocks(nC));
         ktBegin(BNN_KT_PERMUTE);
-        hipLaunchKernelGGL(k_permute, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), alt->view(), perm.as<uint32_t>(),
-                           (uint32_t)np, g);
+        launch(k_permute{cur->view(), alt->view(), perm.as<uint32_t>(),
+                           (uint32_t)np, g}, blocks(np));
         if(affineScheme())
-            hipLaunchKernelGGL(k_permute_affine, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->affine(), alt->affine(),
-                               (const uint32_t*)perm.as<uint32_t>(), (uint32_t)np);
+            launch(k_permute_affine{cur->affine(), alt->affine(),
+                               (const uint32_t*)perm.as<uint32_t>(), (uint32_t)np}, blocks(np));
         ktEnd(BNN_KT_PERMUTE);
         std::swap(cur, alt);
         binned = true;
@@ -624,18 +667,23 @@ public:
         if(!binned) binParticles();
     }
 
-    // FLIP_Solver.cpp:57-95
+    // FLIP_Solver.cpp:57-95; the column segment length p2gSeg (4, 8 or 16) does not change the result
+    template<bool kAffine, int kSeg> void p2gLaunch(const FaceOut& out) {
+        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (((size_t)g.nz + 1 + kSeg - 1) / kSeg);  // columns x segments
+        launch(k_p2g_col<kAffine, kSeg>{cur->view(), kAffine ? cur->affine() : Affine{}, (const uint32_t*)cellStart.as<uint32_t>(), g, out},
+               blocks(n));
+    }
+    template<bool kAffine> void p2gSegLaunch(const FaceOut& out) {
+        if(p2gSeg == 4) p2gLaunch<kAffine, 4>(out);
+        else if(p2gSeg == 16) p2gLaunch<kAffine, 16>(out);
+        else p2gLaunch<kAffine, 8>(out);
+    }
     void mapParticles2Grid() {
         requireBinned();
         FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
-        const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (((size_t)g.nz + 1 + kP2gSeg - 1) / kP2gSeg);  // columns x segments
         ktBegin(BNN_KT_P2G);
-        if(affineScheme())  // APIC_Solver.cpp:52-91, AFLIP_Solver.cpp:48-90
-            hipLaunchKernelGGL(k_p2g_col<true>, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), cur->affine(),
-                               (const uint32_t*)cellStart.as<uint32_t>(), g, out);
-        else
-            hipLaunchKernelGGL(k_p2g_col<false>, dim3(blocks(n)), dim3(kBlock), 0, stream, cur->view(), Affine{},
-                               (const uint32_t*)cellStart.as<uint32_t>(), g, out);
+        if(affineScheme()) p2gSegLaunch<true>(out);  // APIC_Solver.cpp:52-91, AFLIP_Solver.cpp:48-90
+        else p2gSegLaunch<false>(out);
         ktEnd(BNN_KT_P2G);
     }
 
@@ -649,26 +697,25 @@ public:
             const size_t n   = (size_t)c.sx * c.sy * c.sz;
             const int    nbx = (c.sx + 7) / 8, nby = (c.sy + 7) / 8, nbz = (c.sz + 7) / 8, nbk = nbx * nby * nbz;
             uint8_t*     has = extBricks.as<uint8_t>(), *near = has + nbk;
-            hipLaunchKernelGGL(k_extrap_brick_valid, dim3(nbk), dim3(256), 0, stream, (const uint8_t*)c.val->as<uint8_t>(), c.sx, c.sy,
-                               c.sz, has);
-            hipLaunchKernelGGL(k_extrap_brick_near, dim3(blocks(nbk)), dim3(kBlock), 0, stream, (const uint8_t*)has, nbx, nby, nbz, near);
+            launch(k_extrap_brick_valid{(const uint8_t*)c.val->as<uint8_t>(), c.sx, c.sy,
+                               c.sz, has}, nbk);
+            launch(k_extrap_brick_near{(const uint8_t*)has, nbx, nby, nbz, near}, blocks(nbk));
             for(int layer = 0; layer < 4; ++layer) {
                 ktBegin(BNN_KT_EXTRAP);
-                hipLaunchKernelGGL(k_extrap_stamp, dim3((c.sx + 63) / 64, (c.sy + 3) / 4, c.sz), dim3(64, 4), 0, stream, c.a->as<float>(),
-                                   c.val->as<uint8_t>(), c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer);
+                launch(k_extrap_stamp{c.a->as<float>(), c.val->as<uint8_t>(), c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer},
+                       (uint32_t)(((c.sx + 63) / 64) * ((c.sy + 3) / 4) * c.sz));
                 ktEnd(BNN_KT_EXTRAP);
             }
-            hipLaunchKernelGGL(k_extrap_finish, dim3(blocks(n)), dim3(kBlock), 0, stream, c.val->as<uint8_t>(), n);
+            launch(k_extrap_finish{c.val->as<uint8_t>(), n}, blocks(n));
         }
     }
 
-    void ensureWeights() {
-        if(!weightsDirty) return;
-        HIP_OK(hipMemsetAsync(counters.p, 0, 4, stream));
-        hipLaunchKernelGGL(k_weights, dim3(blocks(nN)), dim3(kBlock), 0, stream, (const float*)bsdf.as<float>(), g, wu.as<float>(),
-                           wv.as<float>(), ww.as<float>(), counters.as<uint32_t>());
-        HIP_OK(hipMemcpyAsync(hostU32, counters.p, 4, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+    Co<void> ensureWeights() {
+        if(!weightsDirty) co_return;
+        fill32(counters.p, 0, 4);
+        launch(k_weights{(const float*)bsdf.as<float>(), g, wu.as<float>(), wv.as<float>(), ww.as<float>(), counters.as<uint32_t>()}, blocks(nN));
+        readback(hostU32, counters.p, 4);
+        co_await syncPoint();
         solidFaces      = hostU32[0];
         weightsDirty    = false;
         weightsComputed = true;
@@ -688,33 +735,33 @@ public:
         size_t  cnt[3] = {nU, nV, nW};
         for(int d = 0; d < 3; ++d) {
             VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
-            hipLaunchKernelGGL(k_constrain, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, G, g, d, (const uint8_t*)ext[d]->as<uint8_t>(),
-                               (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(), tmp[d]->as<float>());
-            HIP_OK(hipMemcpyAsync(vel[d]->p, tmp[d]->p, cnt[d] * 4, hipMemcpyDeviceToDevice, stream));
+            launch(k_constrain{G, g, d, (const uint8_t*)ext[d]->as<uint8_t>(),
+                               (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(), tmp[d]->as<float>()}, blocks(cnt[d]));
+            copy32(vel[d]->p, tmp[d]->p, cnt[d] * 4);
         }
     }
 
     void backupGridVelocity() {  // FLIP_Data.h:48-52
-        HIP_OK(hipMemcpyAsync(uo.p, u.p, nU * 4, hipMemcpyDeviceToDevice, stream));
-        HIP_OK(hipMemcpyAsync(vo.p, v.p, nV * 4, hipMemcpyDeviceToDevice, stream));
-        HIP_OK(hipMemcpyAsync(wo.p, w.p, nW * 4, hipMemcpyDeviceToDevice, stream));
+        copy32(uo.p, u.p, nU * 4);
+        copy32(vo.p, v.p, nV * 4);
+        copy32(wo.p, w.p, nW * 4);
     }
 
     void addGravity(float dt) {  // PIC_Solver.cpp:505-529
         if(!prm.apply_gravity) return;
         if(prm.gravity_type == 0) {
             const float gdt = 9.81f * dt;
-            hipLaunchKernelGGL(k_add_const, dim3(blocks(nV)), dim3(kBlock), 0, stream, v.as<float>(), nV, gdt, 1);
+            launch(k_add_const{v.as<float>(), nV, gdt, 1}, blocks(nV));
         } else if(prm.gravity_type == 1) {
             DevBuf* vel[3] = {&u, &v, &w};
             size_t  cnt[3] = {nU, nV, nW};
             for(int d = 0; d < 3; ++d)
-                hipLaunchKernelGGL(k_add_const, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, vel[d]->as<float>(), cnt[d], grav[d] * dt, 0);
+                launch(k_add_const{vel[d]->as<float>(), cnt[d], grav[d] * dt, 0}, blocks(cnt[d]));
         } else {
             DevBuf* vel[3] = {&u, &v, &w};
             for(int d = 0; d < 3; ++d)
-                hipLaunchKernelGGL(k_gravity_center, dim3(blocks(nU)), dim3(kBlock), 0, stream, vel[d]->as<float>(), g, d,
-                                   prm.gravity_center[0], prm.gravity_center[1], prm.gravity_center[2], prm.gravity_type == 2 ? 1 : 0, dt);
+                launch(k_gravity_center{vel[d]->as<float>(), g, d,
+                                   prm.gravity_center[0], prm.gravity_center[1], prm.gravity_center[2], prm.gravity_type == 2 ? 1 : 0, dt}, blocks(nU));
         }
     }
 
@@ -722,40 +769,36 @@ public:
         requireBinned();
         ktBegin(BNN_KT_FLUID_SDF);
         const unsigned tiles = blocks(g.nx, kTileX) * blocks(g.ny, kTileY) * blocks(g.nz, kTileZ);
-        hipLaunchKernelGGL(k_fluid_sdf, dim3(tiles), dim3(kTileThreads), 0, stream, cur->view(),
-                           (const uint32_t*)cellStart.as<uint32_t>(), g, (const float*)bsdf.as<float>(), phi.as<float>());
+        launch(k_fluid_sdf{cur->view(),
+                           (const uint32_t*)cellStart.as<uint32_t>(), g, (const float*)bsdf.as<float>(), phi.as<float>()}, tiles);
         ktEnd(BNN_KT_FLUID_SDF);
     }
 
     // active bricks + matrix-free system (PIC_Solver.cpp:693-893)
-    void computeSystem(float dt) {
-        HIP_OK(hipMemsetAsync(counters.p, 0, 16, stream));
+    Co<void> computeSystem(float dt) {
+        fill32(counters.p, 0, 16);
         uint32_t* cnt = counters.as<uint32_t>();
-        hipLaunchKernelGGL(k_brick_flags, dim3(nb), dim3(512), 0, stream, (const float*)phi.as<float>(), g, bflag.as<uint32_t>(), cnt + 1);
+        launch(k_brick_flags{(const float*)phi.as<float>(), g, bflag.as<uint32_t>(), cnt + 1}, nb);
         scan(bflag.as<uint32_t>(), bscan.as<uint32_t>(), nb, cnt + 2);
-        HIP_OK(hipMemcpyAsync(hostU32, counters.p, 16, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+        readback(hostU32, counters.p, 16);
+        co_await syncPoint();
         nFluid = hostU32[1];
         nSlots = hostU32[2];
         if(nSlots > nb) throw Error(BNN_ERR_DEVICE, "active brick count exceeds the brick grid");
         ensureSlots(std::max<uint32_t>(nSlots, 1));
-        hipLaunchKernelGGL(k_brick_slots, dim3(blocks(nb)), dim3(kBlock), 0, stream, (const uint32_t*)bflag.as<uint32_t>(),
-                           (const uint32_t*)bscan.as<uint32_t>(), nb, brickSlot.as<int32_t>(), slotBrick.as<int32_t>());
+        launch(k_brick_slots{(const uint32_t*)bflag.as<uint32_t>(), (const uint32_t*)bscan.as<uint32_t>(), nb, brickSlot.as<int32_t>(), slotBrick.as<int32_t>()}, blocks(nb));
         if(nSlots == 0) {
             systemValid = true;
-            return;
+            co_return;
         }
-        hipLaunchKernelGGL(k_brick_nbrs, dim3(blocks(nSlots)), dim3(kBlock), 0, stream, (const int32_t*)slotBrick.as<int32_t>(), nSlots,
-                           (const int32_t*)brickSlot.as<int32_t>(), g, nbrTab.as<int32_t>());
+        launch(k_brick_nbrs{(const int32_t*)slotBrick.as<int32_t>(), nSlots, (const int32_t*)brickSlot.as<int32_t>(), g, nbrTab.as<int32_t>()}, blocks(nSlots));
         const bool refined = prm.pressure_mode == BNN_PRESSURE_REFINED;
         SystemOut  S{rhs64.as<double>(), refined ? diag64.as<double>() : nullptr, refined ? cx64.as<double>() : nullptr,
                     refined ? cy64.as<double>() : nullptr, refined ? cz64.as<double>() : nullptr, rhs32.as<float>(), diag32.as<float>(),
                     cx32.as<float>(), cy32.as<float>(), cz32.as<float>(), invd.as<float>()};
         VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
-        hipLaunchKernelGGL(k_assemble, dim3(nSlots), dim3(512), 0, stream, (const int32_t*)slotBrick.as<int32_t>(), g,
-                           (const float*)phi.as<float>(), G, (const float*)wu.as<float>(), (const float*)wv.as<float>(),
-                           (const float*)ww.as<float>(), dt, S);
-        if(prm.preconditioner == BNN_PRECOND_MG) buildHierarchy();
+        launch(k_assemble{(const int32_t*)slotBrick.as<int32_t>(), g, (const float*)phi.as<float>(), G, (const float*)wu.as<float>(), (const float*)wv.as<float>(), (const float*)ww.as<float>(), dt, S}, nSlots);
+        if(prm.preconditioner == BNN_PRECOND_MG) co_await buildHierarchy();
         systemValid = true;
     }
 
@@ -784,7 +827,7 @@ public:
                         L.x.as<float>(), L.b.as<float>(), L.r.as<float>(), L.nSlots, L.ptab.as<int32_t>()};
     }
 
-    void buildHierarchy() {
+    Co<void> buildHierarchy() {
         mgLevels = 1;
         if(mg.empty()) mg.emplace_back(nullptr);
         int      nx = g.nx, ny = g.ny, nz = g.nz;
@@ -803,13 +846,12 @@ public:
             L.flag.reserve(L.nb * 4);
             L.scan.reserve(L.nb * 4);
             L.brickSlot.reserve(L.nb * 4);
-            HIP_OK(hipMemsetAsync(L.flag.p, 0, L.nb * 4, stream));
-            HIP_OK(hipMemsetAsync(counters.p, 0, 16, stream));
-            hipLaunchKernelGGL(k_mg_flag_parents, dim3(blocks(prev.nSlots)), dim3(kBlock), 0, stream, prev.slotBrick, prev.nSlots, prev.nbx,
-                               prev.nby, L.nbx, L.nby, L.flag.as<uint32_t>());
+            fill32(L.flag.p, 0, L.nb * 4);
+            fill32(counters.p, 0, 16);
+            launch(k_mg_flag_parents{prev.slotBrick, prev.nSlots, prev.nbx, prev.nby, L.nbx, L.nby, L.flag.as<uint32_t>()}, blocks(prev.nSlots));
             scan(L.flag.as<uint32_t>(), L.scan.as<uint32_t>(), L.nb, counters.as<uint32_t>() + 2);
-            HIP_OK(hipMemcpyAsync(hostU32, counters.p, 16, hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
+            readback(hostU32, counters.p, 16);
+            co_await syncPoint();
             L.nSlots = hostU32[2];
             if(L.nSlots == 0 || L.nSlots > L.nb) throw Error(BNN_ERR_DEVICE, "multigrid level has an invalid brick count");
             if(L.nSlots > L.slotCap) {
@@ -823,19 +865,16 @@ public:
             GridGeom lg = g;
             lg.nx = nx; lg.ny = ny; lg.nz = nz;
             lg.nbx = L.nbx; lg.nby = L.nby; lg.nbz = L.nbz;
-            hipLaunchKernelGGL(k_brick_slots, dim3(blocks(L.nb)), dim3(kBlock), 0, stream, (const uint32_t*)L.flag.as<uint32_t>(),
-                               (const uint32_t*)L.scan.as<uint32_t>(), L.nb, L.brickSlot.as<int32_t>(), L.slotBrick.as<int32_t>());
-            hipLaunchKernelGGL(k_brick_nbrs, dim3(blocks(L.nSlots)), dim3(kBlock), 0, stream, (const int32_t*)L.slotBrick.as<int32_t>(),
-                               L.nSlots, (const int32_t*)L.brickSlot.as<int32_t>(), lg, L.nbr.as<int32_t>());
+            launch(k_brick_slots{(const uint32_t*)L.flag.as<uint32_t>(), (const uint32_t*)L.scan.as<uint32_t>(), L.nb, L.brickSlot.as<int32_t>(), L.slotBrick.as<int32_t>()}, blocks(L.nb));
+            launch(k_brick_nbrs{(const int32_t*)L.slotBrick.as<int32_t>(), L.nSlots, (const int32_t*)L.brickSlot.as<int32_t>(), lg, L.nbr.as<int32_t>()}, blocks(L.nSlots));
             const LevelDev cur = levelDev(l);
-            hipLaunchKernelGGL(k_mg_coarsen, dim3(L.nSlots), dim3(512), 0, stream, prev, cur, L.diag.as<float>(), L.cx.as<float>(),
-                               L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>());
+            launch(k_mg_coarsen{prev, cur, L.diag.as<float>(), L.cx.as<float>(), L.cy.as<float>(), L.cz.as<float>(), L.inv.as<float>()}, L.nSlots);
             // the restriction writes only coarse cells under an active fine brick; the others stay 0
-            HIP_OK(hipMemsetAsync(L.b.p, 0, (size_t)L.nSlots * 512 * 4, stream));
+            fill32(L.b.p, 0, (size_t)L.nSlots * 512 * 4);
             // the finer level's parent table (its bricks' parents in this level)
             DevBuf& pt = l == 1 ? ptab0 : mg[l - 1]->ptab;
             pt.reserve((size_t)std::max<uint32_t>(prev.nSlots, 1u) * 32);
-            hipLaunchKernelGGL(k_mg_parent_table, dim3(blocks(prev.nSlots)), dim3(kBlock), 0, stream, prev, levelDev(l), pt.as<int32_t>());
+            launch(k_mg_parent_table{prev, levelDev(l), pt.as<int32_t>()}, blocks(prev.nSlots));
             prevSlots = L.nSlots;
             ++mgLevels;
         }
@@ -852,33 +891,33 @@ public:
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
             ktBegin(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
-            hipLaunchKernelGGL((k_mg_smooth_brick<0, 0>), dim3(G), dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
+            launch(k_mg_smooth_brick<0, 0>{L, C, omega, nu, stp, (double*)nullptr}, G);
             ktEnd(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
             ktBegin(l == 0 ? BNN_KT_MG_RESTRICT : BNN_KT_MG_COARSE);
-            hipLaunchKernelGGL(k_mg_residual_restrict, dim3(G), dim3(256), 0, stream, L, C, stp);
+            launch(k_mg_residual_restrict{L, C, stp}, G);
             ktEnd(l == 0 ? BNN_KT_MG_RESTRICT : BNN_KT_MG_COARSE);
         }
         const LevelDev Lc = levelDev(mgLevels - 1);
         ktBegin(BNN_KT_MG_COARSE);
         if(Lc.nSlots == 1) {
-            hipLaunchKernelGGL(k_mg_coarsest1, dim3(1), dim3(64), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
+            launch(k_mg_coarsest1{Lc, prm.mg_coarsest_sweeps, stp}, 1);
         } else {
-            hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(1024), 0, stream, Lc, prm.mg_coarsest_sweeps, stp);
+            launch(k_mg_coarsest{Lc, prm.mg_coarsest_sweeps, stp}, 1);
         }
         ktEnd(BNN_KT_MG_COARSE);
         for(int l = mgLevels - 2; l >= 0; --l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
-            const dim3     G(std::max<uint32_t>(L.nSlots, 1u));
+            const uint32_t G = std::max<uint32_t>(L.nSlots, 1u);
             if(l == 0) {  // finest level: correction as its own streaming pass, then smooth and form z.r
                 ktBegin(BNN_KT_MG_PROLONG);
-                hipLaunchKernelGGL(k_mg_prolong, G, dim3(256), 0, stream, L, C, omega, stp);
+                launch(k_mg_prolong{L, C, omega, stp}, G);
                 ktEnd(BNN_KT_MG_PROLONG);
                 ktBegin(BNN_KT_MG_POST);
-                hipLaunchKernelGGL((k_mg_smooth_brick<2, 1>), G, dim3(256), 0, stream, L, C, omega, nu, stp, pz.as<double>());
+                launch(k_mg_smooth_brick<2, 1>{L, C, omega, nu, stp, pz.as<double>()}, G);
                 ktEnd(BNN_KT_MG_POST);
             } else {
                 ktBegin(BNN_KT_MG_COARSE);
-                hipLaunchKernelGGL((k_mg_smooth_brick<1, 0>), G, dim3(256), 0, stream, L, C, omega, nu, stp, (double*)nullptr);
+                launch(k_mg_smooth_brick<1, 0>{L, C, omega, nu, stp, (double*)nullptr}, G);
                 ktEnd(BNN_KT_MG_COARSE);
             }
         }
@@ -894,29 +933,28 @@ public:
     void formIC0() {  // PCGSolver.cpp:398-466 (formPreconditioner at the start of every solve)
         const Ic0Dev   D  = ic0Dev();
         const unsigned nb = blocks((size_t)g.nx * g.ny);
-        for(int h = 0; h < ic0Planes(); ++h) hipLaunchKernelGGL(k_ic0_factor_plane, dim3(nb), dim3(kBlock), 0, stream, D, g, h);
+        for(int h = 0; h < ic0Planes(); ++h) launch(k_ic0_factor_plane{D, g, h}, nb);
     }
     // z = (L L^T)^-1 r (PCGSolver.cpp:243-269): forward planes ascending into z, then backward in place
     void applyIC0(const float* rv, float* zv, const PcgState* stp) {
         const Ic0Dev   D  = ic0Dev();
         const unsigned nb = blocks((size_t)g.nx * g.ny);
         const int      H  = ic0Planes();
-        for(int h = 0; h < H; ++h) hipLaunchKernelGGL(k_ic0_lower_plane, dim3(nb), dim3(kBlock), 0, stream, D, g, h, rv, zv, stp);
-        for(int h = H - 1; h >= 0; --h) hipLaunchKernelGGL(k_ic0_upper_plane, dim3(nb), dim3(kBlock), 0, stream, D, g, h, zv, stp);
+        for(int h = 0; h < H; ++h) launch(k_ic0_lower_plane{D, g, h, rv, zv, stp}, nb);
+        for(int h = H - 1; h >= 0; --h) launch(k_ic0_upper_plane{D, g, h, zv, stp}, nb);
     }
 
     unsigned redBlocks() const { return std::min<unsigned>(std::max<uint32_t>(nSlots, 1u), (unsigned)kMaxRedBlocks); }
 
     // FP32 PCG on b -> x (PCGSolver.cpp:143-199); returns success
-    bool pcg32(const float* b, float relTol, uint32_t maxIter, uint32_t& iters, float& res, float& bmax) {
+    Co<bool> pcg32(const float* b, float relTol, uint32_t maxIter, uint32_t& iters, float& res, float& bmax) {
         PcgState*      dst = pst.as<PcgState>();
         double*        dpa = pa.as<double>();
         double*        dpb = pb.as<double>();
         float *        sA = s0.as<float>(), *sB = s1.as<float>();
         const unsigned G   = redBlocks();
         const int      jac = prm.preconditioner == BNN_PRECOND_JACOBI ? 1 : 0;
-        hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(512), 0, stream, b, (const float*)invd.as<float>(), x.as<float>(), r.as<float>(),
-                           z.as<float>(), sA, nSlots, dpa, dpb, jac);
+        launch(k_pcg_init{b, (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), sA, nSlots, dpa, dpb, jac}, G);
         // partials of z.r: per block (G) from k_pcg_init / k_pcg_dot_zr, or per brick from the V-cycle
         const double* rz  = dpa;
         int           nrz = (int)G;
@@ -925,53 +963,47 @@ public:
         auto          zr  = [&](const PcgState* stp) {
             if(ic0) {
                 applyIC0(r.as<float>(), z.as<float>(), stp);
-                hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(),
-                                   nSlots, stp, dpa);
+                launch(k_pcg_dot_zr{(const float*)z.as<float>(), (const float*)r.as<float>(), nSlots, stp, dpa}, G);
             } else if(vcycle(stp)) {
                 rz  = pz.as<double>();
                 nrz = (int)nSlots;
             } else {
-                hipLaunchKernelGGL(k_pcg_dot_zr, dim3(G), dim3(512), 0, stream, (const float*)z.as<float>(), (const float*)r.as<float>(),
-                                   nSlots, stp, dpa);
+                launch(k_pcg_dot_zr{(const float*)z.as<float>(), (const float*)r.as<float>(), nSlots, stp, dpa}, G);
             }
         };
         if(!jac) zr(nullptr);
-        hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(1024), 0, stream, dst, rz, nrz, (const double*)dpb, (int)G, relTol, maxIter);
-        uint32_t chunk = 4, launched = 0;
+        launch(k_pcg_init_fin{dst, rz, nrz, (const double*)dpb, (int)G, relTol, maxIter}, 1);
+        uint32_t chunk = pcgChunkFirst, launched = 0;
         for(;;) {
-            HIP_OK(hipMemcpyAsync(hostSt, dst, sizeof(PcgState), hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
+            readback(hostSt, dst, sizeof(PcgState));
+            co_await syncPoint();
             if(hostSt->done) break;
             // never queue more iterations than can remain
             const uint32_t left = hostSt->maxIter > hostSt->iter ? hostSt->maxIter - hostSt->iter : 1u;
             const uint32_t n    = std::min(chunk, left);
             for(uint32_t c = 0; c < n; ++c) {
                 ktBegin(BNN_KT_PCG_SPMV);
-                hipLaunchKernelGGL(k_pcg_spmv, dim3(G), dim3(512), 0, stream, (const int32_t*)nbrTab.as<int32_t>(), (const float*)z.as<float>(),
-                                   (const float*)sA, sB, q.as<float>(), (const float*)diag32.as<float>(), (const float*)cx32.as<float>(),
-                                   (const float*)cy32.as<float>(), (const float*)cz32.as<float>(), nSlots, (const PcgState*)dst, dpa);
+                launch(k_pcg_spmv{(const int32_t*)nbrTab.as<int32_t>(), (const float*)z.as<float>(), (const float*)sA, sB, q.as<float>(), (const float*)diag32.as<float>(), (const float*)cx32.as<float>(), (const float*)cy32.as<float>(), (const float*)cz32.as<float>(), nSlots, (const PcgState*)dst, dpa}, G);
                 ktEnd(BNN_KT_PCG_SPMV);
-                hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(1024), 0, stream, dst, (const double*)dpa, (int)G);
+                launch(k_pcg_alpha{dst, (const double*)dpa, (int)G}, 1);
                 ktBegin(BNN_KT_PCG_UPDATE);
-                hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(512), 0, stream, (const float*)sB, (const float*)q.as<float>(),
-                                   (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst,
-                                   dpa, dpb, jac);
+                launch(k_pcg_update{(const float*)sB, (const float*)q.as<float>(), (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst, dpa, dpb, jac}, G);
                 ktEnd(BNN_KT_PCG_UPDATE);
                 if(!jac) zr(dst);
-                hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1024), 0, stream, dst, rz, nrz, (const double*)dpb, (int)G);
+                launch(k_pcg_beta{dst, rz, nrz, (const double*)dpb, (int)G}, 1);
                 std::swap(sA, sB);
             }
             launched += n;
-            chunk = std::min<uint32_t>(chunk * 2, 64);
+            chunk = std::min<uint32_t>(chunk * 2, pcgChunkMax);
             if(launched > maxIter + 256) throw Error(BNN_ERR_DEVICE, "PCG device state stopped advancing");
         }
         iters = hostSt->iter;
         res   = hostSt->res;
         bmax  = hostSt->bmax;
-        return hostSt->success != 0;
+        co_return hostSt->success != 0;
     }
 
-    bool solveSystem() {  // PIC_Solver.cpp:897-907
+    Co<bool> solveSystem() {  // PIC_Solver.cpp:897-907
         lastRefined = prm.pressure_mode == BNN_PRESSURE_REFINED;
         if(nSlots == 0) {
             st.pcg_iterations   = 0;
@@ -979,18 +1011,18 @@ public:
             st.pcg_residual     = 0;
             st.rhs_max          = 0;
             st.pcg_success      = 1;
-            return true;
+            co_return true;
         }
         if(!lastRefined) {
             uint32_t it;
             float    res, bmax;
-            bool     ok         = pcg32(rhs32.as<float>(), prm.cg_relative_tolerance, prm.max_cg_iteration, it, res, bmax);
+            bool     ok         = co_await pcg32(rhs32.as<float>(), prm.cg_relative_tolerance, prm.max_cg_iteration, it, res, bmax);
             st.pcg_iterations   = it;
             st.outer_iterations = 1;
             st.pcg_residual     = res;
             st.rhs_max          = bmax;
             st.pcg_success      = ok;
-            return ok;
+            co_return ok;
         }
         // FP64 defect correction around FP32 PCG: r = b - A x in FP64, solve A d = r in FP32, x += d
         const double   tol      = prm.refine_tolerance > 0 ? prm.refine_tolerance : 1e-10;
@@ -1001,18 +1033,15 @@ public:
         bool           ok = false;
         double         bmax = 0, res = 0;
         for(int first = 1;; first = 0) {
-            hipLaunchKernelGGL(k_residual64, dim3(G), dim3(512), 0, stream, (const int32_t*)nbrTab.as<int32_t>(),
-                               (const double*)x64.as<double>(), (const double*)rhs64.as<double>(), (const double*)diag64.as<double>(),
-                               (const double*)cx64.as<double>(), (const double*)cy64.as<double>(), (const double*)cz64.as<double>(),
-                               rhs32.as<float>(), nSlots, pa.as<double>(), pb.as<double>(), first);
-            hipLaunchKernelGGL(k_residual64_fin, dim3(1), dim3(1024), 0, stream, ost.as<OuterState>(), (const double*)pa.as<double>(),
-                               (const double*)pb.as<double>(), (int)G);
-            HIP_OK(hipMemcpyAsync(hostOst, ost.p, sizeof(OuterState), hipMemcpyDeviceToHost, stream));
-            HIP_OK(hipStreamSynchronize(stream));
+            co_await barrier();  // the scenes' outer passes start together
+            launch(k_residual64{(const int32_t*)nbrTab.as<int32_t>(), (const double*)x64.as<double>(), (const double*)rhs64.as<double>(), (const double*)diag64.as<double>(), (const double*)cx64.as<double>(), (const double*)cy64.as<double>(), (const double*)cz64.as<double>(), rhs32.as<float>(), nSlots, pa.as<double>(), pb.as<double>(), first}, G);
+            launch(k_residual64_fin{ost.as<OuterState>(), (const double*)pa.as<double>(), (const double*)pb.as<double>(), (int)G}, 1);
+            readback(hostOst, ost.p, sizeof(OuterState));
+            co_await syncPoint();
             if(first) {
                 bmax = hostOst->bmax;
                 if(bmax < 1.17549435e-38) {
-                    HIP_OK(hipMemsetAsync(x64.p, 0, (size_t)nSlots * 512 * 8, stream));
+                    fill32(x64.p, 0, (size_t)nSlots * 512 * 8);
                     ok  = true;
                     res = 0;
                     break;
@@ -1020,16 +1049,16 @@ public:
             }
             res = hostOst->res;
             if(res <= tol * bmax) {
-                if(first) HIP_OK(hipMemsetAsync(x64.p, 0, (size_t)nSlots * 512 * 8, stream));
+                if(first) fill32(x64.p, 0, (size_t)nSlots * 512 * 8);
                 ok = true;
                 break;
             }
             if(outer >= maxOuter) break;
             uint32_t it;
             float    r32, b32;
-            pcg32(rhs32.as<float>(), innerTol, prm.max_cg_iteration, it, r32, b32);
+            co_await pcg32(rhs32.as<float>(), innerTol, prm.max_cg_iteration, it, r32, b32);
             total += it;
-            hipLaunchKernelGGL(k_accumulate64, dim3(G), dim3(512), 0, stream, x64.as<double>(), (const float*)x.as<float>(), nSlots, first);
+            launch(k_accumulate64{x64.as<double>(), (const float*)x.as<float>(), nSlots, first}, G);
             ++outer;
             if(it == 0) break;  // inner solve made no progress
         }
@@ -1038,7 +1067,7 @@ public:
         st.pcg_residual     = res;
         st.rhs_max          = bmax;
         st.pcg_success      = ok;
-        return ok;
+        co_return ok;
     }
 
     void updateProjectedVelocity(float dt) {  // PIC_Solver.cpp:911-981
@@ -1049,33 +1078,35 @@ public:
         for(int d = 0; d < 3; ++d) {
             if(lastRefined && nSlots) {
                 PressView<double> pv{x64.as<double>(), brickSlot.as<int32_t>()};
-                hipLaunchKernelGGL(k_update_velocity<double>, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, vel[d]->as<float>(),
-                                   val[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)phi.as<float>(), pv, g, d, dt);
+                launch(k_update_velocity<double>{vel[d]->as<float>(),
+                                   val[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)phi.as<float>(), pv, g, d, dt}, blocks(cnt[d]));
             } else {
                 PressView<float> pv{x.as<float>(), brickSlot.as<int32_t>()};
-                hipLaunchKernelGGL(k_update_velocity<float>, dim3(blocks(cnt[d])), dim3(kBlock), 0, stream, vel[d]->as<float>(),
-                                   val[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)phi.as<float>(), pv, g, d, dt);
+                launch(k_update_velocity<float>{vel[d]->as<float>(),
+                                   val[d]->as<uint8_t>(), (const float*)wgt[d]->as<float>(), (const float*)phi.as<float>(), pv, g, d, dt}, blocks(cnt[d]));
             }
         }
     }
 
-    bool pressureProjection(float dt) {  // PIC_Solver.cpp:533-539
+    Co<bool> pressureProjection(float dt) {  // PIC_Solver.cpp:533-539
         stageBegin(BNN_STAGE_WEIGHTS);
-        ensureWeights();
+        co_await ensureWeights();
         stageEnd(BNN_STAGE_WEIGHTS);
         stageBegin(BNN_STAGE_PHI);
         computeFluidSDF();
         stageEnd(BNN_STAGE_PHI);
         stageBegin(BNN_STAGE_ASSEMBLY);
-        computeSystem(dt);
+        co_await computeSystem(dt);
         stageEnd(BNN_STAGE_ASSEMBLY);
         stageBegin(BNN_STAGE_PCG);
-        bool ok = solveSystem();
+        co_await barrier();
+        bool ok = co_await solveSystem();
+        co_await barrier();
         stageEnd(BNN_STAGE_PCG);
         stageBegin(BNN_STAGE_UPDATE);
         updateProjectedVelocity(dt);
         stageEnd(BNN_STAGE_UPDATE);
-        return ok;
+        co_return ok;
     }
 
     void mapGrid2Particles() {  // FLIP_Solver.cpp:99-133
@@ -1085,17 +1116,17 @@ public:
         OldGrid O{uo.as<float>(), vo.as<float>(), wo.as<float>()};
         ktBegin(BNN_KT_G2P);
         if(prm.transfer_scheme == BNN_TRANSFER_APIC)
-            hipLaunchKernelGGL(k_g2p_affine<1>, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), cur->affine(), (uint32_t)np, G, O, g,
-                               prm.pic_flip_ratio);
+            launch(k_g2p_affine<1>{cur->view(), cur->affine(), (uint32_t)np, G, O, g,
+                               prm.pic_flip_ratio}, blocks(np));
         else if(prm.transfer_scheme == BNN_TRANSFER_AFLIP)
-            hipLaunchKernelGGL(k_g2p_affine<2>, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), cur->affine(), (uint32_t)np, G, O, g,
-                               prm.pic_flip_ratio);
+            launch(k_g2p_affine<2>{cur->view(), cur->affine(), (uint32_t)np, G, O, g,
+                               prm.pic_flip_ratio}, blocks(np));
         else
-            hipLaunchKernelGGL(k_g2p, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), (uint32_t)np, G, O, g, prm.pic_flip_ratio);
+            launch(k_g2p{cur->view(), (uint32_t)np, G, O, g, prm.pic_flip_ratio}, blocks(np));
         ktEnd(BNN_KT_G2P);
     }
 
-    bool advanceVelocity(float dt) {  // FLIP_Solver.cpp:42-53 (FLIP, AFLIP); APIC_Solver.cpp:41-48 (APIC)
+    Co<bool> advanceVelocity(float dt) {  // FLIP_Solver.cpp:42-53 (FLIP, AFLIP); APIC_Solver.cpp:41-48 (APIC)
         stageBegin(BNN_STAGE_BIN);
         requireBinned();
         stageEnd(BNN_STAGE_BIN);
@@ -1106,7 +1137,7 @@ public:
             stageBegin(BNN_STAGE_CONSTRAIN);
             addGravity(dt);
             stageEnd(BNN_STAGE_CONSTRAIN);
-            bool ok = pressureProjection(dt);
+            bool ok = co_await pressureProjection(dt);
             stageBegin(BNN_STAGE_EXTRAPOLATE);
             extrapolateVelocity();
             stageEnd(BNN_STAGE_EXTRAPOLATE);
@@ -1116,7 +1147,7 @@ public:
             stageBegin(BNN_STAGE_G2P);
             mapGrid2Particles();
             stageEnd(BNN_STAGE_G2P);
-            return ok;
+            co_return ok;
         }
         stageBegin(BNN_STAGE_EXTRAPOLATE);
         extrapolateVelocity();
@@ -1126,7 +1157,7 @@ public:
         backupGridVelocity();
         addGravity(dt);
         stageEnd(BNN_STAGE_CONSTRAIN);
-        bool ok = pressureProjection(dt);
+        bool ok = co_await pressureProjection(dt);
         stageBegin(BNN_STAGE_EXTRAPOLATE);
         extrapolateVelocity();
         stageEnd(BNN_STAGE_EXTRAPOLATE);
@@ -1136,7 +1167,7 @@ public:
         stageBegin(BNN_STAGE_G2P);
         mapGrid2Particles();
         stageEnd(BNN_STAGE_G2P);
-        return ok;
+        co_return ok;
     }
 
     void beginStats() {
@@ -1144,8 +1175,8 @@ public:
         for(double& a : ktAcc) a = 0;
         for(uint64_t& c : ktCnt) c = 0;
     }
-    void endStats(float dt) {
-        HIP_OK(hipStreamSynchronize(stream));
+    Co<void> endStats(float dt) {
+        co_await syncPoint();
         collectTimers();
         for(int s = 0; s < BNN_NUM_STAGES; ++s) st.stage_ms[s] = prm.timing ? stageAcc[s] : 0.0;
         for(int k = 0; k < BNN_NUM_KERNEL_TIMERS; ++k) {
@@ -1184,7 +1215,7 @@ public:
         st.last_substep      = dt;
     }
 
-    bool step(float dt) {  // PIC_Solver.cpp:292-294 (moveParticles, advanceVelocity)
+    Co<bool> step(float dt) {  // PIC_Solver.cpp:292-294 (moveParticles, advanceVelocity)
         if(!(dt > 0)) throw Error(BNN_ERR_INVALID_ARG, "dt must be > 0");
         beginStats();
         stageBegin(BNN_STAGE_MOVE);
@@ -1198,40 +1229,39 @@ public:
         stageBegin(BNN_STAGE_BIN);
         binParticles();
         stageEnd(BNN_STAGE_BIN);
-        bool ok = advanceVelocity(dt);
-        endStats(dt);
+        bool ok = co_await advanceVelocity(dt);
+        co_await endStats(dt);
         ++st.steps;
-        return ok;
+        co_return ok;
     }
 
-    float timestepCFL() {  // PIC_Solver.cpp:342-349
-        HIP_OK(hipMemsetAsync(counters.p, 0, 4, stream));
+    Co<float> timestepCFL() {  // PIC_Solver.cpp:342-349
+        fill32(counters.p, 0, 4);
         DevBuf* vel[3] = {&u, &v, &w};
         size_t  cnt[3] = {nU, nV, nW};
         for(int d = 0; d < 3; ++d)
-            hipLaunchKernelGGL(k_maxabs, dim3(std::min<unsigned>(blocks(cnt[d]), 2048)), dim3(kBlock), 0, stream, (const float*)vel[d]->as<float>(),
-                               cnt[d], counters.as<uint32_t>());
-        HIP_OK(hipMemcpyAsync(hostU32, counters.p, 4, hipMemcpyDeviceToHost, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+            launch(k_maxabs{(const float*)vel[d]->as<float>(), cnt[d], counters.as<uint32_t>()}, std::min<unsigned>(blocks(cnt[d]), 2048));
+        readback(hostU32, counters.p, 4);
+        co_await syncPoint();
         float m;
         std::memcpy(&m, hostU32, 4);
         float maxVel = std::max(kFltMin, m);
         float ts     = maxVel > kFltMin ? prm.cfl_factor * (g.dx / maxVel) : 3.40282347e+38f;
-        return std::min(std::max(ts, prm.min_timestep), prm.max_timestep);
+        co_return std::min(std::max(ts, prm.min_timestep), prm.max_timestep);
     }
 
-    bool advanceFrame() {  // PIC_Solver.cpp:267-311
+    Co<bool> advanceFrame() {  // PIC_Solver.cpp:267-311
         const float fd = prm.frame_duration;
         frameTime      = 0;
         uint32_t count = 0;
         bool     ok    = true;
         float    last  = 0;
         while(frameTime < fd) {
-            float sub = timestepCFL();
+            float sub = co_await timestepCFL();
             float rem = fd - frameTime;
             if(frameTime + sub >= fd) sub = rem;
             else if(frameTime + 1.5f * sub >= fd) sub = rem * 0.5f;
-            ok &= step(sub);
+            ok &= co_await step(sub);
             last = sub;
             frameTime += sub;
             ++count;
@@ -1240,7 +1270,7 @@ public:
         st.finished_frame = finishedFrame;
         st.substep_count = count;
         st.last_substep  = last;
-        return ok;
+        co_return ok;
     }
 
     // PIC_Solver.cpp:315-328: z-sort at frame 0 and then every SortFrequency frames, if enabled
@@ -1274,10 +1304,9 @@ public:
         uint32_t*      hi[2]  = {alt->py.as<uint32_t>(), alt->vy.as<uint32_t>()};
         uint32_t*      val[2] = {alt->pz.as<uint32_t>(), alt->vz.as<uint32_t>()};
         const float    inv    = (float)(1.0 / (double)prm.cell_size);  // NeighborSearch.cpp:33
-        hipLaunchKernelGGL(k_zsort_keys, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->view(), n, inv, lo[0], hi[0], val[0]);
-        hipLaunchKernelGGL(k_zsort_acc_init, dim3(1), dim3(64), 0, stream, sc.acc);
-        hipLaunchKernelGGL(k_zsort_andor, dim3(std::min<unsigned>(blocks(np), 1024)), dim3(kBlock), 0, stream,
-                           (const uint32_t*)lo[0], (const uint32_t*)hi[0], n, sc.acc);
+        launch(k_zsort_keys{cur->view(), n, inv, lo[0], hi[0], val[0]}, blocks(np));
+        launch(k_zsort_acc_init{sc.acc}, 1);
+        launch(k_zsort_andor{(const uint32_t*)lo[0], (const uint32_t*)hi[0], n, sc.acc}, std::min<unsigned>(blocks(np), 1024));
         HIP_OK(hipMemcpyAsync(hostU32, sc.acc, 16, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
         uint64_t acc[2];
@@ -1289,16 +1318,14 @@ public:
         for(int d = 0; d < 8; ++d) {
             if(((varying >> (8 * d)) & 0xFFull) == 0) continue;  // every key has the same digit here
             const int word = d >> 2, shift = (d & 3) * 8;
-            hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kBlock), 0, stream, (const uint32_t*)(word ? hi[s] : lo[s]), n, shift,
-                               ntiles, hist);
+            launch(k_rs_hist{(const uint32_t*)(word ? hi[s] : lo[s]), n, shift, ntiles, hist}, ntiles);
             scan(hist, offs, nh, nullptr, sc.part);
-            hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kBlock), 0, stream, (const uint32_t*)lo[s], (const uint32_t*)hi[s],
-                               (const uint32_t*)val[s], n, word, shift, ntiles, (const uint32_t*)offs, lo[1 - s], hi[1 - s], val[1 - s]);
+            launch(k_rs_scatter{(const uint32_t*)lo[s], (const uint32_t*)hi[s], (const uint32_t*)val[s], n, word, shift, ntiles, (const uint32_t*)offs, lo[1 - s], hi[1 - s], val[1 - s]}, ntiles);
             s = 1 - s;
         }
         uint32_t* rank = alt->gx.as<uint32_t>();
-        hipLaunchKernelGGL(k_zsort_rank, dim3(blocks(np)), dim3(kBlock), 0, stream, (const uint32_t*)val[s], n, rank);
-        hipLaunchKernelGGL(k_zsort_rename, dim3(blocks(np)), dim3(kBlock), 0, stream, cur->id.as<uint32_t>(), n, (const uint32_t*)rank);
+        launch(k_zsort_rank{(const uint32_t*)val[s], n, rank}, blocks(np));
+        launch(k_zsort_rename{cur->id.as<uint32_t>(), n, (const uint32_t*)rank}, blocks(np));
         if(objIndex.size() == np && rankHost.size() == np && objScratch.size() == np) {
             // the user order changed: rank[old user index] = new user index
             HIP_OK(hipMemcpyAsync(rankHost.data(), rank, np * 4, hipMemcpyDeviceToHost, stream));
@@ -1344,8 +1371,7 @@ public:
         } else if(prm.preconditioner == BNN_PRECOND_MG) {
             vcycle(nullptr);  // writes z (with a single level, the coarsest solve alone does)
         } else {
-            hipLaunchKernelGGL(k_mul_arrays, dim3(blocks(rv.size())), dim3(kBlock), 0, stream, (const float*)invd.as<float>(),
-                               (const float*)r.as<float>(), z.as<float>(), rv.size());
+            launch(k_mul_arrays{(const float*)invd.as<float>(), (const float*)r.as<float>(), z.as<float>(), rv.size()}, blocks(rv.size()));
         }
         HIP_OK(hipMemcpyAsync(zv.data(), z.p, zv.size() * 4, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
@@ -1788,27 +1814,28 @@ bnn_status bnn_flip_get_particles(bnn_flip* h, float* pos, float* vel) {
 static bnn_status solve_status(const bnn::FlipSolver& s, bool ok) {
     return (!ok && s.prm.exit_if_pressure_projection_failed) ? BNN_ERR_NOT_CONVERGED : BNN_OK;
 }
+static bnn_status bnn_solve_status_of(const bnn::FlipSolver& s, bool ok) { return solve_status(s, ok); }
 
 bnn_status bnn_flip_step(bnn_flip* h, float dt) {
-    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, s.step(dt)); });
+    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, run_now(s.step(dt))); });
 }
 
 bnn_status bnn_flip_advance_frame(bnn_flip* h) {
-    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, s.advanceFrame()); });
+    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, run_now(s.advanceFrame())); });
 }
 
 bnn_status bnn_flip_do_frame_simulation(bnn_flip* h, uint32_t frame) {
     (void)frame;
     return guard(h, [&](bnn::FlipSolver& s) {
         s.sortParticles();
-        return solve_status(s, s.advanceFrame());
+        return solve_status(s, run_now(s.advanceFrame()));
     });
 }
 
 bnn_status bnn_flip_timestep_cfl(bnn_flip* h, float* dt) {
     if(!dt) return BNN_ERR_INVALID_ARG;
     return guard(h, [&](bnn::FlipSolver& s) {
-        *dt = s.timestepCFL();
+        *dt = run_now(s.timestepCFL());
         return BNN_OK;
     });
 }
@@ -1871,8 +1898,8 @@ bnn_status bnn_flip_sort_particles(bnn_flip* h) {
 bnn_status bnn_flip_advance_velocity(bnn_flip* h, float dt) {
     return guard(h, [&](bnn::FlipSolver& s) {
         s.beginStats();
-        bool ok = s.advanceVelocity(dt);
-        s.endStats(dt);
+        bool ok = run_now(s.advanceVelocity(dt));
+        run_now(s.endStats(dt));
         return solve_status(s, ok);
     });
 }
@@ -1906,7 +1933,7 @@ bnn_status bnn_flip_constrain_grid_velocity(bnn_flip* h) {
 }
 
 bnn_status bnn_flip_pressure_projection(bnn_flip* h, float dt) {
-    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, s.pressureProjection(dt)); });
+    return guard(h, [&](bnn::FlipSolver& s) { return solve_status(s, run_now(s.pressureProjection(dt))); });
 }
 
 bnn_status bnn_flip_compute_fluid_sdf(bnn_flip* h) {
@@ -1919,7 +1946,7 @@ bnn_status bnn_flip_compute_fluid_sdf(bnn_flip* h) {
 bnn_status bnn_flip_compute_cell_weights(bnn_flip* h) {
     return guard(h, [&](bnn::FlipSolver& s) {
         s.weightsDirty = true;
-        s.ensureWeights();
+        run_now(s.ensureWeights());
         return BNN_OK;
     });
 }
@@ -2102,10 +2129,12 @@ bnn_status bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, ui
 // batches: scenes stepped concurrently by a worker pool, each on its own stream
 //--------------------------------------------------------------------------------------------------
 struct bnn_flip_batch {
-    std::unique_ptr<bnn::WorkerPool> pool;
-    std::string                      err;
-    std::mutex                       errMutex;
-    int32_t                          timing = -1;  // < 0: each scene keeps its own setting
+    std::unique_ptr<bnn::BatchRunner> runner;
+    bnn::BatchTuner                   tuner;
+    bool                              pinned = false;  // shape set by the caller: no tuning
+    bnn::BatchRunner::Shape           pinnedShape;
+    std::string                       err;
+    int32_t                           timing = -1;  // < 0: each scene keeps its own setting
 };
 
 struct bnn_server {
@@ -2114,10 +2143,12 @@ struct bnn_server {
 };
 
 namespace {
+// one fused batch call: make(scene, i) gives scene i's task; every scene's status goes to status[i]
 template<class F>
-bnn_status batch_call(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bnn_status* status, F&& f) {
+bnn_status batch_call(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bnn_status* status, F&& make) {
     if(!b || (n && !scenes)) return BNN_ERR_INVALID_ARG;
     b->err.clear();
+    std::vector<bnn::FlipSolver*> sc(n);
     for(uint32_t i = 0; i < n; ++i) {
         if(!scenes[i] || !scenes[i]->s) {
             b->err = "null scene handle";
@@ -2128,22 +2159,59 @@ bnn_status batch_call(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bn
                 b->err = "a scene is listed twice";
                 return BNN_ERR_INVALID_ARG;
             }
-        if(b->timing >= 0) scenes[i]->s->prm.timing = b->timing;
+        sc[i] = scenes[i]->s.get();
+        if(b->timing >= 0) sc[i]->prm.timing = b->timing;
     }
-    std::vector<bnn_status> st(n, BNN_OK);
-    b->pool->run(n, [&](uint32_t i) {
-        st[i] = f(scenes[i], i);
-        if(st[i] != BNN_OK && st[i] != BNN_ERR_NOT_CONVERGED) {
-            std::unique_lock<std::mutex> lk(b->errMutex);
-            if(b->err.empty()) b->err = bnn_flip_last_error(scenes[i]);
+    try {
+        bnn::BatchRunner&             R   = *b->runner;
+        bnn::Recorder&                rec = R.recorder();
+        const uint64_t                sig = bnn::batch_signature(sc);
+        const bool                    tuning = !b->pinned && !b->tuner.fixed(sig);
+        const bnn::BatchRunner::Shape sh     = b->pinned ? b->pinnedShape : b->tuner.next(sig);
+        R.setShape(sh);
+        if(tuning) {
+            rec.timedFns = {bnn::p2g_launch_fn(sh.p2gSeg, false), bnn::p2g_launch_fn(sh.p2gSeg, true)};
+            rec.timedMs  = 0;
         }
-    });
-    bnn_status worst = BNN_OK;
-    for(uint32_t i = 0; i < n; ++i) {
-        if(status) status[i] = st[i];
-        if(st[i] != BNN_OK && (worst == BNN_OK || worst == BNN_ERR_NOT_CONVERGED)) worst = st[i];
+        std::vector<bnn::BatchRunner::Task> tasks(n);
+        std::vector<bnn::BatchRunner::Task*> live;
+        for(uint32_t i = 0; i < n; ++i) {
+            try {
+                R.start(tasks[i], sc[i], make(*sc[i], i));
+                live.push_back(&tasks[i]);
+            } catch(const bnn::Error& e) {
+                tasks[i].done   = true;
+                tasks[i].status = e.code;
+                tasks[i].err    = e.what();
+            }
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        R.runAll(live);
+        const double wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if(tuning) {
+            b->tuner.report(sig, sh, rec.timedMs, wallMs);
+            rec.timedFns.clear();
+        }
+        bnn_status worst = BNN_OK;
+        for(uint32_t i = 0; i < n; ++i) {
+            const bnn::BatchRunner::Task& t = tasks[i];
+            bnn_status                    st = t.status;
+            if(st == BNN_OK) st = bnn_solve_status_of(*sc[i], t.ok);
+            else {
+                scenes[i]->err = t.err;
+                if(b->err.empty()) b->err = t.err;
+            }
+            if(status) status[i] = st;
+            if(st != BNN_OK && (worst == BNN_OK || worst == BNN_ERR_NOT_CONVERGED)) worst = st;
+        }
+        return worst;
+    } catch(const bnn::Error& e) {
+        b->err = e.what();
+        return e.code;
+    } catch(const std::exception& e) {
+        b->err = e.what();
+        return BNN_ERR_DEVICE;
     }
-    return worst;
 }
 }  // namespace
 
@@ -2153,10 +2221,13 @@ bnn_status bnn_flip_batch_create(bnn_flip_batch** out) {
     if(!out) return BNN_ERR_INVALID_ARG;
     *out = nullptr;
     try {
-        auto b  = std::make_unique<bnn_flip_batch>();
-        b->pool = std::make_unique<bnn::WorkerPool>(4);  // one worker per hardware queue of the device
-        *out    = b.release();
+        auto b    = std::make_unique<bnn_flip_batch>();
+        b->runner = std::make_unique<bnn::BatchRunner>();
+        *out      = b.release();
         return BNN_OK;
+    } catch(const bnn::Error& e) {
+        g_createError = e.what();
+        return e.code;
     } catch(const std::exception& e) {
         g_createError = e.what();
         return BNN_ERR_DEVICE;
@@ -2179,17 +2250,52 @@ bnn_status bnn_flip_batch_set_timing(bnn_flip_batch* b, int32_t timing) {
 
 bnn_status bnn_flip_batch_step(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n, bnn_status* status) {
     if(n && !dts) return BNN_ERR_INVALID_ARG;
-    return batch_call(b, scenes, n, status, [&](bnn_flip* h, uint32_t i) { return bnn_flip_step(h, dts[i]); });
+    return batch_call(b, scenes, n, status, [&](bnn::FlipSolver& s, uint32_t i) { return s.step(dts[i]); });
 }
 
 bnn_status bnn_flip_batch_advance_frame(bnn_flip_batch* b, bnn_flip* const* scenes, uint32_t n, bnn_status* status) {
-    return batch_call(b, scenes, n, status, [&](bnn_flip* h, uint32_t) { return bnn_flip_advance_frame(h); });
+    return batch_call(b, scenes, n, status, [&](bnn::FlipSolver& s, uint32_t) { return s.advanceFrame(); });
 }
 
 bnn_status bnn_flip_batch_pressure_projection(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n,
                                               bnn_status* status) {
     if(n && !dts) return BNN_ERR_INVALID_ARG;
-    return batch_call(b, scenes, n, status, [&](bnn_flip* h, uint32_t i) { return bnn_flip_pressure_projection(h, dts[i]); });
+    return batch_call(b, scenes, n, status, [&](bnn::FlipSolver& s, uint32_t i) { return s.pressureProjection(dts[i]); });
+}
+
+bnn_status bnn_flip_batch_set_shape(bnn_flip_batch* b, const bnn_batch_shape* shape) {
+    if(!b) return BNN_ERR_INVALID_ARG;
+    if(!shape) {
+        b->pinned = false;
+        return BNN_OK;
+    }
+    if((shape->p2g_segment != 4 && shape->p2g_segment != 8 && shape->p2g_segment != 16) || shape->pcg_chunk_first < 1 ||
+       shape->pcg_chunk_max < 1 || shape->pcg_chunk_max > 1024 || shape->pcg_chunk_first > 1024)
+        return BNN_ERR_INVALID_ARG;
+    b->pinned                 = true;
+    b->pinnedShape.p2gSeg     = shape->p2g_segment;
+    b->pinnedShape.chunkFirst = shape->pcg_chunk_first;
+    b->pinnedShape.chunkMax   = shape->pcg_chunk_max;
+    b->pinnedShape.graphs     = shape->graphs != 0;
+    return BNN_OK;
+}
+
+bnn_status bnn_flip_batch_get_info(const bnn_flip_batch* b, bnn_batch_info* out) {
+    if(!b || !out) return BNN_ERR_INVALID_ARG;
+    const bnn::Recorder&           rec = const_cast<bnn_flip_batch*>(b)->runner->recorder();
+    const bnn::BatchRunner::Shape& sh  = b->runner->shape();
+    *out                               = bnn_batch_info{};
+    out->shape.p2g_segment             = sh.p2gSeg;
+    out->shape.pcg_chunk_first         = sh.chunkFirst;
+    out->shape.pcg_chunk_max           = sh.chunkMax;
+    out->shape.graphs                  = sh.graphs ? 1 : 0;
+    out->flushes                       = rec.flushes;
+    out->launches                      = rec.kernelLaunches;
+    out->scene_launches                = rec.mergedOps;
+    out->graph_replays                 = rec.graphReplays;
+    out->graph_captures                = rec.graphCaptures;
+    out->mean_scenes_per_flush         = rec.flushes ? (double)rec.laneFlushes / (double)rec.flushes : 0.0;
+    return BNN_OK;
 }
 
 //--------------------------------------------------------------------------------------------------

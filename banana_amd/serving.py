@@ -29,7 +29,8 @@ def footprint(params, n_particles):
 
 
 class Batch:
-    """A worker pool that steps any set of FLIPSolver handles concurrently."""
+    """Steps any set of FLIPSolver handles together: one fused launch per stage over all scenes
+    (bnn_flip_batch_*); each scene's results are those of stepping it alone."""
 
     def __init__(self, timing=-1):
         self.L = _lib.load()
@@ -71,6 +72,27 @@ class Batch:
 
     def pressure_projection(self, solvers, dts):
         return self._call(self.L.bnn_flip_batch_pressure_projection, solvers, dts)
+
+    def set_shape(self, p2g_segment=8, pcg_chunk_first=8, pcg_chunk_max=8, graphs=True):
+        """Pins the launch shape of later calls (none of its settings changes a result)."""
+        sh = _lib.BatchShape(int(p2g_segment), int(pcg_chunk_first), int(pcg_chunk_max), 1 if graphs else 0)
+        st = self.L.bnn_flip_batch_set_shape(self.h, C.byref(sh))
+        if st != _lib.BNN_OK:
+            raise BnnError(st, "invalid batch shape")
+
+    def autotune(self):
+        """Returns later calls to the autotuner."""
+        self.L.bnn_flip_batch_set_shape(self.h, None)
+
+    def info(self):
+        """Launch counters since creation and the shape of the last call."""
+        i = _lib.BatchInfo()
+        st = self.L.bnn_flip_batch_get_info(self.h, C.byref(i))
+        if st != _lib.BNN_OK:
+            raise BnnError(st, "bnn_flip_batch_get_info failed")
+        out = {k: getattr(i, k) for k, _ in _lib.BatchInfo._fields_ if k != "shape"}
+        out["shape"] = {k: getattr(i.shape, k) for k, _ in _lib.BatchShape._fields_}
+        return out
 
 
 class Server:

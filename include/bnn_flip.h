@@ -345,11 +345,15 @@ bnn_status  bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, u
 
 /* --- batching --------------------------------------------------------------------------------
  * Extension with no counterpart in the reference (FLIP_Solver steps one scene per solver object;
- * ParticleSolver.cpp:182-219). For callers that step many independent scenes: a pool of host
- * workers (one per hardware queue) runs the listed scenes concurrently, each on its own handle's
- * HIP stream, so small scenes overlap on the device instead of waiting for each other's launches
- * and readbacks. Each scene's results are bit-identical to stepping it alone. The scenes stay owned
- * by their handles; a handle must not be used elsewhere while a batch call runs on it. */
+ * ParticleSolver.cpp:182-219). For callers that step many independent scenes of any sizes: every
+ * stage kernel of the listed scenes runs as one launch over a flat work list of (scene, block)
+ * items, all on one stream; the host reads device values (brick counts, stop flags) for all scenes
+ * with one copy and one wait; launch sequences that repeat (PCG iterations) are replayed from
+ * hipGraphs. A scene's blocks compute exactly what they compute alone, so each scene's results are
+ * bit-identical to stepping it alone, in any batch. Launch shapes that cannot change results (P2G
+ * column segment, PCG iterations per stop-flag poll, graphs on or off) are chosen by an autotuner
+ * over the first calls of each batch size, or pinned with bnn_flip_batch_set_shape. The scenes
+ * stay owned by their handles; a handle must not be used elsewhere while a batch call runs on it. */
 typedef struct bnn_flip_batch bnn_flip_batch;
 bnn_status  bnn_flip_batch_create(bnn_flip_batch** out);
 bnn_status  bnn_flip_batch_destroy(bnn_flip_batch* b);
@@ -364,6 +368,24 @@ bnn_status  bnn_flip_batch_advance_frame(bnn_flip_batch* b, bnn_flip* const* sce
 /* pressureProjection (PIC_Solver.cpp:533-539) of every scene on its current grid velocities */
 bnn_status  bnn_flip_batch_pressure_projection(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n,
                                                bnn_status* status);
+typedef struct bnn_batch_shape {
+    int32_t  p2g_segment;     /* z nodes per P2G thread: 4, 8 or 16 */
+    uint32_t pcg_chunk_first; /* PCG iterations queued before the first stop-flag poll */
+    uint32_t pcg_chunk_max;   /* ... and at most per poll after it (the chunk doubles up to this) */
+    int32_t  graphs;          /* replay repeated launch sequences from hipGraphs */
+} bnn_batch_shape;
+/* pins the launch shape of later calls (NULL: back to the autotuner) */
+bnn_status bnn_flip_batch_set_shape(bnn_flip_batch* b, const bnn_batch_shape* shape);
+typedef struct bnn_batch_info {
+    bnn_batch_shape shape;                 /* the shape of the last call */
+    uint64_t        flushes;               /* host waits (each reads back every scene's values at once) */
+    uint64_t        launches;              /* kernel launches issued */
+    uint64_t        scene_launches;        /* per-scene launches they stand for */
+    uint64_t        graph_replays, graph_captures;
+    double          mean_scenes_per_flush; /* scenes with work in a flush, on average */
+} bnn_batch_info;
+/* counters since the batch was created */
+bnn_status bnn_flip_batch_get_info(const bnn_flip_batch* b, bnn_batch_info* out);
 
 /* --- serving ---------------------------------------------------------------------------------
  * Extension with no counterpart in the reference (its Simulator drives one solver in a loop,
