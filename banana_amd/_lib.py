@@ -72,7 +72,8 @@ class IoParams(C.Structure):
 
 
 class BatchShape(C.Structure):
-    _fields_ = [("p2g_segment", C.c_int32), ("pcg_chunk_first", C.c_uint32), ("pcg_chunk_max", C.c_uint32), ("graphs", C.c_int32)]
+    _fields_ = [("p2g_segment", C.c_int32), ("pcg_chunk_first", C.c_uint32), ("pcg_chunk_max", C.c_uint32),
+                ("items_per_group", C.c_uint32), ("min_groups", C.c_uint32), ("graphs", C.c_int32)]
 
 
 class BatchInfo(C.Structure):

@@ -106,8 +106,8 @@ template<class T> T run_now(Co<T>&& c) {
 //--------------------------------------------------------------------------------------------------
 class Recorder {
 public:
-    using LaunchFn = void (*)(const void* ks, const uint32_t* first, const uint32_t* count, uint32_t nScenes, uint32_t blocks,
-                              hipStream_t s);
+    using LaunchFn = void (*)(const void* ks, const uint32_t* first, const uint32_t* count, uint32_t nScenes, uint32_t items,
+                              uint32_t per, hipStream_t s);
 
     explicit Recorder(hipStream_t s) : stream(s) {}
     ~Recorder() {
@@ -214,6 +214,13 @@ public:
     }
 
     bool useGraphs = true;
+    // items per workgroup: the launch keeps at least minGroups workgroups, each running at most
+    // maxPer items (1: one workgroup per item)
+    uint32_t maxPer = 16, minGroups = 16384;
+    uint32_t perFor(uint32_t items) const {
+        const uint32_t rows = (items + 7u) / 8u, q = std::max<uint32_t>(minGroups / 8u, 1u);
+        return std::max<uint32_t>(1u, std::min<uint32_t>(maxPer, rows / q));
+    }
     // kernels whose launches are timed with events (the first to the last such launch of a flush,
     // summed into timedMs); flushes with a timed launch are issued directly, never from a graph
     std::vector<LaunchFn> timedFns;
@@ -255,9 +262,12 @@ private:
         uint64_t             lastUse = 0;
     };
 
+    // items = the last scene's end; per = items per workgroup and XCD residue (rows of 8 items)
     template<class K>
-    static void launchMany(const void* ks, const uint32_t* first, const uint32_t* count, uint32_t nScenes, uint32_t blocks, hipStream_t s) {
-        hipLaunchKernelGGL(k_many<K>, dim3(blocks), dim3(K::kThreads), 0, s, static_cast<const K*>(ks), first, count, nScenes);
+    static void launchMany(const void* ks, const uint32_t* first, const uint32_t* count, uint32_t nScenes, uint32_t items, uint32_t per,
+                           hipStream_t s) {
+        const uint32_t rows = (items + 7u) / 8u, groups = 8u * ((rows + per - 1u) / per);
+        hipLaunchKernelGGL(k_many<K>, dim3(groups), dim3(K::kThreads), 0, s, static_cast<const K*>(ks), first, count, nScenes, rows, per);
     }
 
     static void check(hipError_t e) {
@@ -306,7 +316,7 @@ private:
                 while(s1 < members.size()) {
                     const Op&      o   = lanes[members[s1]].ops[head[members[s1]]];
                     const uint64_t pad = (o.nblk + 7u) & ~7u;
-                    if(s1 > s0 && (blk + pad) * o.threads > kMaxItems) break;
+                    if(s1 > s0 && ((blk + pad) * o.threads > kMaxItems || s1 - s0 >= kMaxLaunchScenes)) break;
                     blk += pad;
                     ++s1;
                 }
@@ -392,7 +402,7 @@ private:
                 started = true;
             }
             L.fn(base + L.bodiesOff, reinterpret_cast<const uint32_t*>(base + L.firstOff), reinterpret_cast<const uint32_t*>(base + L.countOff),
-                 L.nScenes, L.blocks, stream);
+                 L.nScenes, L.blocks, perFor(L.blocks), stream);
             if(t) check(hipEventRecord(ev1, stream));
         }
         check(hipGetLastError());

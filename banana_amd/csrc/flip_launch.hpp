@@ -5,8 +5,9 @@
 // (Blk::n), never blockIdx or gridDim. The same functor then runs
 //   - alone:   k_one<K>, one workgroup per scene block (the functor is the kernel argument), or
 //   - batched: k_many<K>, one launch over a flat work list of many scenes. Scene s owns the
-//              workgroups [first[s], first[s] + count[s]) and its functor is ks[s]. first[s] is a
-//              multiple of 8, so a workgroup's index mod 8 (its XCD) is the same in both forms.
+//              items [first[s], first[s] + count[s]) and its functor is ks[s]; a workgroup runs
+//              several items of one XCD residue in turn. first[s] is a multiple of 8, so an item's
+//              index mod 8 (the XCD it runs on) is the same in both forms.
 // A scene's blocks therefore do exactly the same arithmetic in either form, and its results are the
 // same bits alone and in any batch.
 #pragma once
@@ -23,22 +24,42 @@ struct Blk {
 
 template<class K> __global__ void __launch_bounds__(K::kThreads) k_one(K k) { k(Blk{blockIdx.x, gridDim.x}); }
 
+// Scenes per batched launch at most (the scene search below is two rounds of 64 lanes)
+constexpr uint32_t kMaxLaunchScenes = 4096;
+
+// Batched launch. Items are the scenes' blocks laid end to end (scene s: items first[s] ..
+// first[s] + count[s] - 1, first[s] a multiple of 8); workgroup w runs the items 8 r + (w mod 8) for
+// rows r in [(w / 8) per, (w / 8 + 1) per) one after the other, so an item keeps its XCD residue and
+// one scene search serves `per` items.
 template<class K>
 __global__ void __launch_bounds__(K::kThreads) k_many(const K* __restrict__ ks, const uint32_t* __restrict__ first,
-                                                      const uint32_t* __restrict__ count, uint32_t nScenes) {
-    // the scene of this workgroup: the number of scenes s >= 1 with first[s] <= blockIdx.x
+                                                      const uint32_t* __restrict__ count, uint32_t nScenes, uint32_t rows, uint32_t per) {
+    const uint32_t x = blockIdx.x & 7u, r0 = (blockIdx.x >> 3) * per;
+    const uint32_t r1 = min(r0 + per, rows);
+    if(r0 >= r1) return;
+    // the scene of the first item: the last s with first[s] <= i0, found by the first wavefront in
+    // two rounds (64 buckets of ceil(nScenes / 64) scenes, then the scenes of one bucket)
     __shared__ uint32_t s_scene;
-    if(threadIdx.x == 0) s_scene = 0;
+    const uint32_t      i0 = 8u * r0 + x;
+    if(threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x, stride = (nScenes + 63u) / 64u;
+        const uint32_t b0   = lane * stride;
+        const uint64_t m1   = __ballot(b0 < nScenes && first[b0] <= i0);
+        const uint32_t base = ((uint32_t)__popcll(m1) - 1u) * stride;
+        const uint32_t sc   = base + lane;
+        const uint64_t m2   = __ballot(lane < stride && sc < nScenes && first[sc] <= i0);
+        if(lane == 0) s_scene = base + (uint32_t)__popcll(m2) - 1u;
+    }
     __syncthreads();
-    uint32_t c = 0;
-    for(uint32_t i = 1 + threadIdx.x; i < nScenes; i += K::kThreads) c += first[i] <= blockIdx.x ? 1u : 0u;
-    if(c) atomicAdd(&s_scene, c);
-    __syncthreads();
-    const uint32_t s     = __builtin_amdgcn_readfirstlane(s_scene);
-    const uint32_t local = blockIdx.x - first[s];
-    const uint32_t n     = count[s];
-    if(local >= n) return;  // padding up to the next scene's multiple of 8
-    ks[s](Blk{local, n});
+    uint32_t s = __builtin_amdgcn_readfirstlane(s_scene);
+    for(uint32_t r = r0; r < r1; ++r) {
+        const uint32_t i = 8u * r + x;
+        while(s + 1u < nScenes && first[s + 1u] <= i) ++s;
+        s                    = __builtin_amdgcn_readfirstlane(s);
+        const uint32_t local = i - first[s], n = count[s];
+        if(local < n) ks[s](Blk{local, n});  // else: padding up to the next scene's multiple of 8
+        if(r + 1u < r1) __syncthreads();      // the next item may reuse this item's LDS
+    }
 }
 
 // dst[i] = v for i < n words

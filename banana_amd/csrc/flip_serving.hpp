@@ -120,6 +120,8 @@ public:
     struct Shape {
         int      p2gSeg     = 8;
         uint32_t chunkFirst = 8, chunkMax = 8;
+        uint32_t maxPer     = 16;     // work items per workgroup of a batched launch, at most ...
+        uint32_t minGroups  = 16384;  // ... keeping at least this many workgroups
         bool     graphs     = true;
     };
 
@@ -185,6 +187,8 @@ public:
         }
         try {
             rec->useGraphs = cur.graphs;
+            rec->maxPer    = cur.maxPer;
+            rec->minGroups = cur.minGroups;
             rec->flushAndSync();
         } catch(const std::exception& e) {
             tl_batch = prev;
@@ -246,10 +250,11 @@ private:
 };
 
 // The autotuner of a batch: the first batched calls try launch shapes, one per call: P2G segment
-// lengths 4, 8 and 16 (scored by the P2G kernel time, measured with events) together with PCG
-// stop-flag chunks of 4, 8 and 16 iterations (scored by the call's wall time less the P2G time),
-// then graphs off at the best pair. The best shape is kept for every later call with the same
-// signature (scene count and total cells, in powers of two).
+// lengths 8, 4 and 16 (scored by the P2G kernel time, measured with events) together with PCG
+// stop-flag chunks of 8, 4 and 16 iterations and at most 16, 4 and 1 work items per workgroup
+// (scored by the call's wall time less the P2G time), then graphs off at the best shape. The best
+// shape is kept for every later call with the same signature (scene count and total cells, in
+// powers of two).
 class BatchTuner {
 public:
     struct Trial {
@@ -263,8 +268,9 @@ public:
         const size_t k = e.trials.size();
         if(k < 3) {
             BatchRunner::Shape sh;
-            sh.p2gSeg     = k == 0 ? 4 : (k == 1 ? 8 : 16);
-            sh.chunkFirst = sh.chunkMax = k == 0 ? 4u : (k == 1 ? 8u : 16u);
+            sh.p2gSeg     = k == 0 ? 8 : (k == 1 ? 4 : 16);
+            sh.chunkFirst = sh.chunkMax = k == 0 ? 8u : (k == 1 ? 4u : 16u);
+            sh.maxPer     = k == 0 ? 16u : (k == 1 ? 4u : 1u);
             sh.graphs     = true;
             return sh;
         }
@@ -282,7 +288,7 @@ public:
             const Trial&       g = e.trials[3];
             double             graphRest = 1e300;
             for(size_t i = 0; i < 3; ++i)
-                if(e.trials[i].shape.chunkFirst == b.chunkFirst) graphRest = e.trials[i].restMs;
+                if(e.trials[i].shape.chunkFirst == b.chunkFirst && e.trials[i].shape.maxPer == b.maxPer) graphRest = e.trials[i].restMs;
             b.graphs = !(g.restMs < graphRest);
             e.best   = b;
             e.fixed  = true;
@@ -313,6 +319,7 @@ private:
         sh.p2gSeg     = e.trials[bp].shape.p2gSeg;
         sh.chunkFirst = e.trials[bc].shape.chunkFirst;
         sh.chunkMax   = e.trials[bc].shape.chunkMax;
+        sh.maxPer     = e.trials[bc].shape.maxPer;
         sh.graphs     = true;
         return sh;
     }

@@ -2270,12 +2270,15 @@ bnn_status bnn_flip_batch_set_shape(bnn_flip_batch* b, const bnn_batch_shape* sh
         return BNN_OK;
     }
     if((shape->p2g_segment != 4 && shape->p2g_segment != 8 && shape->p2g_segment != 16) || shape->pcg_chunk_first < 1 ||
-       shape->pcg_chunk_max < 1 || shape->pcg_chunk_max > 1024 || shape->pcg_chunk_first > 1024)
+       shape->pcg_chunk_max < 1 || shape->pcg_chunk_max > 1024 || shape->pcg_chunk_first > 1024 || shape->items_per_group < 1 ||
+       shape->items_per_group > 256)
         return BNN_ERR_INVALID_ARG;
     b->pinned                 = true;
     b->pinnedShape.p2gSeg     = shape->p2g_segment;
     b->pinnedShape.chunkFirst = shape->pcg_chunk_first;
     b->pinnedShape.chunkMax   = shape->pcg_chunk_max;
+    b->pinnedShape.maxPer     = shape->items_per_group;
+    b->pinnedShape.minGroups  = shape->min_groups ? shape->min_groups : 16384u;
     b->pinnedShape.graphs     = shape->graphs != 0;
     return BNN_OK;
 }
@@ -2288,6 +2291,8 @@ bnn_status bnn_flip_batch_get_info(const bnn_flip_batch* b, bnn_batch_info* out)
     out->shape.p2g_segment             = sh.p2gSeg;
     out->shape.pcg_chunk_first         = sh.chunkFirst;
     out->shape.pcg_chunk_max           = sh.chunkMax;
+    out->shape.items_per_group         = sh.maxPer;
+    out->shape.min_groups              = sh.minGroups;
     out->shape.graphs                  = sh.graphs ? 1 : 0;
     out->flushes                       = rec.flushes;
     out->launches                      = rec.kernelLaunches;

@@ -73,9 +73,10 @@ class Batch:
     def pressure_projection(self, solvers, dts):
         return self._call(self.L.bnn_flip_batch_pressure_projection, solvers, dts)
 
-    def set_shape(self, p2g_segment=8, pcg_chunk_first=8, pcg_chunk_max=8, graphs=True):
+    def set_shape(self, p2g_segment=8, pcg_chunk_first=8, pcg_chunk_max=8, items_per_group=16, min_groups=0, graphs=True):
         """Pins the launch shape of later calls (none of its settings changes a result)."""
-        sh = _lib.BatchShape(int(p2g_segment), int(pcg_chunk_first), int(pcg_chunk_max), 1 if graphs else 0)
+        sh = _lib.BatchShape(int(p2g_segment), int(pcg_chunk_first), int(pcg_chunk_max), int(items_per_group), int(min_groups),
+                             1 if graphs else 0)
         st = self.L.bnn_flip_batch_set_shape(self.h, C.byref(sh))
         if st != _lib.BNN_OK:
             raise BnnError(st, "invalid batch shape")

@@ -351,9 +351,10 @@ bnn_status  bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, u
  * with one copy and one wait; launch sequences that repeat (PCG iterations) are replayed from
  * hipGraphs. A scene's blocks compute exactly what they compute alone, so each scene's results are
  * bit-identical to stepping it alone, in any batch. Launch shapes that cannot change results (P2G
- * column segment, PCG iterations per stop-flag poll, graphs on or off) are chosen by an autotuner
- * over the first calls of each batch size, or pinned with bnn_flip_batch_set_shape. The scenes
- * stay owned by their handles; a handle must not be used elsewhere while a batch call runs on it. */
+ * column segment, PCG iterations per stop-flag poll, scene blocks per workgroup, graphs on or off)
+ * are chosen by an autotuner over the first calls of each batch size, or pinned with
+ * bnn_flip_batch_set_shape. The scenes stay owned by their handles; a handle must not be used
+ * elsewhere while a batch call runs on it. */
 typedef struct bnn_flip_batch bnn_flip_batch;
 bnn_status  bnn_flip_batch_create(bnn_flip_batch** out);
 bnn_status  bnn_flip_batch_destroy(bnn_flip_batch* b);
@@ -372,6 +373,8 @@ typedef struct bnn_batch_shape {
     int32_t  p2g_segment;     /* z nodes per P2G thread: 4, 8 or 16 */
     uint32_t pcg_chunk_first; /* PCG iterations queued before the first stop-flag poll */
     uint32_t pcg_chunk_max;   /* ... and at most per poll after it (the chunk doubles up to this) */
+    uint32_t items_per_group; /* scene blocks one workgroup of a batched launch runs in turn, at most */
+    uint32_t min_groups;      /* ... while keeping at least this many workgroups (0: 16384) */
     int32_t  graphs;          /* replay repeated launch sequences from hipGraphs */
 } bnn_batch_shape;
 /* pins the launch shape of later calls (NULL: back to the autotuner) */

@@ -68,9 +68,9 @@ def reference(bnn):
     return cs, refs
 
 
-@pytest.mark.parametrize("shape", [dict(p2g_segment=4, pcg_chunk_first=1, pcg_chunk_max=1, graphs=True),
-                                   dict(p2g_segment=8, pcg_chunk_first=4, pcg_chunk_max=64, graphs=False),
-                                   dict(p2g_segment=16, pcg_chunk_first=8, pcg_chunk_max=8, graphs=True)])
+@pytest.mark.parametrize("shape", [dict(p2g_segment=4, pcg_chunk_first=1, pcg_chunk_max=1, items_per_group=1, min_groups=8, graphs=True),
+                                   dict(p2g_segment=8, pcg_chunk_first=4, pcg_chunk_max=64, items_per_group=64, min_groups=8, graphs=False),
+                                   dict(p2g_segment=16, pcg_chunk_first=8, pcg_chunk_max=8, items_per_group=5, min_groups=64, graphs=True)])
 def test_fused_batch_is_bit_identical_to_alone(bnn, reference, shape):
     cs, refs = reference
     mixed = build(bnn, cs)
