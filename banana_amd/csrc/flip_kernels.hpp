@@ -143,20 +143,25 @@ struct k_cell_key {
     GridGeom g;
     uint32_t* __restrict__ key;
     uint32_t* __restrict__ count;
+    uint8_t* __restrict__  occ;  // per 8^3 cell brick: 1 when a particle lies in it (cleared before)
     __device__ void operator()(Blk blk) const {
         const uint32_t p      = blk.id * blockDim.x + threadIdx.x;
         const bool     active = p < n;
-        uint32_t       c      = 0xFFFFFFFFu;
+        uint32_t       c = 0xFFFFFFFFu, brick = 0;
         if(active) {
             const int i = cell_axis(px[p], g.bminx, g.dx, g.nx);
             const int j = cell_axis(py[p], g.bminy, g.dx, g.ny);
             const int k = cell_axis(pz[p], g.bminz, g.dx, g.nz);
             c           = (uint32_t)lin3(i, j, k, g.nx, g.ny);
+            brick       = (uint32_t)(((k >> 3) * g.nby + (j >> 3)) * g.nbx + (i >> 3));
             key[p]      = c;
         }
         int            leader;
         const uint32_t len = key_run(c, active, leader);
-        if(len) atomicAdd(&count[c], len);
+        if(len) {
+            atomicAdd(&count[c], len);
+            occ[brick] = 1;
+        }
     }
 };
 
@@ -827,6 +832,7 @@ struct k_p2g_col {
     const uint32_t* __restrict__ cell_start;
     GridGeom g;
     FaceOut out;
+    const uint8_t* __restrict__ occ;  // particle occupancy of the 8^3 cell bricks (binning)
     __device__ void operator()(Blk blk) const {
         const int      ex = g.nx + 1, ey = g.ny + 1, ez = g.nz + 1;
         const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
@@ -837,7 +843,21 @@ struct k_p2g_col {
         if(t >= cols * nseg) return;
         const int  i = (int)(t % (uint32_t)ex), j = (int)((t / (uint32_t)ex) % (uint32_t)ey), seg = (int)(t / cols);
         const int  k0 = seg * kSeg, k1 = min(k0 + kSeg, ez);  // nodes [k0, k1)
-        // cells read: x in [i-1, i+1], y in [j-1, j+1], z in [k0-1, k1]
+        // cells read: x in [i-1, i+1], y in [j-1, j+1], z in [k0-1, k1] (clamped to the grid). With no
+        // particle in any brick covering them every node of the segment has weight 0: u = 0, invalid.
+        {
+            const int x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1), y0 = max(j - 1, 0), y1 = min(j + 1, g.ny - 1);
+            const int zs = max(k0 - 1, 0), ze = min(k1, g.nz - 1);
+            bool      any = false;
+            for(int bz = zs >> 3; bz <= (ze >> 3); ++bz)
+                for(int by = y0 >> 3; by <= (y1 >> 3); ++by)
+                    for(int bx = x0 >> 3; bx <= (x1 >> 3); ++bx) any |= occ[(bz * g.nby + by) * g.nbx + bx] != 0;
+            if(!any) {
+                const P2gAcc zero{};
+                for(int k = k0; k < k1; ++k) p2g_emit(zero, i, j, k, j < g.ny, i < g.nx, i < g.nx && j < g.ny, g, out);
+                return;
+            }
+        }
         const bool interior = i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k0 >= 2 && k1 <= g.nz - 2;
         if(interior) p2g_column<true, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
         else p2g_column<false, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);

@@ -30,7 +30,7 @@ static size_t scene_footprint(const bnn_flip_params& prm, uint64_t n) {
     const size_t   C = nx * ny * nz, N = (nx + 1) * (ny + 1) * (nz + 1);
     const size_t   nU = (nx + 1) * ny * nz, nV = nx * (ny + 1) * nz, nW = nx * ny * (nz + 1), F = nU + nV + nW;
     size_t         nbx = (nx + 7) / 8, nby = (ny + 7) / 8, nbz = (nz + 7) / 8, nb = nbx * nby * nbz;
-    size_t         b = 16 * F + 2 * F + 12 * C + 4 * N + 4 + 4 * (blocks(std::max(C, nb), kScanTile) + 1) +
+    size_t         b = 16 * F + 2 * F + 12 * C + 4 * N + 4 + 4 * (blocks(std::max(C, nb), kScanTile) + 1) + nb +
                256 + 12 * nb + 2 * sizeof(PcgState);
     const size_t slotsCap = nb + nb / 4 + 16;
     b += 12 * 4 * 512 * slotsCap + 28 * slotsCap + 2 * 8 * kMaxRedBlocks;

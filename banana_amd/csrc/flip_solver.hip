@@ -267,6 +267,7 @@ public:
     DevBuf u, v, w, ut, vt, wt, uo, vo, wo, wu, wv, ww;
     DevBuf vu, vv, vw, eu, ew;
     DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters, extBricks;
+    DevBuf occ;  // per 8^3 cell brick: holds a particle (binning), so P2G can skip empty neighbourhoods
     bool     weightsDirty    = true;   // boundary SDF changed since the weights were computed
     bool     weightsComputed = false;  // computeCellWeights has run at least once
     bool     bsdfNonzero     = false;  // the boundary SDF has been computed or set
@@ -403,6 +404,7 @@ public:
         scanPart.reserve((blocks(std::max(nC, (size_t)nb), kScanTile) + 1) * 4);
         counters.reserve(256);
         bflag.reserve(nb * 4);
+        occ.reserve((size_t)(nb + 3) / 4 * 4);
         bscan.reserve(nb * 4);
         brickSlot.reserve(nb * 4);
         pst.reserve(sizeof(PcgState));
@@ -644,11 +646,15 @@ public:
     void binParticles() {
         if(!np) {
             fill32(cellStart.p, 0, (nC + 1) * 4);
+            fill32(occ.p, 0, occ.bytes);
             binned = true;
             return;
         }
         const ParticleBufs& P = *cur;
-        launch(k_cell_key{P.px.as<float>(), P.py.as<float>(), P.pz.as<float>(), (uint32_t)np, g, key.as<uint32_t>(), cellCount.as<uint32_t>()}, blocks(np));
+        fill32(occ.p, 0, occ.bytes);
+        launch(k_cell_key{P.px.as<float>(), P.py.as<float>(), P.pz.as<float>(), (uint32_t)np, g, key.as<uint32_t>(), cellCount.as<uint32_t>(),
+                          occ.as<uint8_t>()},
+               blocks(np));
         scan(cellCount.as<uint32_t>(), cellStart.as<uint32_t>(), nC, cellStart.as<uint32_t>() + nC);
         launch(k_scatter{key.as<uint32_t>(), (uint32_t)np, cellStart.as<uint32_t>(), cellCount.as<uint32_t>(), perm.as<uint32_t>()}, blocks(np));
         launch(k_segsort{cellStart.as<uint32_t>(), (uint32_t)nC, perm.as<uint32_t>()}, blocks(nC));
@@ -670,7 +676,8 @@ public:
     // FLIP_Solver.cpp:57-95; the column segment length p2gSeg (4, 8 or 16) does not change the result
     template<bool kAffine, int kSeg> void p2gLaunch(const FaceOut& out) {
         const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (((size_t)g.nz + 1 + kSeg - 1) / kSeg);  // columns x segments
-        launch(k_p2g_col<kAffine, kSeg>{cur->view(), kAffine ? cur->affine() : Affine{}, (const uint32_t*)cellStart.as<uint32_t>(), g, out},
+        launch(k_p2g_col<kAffine, kSeg>{cur->view(), kAffine ? cur->affine() : Affine{}, (const uint32_t*)cellStart.as<uint32_t>(), g, out,
+                                        (const uint8_t*)occ.as<uint8_t>()},
                blocks(n));
     }
     template<bool kAffine> void p2gSegLaunch(const FaceOut& out) {
