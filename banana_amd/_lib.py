@@ -83,7 +83,8 @@ class BatchInfo(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("workers", C.c_uint32),
-                ("batching", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("batching", C.c_int32), ("barrier_rounds", C.c_int32), ("max_batch_mcells", C.c_uint32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class RequestResult(C.Structure):

@@ -108,6 +108,8 @@ class Recorder {
 public:
     using LaunchFn = void (*)(const void* ks, const uint32_t* first, const uint32_t* count, uint32_t nScenes, uint32_t items,
                               uint32_t per, hipStream_t s);
+    // one scene's launch of the same kernel: k_one with the functor (host bytes) as its argument
+    using LaunchOneFn = void (*)(const void* k, uint32_t blocks, hipStream_t s);
 
     explicit Recorder(hipStream_t s) : stream(s) {}
     ~Recorder() {
@@ -144,7 +146,7 @@ public:
         const size_t off = (L.bytes.size() + alignof(K) - 1) / alignof(K) * alignof(K);
         L.bytes.resize(off + sizeof(K));
         std::memcpy(L.bytes.data() + off, &k, sizeof(K));
-        L.ops.push_back(Op{&launchMany<K>, (uint32_t)sizeof(K), (uint32_t)alignof(K), (uint32_t)K::kThreads, nblk, off});
+        L.ops.push_back(Op{&launchMany<K>, &launchOne<K>, (uint32_t)sizeof(K), (uint32_t)alignof(K), (uint32_t)K::kThreads, nblk, off});
     }
     void readback(uint32_t lane, void* host, const void* dev, size_t bytes) {
         if(bytes % 4 || bytes > 256) throw std::logic_error("readbacks are whole words, at most 64");
@@ -235,9 +237,10 @@ private:
     static constexpr uint64_t kMaxItems       = 1ull << 31;  // work-items per launch
 
     struct Op {
-        LaunchFn fn;
-        uint32_t size, align, threads, nblk;
-        size_t   off;
+        LaunchFn    fn;
+        LaunchOneFn one;
+        uint32_t    size, align, threads, nblk;
+        size_t      off;
     };
     struct Rb {
         void*       host;
@@ -250,9 +253,10 @@ private:
         std::vector<Rb>      rbs;
     };
     struct Launch {
-        LaunchFn fn;
-        size_t   bodiesOff, firstOff, countOff;
-        uint32_t nScenes, blocks;
+        LaunchFn    fn;
+        LaunchOneFn one;  // used when the launch holds one scene (its functor at bodiesOff)
+        size_t      bodiesOff, firstOff, countOff;
+        uint32_t    nScenes, blocks;
     };
     struct Graph {
         std::vector<uint8_t> blob;
@@ -263,6 +267,9 @@ private:
     };
 
     // items = the last scene's end; per = items per workgroup and XCD residue (rows of 8 items)
+    template<class K> static void launchOne(const void* k, uint32_t blocks, hipStream_t s) {
+        hipLaunchKernelGGL(k_one<K>, dim3(blocks), dim3(K::kThreads), 0, s, *static_cast<const K*>(k));
+    }
     template<class K>
     static void launchMany(const void* ks, const uint32_t* first, const uint32_t* count, uint32_t nScenes, uint32_t items, uint32_t per,
                            hipStream_t s) {
@@ -321,7 +328,7 @@ private:
                     ++s1;
                 }
                 const uint32_t m = (uint32_t)(s1 - s0);
-                Launch         L{best, put((size_t)o0.size * m, std::max<size_t>(o0.align, 16)), 0, 0, m, 0};
+                Launch         L{best, o0.one, put((size_t)o0.size * m, std::max<size_t>(o0.align, 16)), 0, 0, m, 0};
                 for(uint32_t t = 0; t < m; ++t) {
                     const Lane& ln = lanes[members[s0 + t]];
                     const Op&   o  = ln.ops[head[members[s0 + t]]];
@@ -401,8 +408,11 @@ private:
                 check(hipEventRecord(ev0, stream));
                 started = true;
             }
-            L.fn(base + L.bodiesOff, reinterpret_cast<const uint32_t*>(base + L.firstOff), reinterpret_cast<const uint32_t*>(base + L.countOff),
-                 L.nScenes, L.blocks, perFor(L.blocks), stream);
+            if(L.nScenes == 1)  // the functor travels as the kernel argument (host staging copy)
+                L.one(staging.data() + L.bodiesOff, L.blocks, stream);
+            else
+                L.fn(base + L.bodiesOff, reinterpret_cast<const uint32_t*>(base + L.firstOff),
+                     reinterpret_cast<const uint32_t*>(base + L.countOff), L.nScenes, L.blocks, perFor(L.blocks), stream);
             if(t) check(hipEventRecord(ev1, stream));
         }
         check(hipGetLastError());

@@ -46,9 +46,13 @@ __global__ void __launch_bounds__(K::kThreads) k_many(const K* __restrict__ ks, 
         const uint32_t b0   = lane * stride;
         const uint64_t m1   = __ballot(b0 < nScenes && first[b0] <= i0);
         const uint32_t base = ((uint32_t)__popcll(m1) - 1u) * stride;
-        const uint32_t sc   = base + lane;
-        const uint64_t m2   = __ballot(lane < stride && sc < nScenes && first[sc] <= i0);
-        if(lane == 0) s_scene = base + (uint32_t)__popcll(m2) - 1u;
+        if(stride == 1u) {  // at most 64 scenes: the first round found it
+            if(lane == 0) s_scene = base;
+        } else {
+            const uint32_t sc = base + lane;
+            const uint64_t m2 = __ballot(lane < stride && sc < nScenes && first[sc] <= i0);
+            if(lane == 0) s_scene = base + (uint32_t)__popcll(m2) - 1u;
+        }
     }
     __syncthreads();
     uint32_t s = __builtin_amdgcn_readfirstlane(s_scene);

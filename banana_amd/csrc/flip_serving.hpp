@@ -109,6 +109,7 @@ public:
         std::optional<Co<bool>> co;
         std::coroutine_handle<> next;
         uint32_t                lane        = 0;
+        uint32_t                barrierWait = 0;  // rounds spent waiting at the current barrier
         int32_t                 savedTiming = 0;
         bool                    done        = false;
         bool                    ok          = false;
@@ -137,6 +138,7 @@ public:
     BatchRunner(const BatchRunner&)            = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
 
+    uint32_t     barrierRounds = 0xFFFFFFFFu;  // rounds a scene may wait at a barrier
     Recorder&    recorder() { return *rec; }
     const Shape& shape() const { return cur; }
     void         setShape(const Shape& sh) { cur = sh; }
@@ -157,8 +159,9 @@ public:
         s->pcgChunkFirst = cur.chunkFirst;
         s->pcgChunkMax   = cur.chunkMax;
         t.co.emplace(std::move(co));
-        t.next = t.co->h;
-        t.done = false;
+        t.next        = t.co->h;
+        t.done        = false;
+        t.barrierWait = 0;
     }
 
     // resumes every task waiting at a sync point (tasks started since the last round run to their
@@ -167,15 +170,18 @@ public:
     void round(const std::vector<Task*>& tasks) {
         Recorder* const prev = tl_batch;
         tl_batch             = rec.get();
-        // scenes at a barrier wait while any other scene can still advance
+        // scenes at a barrier wait while any other scene can still advance (at most barrierRounds
+        // rounds: requests that keep arriving must not hold them back for ever)
         bool others = false;
         for(Task* t : tasks) others |= !t->done && t->next && !t->s->atBarrier;
         for(Task* t : tasks) {
-            if(t->done || !t->next || (others && t->s->atBarrier)) continue;
+            if(t->done || !t->next) continue;
+            if(others && t->s->atBarrier && ++t->barrierWait <= barrierRounds) continue;
             const std::coroutine_handle<> h = t->next;
             t->next                         = {};
             t->s->pending                   = {};
             t->s->atBarrier                 = false;
+            t->barrierWait                  = 0;
             h.resume();
             if(!t->co->done()) {
                 t->next = t->s->pending;
@@ -350,14 +356,24 @@ static inline uint64_t batch_signature(const std::vector<FlipSolver*>& sc) {
 //==================================================================================================
 // request server
 //==================================================================================================
+// With batching (the default) one engine thread runs every admitted request as a task of a fused
+// batch (BatchRunner): each round it admits the ready requests (up to max_batch in flight), resumes
+// the tasks to their next sync point and issues all their launches as merged launches, so requests
+// that arrive while others run join at the next sync point. Worker threads only create scenes.
+// Without batching, one worker runs one request at a time through the scene's own stream.
 class Server {
 public:
     explicit Server(const bnn_server_config& c) : cfg(c) {
         if(!cfg.hbm_budget_bytes) cfg.hbm_budget_bytes = 200000000000ull;
         if(!cfg.workers) cfg.workers = 4;
         if(!cfg.batching) cfg.workers = 1;
+        if(!cfg.max_batch) cfg.max_batch = 1024;
         if(cfg.workers > 16) throw Error(BNN_ERR_INVALID_ARG, "at most 16 server workers");
         t0 = std::chrono::steady_clock::now();
+        if(cfg.batching) {
+            HIP_OK(hipGetDevice(&device));
+            engine = std::thread([this] { engineLoop(); });
+        }
         for(uint32_t w = 0; w < cfg.workers; ++w) threads.emplace_back([this] { workerLoop(); });
     }
 
@@ -368,6 +384,7 @@ public:
         }
         cv.notify_all();
         for(auto& t : threads) t.join();
+        if(engine.joinable()) engine.join();
         scenes.clear();
     }
 
@@ -513,7 +530,7 @@ public:
         o.latency_mean_ms = lat.empty() ? 0 : sum / lat.size() * 1e3;
         o.latency_max_ms  = mx * 1e3;
         o.throughput_rps  = (firstSubmit >= 0 && lastDone > firstSubmit) ? (double)lat.size() / (lastDone - firstSubmit) : 0;
-        o.mean_batch      = (double)cfg.workers;
+        o.mean_batch      = cfg.batching ? (roundsSince ? (double)lanesSince / (double)roundsSince : 0.0) : 1.0;
         o.busy_s          = busy;
         return o;
     }
@@ -524,6 +541,7 @@ public:
         latencies.clear();
         firstSubmit = lastDone = -1;
         busy                   = 0;
+        lanesSince = roundsSince = 0;
     }
 
     std::string lastError() const {
@@ -551,6 +569,9 @@ private:
     };
 
     bnn_server_config                           cfg;
+    int                                         device = 0;
+    std::thread                                 engine;
+    uint64_t                                    lanesSince = 0, roundsSince = 0;  // requests in flight summed over engine rounds
     std::chrono::steady_clock::time_point       t0;
     mutable std::mutex                          m, errMutex;
     std::condition_variable                     cv, doneCv;
@@ -659,7 +680,8 @@ private:
             {
                 std::unique_lock<std::mutex> lk(m);
                 for(;;) {
-                    if(admissibleLocked() || runnableLocked()) break;
+                    if(admissibleLocked() || (!cfg.batching && runnableLocked())) break;
+                    if(stopping && cfg.batching) return;  // the engine fails what cannot run
                     if(stopping) {
                         if(creating || running) {  // another worker may still unblock queued requests
                             cv.wait(lk);
@@ -681,11 +703,170 @@ private:
                 }
                 toCreate = admitLocked();
                 if(toCreate) ++creating;
-                else if((haveReq = pickLocked(rid))) ++running;
+                else if(!cfg.batching && (haveReq = pickLocked(rid))) ++running;
             }
             if(toCreate) create(*toCreate);
             if(haveReq) run(rid);
         }
+    }
+
+    // one request in flight on the engine
+    struct Flight {
+        uint64_t                    rid = 0, sid = 0;
+        FlipSolver*                 sv  = nullptr;
+        int                         kind = 0;
+        float                       dt   = 0;
+        double                      t1   = 0;
+        BatchRunner::Task           task;
+    };
+
+    void engineLoop() {
+        std::string                          msg;
+        std::vector<std::unique_ptr<Flight>> fl;
+        try {
+            HIP_OK(hipSetDevice(device));
+            BatchRunner R;
+            R.barrierRounds = (uint32_t)std::max(cfg.barrier_rounds, 0);
+            BatchRunner::Shape sh;  // fixed shape: requests come and go, so no per-size tuning
+            sh.chunkFirst = 4;
+            sh.chunkMax   = 16;
+            sh.maxPer     = 4;
+            R.setShape(sh);
+            std::vector<BatchRunner::Task*> ptrs;
+            for(;;) {
+                std::vector<Flight*> fresh;
+                {
+                    std::unique_lock<std::mutex> lk(m);
+                    for(;;) {
+                        uint64_t rid = 0;
+                        while(fl.size() < cfg.max_batch && (fl.empty() || !cfg.max_batch_mcells || cellsInFlight(fl) < 1e6 * cfg.max_batch_mcells) &&
+                              pickLocked(rid)) {
+                            auto f  = std::make_unique<Flight>();
+                            Req& r  = reqs[rid];
+                            f->rid  = rid;
+                            f->sid  = r.res.scene_id;
+                            f->sv   = findLocked(f->sid)->solver.get();  // busy scenes are never erased
+                            f->kind = r.res.kind;
+                            f->dt   = r.dt;
+                            ++running;
+                            fresh.push_back(f.get());
+                            fl.push_back(std::move(f));
+                        }
+                        if(!fl.empty()) break;
+                        if(stopping && creating == 0 && !admissibleLocked()) {
+                            for(uint64_t id : queue) {  // nothing can run them any more
+                                Req& r       = reqs[id];
+                                r.res.status = BNN_ERR_STATE;
+                                r.res.t_done = now();
+                                r.done       = true;
+                                ++st.failed;
+                            }
+                            queue.clear();
+                            doneCv.notify_all();
+                            return;
+                        }
+                        cv.wait(lk);
+                    }
+                    const double tn = now();
+                    for(Flight* f : fresh) {
+                        Req& r           = reqs[f->rid];
+                        r.res.t_start    = tn;
+                        r.res.batch_size = (uint32_t)fl.size();
+                        f->t1            = tn;
+                    }
+                }
+                for(Flight* f : fresh) {
+                    try {
+                        R.start(f->task, f->sv, f->kind == BNN_REQ_STEP ? f->sv->step(f->dt) : f->sv->projectionRequest(f->dt));
+                    } catch(const Error& e) {
+                        f->task.done   = true;
+                        f->task.status = e.code;
+                        f->task.err    = e.what();
+                    }
+                }
+                ptrs.clear();
+                for(auto& f : fl) ptrs.push_back(&f->task);
+                R.round(ptrs);
+                std::vector<std::unique_ptr<Flight>> keep, ended;
+                for(auto& f : fl) (f->task.done ? ended : keep).push_back(std::move(f));
+                fl.swap(keep);
+                {
+                    std::unique_lock<std::mutex> lk(m);
+                    lanesSince += ptrs.size();
+                    roundsSince += 1;
+                }
+                for(auto& f : ended) finish(*f);
+            }
+        } catch(const std::exception& e) {
+            msg = e.what();
+        }
+        setError("serving engine stopped: " + msg);
+        for(auto& f : fl) {  // requests in flight when the engine failed
+            f->sv->rec       = nullptr;
+            f->sv->pending   = {};
+            f->sv->atBarrier = false;
+            f->task.done   = true;
+            f->task.status = BNN_ERR_DEVICE;
+            f->task.err    = msg;
+            finish(*f);
+        }
+        std::unique_lock<std::mutex> lk(m);
+        for(uint64_t id : queue) {
+            Req& r       = reqs[id];
+            r.res.status = BNN_ERR_DEVICE;
+            r.res.t_done = now();
+            r.done       = true;
+            ++st.failed;
+        }
+        queue.clear();
+        doneCv.notify_all();
+    }
+
+    static double cellsInFlight(const std::vector<std::unique_ptr<Flight>>& fl) {
+        double c = 0;
+        for(const auto& f : fl) c += (double)f->sv->nC;
+        return c;
+    }
+
+    // records a finished engine request and frees its scene
+    void finish(Flight& f) {
+        const BatchRunner::Task& t    = f.task;
+        bnn_status               code = t.status;
+        if(code != BNN_OK) setError(t.err);
+        std::unique_ptr<Served> dead;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            const double t2 = now();
+            busy += t2 - f.t1;
+            Req& r       = reqs[f.rid];
+            r.res.t_done = t2;
+            r.res.status = code;
+            if(code == BNN_OK) {
+                r.res.pcg_iterations = f.sv->st.pcg_iterations;
+                r.res.pcg_success    = f.sv->st.pcg_success;
+                r.res.pcg_residual   = f.sv->st.pcg_residual;
+                if(!t.ok && f.sv->prm.exit_if_pressure_projection_failed) r.res.status = BNN_ERR_NOT_CONVERGED;
+            }
+            r.done = true;
+            if(r.res.status == BNN_OK) {
+                ++st.completed;
+                latencies.push_back(r.res.t_done - r.res.t_submit);
+                lastDone = std::max(lastDone, r.res.t_done);
+            } else {
+                ++st.failed;
+            }
+            ++st.batches;
+            auto it = scenes.find(f.sid);
+            if(it != scenes.end()) {
+                Served& sc = *it->second;
+                sc.busy    = false;
+                sc.queued -= 1;
+                if(sc.release && sc.queued == 0) dead = takeLocked(it);
+            }
+            --running;
+        }
+        cv.notify_all();
+        doneCv.notify_all();
     }
 
     void create(Served& s) {
@@ -749,13 +930,7 @@ private:
         bnn_status   code = BNN_OK;
         bool         ok   = true;
         try {
-            if(kind == BNN_REQ_STEP) {
-                ok = run_now(sv->step(dt));
-            } else {
-                sv->beginStats();
-                ok = run_now(sv->pressureProjection(dt));
-                run_now(sv->endStats(dt));
-            }
+            ok = run_now(kind == BNN_REQ_STEP ? sv->step(dt) : sv->projectionRequest(dt));
         } catch(const Error& e) {
             code = e.code;
             setError(e.what());

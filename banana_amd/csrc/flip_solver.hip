@@ -1222,6 +1222,15 @@ public:
         st.last_substep      = dt;
     }
 
+    // a served pressure-solve request: pressureProjection on the current grid velocities, with stats
+    Co<bool> projectionRequest(float dt) {
+        if(!(dt > 0)) throw Error(BNN_ERR_INVALID_ARG, "dt must be > 0");
+        beginStats();
+        bool ok = co_await pressureProjection(dt);
+        co_await endStats(dt);
+        co_return ok;
+    }
+
     Co<bool> step(float dt) {  // PIC_Solver.cpp:292-294 (moveParticles, advanceVelocity)
         if(!(dt > 0)) throw Error(BNN_ERR_INVALID_ARG, "dt must be > 0");
         beginStats();

@@ -99,13 +99,15 @@ class Batch:
 class Server:
     """Request loop over resident scenes (bnn_server_*)."""
 
-    def __init__(self, hbm_budget_bytes=0, max_batch=0, workers=0, batching=True):
+    def __init__(self, hbm_budget_bytes=0, max_batch=0, workers=0, batching=True, barrier_rounds=0, max_batch_mcells=0):
         self.L = _lib.load()
         cfg = ServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_batch = int(max_batch)
         cfg.workers = int(workers)
         cfg.batching = 1 if batching else 0
+        cfg.barrier_rounds = int(barrier_rounds)
+        cfg.max_batch_mcells = int(max_batch_mcells)
         h = C.c_void_p()
         st = self.L.bnn_server_create(C.byref(cfg), C.byref(h))
         if st != _lib.BNN_OK:

@@ -23,9 +23,12 @@ Two more legs ride on the same line:
               blocks, one host wait per sync point for all scenes, repeated PCG launch sequences
               replayed from hipGraphs, launch shape chosen by the autotuner during warm-up) against
               the plain path (each scene stepped through its own handle, one after the other).
-  served   -- Poisson arrivals (200/s) of step or pressure-solve requests (50/50) on a resident pool
-              of scenes with edge length log-uniform 32..192, HBM admission cap 200 GB: completed
-              requests/s and p50/p99 latency, with four workers and with one.
+  served   -- Poisson arrivals (200/s for 120 s) of step or pressure-solve requests (50/50) on a
+              resident pool of scenes with edge length log-uniform 32..192, HBM admission cap
+              200 GB: completed requests/s, p50/p99 latency and the measured mean number of requests
+              in flight, served with continuous batching (every request in flight joins the fused
+              launches) and one request at a time; then doubled rates until the server falls behind
+              (its saturation throughput).
 """
 import argparse
 import json
@@ -222,7 +225,9 @@ def batched_leg(args, log):
 
 
 def served_leg(args, log, batching):
-    """Poisson request stream against a resident pool; returns the server's own accounting."""
+    """Poisson request streams against a resident pool, with the server's own accounting: the
+    configured rate for serve_seconds, then higher rates for sat_seconds each until the server no
+    longer keeps up (completions below 90 % of arrivals) -- its saturation throughput."""
     import numpy as np
 
     from banana_amd import scenes
@@ -230,7 +235,8 @@ def served_leg(args, log, batching):
 
     rng = np.random.default_rng(11)
     dt = np.float32(1.0 / 600.0)
-    srv = Server(hbm_budget_bytes=200e9, workers=4 if batching else 1, batching=batching)
+    srv = Server(hbm_budget_bytes=200e9, workers=4, batching=batching, barrier_rounds=args.barrier_rounds,
+                 max_batch_mcells=args.batch_mcells)
     t0 = time.perf_counter()
     ids = []
     for i in range(args.serve_pool):
@@ -240,31 +246,47 @@ def served_leg(args, log, batching):
     warm = [srv.submit(i, 0, dt) for i in ids]  # every scene resident and stepped once
     for r in warm:
         srv.wait(r)
-    srv.reset_stats()
-    log(f"served ({'batched' if batching else 'one at a time'}): pool of {len(ids)} resident in "
-        f"{time.perf_counter() - t0:.1f}s, {srv.stats()['resident_bytes'] / 1e9:.1f} GB")
+    mode = "batched" if batching else "one at a time"
+    log(f"served ({mode}): pool of {len(ids)} resident in {time.perf_counter() - t0:.1f}s, "
+        f"{srv.stats()['resident_bytes'] / 1e9:.1f} GB")
     arr = np.random.default_rng(13)
-    t_next = srv.now()
-    end = t_next + args.serve_seconds
-    n = 0
-    while True:
-        t_next += arr.exponential(1.0 / args.serve_rate)
-        if t_next > end:
+
+    def stream(rate, seconds):
+        srv.reset_stats()
+        t_next = srv.now()
+        end = t_next + seconds
+        n = 0
+        while True:
+            t_next += arr.exponential(1.0 / rate)
+            if t_next > end:
+                break
+            wait = t_next - srv.now()
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(ids[int(arr.integers(len(ids)))], int(arr.integers(2)), dt)
+            n += 1
+        srv.drain(600.0)
+        st = srv.stats()
+        out = {"arrival_rate_per_s": rate, "seconds": seconds, "requests": n, "completed": st["completed"], "failed": st["failed"],
+               "completed_per_s": round(st["throughput_rps"], 2), "latency_p50_ms": round(st["latency_p50_ms"], 2),
+               "latency_p99_ms": round(st["latency_p99_ms"], 2), "latency_mean_ms": round(st["latency_mean_ms"], 2),
+               "mean_batch": round(st["mean_batch"], 2), "peak_resident_gb": round(st["peak_resident_bytes"] / 1e9, 2)}
+        log(f"served ({mode}) at {rate}/s: {out}")
+        return out
+
+    main = stream(args.serve_rate, args.serve_seconds if batching else min(args.serve_seconds, args.plain_serve_seconds))
+    sweep = []
+    rate = args.serve_rate
+    for _ in range(args.sat_steps):
+        rate *= 2
+        r = stream(rate, args.sat_seconds)
+        sweep.append(r)
+        if r["completed_per_s"] < 0.9 * rate:
             break
-        wait = t_next - srv.now()
-        if wait > 0:
-            time.sleep(wait)
-        srv.submit(ids[int(arr.integers(len(ids)))], int(arr.integers(2)), dt)
-        n += 1
-    srv.drain(600.0)
-    st = srv.stats()
     srv.close()
-    out = {"requests": n, "completed": st["completed"], "failed": st["failed"],
-           "completed_per_s": round(st["throughput_rps"], 2), "latency_p50_ms": round(st["latency_p50_ms"], 2),
-           "latency_p99_ms": round(st["latency_p99_ms"], 2), "latency_mean_ms": round(st["latency_mean_ms"], 2),
-           "mean_batch": round(st["mean_batch"], 2), "peak_resident_gb": round(st["peak_resident_bytes"] / 1e9, 2)}
-    log(f"served: {out}")
-    return out
+    main["saturation"] = sweep
+    main["saturation_completed_per_s"] = max([main["completed_per_s"]] + [r["completed_per_s"] for r in sweep])
+    return main
 
 
 def measured_traffic(kernel, grid):
@@ -324,8 +346,13 @@ def main():
     ap.add_argument("--cpu-configs", type=int, default=1, help="also time the 64^3 and 128^3 dam-break CPU samples")
     ap.add_argument("--batch-scenes", type=int, default=512, help="scenes of the batched leg (0: skip the leg)")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the Poisson request stream (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=120.0, help="length of the Poisson request stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="request arrivals per second")
+    ap.add_argument("--plain-serve-seconds", type=float, default=30.0, help="length of the stream served one request at a time")
+    ap.add_argument("--sat-seconds", type=float, default=10.0, help="length of each higher-rate stream of the saturation sweep")
+    ap.add_argument("--sat-steps", type=int, default=4, help="rate doublings of the saturation sweep at most")
+    ap.add_argument("--barrier-rounds", type=int, default=0, help="server rounds a request may wait at a phase boundary")
+    ap.add_argument("--batch-mcells", type=int, default=0, help="server admission cap on grid cells in flight, millions (0: none)")
     ap.add_argument("--serve-pool", type=int, default=48, help="resident scenes the requests address")
     args = ap.parse_args()
 
@@ -458,7 +485,7 @@ def main():
     batched = batched_leg(args, log) if args.batch_scenes > 0 else None
     served = None
     if args.serve_seconds > 0:
-        served = {"arrival_rate_per_s": args.serve_rate, "seconds": args.serve_seconds, "pool": args.serve_pool,
+        served = {"arrival_rate_per_s": args.serve_rate, "pool": args.serve_pool,
                   "batched": served_leg(args, log, True), "plain": served_leg(args, log, False)}
 
     out = {

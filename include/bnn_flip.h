@@ -394,10 +394,12 @@ bnn_status bnn_flip_batch_get_info(const bnn_flip_batch* b, bnn_batch_info* out)
  * Extension with no counterpart in the reference (its Simulator drives one solver in a loop,
  * Simulator.cpp:42-69). A request loop for step and pressure-solve requests on resident scenes:
  * scenes are admitted while the sum of their footprints (bnn_flip_footprint) fits the HBM budget
- * (later scenes wait in arrival order, smaller ones may pass); each worker thread takes the oldest
- * ready request (its scene resident and idle; a scene's requests run in submission order) as soon
- * as it is free, so requests that arrive while others run start at once on their own scenes'
- * streams and overlap on the device. */
+ * (later scenes wait in arrival order, smaller ones may pass). With batching, one engine thread runs
+ * every request in flight as part of one fused batch (see batching above): each round it admits the
+ * oldest ready requests (a scene's requests run one at a time, in submission order), advances every
+ * request to its next device readback with merged launches and one wait, and completes the finished
+ * ones, so a request that arrives while others run joins them at the next round. Each result is the
+ * same bits as the request run alone. */
 typedef struct bnn_server bnn_server;
 typedef enum bnn_request_kind {
     BNN_REQ_STEP     = 0, /* moveParticles + advanceVelocity with the request's dt */
@@ -405,10 +407,17 @@ typedef enum bnn_request_kind {
 } bnn_request_kind;
 typedef struct bnn_server_config {
     uint64_t hbm_budget_bytes; /* admission cap on the summed resident footprint (0: 200e9) */
-    uint32_t max_batch;        /* reserved (0) */
-    uint32_t workers;          /* requests in flight at once (0: 4) */
-    int32_t  batching;         /* 1: concurrent workers (default with a NULL config); 0: one request at a time */
-    int32_t  reserved[5];
+    uint32_t max_batch;        /* batching: requests in flight at once, at most (0: 1024) */
+    uint32_t workers;          /* threads that create admitted scenes (0: 4); without batching 1 */
+    int32_t  batching;         /* 1 (default with a NULL config): continuous batching of all requests in
+                                  flight into fused launches; 0: one request at a time, alone */
+    int32_t  barrier_rounds;   /* batching: engine rounds a request waits at a phase boundary for the
+                                  others to reach theirs, at most (0: none; aligned phases merge
+                                  better, unaligned ones finish sooner) */
+    uint32_t max_batch_mcells; /* batching: admit no further request while the requests in flight hold
+                                  this many million grid cells (0: no limit); the oldest ready
+                                  request is always admitted when none is in flight */
+    int32_t  reserved[3];
 } bnn_server_config;
 typedef struct bnn_request_result {
     uint64_t   request_id, scene_id;
@@ -426,8 +435,8 @@ typedef struct bnn_server_stats {
     uint64_t scenes_resident, scenes_waiting, resident_bytes, peak_resident_bytes;
     double   latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms; /* submit -> done */
     double   throughput_rps; /* completed requests / (last completion - first submission) */
-    double   mean_batch;     /* workers (requests that may run at once) */
-    double   busy_s;         /* summed wall time of requests (all workers) */
+    double   mean_batch;     /* requests in flight per engine round, on average (1 without batching) */
+    double   busy_s;         /* summed wall time of requests, start to done */
 } bnn_server_stats;
 bnn_status  bnn_server_create(const bnn_server_config* cfg, bnn_server** out);
 bnn_status  bnn_server_destroy(bnn_server* s); /* finishes queued work first */
