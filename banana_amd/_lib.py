@@ -40,7 +40,7 @@ class Params(C.Structure):
         ("preconditioner", C.c_int32), ("timing", C.c_int32), ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float),
         ("mg_coarsest_sweeps", C.c_int32), ("start_frame", C.c_uint32), ("final_frame", C.c_uint32),
         ("repulsive_force_stiffness", C.c_float), ("random_seed", C.c_uint64), ("refine_inner_tolerance", C.c_float),
-        ("transfer_scheme", C.c_int32),
+        ("transfer_scheme", C.c_int32), ("warm_start", C.c_int32),
     ]
 
 

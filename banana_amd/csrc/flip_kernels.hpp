@@ -1752,6 +1752,31 @@ struct k_residual64_fin {
     }
 };
 
+// the previous solve's pressure as the initial guess of the next refined solve: cell l of new slot s
+// takes the old value of the same cell when it is fluid now (phi < 0) and its brick had a slot then
+// (prevBrickSlot: the old brick -> slot table), else 0
+struct k_warm_start {
+    static constexpr int kThreads = 512;
+    const int32_t* __restrict__ slotBrick;
+    const int32_t* __restrict__ prevBrickSlot;
+    const double* __restrict__  xprev;
+    const float* __restrict__   phi;
+    GridGeom                    g;
+    uint32_t                    nSlots;
+    double* __restrict__        x;
+    __device__ void operator()(Blk blk) const {
+        const int l = threadIdx.x;
+        for(uint32_t slot = blk.id; slot < nSlots; slot += blk.n) {
+            const int32_t b  = slotBrick[slot];
+            const int32_t os = prevBrickSlot[b];
+            const int     bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+            const int     i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
+            const bool    fluid = i < g.nx && j < g.ny && k < g.nz && phi[lin3(i, j, k, g.nx, g.ny)] < 0.f;
+            x[(size_t)slot * 512 + l] = (fluid && os >= 0) ? xprev[(size_t)os * 512 + l] : 0.0;
+        }
+    }
+};
+
 struct k_accumulate64 {
     static constexpr int kThreads = 512;
     double* __restrict__ x64;

@@ -89,6 +89,10 @@ struct DevBuf {
         bytes = b;
     }
     template<class T> T* as() const { return static_cast<T*>(p); }
+    void swapWith(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 struct ParticleBufs {
@@ -278,6 +282,9 @@ public:
     DevBuf   bflag, bscan, brickSlot, slotBrick, nbrTab;
     DevBuf   x, r, z, s0, s1, q, rhs32, diag32, cx32, cy32, cz32, invd;
     DevBuf   rhs64, diag64, cx64, cy64, cz64, x64;
+    DevBuf   x64prev, prevBrickSlot;  // the last refined solve's pressure and brick -> slot table
+    bool     prevPressure = false;   // they hold a converged pressure of this grid
+    bool     warmNext     = false;   // this solve starts from them (x64 filled by k_warm_start)
     DevBuf   pa, pb, pz, pst, ost;  // PCG partials (per block: pa, pb; per brick: pz), device states
     DevBuf   icInv, icLx, icLy, icLz;  // IC(0) factor (BNN_PRECOND_IC0 only)
     PcgState*   hostSt  = nullptr;  // pinned
@@ -783,6 +790,14 @@ public:
 
     // active bricks + matrix-free system (PIC_Solver.cpp:693-893)
     Co<void> computeSystem(float dt) {
+        // warm start: keep the previous pressure and its brick table before the slots are rebuilt
+        warmNext = prm.warm_start && prm.pressure_mode == BNN_PRESSURE_REFINED && prevPressure;
+        if(warmNext) {
+            prevBrickSlot.reserve((size_t)nb * 4);
+            copy32(prevBrickSlot.p, brickSlot.p, (size_t)nb * 4);
+            x64prev.swapWith(x64);
+        }
+        prevPressure = false;
         fill32(counters.p, 0, 16);
         uint32_t* cnt = counters.as<uint32_t>();
         launch(k_brick_flags{(const float*)phi.as<float>(), g, bflag.as<uint32_t>(), cnt + 1}, nb);
@@ -793,8 +808,12 @@ public:
         nSlots = hostU32[2];
         if(nSlots > nb) throw Error(BNN_ERR_DEVICE, "active brick count exceeds the brick grid");
         ensureSlots(std::max<uint32_t>(nSlots, 1));
+        // the warm start swapped x64 with the previous-pressure buffer, which may be smaller than the
+        // slot capacity (or not yet allocated): ensureSlots only grows buffers when the capacity grows
+        if(prm.pressure_mode == BNN_PRESSURE_REFINED) x64.reserve((size_t)slotCap * 512 * 8);
         launch(k_brick_slots{(const uint32_t*)bflag.as<uint32_t>(), (const uint32_t*)bscan.as<uint32_t>(), nb, brickSlot.as<int32_t>(), slotBrick.as<int32_t>()}, blocks(nb));
         if(nSlots == 0) {
+            warmNext    = false;
             systemValid = true;
             co_return;
         }
@@ -805,6 +824,14 @@ public:
                     cx32.as<float>(), cy32.as<float>(), cz32.as<float>(), invd.as<float>()};
         VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
         launch(k_assemble{(const int32_t*)slotBrick.as<int32_t>(), g, (const float*)phi.as<float>(), G, (const float*)wu.as<float>(), (const float*)wv.as<float>(), (const float*)ww.as<float>(), dt, S}, nSlots);
+        if(warmNext) {
+            const size_t need = (size_t)nSlots * 512 * 8;
+            if(!x64.p || x64.bytes < need || !x64prev.p || !prevBrickSlot.p || prevBrickSlot.bytes < (size_t)nb * 4)
+                throw Error(BNN_ERR_DEVICE, "warm start buffers are not allocated");
+            launch(k_warm_start{(const int32_t*)slotBrick.as<int32_t>(), (const int32_t*)prevBrickSlot.as<int32_t>(),
+                                (const double*)x64prev.as<double>(), (const float*)phi.as<float>(), g, nSlots, x64.as<double>()},
+                   redBlocks());
+        }
         if(prm.preconditioner == BNN_PRECOND_MG) co_await buildHierarchy();
         systemValid = true;
     }
@@ -1041,7 +1068,8 @@ public:
         double         bmax = 0, res = 0;
         for(int first = 1;; first = 0) {
             co_await barrier();  // the scenes' outer passes start together
-            launch(k_residual64{(const int32_t*)nbrTab.as<int32_t>(), (const double*)x64.as<double>(), (const double*)rhs64.as<double>(), (const double*)diag64.as<double>(), (const double*)cx64.as<double>(), (const double*)cy64.as<double>(), (const double*)cz64.as<double>(), rhs32.as<float>(), nSlots, pa.as<double>(), pb.as<double>(), first}, G);
+            const int zeroX = first && !warmNext;  // x = 0 (else x64 holds the warm start or the sum so far)
+            launch(k_residual64{(const int32_t*)nbrTab.as<int32_t>(), (const double*)x64.as<double>(), (const double*)rhs64.as<double>(), (const double*)diag64.as<double>(), (const double*)cx64.as<double>(), (const double*)cy64.as<double>(), (const double*)cz64.as<double>(), rhs32.as<float>(), nSlots, pa.as<double>(), pb.as<double>(), zeroX}, G);
             launch(k_residual64_fin{ost.as<OuterState>(), (const double*)pa.as<double>(), (const double*)pb.as<double>(), (int)G}, 1);
             readback(hostOst, ost.p, sizeof(OuterState));
             co_await syncPoint();
@@ -1056,16 +1084,20 @@ public:
             }
             res = hostOst->res;
             if(res <= tol * bmax) {
-                if(first) fill32(x64.p, 0, (size_t)nSlots * 512 * 8);
+                if(zeroX) fill32(x64.p, 0, (size_t)nSlots * 512 * 8);
                 ok = true;
                 break;
             }
             if(outer >= maxOuter) break;
-            uint32_t it;
-            float    r32, b32;
-            co_await pcg32(rhs32.as<float>(), innerTol, prm.max_cg_iteration, it, r32, b32);
+            // the inner solve need only reduce the residual by the factor still missing to the bar
+            // (with a margin of 4), when that is less than refine_inner_tolerance asks for
+            const double missing = tol * bmax / res;
+            const float  passTol = missing > 4.0 * innerTol ? (float)std::min(0.25 * missing, 0.5) : innerTol;
+            uint32_t     it;
+            float        r32, b32;
+            co_await pcg32(rhs32.as<float>(), passTol, prm.max_cg_iteration, it, r32, b32);
             total += it;
-            launch(k_accumulate64{x64.as<double>(), (const float*)x.as<float>(), nSlots, first}, G);
+            launch(k_accumulate64{x64.as<double>(), (const float*)x.as<float>(), nSlots, zeroX}, G);
             ++outer;
             if(it == 0) break;  // inner solve made no progress
         }
@@ -1074,6 +1106,7 @@ public:
         st.pcg_residual     = res;
         st.rhs_max          = bmax;
         st.pcg_success      = ok;
+        prevPressure        = ok;
         co_return ok;
     }
 

@@ -330,6 +330,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge in cells (256 = the headline workload)")
     ap.add_argument("--pressure", choices=["refined", "fp32"], default="refined")
     ap.add_argument("--precond", choices=["mg", "jacobi"], default="mg")
+    ap.add_argument("--warm-start", type=int, default=1, help="1: refined solves start from the previous pressure (0: from 0)")
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="relative tolerance of each inner FP32 solve under the FP64 correction (library default when omitted)")
     ap.add_argument("--mg-sweeps", type=int, default=None, help="smoothing sweeps per multigrid level (library default when omitted)")
@@ -371,6 +372,7 @@ def main():
                   preconditioner=1 if args.precond == "mg" else 0)
     if args.inner_tol is not None:
         params["refine_inner_tolerance"] = args.inner_tol
+    params["warm_start"] = args.warm_start
     if args.mg_sweeps is not None:
         params["mg_sweeps"] = args.mg_sweeps
     if args.mg_coarsest_sweeps is not None:
@@ -506,7 +508,7 @@ def main():
                    "grid": list(sim.n), "particles": np_, "pcg_iterations_per_step": pcg_iters / args.steps,
                    "pcg_iterations_per_s": round(pcg_iters / elapsed, 1),
                    "fp64_relative_residual_max": max(rel_res) if rel_res else None,
-                   "preconditioner": args.precond,
+                   "preconditioner": args.precond, "warm_start": bool(args.warm_start),
                    "stage_ms_per_step": {k: round(v / args.profile_steps, 3) for k, v in stage_ms.items()} if args.profile_steps else None},
         "roofline": roofline,
         "kernels": kernels,

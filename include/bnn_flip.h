@@ -117,6 +117,10 @@ typedef struct bnn_flip_params {
                                                PCG solve reaches before the next FP64 correction */
     int32_t  transfer_scheme;               /* bnn_transfer_scheme (FLIP); set from GlobalParameters
                                                Solver by bnn_scene_from_json */
+    int32_t  warm_start;                    /* extension, REFINED (0): start the FP64 defect correction
+                                               from the previous substep's pressure (same cells, where
+                                               still fluid) instead of 0 (the reference's PCG starts at
+                                               0, PCGSolver.cpp:147-149); same stopping rule */
 } bnn_flip_params;
 
 /* Seeded scene generator input (replaces ParticleGenerator, ParticleGenerator.hpp:87-136): one

