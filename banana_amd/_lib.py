@@ -44,6 +44,19 @@ class Params(C.Structure):
     ]
 
 
+class KeyFrame(C.Structure):
+    _fields_ = [("frame", C.c_uint32), ("translation", C.c_float * 3), ("rotation_axis_angle", C.c_float * 4), ("scale", C.c_float),
+                ("has_rotation", C.c_int32), ("has_scale", C.c_int32)]
+
+
+class BoundaryObject(C.Structure):
+    _fields_ = [("geometry", C.c_int32), ("box_min", C.c_float * 3), ("box_max", C.c_float * 3), ("translation", C.c_float * 3),
+                ("rotation_axis_angle", C.c_float * 4), ("scale", C.c_float), ("has_translation", C.c_int32),
+                ("has_rotation", C.c_int32), ("has_scale", C.c_int32), ("n_keyframes", C.c_uint32), ("keyframes", KeyFrame * 16),
+                ("periodic", C.c_int32), ("start_frame", C.c_uint32), ("cubic_translation", C.c_int32),
+                ("cubic_rotation", C.c_int32), ("cubic_scale", C.c_int32)]
+
+
 class SceneSpec(C.Structure):
     _fields_ = [
         ("n_boxes", C.c_int32), ("box_min", (C.c_float * 3) * 8), ("box_max", (C.c_float * 3) * 8),
@@ -167,6 +180,8 @@ SIGNATURES = {
     "bnn_flip_batch_advance_frame": (_i32, [_vp, _P(_vp), _u32, _P(_i32)]),
     "bnn_flip_batch_pressure_projection": (_i32, [_vp, _P(_vp), _P(_f32), _u32, _P(_i32)]),
     "bnn_flip_batch_set_shape": (_i32, [_vp, _vp]),
+    "bnn_flip_set_boundary_objects": (_i32, [_vp, _vp, _u32]),
+    "bnn_boundaries_from_json": (_i32, [C.c_char_p, _vp, _u32, _P(_u32)]),
     "bnn_flip_batch_get_info": (_i32, [_vp, _vp]),
     "bnn_server_create": (_i32, [_P(ServerConfig), _P(_vp)]),
     "bnn_server_destroy": (_i32, [_vp]),

@@ -243,6 +243,108 @@ __device__ inline void box_grad_out(const BoxDev& b, float x, float y, float z, 
     gz = (((ddz00 + ddz01) + ddz10) + ddz11) * 0.25f;
 }
 
+//--------------------------------------------------------------------------------------------------
+// Boundary objects (BoundaryObject.h:43-48, BoundaryObject.cpp:55-90, GeometryObjects.cpp:37-85,
+// 246-266, 379-383, 861-896). A boundary object's signed distance is its geometry's with
+// bNegativeInside = false: positive inside the object, where the fluid may be. Static objects are
+// merged into one CSG union (min of their distances, ParticleSolver.cpp:262-287); dynamic objects
+// follow as groups of their own. A group = shapes [end[g-1], end[g]).
+//--------------------------------------------------------------------------------------------------
+constexpr int kMaxBndShapes = 8;
+struct BndShape {
+    int32_t type;     // 0 box (lo, hi in object space), 1 unit sphere
+    int32_t xform;    // 1: object space = inv * world (3 x 4, row-major), distances times scale
+    float   inv[12];
+    float   scale;
+    float   lo[3], hi[3];
+};
+struct BndSet {
+    int32_t  nShapes, nGroups;
+    int32_t  end[kMaxBndShapes];
+    BndShape s[kMaxBndShapes];
+};
+
+// BoxObject / SphereObject::signedDistance(p, false) (GeometryObjects.cpp:246-266, 379-383)
+__device__ __forceinline__ float bnd_shape_sd(const BndShape& s, float x, float y, float z) {
+    if(s.xform) {
+        const float* m = s.inv;
+        const float  px = ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
+        const float  py = ((m[4] * x + m[5] * y) + m[6] * z) + m[7];
+        const float  pz = ((m[8] * x + m[9] * y) + m[10] * z) + m[11];
+        x = px;
+        y = py;
+        z = pz;
+    }
+    float r;
+    if(s.type == 0) {
+        BoxDev b;
+        for(int d = 0; d < 3; ++d) {
+            b.lo[d] = s.lo[d];
+            b.hi[d] = s.hi[d];
+        }
+        r = box_sd_out(b, x, y, z);
+    } else {
+        r = -(len3f(x, y, z) - 1.0f);
+    }
+    return s.scale * r;
+}
+// the signed distance of group g: the CSG union of its shapes (CSGObject::signedDistance, Union)
+__device__ __forceinline__ float bnd_group_sd(const BndSet& B, int g, float x, float y, float z) {
+    const int b = g == 0 ? 0 : B.end[g - 1], e = B.end[g];
+    float     sd = bnd_shape_sd(B.s[b], x, y, z);
+    for(int i = b + 1; i < e; ++i) sd = smin(sd, bnd_shape_sd(B.s[i], x, y, z));
+    return sd;
+}
+// GeometryObject::gradSignedDistance (GeometryObjects.cpp:37-85), dx = 1e-4, on group g
+__device__ inline void bnd_group_grad(const BndSet& B, int g, float x, float y, float z, float& gx, float& gy, float& gz) {
+    const float h    = 1e-4f;
+    float       v000 = bnd_group_sd(B, g, x - h, y - h, z - h);
+    float       v001 = bnd_group_sd(B, g, x - h, y - h, z + h);
+    float       v010 = bnd_group_sd(B, g, x - h, y + h, z - h);
+    float       v011 = bnd_group_sd(B, g, x - h, y + h, z + h);
+    float       v100 = bnd_group_sd(B, g, x + h, y - h, z - h);
+    float       v101 = bnd_group_sd(B, g, x + h, y - h, z + h);
+    float       v110 = bnd_group_sd(B, g, x + h, y + h, z - h);
+    float       v111 = bnd_group_sd(B, g, x + h, y + h, z + h);
+    float ddx00 = v100 - v000, ddx10 = v110 - v010, ddx01 = v101 - v001, ddx11 = v111 - v011;
+    float ddy00 = v010 - v000, ddy10 = v110 - v100, ddy01 = v011 - v001, ddy11 = v111 - v101;
+    float ddz00 = v001 - v000, ddz10 = v101 - v100, ddz01 = v011 - v010, ddz11 = v111 - v110;
+    gx = (((ddx00 + ddx01) + ddx10) + ddx11) * 0.25f;
+    gy = (((ddy00 + ddy01) + ddy10) + ddy11) * 0.25f;
+    gz = (((ddz00 + ddz01) + ddz10) + ddz11) * 0.25f;
+}
+// BoundaryObject::constrainToBoundary for every group in order (PIC_Solver.cpp:362-366); returns
+// true when a reflection changed the velocity
+__device__ inline bool bnd_constrain(const BndSet& B, float& x, float& y, float& z, float& vx, float& vy, float& vz, int reflect,
+                                     float reflMult) {
+    bool changed = false;
+    for(int g = 0; g < B.nGroups; ++g) {
+        const float phi = bnd_group_sd(B, g, x, y, z);
+        if(phi < 0) {
+            float gx, gy, gz;
+            bnd_group_grad(B, g, x, y, z, gx, gy, gz);
+            const float m2 = gx * gx + gy * gy + gz * gz;
+            if(m2 > kFltMin) {
+                const float s = sqrtf(m2);
+                gx /= s;
+                gy /= s;
+                gz /= s;
+                x -= phi * gx;
+                y -= phi * gy;
+                z -= phi * gz;
+                if(reflect) {
+                    const float dn = gx * vx + gy * vy + gz * vz;
+                    vx             = (vx - 2.f * dn * gx) * reflMult;
+                    vy             = (vy - 2.f * dn * gy) * reflMult;
+                    vz             = (vz - 2.f * dn * gz) * reflMult;
+                    changed        = true;
+                }
+            }
+        }
+    }
+    return changed;
+}
+
 // wave64 reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

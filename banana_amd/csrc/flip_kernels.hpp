@@ -43,7 +43,7 @@ struct k_move {
     uint32_t n;
     VelGrid G;
     GridGeom g;
-    BoxDev dom;
+    BndSet bnd;
     float dt;
     uint32_t advSteps;
     int reflect;
@@ -65,29 +65,8 @@ struct k_move {
             x = x + sub * ux;
             y = y + sub * uy;
             z = z + sub * uz;
-            // constrainToBoundary(ppos, pvel)
-            const float phi = box_sd_out(dom, x, y, z);
-            if(phi < 0) {
-                float gx, gy, gz;
-                box_grad_out(dom, x, y, z, gx, gy, gz);
-                const float m2 = gx * gx + gy * gy + gz * gz;
-                if(m2 > kFltMin) {
-                    const float s = sqrtf(m2);
-                    gx /= s;
-                    gy /= s;
-                    gz /= s;
-                    x -= phi * gx;
-                    y -= phi * gy;
-                    z -= phi * gz;
-                    if(reflect) {
-                        const float dn = gx * vx + gy * vy + gz * vz;
-                        vx             = (vx - 2.f * dn * gx) * reflMult;
-                        vy             = (vy - 2.f * dn * gy) * reflMult;
-                        vz             = (vz - 2.f * dn * gz) * reflMult;
-                        changed        = true;
-                    }
-                }
-            }
+            // constrainToBoundary(ppos, pvel) of every boundary object
+            if(bnd_constrain(bnd, x, y, z, vx, vy, vz, reflect, reflMult)) changed = true;
         }
         P.px[p] = x;
         P.py[p] = y;
@@ -443,7 +422,7 @@ struct k_correct_positions {
     const uint32_t* __restrict__ cell_start;
     uint32_t n;
     GridGeom g;
-    BoxDev dom;
+    BndSet bnd;
     CorrectParams c;
     float* __restrict__ ox;
     float* __restrict__ oy;
@@ -491,21 +470,8 @@ struct k_correct_positions {
             nx_ += sx * c.K;
             ny_ += sy * c.K;
             nz_ += sz * c.K;
-            const float phi = box_sd_out(dom, nx_, ny_, nz_);
-            if(phi < 0) {
-                float gx, gy, gz;
-                box_grad_out(dom, nx_, ny_, nz_, gx, gy, gz);
-                const float m2 = gx * gx + gy * gy + gz * gz;
-                if(m2 > kFltMin) {
-                    const float s = sqrtf(m2);
-                    gx /= s;
-                    gy /= s;
-                    gz /= s;
-                    nx_ -= phi * gx;
-                    ny_ -= phi * gy;
-                    nz_ -= phi * gz;
-                }
-            }
+            float vx = 0.f, vy = 0.f, vz = 0.f;  // constrainToBoundary(ppos): no velocity
+            bnd_constrain(bnd, nx_, ny_, nz_, vx, vy, vz, 0, 1.f);
         }
         ox[p] = nx_;
         oy[p] = ny_;
@@ -1138,19 +1104,22 @@ struct k_weights {
     }
 };
 
-// computeBoundarySDF with the domain box as the boundary (PIC_Solver.cpp:1158-1168)
+// computeBoundarySDF: the smallest signed distance over the boundary objects (PIC_Solver.cpp:1158-1168)
 struct k_boundary_sdf {
     static constexpr int kThreads = kBlock;
     float* __restrict__ b;
     GridGeom g;
-    BoxDev dom;
+    BndSet bnd;
     __device__ void operator()(Blk blk) const {
         const int    sx = g.nx + 1, sy = g.ny + 1, sz = g.nz + 1;
         const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
         if(t >= (size_t)sx * sy * sz) return;
         int i, j, k;
         unlin3((uint32_t)t, sx, sy, i, j, k);
-        b[t]        = box_sd_out(dom, (float)i * g.dx + g.bminx, (float)j * g.dx + g.bminy, (float)k * g.dx + g.bminz);
+        const float x = (float)i * g.dx + g.bminx, y = (float)j * g.dx + g.bminy, z = (float)k * g.dx + g.bminz;
+        float       m = 3.40282347e+38f;  // Huge<float>
+        for(int q = 0; q < bnd.nGroups; ++q) m = smin(m, bnd_group_sd(bnd, q, x, y, z));
+        b[t] = m;
     }
 };
 

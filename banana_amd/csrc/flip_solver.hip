@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/bnn_flip.h"
+#include "flip_boundary.hpp"
 #include "flip_io.hpp"
 #include "scene_json.hpp"
 
@@ -218,6 +219,9 @@ public:
     bnn_flip_params prm;
     GridGeom        g{};
     BoxDev          dom{};
+    // the scene's additional boundary objects and the device set for their current transforms
+    std::vector<BoundaryObjectHost> bobjs;
+    BndSet                          bnd{};
     float           grav[3] = {0, -9.81f, 0};
     std::string     lastError;
     hipStream_t     stream = nullptr;
@@ -330,6 +334,7 @@ public:
             dom.lo[d] = prm.box_min[d];
             dom.hi[d] = prm.box_max[d];
         }
+        bnd = make_bnd_set(dom.lo, dom.hi, bobjs);
         // gravity (ParticleSolverData.cpp:256-271)
         if(prm.gravity_type == 1) {
             const float* gd = prm.gravity_direction;
@@ -608,7 +613,7 @@ public:
     }
 
     void computeBoundarySDF() {
-        launch(k_boundary_sdf{bsdf.as<float>(), g, dom}, blocks(nN));
+        launch(k_boundary_sdf{bsdf.as<float>(), g, bnd}, blocks(nN));
         weightsDirty = true;
         bsdfNonzero  = true;
     }
@@ -618,7 +623,7 @@ public:
         if(!np) return;
         VelGrid G{u.as<float>(), v.as<float>(), w.as<float>()};
         ktBegin(BNN_KT_MOVE);
-        launch(k_move{cur->view(), (uint32_t)np, G, g, dom, dt,
+        launch(k_move{cur->view(), (uint32_t)np, G, g, bnd, dt,
                            prm.advection_steps, prm.reflect_velocity_at_boundary, prm.boundary_reflection_multiplier}, blocks(np));
         ktEnd(BNN_KT_MOVE);
         binned = false;
@@ -642,7 +647,7 @@ public:
         c.advSteps   = prm.advection_steps;
         c.seed       = correctSeed();
         launch(k_correct_positions{cur->view(), (const uint32_t*)cellStart.as<uint32_t>(),
-                           (uint32_t)np, g, dom, c, alt->px.as<float>(), alt->py.as<float>(), alt->pz.as<float>()}, blocks(np));
+                           (uint32_t)np, g, bnd, c, alt->px.as<float>(), alt->py.as<float>(), alt->pz.as<float>()}, blocks(np));
         copy32(cur->px.p, alt->px.p, np * 4);
         copy32(cur->py.p, alt->py.p, np * 4);
         copy32(cur->pz.p, alt->pz.p, np * 4);
@@ -1306,6 +1311,7 @@ public:
         bool     ok    = true;
         float    last  = 0;
         while(frameTime < fd) {
+            if(finishedFrame > 0) advanceScene(frameTime / fd);  // PIC_Solver.cpp:279-281
             float sub = co_await timestepCFL();
             float rem = fd - frameTime;
             if(frameTime + sub >= fd) sub = rem;
@@ -1320,6 +1326,35 @@ public:
         st.substep_count = count;
         st.last_substep  = last;
         co_return ok;
+    }
+
+    // the scene's additional boundary objects (ParticleSolver::generateBoundaries)
+    void setBoundaryObjects(const bnn_boundary_object* objs, uint32_t n) {
+        if(n > BNN_MAX_BOUNDARY_OBJECTS) throw Error(BNN_ERR_INVALID_ARG, "too many boundary objects");
+        if(n && !objs) throw Error(BNN_ERR_INVALID_ARG, "null boundary object array");
+        std::vector<BoundaryObjectHost> v(n);
+        try {
+            for(uint32_t i = 0; i < n; ++i) v[i].load(objs[i]);
+        } catch(const std::invalid_argument& e) {
+            throw Error(BNN_ERR_INVALID_ARG, e.what());
+        }
+        bobjs.swap(v);
+        bnd = make_bnd_set(dom.lo, dom.hi, bobjs);
+        if(bsdfNonzero || prm.boundary_sdf_at_init) computeBoundarySDF();
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    bool hasDynamicBoundaries() const {
+        for(const auto& o : bobjs)
+            if(o.isDynamic()) return true;
+        return false;
+    }
+    // PIC_Solver::advanceScene (PIC_Solver.cpp:70-118): animated boundary objects move to
+    // (finishedFrame + fraction) and the boundary SDF is recomputed from every object
+    void advanceScene(float fraction) {
+        if(!hasDynamicBoundaries()) return;
+        for(auto& o : bobjs) o.update(finishedFrame, (double)fraction);
+        bnd = make_bnd_set(dom.lo, dom.hi, bobjs);
+        computeBoundarySDF();
     }
 
     // PIC_Solver.cpp:315-328: z-sort at frame 0 and then every SortFrequency frames, if enabled
@@ -1769,6 +1804,35 @@ bnn_status bnn_scene_from_json(const char* json_text, bnn_flip_params* p, bnn_sc
     }
 }
 
+bnn_status bnn_boundaries_from_json(const char* json_text, bnn_boundary_object* objs, uint32_t cap, uint32_t* n) {
+    if(!json_text || !n || (cap && !objs)) return BNN_ERR_INVALID_ARG;
+    try {
+        const bnn::json::Value           root = bnn::json::Parser(json_text).parse();
+        std::vector<bnn_boundary_object> v;
+        bnn::boundaries_from_json(root, v);
+        *n = (uint32_t)v.size();
+        if(v.size() > cap) {
+            g_createError = "more boundary objects than the array holds";
+            return BNN_ERR_INVALID_ARG;
+        }
+        for(size_t i = 0; i < v.size(); ++i) objs[i] = v[i];
+        return BNN_OK;
+    } catch(const bnn::SceneLoadError& e) {
+        g_createError = e.what();
+        return e.code;
+    } catch(const std::exception& e) {
+        g_createError = e.what();
+        return BNN_ERR_INVALID_ARG;
+    }
+}
+
+bnn_status bnn_flip_set_boundary_objects(bnn_flip* h, const bnn_boundary_object* objs, uint32_t n) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.setBoundaryObjects(objs, n);
+        return BNN_OK;
+    });
+}
+
 bnn_status bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_flip** out) {
     if(!out) return BNN_ERR_INVALID_ARG;
     *out = nullptr;
@@ -1776,9 +1840,14 @@ bnn_status bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_f
     bnn_scene_spec  sp;
     bnn_status      st = bnn_scene_from_json(json_text, &p, &sp, nullptr, 0);
     if(st != BNN_OK) return st;
+    bnn_boundary_object objs[BNN_MAX_BOUNDARY_OBJECTS];
+    uint32_t            nobj = 0;
+    st                       = bnn_boundaries_from_json(json_text, objs, BNN_MAX_BOUNDARY_OBJECTS, &nobj);
+    if(st != BNN_OK) return st;
     st = bnn_flip_create(&p, out);
     if(st != BNN_OK) return st;
-    st = bnn_flip_generate_scene(*out, &sp, seed);
+    if(nobj) st = bnn_flip_set_boundary_objects(*out, objs, nobj);
+    if(st == BNN_OK) st = bnn_flip_generate_scene(*out, &sp, seed);
     if(st != BNN_OK) {
         g_createError = bnn_flip_last_error(*out);
         bnn_flip_destroy(*out);

@@ -198,6 +198,94 @@ struct SceneLoadError : std::runtime_error {
 };
 
 // Reads a reference scene (already parsed) into params/spec; returns warnings (one per line).
+// AdditionalBoundaryObjects (ParticleSolver.cpp:242-258, GeometryObjects.cpp:143-201): box or
+// sphere geometry, Translation / RotationAxisAngle (degrees) / Scale, and an Animation with key
+// frames. CSGOperator and UniqueName are accepted and not used (the reference merges every static
+// object with a union whatever its operator).
+inline void boundaries_from_json(const json::Value& root, std::vector<bnn_boundary_object>& out) {
+    using json::Value;
+    out.clear();
+    const Value* ab = root.find("AdditionalBoundaryObjects");
+    if(!ab) return;
+    if(ab->kind != Value::Array) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: AdditionalBoundaryObjects must be an array");
+    auto vec = [](const Value& o, const char* k, float* dst, int n) {
+        const Value* v = o.find(k);
+        if(!v) return false;
+        if(v->kind != Value::Array || (int)v->arr.size() != n)
+            throw SceneLoadError(BNN_ERR_INVALID_ARG, std::string("scene JSON: boundary key '") + k + "' has the wrong shape");
+        for(int d = 0; d < n; ++d) dst[d] = (float)v->arr[d].num;
+        return true;
+    };
+    auto number = [](const Value& o, const char* k, float& dst) {
+        const Value* v = o.find(k);
+        if(!v) return false;
+        if(v->kind != Value::Number) throw SceneLoadError(BNN_ERR_INVALID_ARG, std::string("scene JSON: boundary key '") + k + "' must be a number");
+        dst = (float)v->num;
+        return true;
+    };
+    auto flag = [](const Value& o, const char* k, int32_t& dst) {
+        const Value* v = o.find(k);
+        if(!v) return false;
+        if(v->kind == Value::Bool) dst = v->b ? 1 : 0;
+        else if(v->kind == Value::Number) dst = v->num != 0 ? 1 : 0;
+        else throw SceneLoadError(BNN_ERR_INVALID_ARG, std::string("scene JSON: boundary key '") + k + "' must be a boolean");
+        return true;
+    };
+    for(const Value& jo : ab->arr) {
+        if(jo.kind != Value::Object) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: a boundary object must be an object");
+        if(out.size() >= BNN_MAX_BOUNDARY_OBJECTS) throw SceneLoadError(BNN_ERR_UNSUPPORTED, "scene JSON: too many boundary objects");
+        bnn_boundary_object o;
+        std::memset(&o, 0, sizeof(o));
+        for(int d = 0; d < 3; ++d) {
+            o.box_min[d] = -1.f;
+            o.box_max[d] = 1.f;
+        }
+        o.rotation_axis_angle[0] = o.rotation_axis_angle[1] = o.rotation_axis_angle[2] = 1.f;
+        o.scale             = 1.f;
+        o.cubic_translation = o.cubic_rotation = o.cubic_scale = 1;
+        const Value* ty = jo.find("GeometryType");
+        if(!ty || ty->kind != Value::String) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: boundary object without GeometryType");
+        const std::string& g = ty->str;
+        if(g == "Box" || g == "box" || g == "BOX") o.geometry = BNN_GEOMETRY_BOX;
+        else if(g == "Sphere" || g == "sphere" || g == "SPHERE") o.geometry = BNN_GEOMETRY_SPHERE;
+        else throw SceneLoadError(BNN_ERR_UNSUPPORTED, "scene JSON: boundary geometry '" + g + "' is not provided (Box, Sphere)");
+        for(const char* k : {"RotationEulerAngles", "RotationEulerAngle", "MeshFile"})
+            if(jo.find(k)) throw SceneLoadError(BNN_ERR_UNSUPPORTED, std::string("scene JSON: boundary key '") + k + "' is not provided");
+        vec(jo, "BoxMin", o.box_min, 3);
+        vec(jo, "BoxMax", o.box_max, 3);
+        o.has_translation = vec(jo, "Translation", o.translation, 3) ? 1 : 0;
+        o.has_rotation    = vec(jo, "RotationAxisAngle", o.rotation_axis_angle, 4) ? 1 : 0;
+        o.has_scale       = number(jo, "Scale", o.scale) ? 1 : 0;
+        if(const Value* an = jo.find("Animation")) {
+            if(an->kind != Value::Object) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: Animation must be an object");
+            flag(*an, "CubicInterpolationTranslation", o.cubic_translation);
+            flag(*an, "CubicInterpolationRotation", o.cubic_rotation);
+            flag(*an, "CubicInterpolationScale", o.cubic_scale);
+            if(flag(*an, "Periodic", o.periodic)) {
+                float sf = 0;
+                if(number(*an, "StartFrame", sf)) o.start_frame = (uint32_t)sf;
+            }
+            const Value* kf = an->find("KeyFrames");
+            if(!kf || kf->kind != Value::Array) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: Animation without KeyFrames");
+            if(kf->arr.size() > BNN_MAX_KEYFRAMES) throw SceneLoadError(BNN_ERR_UNSUPPORTED, "scene JSON: too many key frames");
+            for(const Value& jk : kf->arr) {
+                if(jk.kind != Value::Object) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: a key frame must be an object");
+                for(const char* k : {"RotationEulerAngles", "RotationEulerAngle"})
+                    if(jk.find(k)) throw SceneLoadError(BNN_ERR_UNSUPPORTED, std::string("scene JSON: key frame key '") + k + "' is not provided");
+                bnn_keyframe& k = o.keyframes[o.n_keyframes++];
+                float         fr = 0;
+                if(!number(jk, "Frame", fr)) throw SceneLoadError(BNN_ERR_INVALID_ARG, "scene JSON: a key frame without Frame");
+                k.frame = (uint32_t)fr;
+                vec(jk, "Translation", k.translation, 3);
+                k.has_rotation = vec(jk, "RotationAxisAngle", k.rotation_axis_angle, 4) ? 1 : 0;
+                k.scale        = 1.f;
+                k.has_scale    = number(jk, "Scale", k.scale) ? 1 : 0;
+            }
+        }
+        out.push_back(o);
+    }
+}
+
 inline std::string scene_from_json(const json::Value& root, bnn_flip_params& p, bnn_scene_spec& sp) {
     using json::Value;
     std::string warn;
@@ -310,9 +398,10 @@ inline std::string scene_from_json(const json::Value& root, bnn_flip_params& p, 
         p.box_max[d] = s * bmax[d] + t[d];
     }
 
-    if(const Value* ab = root.find("AdditionalBoundaryObjects"))
-        if(ab->kind == Value::Array && !ab->arr.empty())
-            throw SceneLoadError(BNN_ERR_UNSUPPORTED, "scene JSON: AdditionalBoundaryObjects are not provided (domain box only)");
+    if(const Value* ab = root.find("AdditionalBoundaryObjects")) {  // parsed by boundaries_from_json
+        std::vector<bnn_boundary_object> objs;
+        boundaries_from_json(root, objs);
+    }
     if(const Value* pr = root.find("ParticleRemovers"))
         if(pr->kind == Value::Array && !pr->arr.empty()) note("ParticleRemovers are not provided; ignored");
 

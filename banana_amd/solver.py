@@ -63,6 +63,58 @@ def io_from_json(text):
     return {k: (getattr(io, k).decode() if isinstance(getattr(io, k), bytes) else getattr(io, k)) for k, _ in _lib.IoParams._fields_}
 
 
+def boundary_objects(objs):
+    """ctypes array of bnn_boundary_object from dicts: geometry ("box" / "sphere"), box_min, box_max,
+    translation, rotation_axis_angle (degrees), scale, and an optional animation dict with
+    keyframes [{frame, translation, rotation_axis_angle, scale}], periodic, start_frame and
+    cubic_translation / cubic_rotation / cubic_scale (default True)."""
+    arr = (_lib.BoundaryObject * max(len(objs), 1))()
+    for o, d in zip(arr, objs):
+        o.geometry = 1 if d.get("geometry", "box").lower() == "sphere" else 0
+        for a in range(3):
+            o.box_min[a] = d.get("box_min", (-1, -1, -1))[a]
+            o.box_max[a] = d.get("box_max", (1, 1, 1))[a]
+            o.translation[a] = d.get("translation", (0, 0, 0))[a]
+        r = d.get("rotation_axis_angle", (1, 1, 1, 0))
+        for a in range(4):
+            o.rotation_axis_angle[a] = r[a]
+        o.scale = d.get("scale", 1.0)
+        o.has_translation = int("translation" in d)
+        o.has_rotation = int("rotation_axis_angle" in d)
+        o.has_scale = int("scale" in d)
+        an = d.get("animation")
+        o.cubic_translation = o.cubic_rotation = o.cubic_scale = 1
+        if an:
+            o.periodic = int(an.get("periodic", False))
+            o.start_frame = int(an.get("start_frame", 0))
+            o.cubic_translation = int(an.get("cubic_translation", True))
+            o.cubic_rotation = int(an.get("cubic_rotation", True))
+            o.cubic_scale = int(an.get("cubic_scale", True))
+            o.n_keyframes = len(an["keyframes"])
+            for k, kd in zip(o.keyframes, an["keyframes"]):
+                k.frame = int(kd["frame"])
+                for a in range(3):
+                    k.translation[a] = kd.get("translation", (0, 0, 0))[a]
+                rk = kd.get("rotation_axis_angle", (1, 1, 1, 0))
+                for a in range(4):
+                    k.rotation_axis_angle[a] = rk[a]
+                k.scale = kd.get("scale", 1.0)
+                k.has_rotation = int("rotation_axis_angle" in kd)
+                k.has_scale = int("scale" in kd)
+    return arr
+
+
+def boundaries_from_json(text):
+    """The AdditionalBoundaryObjects of a scene file as a ctypes array (bnn_boundaries_from_json)."""
+    L = _lib.load()
+    arr = (_lib.BoundaryObject * 8)()
+    n = C.c_uint32(0)
+    st = L.bnn_boundaries_from_json(text.encode() if isinstance(text, str) else text, C.cast(arr, C.c_void_p), 8, C.byref(n))
+    if st != 0:
+        raise BnnError(st, (L.bnn_flip_last_error(None) or b"").decode())
+    return arr, n.value
+
+
 def data_file_path(data_path, folder, frame):
     """DataIO::getFilePath: <data_path>/<folder>/frame.%04d.bnn"""
     L = _lib.load()
@@ -181,6 +233,15 @@ class FLIPSolver:
         c = np.ascontiguousarray(c, np.float32)
         assert c.shape == (self.num_particles(), 3, 3)
         self._c(self.L.bnn_flip_set_affine(self.h, _ptr(c)))
+
+    def set_boundary_objects(self, objs):
+        """The scene's additional boundary objects: a list of dicts (see boundary_objects) or a
+        ctypes array with its count, as (array, n)."""
+        if isinstance(objs, tuple):
+            arr, n = objs
+        else:
+            arr, n = boundary_objects(objs), len(objs)
+        self._c(self.L.bnn_flip_set_boundary_objects(self.h, C.cast(arr, C.c_void_p), n))
 
     def correct_particle_positions(self, dt):
         self._c(self.L.bnn_flip_correct_particle_positions(self.h, dt))

@@ -123,6 +123,40 @@ typedef struct bnn_flip_params {
                                                0, PCGSolver.cpp:147-149); same stopping rule */
 } bnn_flip_params;
 
+/* --- boundary objects ----------------------------------------------------------------------
+ * The scene's AdditionalBoundaryObjects (ParticleSolver.cpp:242-298): box or sphere geometry (a
+ * sphere is the unit sphere, a box spans box_min .. box_max, both in object space) under
+ * translate * rotate(axis, angle) * uniform scale, optionally animated by key frames (Animation.h:
+ * cubic splines with zero end slopes, or linear, per channel; periodic from start_frame). The fluid
+ * stays inside every boundary object (signed distance positive inside). Static objects and the
+ * domain box act as one CSG union, each animated object on its own; animated objects move at every
+ * substep of bnn_flip_advance_frame after the first frame (PIC_Solver.cpp:279-281), and the
+ * boundary SDF is then recomputed from all objects (PIC_Solver.cpp:103-115,1158-1168). */
+#define BNN_GEOMETRY_BOX 0
+#define BNN_GEOMETRY_SPHERE 1
+#define BNN_MAX_KEYFRAMES 16
+#define BNN_MAX_BOUNDARY_OBJECTS 7
+typedef struct bnn_keyframe {
+    uint32_t frame;
+    float    translation[3];
+    float    rotation_axis_angle[4]; /* axis, angle in degrees (RotationAxisAngle) */
+    float    scale;
+    int32_t  has_rotation, has_scale; /* the key frame sets RotationAxisAngle / Scale */
+} bnn_keyframe;
+typedef struct bnn_boundary_object {
+    int32_t      geometry;               /* BNN_GEOMETRY_BOX or BNN_GEOMETRY_SPHERE */
+    float        box_min[3], box_max[3]; /* BoxMin / BoxMax (box) */
+    float        translation[3];         /* Translation */
+    float        rotation_axis_angle[4]; /* RotationAxisAngle, degrees */
+    float        scale;                  /* Scale */
+    int32_t      has_translation, has_rotation, has_scale;
+    uint32_t     n_keyframes;            /* Animation.KeyFrames (0: static object) */
+    bnn_keyframe keyframes[BNN_MAX_KEYFRAMES];
+    int32_t      periodic;               /* Animation.Periodic */
+    uint32_t     start_frame;            /* Animation.StartFrame */
+    int32_t      cubic_translation, cubic_rotation, cubic_scale; /* Animation.CubicInterpolation* (1) */
+} bnn_boundary_object;
+
 /* Seeded scene generator input (replaces ParticleGenerator, ParticleGenerator.hpp:87-136): one
  * particle per sub-cell of the grid's sub-cell lattice (ppa^3 per cell) whose centre lies inside an
  * object, jittered uniformly (fraction `jitter`) inside its sub-cell; objects in order, a sub-cell
@@ -205,6 +239,14 @@ typedef struct bnn_flip_stats {
 } bnn_flip_stats;
 
 typedef struct bnn_flip bnn_flip;
+
+/* replaces the scene's additional boundary objects (at most BNN_MAX_BOUNDARY_OBJECTS); with
+ * boundary_sdf_at_init the boundary SDF is recomputed from the domain box and the objects */
+bnn_status bnn_flip_set_boundary_objects(bnn_flip* h, const bnn_boundary_object* objs, uint32_t n);
+/* the AdditionalBoundaryObjects of a scene file (GeometryType Box / Sphere, Translation,
+ * RotationAxisAngle, Scale, Animation); *n receives their count, at most cap are written.
+ * bnn_flip_create_from_json applies them. */
+bnn_status bnn_boundaries_from_json(const char* json_text, bnn_boundary_object* objs, uint32_t cap, uint32_t* n);
 
 /* --- lifecycle (ParticleSolver::loadScene, ParticleSolver.cpp:30-168) ------------------------ */
 void        bnn_flip_default_params(bnn_flip_params* p);

@@ -264,8 +264,106 @@ template<class R> struct BoxSDF {
 };
 
 //------------------------------------------------------------------------------------------------
-// Parameters (ParticleSolverData.h:38-175, PIC_Data.h:38-44, Constants.h:31-45)
+// Additional boundary objects (ParticleSolver.cpp:242-298; GeometryObjects.cpp:93-245, 246-266,
+// 379-383, 861-896; Animation.h; CubicSpline.h). A shape's distance: query point through the
+// inverse of translate * rotate(axis, angle) * scale, then the box / unit-sphere distance with
+// bNegativeInside = false, times the uniform scale. Static shapes and the domain box form one
+// union group (min), each animated object a group of its own.
 //------------------------------------------------------------------------------------------------
+struct Shape {
+    int    type  = 0;  // 0 box, 1 unit sphere
+    bool   xform = false;
+    float  inv[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    float  scale   = 1;
+    float  lo[3] = {-1, -1, -1}, hi[3] = {1, 1, 1};
+    template<class R> R sd(const V3<R>& p0) const {
+        V3<R> p = p0;
+        if(xform) {
+            const float* m = inv;
+            p = V3<R>{((R(m[0]) * p0.x + R(m[1]) * p0.y) + R(m[2]) * p0.z) + R(m[3]),
+                      ((R(m[4]) * p0.x + R(m[5]) * p0.y) + R(m[6]) * p0.z) + R(m[7]),
+                      ((R(m[8]) * p0.x + R(m[9]) * p0.y) + R(m[10]) * p0.z) + R(m[11])};
+        }
+        R r;
+        if(type == 0) {
+            BoxSDF<R> b;
+            for(int d = 0; d < 3; ++d) {
+                b.lo[d] = R(lo[d]);
+                b.hi[d] = R(hi[d]);
+            }
+            r = b.signedDistance(p, false);
+        } else {
+            r = -(len3(p.x, p.y, p.z) - R(1));
+        }
+        return R(scale) * r;
+    }
+};
+
+// a clamped cubic (end slopes 0) or piecewise linear interpolant through (x_i, y_i), with the
+// reference's extrapolation (the end polynomials continued); the cubic's second-derivative
+// coefficients come from a dense Gaussian elimination of the end-conditioned system
+struct Interp {
+    std::vector<double> x, y, a, b, c;
+    double              b0 = 0, c0 = 0;
+    void build(const std::vector<double>& X, const std::vector<double>& Y, bool cubic) {
+        x = X;
+        y = Y;
+        const size_t n = x.size();
+        a.assign(n, 0);
+        b.assign(n, 0);
+        c.assign(n, 0);
+        if(cubic) {
+            std::vector<std::vector<double>> A(n, std::vector<double>(n + 1, 0.0));
+            auto slope = [&](size_t i) { return (y[i + 1] - y[i]) / (x[i + 1] - x[i]); };
+            A[0][0] = 2 * (x[1] - x[0]);
+            A[0][1] = x[1] - x[0];
+            A[0][n] = 3 * slope(0);
+            for(size_t i = 1; i + 1 < n; ++i) {
+                A[i][i - 1] = (x[i] - x[i - 1]) / 3;
+                A[i][i]     = 2 * (x[i + 1] - x[i - 1]) / 3;
+                A[i][i + 1] = (x[i + 1] - x[i]) / 3;
+                A[i][n]     = slope(i) - slope(i - 1);
+            }
+            A[n - 1][n - 2] = x[n - 1] - x[n - 2];
+            A[n - 1][n - 1] = 2 * (x[n - 1] - x[n - 2]);
+            A[n - 1][n]     = -3 * slope(n - 2);
+            for(size_t k = 0; k < n; ++k)  // no pivoting: the system is diagonally dominant
+                for(size_t i = k + 1; i < n; ++i) {
+                    const double f = A[i][k] / A[k][k];
+                    for(size_t j = k; j <= n; ++j) A[i][j] -= f * A[k][j];
+                }
+            for(size_t i = n; i-- > 0;) {
+                double v = A[i][n];
+                for(size_t j = i + 1; j < n; ++j) v -= A[i][j] * b[j];
+                b[i] = v / A[i][i];
+            }
+            for(size_t i = 0; i + 1 < n; ++i) {
+                a[i] = (b[i + 1] - b[i]) / (3 * (x[i + 1] - x[i]));
+                c[i] = slope(i) - (2 * b[i] + b[i + 1]) * (x[i + 1] - x[i]) / 3;
+            }
+        } else {
+            for(size_t i = 0; i + 1 < n; ++i) c[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        }
+        b0 = b[0];
+        c0 = c[0];
+        const double h = x[n - 1] - x[n - 2];
+        c[n - 1]       = 3 * a[n - 2] * h * h + 2 * b[n - 2] * h + c[n - 2];
+        a[n - 1]       = 0;
+    }
+    double operator()(double t) const {
+        const size_t n = x.size();
+        size_t       k = 0;
+        while(k + 1 < n && x[k + 1] < t) ++k;  // the last knot below t (0 to the left of x_0)
+        const double h = t - x[k];
+        if(t < x[0]) return (b0 * h + c0) * h + y[0];
+        if(t > x[n - 1]) {
+            const double hr = t - x[n - 1];
+            return (b[n - 1] * hr + c[n - 1]) * hr + y[n - 1];
+        }
+        return ((a[k] * h + b[k]) * h + c[k]) * h + y[k];
+    }
+};
+
 }  // namespace orc
 
 extern "C" {
@@ -289,9 +387,136 @@ struct orc_params {
     float    boundary_reflection_multiplier;   // BoundaryReflectionMultiplier
     int32_t  boundary_sdf_at_init;             // (extension) compute boundary SDF once at scene load
 };
+// a scene's AdditionalBoundaryObjects entry (same layout as the library's bnn_boundary_object)
+struct orc_keyframe {
+    uint32_t frame;
+    float    translation[3];
+    float    rotation_axis_angle[4];  // degrees
+    float    scale;
+    int32_t  has_rotation, has_scale;
+};
+struct orc_boundary_object {
+    int32_t      geometry;  // 0 box, 1 sphere
+    float        box_min[3], box_max[3];
+    float        translation[3];
+    float        rotation_axis_angle[4];  // degrees
+    float        scale;
+    int32_t      has_translation, has_rotation, has_scale;
+    uint32_t     n_keyframes;
+    orc_keyframe keyframes[16];
+    int32_t      periodic;
+    uint32_t     start_frame;
+    int32_t      cubic_translation, cubic_rotation, cubic_scale;
+};
 }
 
 namespace orc {
+
+// one boundary object: its description, its animation channels and its current shape
+struct BoundaryObj {
+    orc_boundary_object d{};
+    bool                dynamic = false, periodic = false;
+    uint32_t            start = 0, maxFrame = 0;
+    struct Key {
+        uint32_t frame;
+        double   t[3], r[4], s;
+    };
+    std::vector<Key> keys;
+    Interp           tr[3], rot[4], sc;
+    Shape            shape;
+
+    void load(const orc_boundary_object& o) {
+        d = o;
+        shape.type = o.geometry == 1 ? 1 : 0;
+        for(int k = 0; k < 3; ++k) {
+            shape.lo[k] = o.box_min[k];
+            shape.hi[k] = o.box_max[k];
+        }
+        const double deg = 3.14159265358979323846 / 180.0;
+        Key          k0{0, {o.translation[0], o.translation[1], o.translation[2]}, {1, 1, 1, 0}, o.has_scale ? (double)o.scale : 1.0};
+        if(o.rotation_axis_angle[3] != 0.f) {
+            for(int k = 0; k < 3; ++k) k0.r[k] = o.rotation_axis_angle[k];
+            k0.r[3] = o.rotation_axis_angle[3] * deg;
+        }
+        keys.assign(1, k0);
+        for(uint32_t i = 0; i < o.n_keyframes; ++i) {
+            const orc_keyframe& f = o.keyframes[i];
+            Key                 k{f.frame, {f.translation[0], f.translation[1], f.translation[2]}, {1, 1, 1, 0}, f.has_scale ? (double)f.scale : 1.0};
+            for(int q = 0; q < 4; ++q) k.r[q] = (f.has_rotation ? (double)f.rotation_axis_angle[q] : (q < 3 ? 1.0 : 0.0)) * deg;
+            if(k.frame == 0) keys[0] = k;
+            else keys.push_back(k);
+        }
+        dynamic  = o.n_keyframes > 0;
+        periodic = o.periodic != 0;
+        start    = o.start_frame;
+        if(dynamic) {
+            std::vector<double> x, t[3], r[4], s;
+            for(const Key& k : keys) {
+                if(periodic && k.frame < start) continue;
+                maxFrame = std::max(maxFrame, k.frame);
+                x.push_back(k.frame);
+                for(int q = 0; q < 3; ++q) t[q].push_back(k.t[q]);
+                for(int q = 0; q < 4; ++q) r[q].push_back(k.r[q]);
+                s.push_back(k.s);
+            }
+            for(int q = 0; q < 3; ++q) tr[q].build(x, t[q], o.cubic_translation != 0);
+            for(int q = 0; q < 4; ++q) rot[q].build(x, r[q], o.cubic_rotation != 0);
+            sc.build(x, s, o.cubic_scale != 0);
+        }
+        shape.xform = false;
+        if(o.has_translation || (o.has_rotation && o.rotation_axis_angle[3] != 0.f) || o.has_scale) place(keys[0], keys[0].s);
+    }
+    // Animation::getTransformation / getUniformScale at (frame + fraction) (Animation.h)
+    void update(uint32_t frame, double fraction) {
+        if(!dynamic) return;
+        Key k = keys[0];
+        if(!(keys.size() == 1 || (periodic && frame < start))) {
+            uint32_t f = frame;
+            if(periodic && f > maxFrame) f = (f - start) % (maxFrame - start) + start;
+            const double x = f + fraction;
+            for(int q = 0; q < 3; ++q) k.t[q] = tr[q](x);
+            for(int q = 0; q < 4; ++q) k.r[q] = rot[q](x);
+            k.s = sc(x);
+        }
+        double us = keys[0].s;
+        if(keys.size() > 1) {
+            uint32_t f = frame;
+            if(periodic && f > maxFrame) f = f % maxFrame;
+            us = sc(f + fraction);
+        }
+        place(k, us);
+    }
+    // the shape's inverse map: M = T R(axis, angle) S inverted by cofactors
+    void place(const Key& k, double uniformScale) {
+        const double n  = std::sqrt(k.r[0] * k.r[0] + k.r[1] * k.r[1] + k.r[2] * k.r[2]);
+        const double ux = n > 0 ? k.r[0] / n : 0, uy = n > 0 ? k.r[1] / n : 0, uz = n > 0 ? k.r[2] / n : 0;
+        const double c = std::cos(k.r[3]), sn = std::sin(k.r[3]);
+        // R = c I + (1 - c) u u^T + sin [u]x
+        double M[3][3] = {{c + (1 - c) * ux * ux, (1 - c) * ux * uy - sn * uz, (1 - c) * ux * uz + sn * uy},
+                          {(1 - c) * uy * ux + sn * uz, c + (1 - c) * uy * uy, (1 - c) * uy * uz - sn * ux},
+                          {(1 - c) * uz * ux - sn * uy, (1 - c) * uz * uy + sn * ux, c + (1 - c) * uz * uz}};
+        for(auto& row : M)
+            for(double& e : row) e *= k.s;
+        const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                           M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+        double I[3][3];
+        for(int i = 0; i < 3; ++i)
+            for(int j = 0; j < 3; ++j) {
+                const int i1 = (j + 1) % 3, i2 = (j + 2) % 3, j1 = (i + 1) % 3, j2 = (i + 2) % 3;
+                I[i][j] = (M[i1][j1] * M[i2][j2] - M[i1][j2] * M[i2][j1]) / det;
+            }
+        for(int i = 0; i < 3; ++i) {
+            double off = 0;
+            for(int j = 0; j < 3; ++j) {
+                shape.inv[4 * i + j] = (float)I[i][j];
+                off -= I[i][j] * k.t[j];
+            }
+            shape.inv[4 * i + 3] = (float)off;
+        }
+        shape.scale = (float)uniformScale;
+        shape.xform = true;
+    }
+};
 
 // uniform in [-1, 1) from (seed, p, q, component): 24 bits of a splitmix64 hash, so the value is
 // exact in float and the same on the device
@@ -331,6 +556,8 @@ public:
     int   n[3];
     R     gravity[3];
     BoxSDF<R> domain;
+    // the scene's additional boundary objects (none: the domain box alone)
+    std::vector<BoundaryObj> bobjs;
     // particles
     std::vector<Vec> pos, vel;
     // transfer scheme: 0 FLIP (FLIP_Solver), 1 APIC (APIC_Solver), 2 AFLIP (AFLIP_Solver); the
@@ -449,26 +676,81 @@ public:
         return Vec{p.x + dt * pvel.x, p.y + dt * pvel.y, p.z + dt * pvel.z};
     }
 
-    // BoundaryObject.cpp:72-90 (constrainToBoundary with velocity)
+    // the boundary objects after ParticleSolver::generateBoundaries (ParticleSolver.cpp:262-287):
+    // group 0 = the domain box (and, with additional objects, every static one in a CSG union),
+    // then one group per animated object
+    int nGroups() const {
+        int n = 1;
+        for(const auto& o : bobjs) n += o.dynamic ? 1 : 0;
+        return n;
+    }
+    R groupSD(int g, const Vec& p) const {
+        if(g == 0) {
+            R sd = domain.signedDistance(p, false);
+            for(const auto& o : bobjs)
+                if(!o.dynamic) sd = std::min(sd, o.shape.sd(p));
+            return sd;
+        }
+        int k = 0;
+        for(const auto& o : bobjs)
+            if(o.dynamic && ++k == g) return o.shape.sd(p);
+        return Huge<R>();
+    }
+    // GeometryObjects.cpp:37-85 on a group
+    Vec groupGrad(int g, const Vec& p) const {
+        if(g == 0 && bobjs.empty()) return domain.grad(p, false);
+        const R dx   = R(1e-4);
+        auto    sd   = [&](R a, R b, R c) { return groupSD(g, Vec{a, b, c}); };
+        R       v000 = sd(p.x - dx, p.y - dx, p.z - dx), v001 = sd(p.x - dx, p.y - dx, p.z + dx);
+        R       v010 = sd(p.x - dx, p.y + dx, p.z - dx), v011 = sd(p.x - dx, p.y + dx, p.z + dx);
+        R       v100 = sd(p.x + dx, p.y - dx, p.z - dx), v101 = sd(p.x + dx, p.y - dx, p.z + dx);
+        R       v110 = sd(p.x + dx, p.y + dx, p.z - dx), v111 = sd(p.x + dx, p.y + dx, p.z + dx);
+        R ddx00 = v100 - v000, ddx10 = v110 - v010, ddx01 = v101 - v001, ddx11 = v111 - v011;
+        R ddy00 = v010 - v000, ddy10 = v110 - v100, ddy01 = v011 - v001, ddy11 = v111 - v101;
+        R ddz00 = v001 - v000, ddz10 = v101 - v100, ddz01 = v011 - v010, ddz11 = v111 - v110;
+        return Vec{(((ddx00 + ddx01) + ddx10) + ddx11) * R(0.25), (((ddy00 + ddy01) + ddy10) + ddy11) * R(0.25),
+                   (((ddz00 + ddz01) + ddz10) + ddz11) * R(0.25)};
+    }
+
+    // BoundaryObject.cpp:72-90 (constrainToBoundary with velocity), every boundary object in turn
+    // (PIC_Solver.cpp:362-366)
     bool constrainToBoundary(Vec& p, Vec& v) const {
-        R phi = domain.signedDistance(p, false);
-        if(phi < 0) {
-            Vec g  = domain.grad(p, false);
-            R   m2 = g.x * g.x + g.y * g.y + g.z * g.z;
-            if(m2 > Tiny<R>()) {
-                R s = std::sqrt(m2);
-                g.x /= s; g.y /= s; g.z /= s;
-                p.x -= phi * g.x; p.y -= phi * g.y; p.z -= phi * g.z;
-                if(prm.reflect_velocity_at_boundary) {
-                    // glm::reflect(I, N) = I - 2 dot(N, I) N
-                    R dn = g.x * v.x + g.y * v.y + g.z * v.z;
-                    R m  = R(prm.boundary_reflection_multiplier);
-                    v    = Vec{(v.x - R(2) * dn * g.x) * m, (v.y - R(2) * dn * g.y) * m, (v.z - R(2) * dn * g.z) * m};
-                    return true;
+        bool changed = false;
+        for(int grp = 0; grp < nGroups(); ++grp) {
+            R phi = groupSD(grp, p);
+            if(phi < 0) {
+                Vec g  = groupGrad(grp, p);
+                R   m2 = g.x * g.x + g.y * g.y + g.z * g.z;
+                if(m2 > Tiny<R>()) {
+                    R s = std::sqrt(m2);
+                    g.x /= s; g.y /= s; g.z /= s;
+                    p.x -= phi * g.x; p.y -= phi * g.y; p.z -= phi * g.z;
+                    if(prm.reflect_velocity_at_boundary) {
+                        // glm::reflect(I, N) = I - 2 dot(N, I) N
+                        R dn = g.x * v.x + g.y * v.y + g.z * v.z;
+                        R m  = R(prm.boundary_reflection_multiplier);
+                        v    = Vec{(v.x - R(2) * dn * g.x) * m, (v.y - R(2) * dn * g.y) * m, (v.z - R(2) * dn * g.z) * m};
+                        changed = true;
+                    }
                 }
             }
         }
-        return false;
+        return changed;
+    }
+    void setBoundaryObjects(const orc_boundary_object* o, uint32_t n) {
+        bobjs.assign(n, BoundaryObj{});
+        for(uint32_t i = 0; i < n; ++i) bobjs[i].load(o[i]);
+        if(prm.boundary_sdf_at_init) computeBoundarySDF();
+    }
+    // PIC_Solver::advanceScene (PIC_Solver.cpp:70-118)
+    void advanceScene(R fraction) {
+        bool dyn = false;
+        for(auto& o : bobjs)
+            if(o.dynamic) {
+                o.update(finishedFrame, (double)fraction);
+                dyn = true;
+            }
+        if(dyn) computeBoundarySDF();
     }
 
     // PIC_Solver.cpp:353-373
@@ -1185,6 +1467,7 @@ public:
         substepCount = 0;
         bool ok      = true;
         while(frameTime < fd) {
+            if(finishedFrame > 0) advanceScene(frameTime / fd);  // PIC_Solver.cpp:279-281
             R sub = timestepCFL();
             R rem = fd - frameTime;
             if(frameTime + sub >= fd) sub = rem;
@@ -1198,13 +1481,16 @@ public:
         return ok;
     }
 
-    // PIC_Solver.cpp:1158-1168 with the domain box as the only boundary object
+    // PIC_Solver.cpp:1158-1168: the smallest distance over the boundary objects
     void computeBoundarySDF() {
+        const int ng = nGroups();
         for(int k = 0; k <= n[2]; ++k)
             for(int j = 0; j <= n[1]; ++j)
                 for(int i = 0; i <= n[0]; ++i) {
                     Vec w{R(i) * dx + bmin[0], R(j) * dx + bmin[1], R(k) * dx + bmin[2]};
-                    boundarySDF(i, j, k) = std::min(Huge<R>(), domain.signedDistance(w, false));
+                    R   m = Huge<R>();
+                    for(int g = 0; g < ng; ++g) m = std::min(m, groupSD(g, w));
+                    boundarySDF(i, j, k) = m;
                 }
     }
 
@@ -1588,6 +1874,32 @@ void orc_update_projected_velocity(void* h, float dt) { dispatch(h, [&](auto& s)
 void orc_compute_changes_grid_velocity(void* h) { dispatch(h, [&](auto& s) { s.computeChangesGridVelocity(); return 0; }); }
 void orc_map_grid_to_particles(void* h) { dispatch(h, [&](auto& s) { s.mapGrid2Particles(); return 0; }); }
 void orc_compute_boundary_sdf(void* h) { dispatch(h, [&](auto& s) { s.computeBoundarySDF(); return 0; }); }
+void orc_set_boundary_objects(void* h, const orc_boundary_object* o, uint32_t n) {
+    dispatch(h, [&](auto& s) {
+        s.setBoundaryObjects(o, n);
+        return 0;
+    });
+}
+void orc_advance_scene(void* h, float fraction) {
+    dispatch(h, [&](auto& s) {
+        s.advanceScene(fraction);
+        return 0;
+    });
+}
+void orc_set_finished_frame(void* h, uint32_t f) {
+    dispatch(h, [&](auto& s) {
+        s.finishedFrame = f;
+        return 0;
+    });
+}
+// the signed distance of group g of the boundary objects at the given points (n x 3)
+void orc_boundary_sd(void* h, int g, const float* p, uint64_t n, double* out) {
+    dispatch(h, [&](auto& s) {
+        using Rt = std::decay_t<decltype(s.dx)>;
+        for(uint64_t i = 0; i < n; ++i) out[i] = (double)s.groupSD(g, orc::V3<Rt>{Rt(p[3 * i]), Rt(p[3 * i + 1]), Rt(p[3 * i + 2])});
+        return 0;
+    });
+}
 float orc_timestep_cfl(void* h) { return dispatch(h, [&](auto& s) { return (float)s.timestepCFL(); }); }
 
 // stable counting sort by linear cell index; out_perm[i] = source index of sorted slot i,

@@ -107,6 +107,10 @@ def lib():
         L.orc_cell_index.argtypes = [vp, u64, vp, f32, vp]
         L.orc_morton_keys.argtypes = [vp, u64, f32, vp]
         L.orc_morton_sort.argtypes = [vp, u64, f32, vp]
+        L.orc_set_boundary_objects.argtypes = [vp, vp, u32]
+        L.orc_advance_scene.argtypes = [vp, f32]
+        L.orc_set_finished_frame.argtypes = [vp, u32]
+        L.orc_boundary_sd.argtypes = [vp, i32, vp, u64, vp]
         _lib = L
     return _lib
 
@@ -284,6 +288,17 @@ class Oracle:
 
     def set_cg(self, tol, max_iters):
         self.L.orc_set_cg(self.h, tol, max_iters)
+
+    def set_boundary_objects(self, arr, n):
+        """arr: a ctypes array of the library's bnn_boundary_object (the oracle's struct has the
+        same layout)."""
+        self.L.orc_set_boundary_objects(self.h, C.cast(arr, C.c_void_p), n)
+
+    def boundary_sd(self, group, pts):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.zeros(pts.shape[0], np.float64)
+        self.L.orc_boundary_sd(self.h, group, ptr(pts), pts.shape[0], ptr(out))
+        return out
 
     def __getattr__(self, name):
         # stage calls: o.map_particles_to_grid(), o.add_gravity(dt), ...

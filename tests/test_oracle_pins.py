@@ -404,3 +404,101 @@ def test_aflip_with_pic_ratio_zero_is_apic(oracle):
         o.map_grid_to_particles()
     assert np.array_equal(a.particles()[1], b.particles()[1])
     assert np.array_equal(a.affine(), b.affine())
+
+
+# --------------------------------------------------------------------------------------------
+# f1 / f3 boundary objects (ParticleSolver.cpp:242-298, GeometryObjects.cpp, Animation.h)
+# --------------------------------------------------------------------------------------------
+def _bnd_oracle(oracle, objs, n=16, precision=64):
+    from banana_amd import boundary_objects
+    params = scenes.base_params(n)
+    params.update(box_min=(-1.0, -1.0, -1.0), box_max=(1.0, 1.0, 1.0), cell_size=2.0 / n)
+    o = oracle.Oracle(params, precision=precision)
+    o.set_boundary_objects(boundary_objects(objs), len(objs))
+    return o
+
+
+def test_sphere_boundary_distance_closed_form(oracle):
+    """A sphere container of radius s at t: distance s - |p - t| (positive inside), whatever the
+    direction; the domain box joins it in one union (the smaller distance counts)."""
+    o = _bnd_oracle(oracle, [dict(geometry="sphere", translation=(0.1, -0.2, 0.05), scale=0.5)])
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-0.9, 0.9, (200, 3)).astype(np.float32).astype(np.float64)  # the points go in as FP32
+    ref = np.minimum(0.5 - np.linalg.norm(p - (0.1, -0.2, 0.05), axis=1), np.min(np.minimum(p + 1, 1 - p), axis=1))
+    # the inverse transform is stored in FP32 (as on the device): agreement to its rounding
+    assert np.allclose(o.boundary_sd(0, p), ref, atol=2e-7)
+
+
+def test_rotated_box_boundary_closed_form(oracle):
+    """A box [-0.5, 0.5] x [-0.2, 0.2] x [-0.3, 0.3] turned 90 degrees about z and moved to t: at
+    p its distance is the unturned box's at R^T (p - t) (inside: the nearest face, positive)."""
+    t = np.array([0.2, 0.1, -0.1])
+    o = _bnd_oracle(oracle, [dict(geometry="box", box_min=(-0.5, -0.2, -0.3), box_max=(0.5, 0.2, 0.3), translation=tuple(t),
+                                  rotation_axis_angle=(0, 0, 1, 90))])
+    p = np.array([[0.2, 0.1, -0.1], [0.25, 0.4, 0.0], [0.05, 0.1, 0.1], [0.2, 0.65, -0.1]])
+    q = p - t
+    q = np.stack([q[:, 1], -q[:, 0], q[:, 2]], axis=1)  # R^T for +90 degrees about z
+    lo, hi = np.array([-0.5, -0.2, -0.3]), np.array([0.5, 0.2, 0.3])
+    inside = np.all((q >= lo) & (q <= hi), axis=1)
+    d_in = np.min(np.minimum(q - lo, hi - q), axis=1)
+    d_out = -np.linalg.norm(q - np.clip(q, lo, hi), axis=1)
+    ref = np.minimum(np.where(inside, d_in, d_out), np.min(np.minimum(p + 1, 1 - p), axis=1))
+    assert np.allclose(o.boundary_sd(0, p), ref, atol=2e-7)
+    # the centre: the nearest face is the short half-width; 0.3 above the centre lies inside only
+    # because the box is turned (its long axis now along y); 0.55 above it lies outside
+    assert np.isclose(ref[0], 0.2) and np.isclose(ref[1], 0.15) and np.isclose(ref[3], -0.05)
+
+
+def test_linear_key_frames_and_periodic_wrap(oracle):
+    """Linear translation key frames at frames 10 and 20 (and frame 0 at the origin): at frame 15 +
+    0.5 the object sits at the interpolated point; periodic from frame 10, frame 27 maps to
+    (27 - 10) % (20 - 10) + 10 = 17 (Animation::getTransformation), and before frame 10 the object
+    keeps its static transform. (With periodic key frames the frame-0 key frame is left out of the
+    channels, so the key frames carry the scale themselves.)"""
+    an = dict(periodic=True, start_frame=10, cubic_translation=False, cubic_rotation=False, cubic_scale=False,
+              keyframes=[dict(frame=10, translation=(0.0, 0.0, 0.0), scale=0.3), dict(frame=20, translation=(0.4, 0.0, 0.0), scale=0.3)])
+    o = _bnd_oracle(oracle, [dict(geometry="sphere", scale=0.3, animation=an)])
+    centre_probe = lambda x: o.boundary_sd(1, np.array([[x, 0.0, 0.0]]))[0]
+    for frame, frac, cx in ((15, 0.5, 0.22), (27, 0.0, 0.28), (5, 0.25, 0.0)):
+        o.set_finished_frame(frame)
+        o.advance_scene(frac)
+        # the distance is largest (0.3) at the centre, and 0.3 - |x - cx| along the x axis
+        assert np.isclose(centre_probe(cx), 0.3, atol=1e-7), (frame, frac)
+        assert np.isclose(centre_probe(cx + 0.1), 0.2, atol=1e-7)
+
+
+def test_cubic_key_frames_interpolate_with_flat_ends(oracle):
+    """Cubic translation channel (clamped: end slopes 0): the key frames (0 at the origin, then 5, 9,
+    20) are hit exactly, the motion leaves the first key frame with zero velocity, and the velocity
+    is continuous across an interior key frame."""
+    an = dict(keyframes=[dict(frame=5, translation=(0.2, 0.0, 0.0), scale=0.25), dict(frame=9, translation=(0.5, 0.0, 0.0), scale=0.25),
+                         dict(frame=20, translation=(0.6, 0.0, 0.0), scale=0.25)])
+    o = _bnd_oracle(oracle, [dict(geometry="sphere", scale=0.25, animation=an)])
+
+    def centre(t):
+        frame = int(np.floor(t))
+        o.set_finished_frame(frame)
+        o.advance_scene(np.float32(t - frame))
+        # a probe left of the centre on the x axis: distance 0.25 - (c - x)
+        x = -0.45
+        return x + 0.25 - o.boundary_sd(1, np.array([[x, 0.0, 0.0]]))[0]
+
+    for t, c in ((0, 0.0), (5, 0.2), (9, 0.5), (20, 0.6)):
+        assert np.isclose(centre(t), c, atol=1e-6), t
+    h = 0.0625  # exact in FP32
+    assert abs(centre(h) - centre(0)) < 1e-3 * h  # zero start velocity (the offset is O(h^2))
+    left, right = (centre(9) - centre(9 - h)) / h, (centre(9 + h) - centre(9)) / h
+    assert abs(left - right) < 5e-3 and abs(left) > 0.02  # C1 at the key frame, and moving
+
+
+def test_particle_pushed_back_into_sphere_container(oracle):
+    """A particle outside a sphere container of radius 0.4 ends on its surface after one move (zero
+    grid velocity): constrainToBoundary moves it by -phi along the numerical gradient."""
+    o = _bnd_oracle(oracle, [dict(geometry="sphere", scale=0.4)])
+    pos = np.array([[0.5, 0.1, -0.2], [0.0, 0.0, 0.1]], np.float64)
+    o.set_particles(pos, np.zeros_like(pos))
+    o.move_particles(np.float32(1e-3))
+    p, _ = o.particles()
+    assert np.isclose(np.linalg.norm(p[0]), 0.4, atol=1e-6)
+    assert np.allclose(p[0] / np.linalg.norm(p[0]), pos[0] / np.linalg.norm(pos[0]), atol=1e-6)
+    assert np.allclose(p[1], pos[1])  # inside: untouched

@@ -67,7 +67,8 @@ def test_comments_and_defaults(bnn):
 
 @pytest.mark.parametrize("mutate,status", [
     (lambda d: d["GlobalParameters"].update(Solver="SPH_3DSolver"), 5),
-    (lambda d: d.update(AdditionalBoundaryObjects=[{"GeometryType": "Box"}]), 5),
+    (lambda d: d.update(AdditionalBoundaryObjects=[{"GeometryType": "Torus"}]), 5),
+    (lambda d: d.update(AdditionalBoundaryObjects=[{"GeometryType": "Box", "RotationEulerAngles": [0, 0, 90]}]), 5),
     (lambda d: d["ParticleGenerators"].append({"GeometryType": "Torus"}), 5),
     (lambda d: d["SimulationParameters"].pop("SimulationDomainBox"), 1),
     (lambda d: d.pop("ParticleGenerators"), 1),
@@ -110,3 +111,30 @@ def test_solver_names_select_the_transfer_scheme(bnn):
         st, p, s, warn = parse(d)
         assert st == 0 and p.transfer_scheme == scheme, name
         assert p.correct_position == 1 and p.repulsive_force_stiffness == 20.0
+
+
+def test_boundary_objects_from_json(bnn):
+    """AdditionalBoundaryObjects as the reference's scene files write them (FLIP3D_BallDrop.json,
+    FLIP3D_RotationalBoxBoundary.json): geometry, transform keys, animation key frames (angles in
+    degrees), the Periodic / StartFrame pair and the interpolation flags."""
+    import banana_amd
+    d = json.loads(json.dumps(DAM))
+    d["AdditionalBoundaryObjects"] = [
+        {"GeometryType": "Sphere", "UniqueName": "SphereBoundary", "CSGOperator": "Intersection", "Translation": [0, 0, 0], "Scale": 1},
+        {"GeometryType": "Box", "UniqueName": "BoxBoundary", "CSGOperator": "Intersection", "BoxMin": [-1, -0.5, -1],
+         "BoxMax": [1, 0.5, 1], "Animation": {"Periodic": True, "StartFrame": 50, "CubicInterpolationRotation": False,
+                                              "KeyFrames": [{"Frame": 50, "RotationAxisAngle": [1, 0, 0, 0]},
+                                                            {"Frame": 150, "RotationAxisAngle": [1, 0, 0, 180]},
+                                                            {"Frame": 250, "RotationAxisAngle": [1, 0, 0, 360]}]}}]
+    arr, n = banana_amd.boundaries_from_json(json.dumps(d))
+    assert n == 2
+    s, b = arr[0], arr[1]
+    assert s.geometry == 1 and s.has_translation == 1 and s.has_scale == 1 and s.scale == 1.0 and s.n_keyframes == 0
+    assert b.geometry == 0 and list(b.box_min) == [-1, -0.5, -1] and list(b.box_max) == [1, 0.5, 1]
+    assert b.n_keyframes == 3 and b.periodic == 1 and b.start_frame == 50
+    assert (b.cubic_translation, b.cubic_rotation, b.cubic_scale) == (1, 0, 1)
+    assert [k.frame for k in b.keyframes[:3]] == [50, 150, 250]
+    assert [k.rotation_axis_angle[3] for k in b.keyframes[:3]] == [0, 180, 360] and all(k.has_rotation for k in b.keyframes[:3])
+    # a scene with them still parses (the boundary objects are not part of the parameters)
+    st, p, sp, warn = parse(d)
+    assert st == 0
