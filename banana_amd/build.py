@@ -6,8 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "flip_solver.hip")
 OUT = os.path.join(HERE, "libbnn_flip.so")
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("flip_solver.hip", "flip_kernels.hpp", "flip_device.hpp", "flip_serving.hpp",
-                                                       "scene_json.hpp")] + [
+DEPS = sorted(os.path.join(HERE, "csrc", f) for f in os.listdir(os.path.join(HERE, "csrc")) if f.endswith((".hip", ".hpp"))) + [
     os.path.join(os.path.dirname(HERE), "include", "bnn_flip.h")]
 
 HIPCC_FLAGS = [

@@ -22,7 +22,17 @@ struct Blk {
     uint32_t id, n;  // this workgroup's index among its scene's workgroups, and their count
 };
 
-template<class K> __global__ void __launch_bounds__(K::kThreads) k_one(K k) { k(Blk{blockIdx.x, gridDim.x}); }
+// Single scene. The functor is the kernel's only argument, so it lies at offset 0 of the
+// kernel-argument segment, and it is called in place there (constant memory, read with scalar loads
+// even where it indexes one of its arrays at run time, as k_move does with the boundary shapes). Used
+// as the by-value parameter instead, such a functor is first copied to per-lane scratch. The call is
+// inlined by force here and in k_many: the larger functors exceed the inliner's threshold, and a real
+// call doubles their registers.
+template<class K> __global__ void __launch_bounds__(K::kThreads) k_one(K k) {
+    (void)k;
+    const K* kp = (const K*)__builtin_amdgcn_kernarg_segment_ptr();  // an address-space conversion
+    [[clang::always_inline]] (*kp)(Blk{blockIdx.x, gridDim.x});
+}
 
 // Scenes per batched launch at most (the scene search below is two rounds of 64 lanes)
 constexpr uint32_t kMaxLaunchScenes = 4096;
@@ -61,7 +71,7 @@ __global__ void __launch_bounds__(K::kThreads) k_many(const K* __restrict__ ks, 
         while(s + 1u < nScenes && first[s + 1u] <= i) ++s;
         s                    = __builtin_amdgcn_readfirstlane(s);
         const uint32_t local = i - first[s], n = count[s];
-        if(local < n) ks[s](Blk{local, n});  // else: padding up to the next scene's multiple of 8
+        if(local < n) [[clang::always_inline]] ks[s](Blk{local, n});  // else: padding up to the next scene's multiple of 8
         if(r + 1u < r1) __syncthreads();      // the next item may reuse this item's LDS
     }
 }
