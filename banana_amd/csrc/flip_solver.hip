@@ -985,6 +985,19 @@ public:
 
     unsigned redBlocks() const { return std::min<unsigned>(std::max<uint32_t>(nSlots, 1u), (unsigned)kMaxRedBlocks); }
 
+    // Iterations to queue after a poll (the stop flag ends the ones not needed, but each still costs a
+    // few launches): as many as a geometric fit of max|r| from iteration 0 (res0) to now (res after
+    // iter iterations) says remain until max|r| < tol, at least 1 and at most pcgChunkMax; doubling the
+    // last chunk while the fit has no progress to go by.
+    uint32_t pcgNextChunk(float res0, float res, float tol, uint32_t iter, uint32_t last) const {
+        const uint32_t doubled = std::min<uint32_t>(last * 2, pcgChunkMax);
+        if(iter == 0 || !(res > 0.f) || !(res < res0) || !(tol > 0.f) || !(tol < res)) return doubled;
+        const double rate = std::log((double)res / (double)res0) / (double)iter;  // log of the mean reduction per iteration
+        const double need = std::ceil(std::log((double)tol / (double)res) / rate);
+        if(!(need >= 1.0)) return 1u;
+        return (uint32_t)std::min<double>(need, (double)pcgChunkMax);
+    }
+
     // FP32 PCG on b -> x (PCGSolver.cpp:143-199); returns success
     Co<bool> pcg32(const float* b, float relTol, uint32_t maxIter, uint32_t& iters, float& res, float& bmax) {
         PcgState*      dst = pst.as<PcgState>();
@@ -1013,10 +1026,16 @@ public:
         if(!jac) zr(nullptr);
         launch(k_pcg_init_fin{dst, rz, nrz, (const double*)dpb, (int)G, relTol, maxIter}, 1);
         uint32_t chunk = pcgChunkFirst, launched = 0;
+        float    res0  = -1.f;  // max|r| at iteration 0 (= max|b|, x starts at 0)
         for(;;) {
             readback(hostSt, dst, sizeof(PcgState));
             co_await syncPoint();
             if(hostSt->done) break;
+            if(res0 < 0.f) {
+                res0 = hostSt->res;
+            } else {
+                chunk = pcgNextChunk(res0, hostSt->res, hostSt->tol, hostSt->iter, chunk);
+            }
             // never queue more iterations than can remain
             const uint32_t left = hostSt->maxIter > hostSt->iter ? hostSt->maxIter - hostSt->iter : 1u;
             const uint32_t n    = std::min(chunk, left);
@@ -1033,7 +1052,6 @@ public:
                 std::swap(sA, sB);
             }
             launched += n;
-            chunk = std::min<uint32_t>(chunk * 2, pcgChunkMax);
             if(launched > maxIter + 256) throw Error(BNN_ERR_DEVICE, "PCG device state stopped advancing");
         }
         iters = hostSt->iter;

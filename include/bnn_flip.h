@@ -418,7 +418,8 @@ bnn_status  bnn_flip_batch_pressure_projection(bnn_flip_batch* b, bnn_flip* cons
 typedef struct bnn_batch_shape {
     int32_t  p2g_segment;     /* z nodes per P2G thread: 4, 8 or 16 */
     uint32_t pcg_chunk_first; /* PCG iterations queued before the first stop-flag poll */
-    uint32_t pcg_chunk_max;   /* ... and at most per poll after it (the chunk doubles up to this) */
+    uint32_t pcg_chunk_max;   /* ... and at most per poll after it (later chunks: the iterations a geometric
+                                 fit of max|r| so far says remain) */
     uint32_t items_per_group; /* scene blocks one workgroup of a batched launch runs in turn, at most */
     uint32_t min_groups;      /* ... while keeping at least this many workgroups (0: 16384) */
     int32_t  graphs;          /* replay repeated launch sequences from hipGraphs */
