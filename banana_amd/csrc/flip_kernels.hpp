@@ -39,6 +39,7 @@ __device__ __forceinline__ void vel_from_grid(const VelGrid& G, const GridGeom& 
 
 struct k_move {
     static constexpr int kThreads = kBlock;
+    static constexpr bool kByAddress = true;
     Particles P;
     uint32_t n;
     VelGrid G;
@@ -830,118 +831,143 @@ struct k_p2g_col {
     }
 };
 
-//==================================================================================================
-// extrapolateVelocity: one Jacobi layer of one component (PIC_Solver.cpp:992-1090), in place.
-// An invalid interior face takes the mean of its valid neighbours (order +x,-x,+y,-y,+z,-z) and
-// becomes valid and extrapolated. `val` holds stamps: 0 invalid, 1 valid on entry, l + 2 for a face
-// filled in layer l, and layer l reads the neighbours valid before it (stamps 1 .. l + 1). A face
-// filled in this very layer reads as 0 or l + 2 to its neighbours, so concurrent writes never feed the
-// same layer and faces valid before a layer never change: the values equal the reference's
-// two-buffer Jacobi bit for bit without copying the faces a layer leaves alone.
-// Layer 0 also clears the extrapolated flag of every face it does not fill.
-//==================================================================================================
-// Faces that no layer can reach: `near` marks the 8^3 face bricks that hold a valid face or touch
-// one that does (26 neighbours). A face outside them is at least 9 face steps from every valid face,
-// so four layers cannot fill it: it is skipped (layer 0 still clears its extrapolated flag).
-struct k_extrap_brick_valid {
-    static constexpr int kThreads = 256;
-    const uint8_t* __restrict__ val;
-    int sx;
-    int sy;
-    int sz;
-    uint8_t* __restrict__ has;
-    __device__ void operator()(Blk blk) const {
-        const int nbx = (sx + 7) / 8, nby = (sy + 7) / 8;
-        const int b = blk.id, bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-        int       any = 0;
-        for(int l = threadIdx.x; l < 512; l += 256) {
-            const int i = bx * 8 + (l & 7), j = by * 8 + ((l >> 3) & 7), k = bz * 8 + (l >> 6);
-            if(i < sx && j < sy && k < sz && val[lin3(i, j, k, sx, sy)] != 0) any = 1;
-        }
-        any = __syncthreads_or(any);
-        if(threadIdx.x == 0) has[b] = (uint8_t)any;
-    }
-};
-struct k_extrap_brick_near {
-    static constexpr int kThreads = kBlock;
-    const uint8_t* __restrict__ has;
-    int nbx;
-    int nby;
-    int nbz;
-    uint8_t* __restrict__ near;
-    __device__ void operator()(Blk blk) const {
-        const int b = blk.id * blockDim.x + threadIdx.x;
-        if(b >= nbx * nby * nbz) return;
-        const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-        uint8_t   v = 0;
-        for(int dz = -1; dz <= 1; ++dz)
-            for(int dy = -1; dy <= 1; ++dy)
-                for(int dx = -1; dx <= 1; ++dx) {
-                    const int x = bx + dx, y = by + dy, z = bz + dz;
-                    if(x >= 0 && y >= 0 && z >= 0 && x < nbx && y < nby && z < nbz) v |= has[(z * nby + y) * nbx + x];
-                }
-        near[b] = v;
-    }
-};
+// Tile (tx, ty, tz) of an ntx x nty x ntz grid of tiles for workgroup blk: blocks are dispatched
+// round-robin over the 8 XCDs, so each XCD is handed a contiguous run of tiles and shares the reads
+// of neighbouring tiles through its own L2.
+__device__ __forceinline__ void tile_of_block(Blk blk, int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
+    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blk.id, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
+    const uint32_t t = xcd * q + min(xcd, r) + b / 8u;
+    tx = (int)(t % (uint32_t)ntx);
+    ty = (int)((t / (uint32_t)ntx) % (uint32_t)nty);
+    tz = (int)(t / (uint32_t)(ntx * nty));
+}
 
-// one block per 64 x 4 x 1 run of faces: blocks numbered x-fastest over (ceil(sx / 64), ceil(sy / 4), sz)
-struct k_extrap_stamp {
-    static constexpr int kThreads = 256;
+//==================================================================================================
+// extrapolateVelocity of one component (PIC_Solver.cpp:985-1090): 4 Jacobi layers. Layer l fills an
+// invalid face that is not on the array's outer layer with the mean of its neighbours valid before
+// the layer (order +x, -x, +y, -y, +z, -z) and makes it valid and extrapolated.
+//
+// One workgroup per 8^3 brick of faces. A face changes only through faces within 4 face steps, so
+// the brick and a halo of 4 faces on every side (a 16^3 tile) are read into LDS once and the four
+// layers run there, in place with layer stamps: 0 invalid, 1 valid on entry, l + 2 filled in layer l;
+// layer l reads the stamps 1 .. l + 1, so a face filled in the same layer is never read by it and the
+// values equal the two-buffer Jacobi's bit for bit. A halo face at distance d from the brick can go
+// wrong (its own neighbours beyond the tile are missing) only from layer 4 - d on, too late to reach
+// the brick. Faces on the tile's outer shell are never computed.
+//
+// Other workgroups read this brick's faces as their halo while it runs, so what they read as valid
+// on entry is never changed: a face filled here gets its value, ext = 1 and val = gen (2 or 3, the
+// host alternates it per call), which this launch's loaders read as invalid; the other generation,
+// left by the previous call, reads as valid and is rewritten as 1. Every other face of the brick gets
+// ext = 0. Any nonzero val therefore means valid.
+//==================================================================================================
+struct k_extrap_tile {
+    static constexpr int kThreads = 512;
     float* __restrict__ a;
     uint8_t* __restrict__ val;
     uint8_t* __restrict__ ext;
-    const uint8_t* __restrict__ near;
     int sx;
     int sy;
     int sz;
-    int layer;
+    uint8_t gen;
     __device__ void operator()(Blk blk) const {
-        const uint32_t gx = (uint32_t)(sx + 63) / 64u, gy = (uint32_t)(sy + 3) / 4u, r = blk.id / gx;
-        const int      i = (int)((blk.id - r * gx) * 64u + (threadIdx.x & 63u)), j = (int)((r % gy) * 4u + (threadIdx.x >> 6));
-        const int      k = (int)(r / gy);
-        if(i >= sx || j >= sy) return;
-        const size_t t = lin3(i, j, k, sx, sy);
-        if(!near[((k >> 3) * ((sy + 7) / 8) + (j >> 3)) * ((sx + 7) / 8) + (i >> 3)]) {
-            if(layer == 0) ext[t] = 0;
-            return;
+        __shared__ float   as[4096];  // 16^3, x fastest
+        __shared__ uint8_t ss[4096];
+        int bx, by, bz;
+        tile_of_block(blk, (sx + 7) / 8, (sy + 7) / 8, (sz + 7) / 8, bx, by, bz);
+        const int    t = threadIdx.x, x = t & 15, y = (t >> 4) & 15, z0 = t >> 8;  // faces z0, z0 + 2, .., z0 + 14
+        const int    gi = bx * 8 - 4 + x, gj = by * 8 - 4 + y;
+        const bool   inXY = gi >= 0 && gi < sx && gj >= 0 && gj < sy;
+        const bool   brickXY = x >= 4 && x < 12 && y >= 4 && y < 12;
+        const size_t sxy = (size_t)sx * (size_t)sy;
+        uint8_t      v0[8];  // val on entry
+        size_t       off[8];
+        int          anyValid = 0, anyOpen = 0;
+#pragma unroll
+        for(int s = 0; s < 8; ++s) {
+            const int z = z0 + 2 * s, gk = bz * 8 - 4 + z;
+            off[s]      = (size_t)gk * sxy + (size_t)gj * (size_t)sx + (size_t)gi;
+            v0[s]       = (inXY && gk >= 0 && gk < sz) ? val[off[s]] : (uint8_t)0;
         }
-        bool filled = false;
-        if(val[t] == 0) {
-            if(i >= 1 && i <= sx - 2 && j >= 1 && j <= sy - 2 && k >= 1 && k <= sz - 2) {
-                const size_t  sxy = (size_t)sx * sy;
+#pragma unroll
+        for(int s = 0; s < 8; ++s) {
+            const int     z = z0 + 2 * s, q = (z << 8) | (y << 4) | x;
+            const uint8_t st = (v0[s] != 0 && v0[s] != gen) ? 1 : 0;
+            const int     gk = bz * 8 - 4 + z;
+            ss[q]            = st;
+            anyValid |= st;
+            if(!st && brickXY && z >= 4 && z < 12 && gk < sz && inXY) anyOpen = 1;
+        }
+        const bool work = __syncthreads_or(anyValid) != 0 && __syncthreads_or(anyOpen) != 0;
+        if(work) {
+            // the values of the faces valid on entry (the others are written before they are read)
+#pragma unroll
+            for(int s = 0; s < 8; ++s) {
+                const int z = z0 + 2 * s, q = (z << 8) | (y << 4) | x;
+                if(v0[s] != 0 && v0[s] != gen) as[q] = a[off[s]];
+            }
+            __syncthreads();
+            // faces that can be filled: not on the tile's outer shell, not on the array's outer layer
+            uint32_t open = 0;
+#pragma unroll
+            for(int s = 0; s < 8; ++s) {
+                const int z = z0 + 2 * s, gk = bz * 8 - 4 + z;
+                if(x >= 1 && x <= 14 && y >= 1 && y <= 14 && z >= 1 && z <= 14 && gi >= 1 && gi <= sx - 2 && gj >= 1 && gj <= sy - 2 &&
+                   gk >= 1 && gk <= sz - 2 && !(v0[s] != 0 && v0[s] != gen))
+                    open |= 1u << s;
+            }
+            for(int layer = 0; layer < 4; ++layer) {
                 const uint8_t lim = (uint8_t)(layer + 1);
-                auto          ok  = [&](size_t n) {
-                    const uint8_t s = val[n];
-                    return s >= 1 && s <= lim;
-                };
-                float    sum   = 0.f;
-                uint32_t count = 0;
-                if(ok(t + 1)) { sum += a[t + 1]; ++count; }
-                if(ok(t - 1)) { sum += a[t - 1]; ++count; }
-                if(ok(t + sx)) { sum += a[t + sx]; ++count; }
-                if(ok(t - sx)) { sum += a[t - sx]; ++count; }
-                if(ok(t + sxy)) { sum += a[t + sxy]; ++count; }
-                if(ok(t - sxy)) { sum += a[t - sxy]; ++count; }
-                if(count > 0) {
-                    a[t]   = sum / (float)count;
-                    val[t] = (uint8_t)(layer + 2);
-                    ext[t] = 1;
-                    filled = true;
+                float         res[8];
+                uint32_t      fill = 0;
+                // all reads of the layer first (a face this layer fills is never read by it), then the writes
+#pragma unroll
+                for(int s = 0; s < 8; ++s) {
+                    res[s] = 0.f;
+                    if(!(open & (1u << s))) continue;
+                    const int q   = ((z0 + 2 * s) << 8) | (y << 4) | x;
+                    float     sum = 0.f;  // starts at +0 and never becomes -0, so adding +0 for a neighbour not taken is exact
+                    uint32_t  count = 0;
+                    const int nb[6] = {q + 1, q - 1, q + 16, q - 16, q + 256, q - 256};
+#pragma unroll
+                    for(int e = 0; e < 6; ++e) {
+                        const uint8_t m  = ss[nb[e]];
+                        const float   av = as[nb[e]];
+                        const bool    ok = m >= 1 && m <= lim;
+                        sum += ok ? av : 0.f;
+                        count += ok ? 1u : 0u;
+                    }
+                    if(count > 0) {
+                        res[s] = sum / (float)count;
+                        fill |= 1u << s;
+                    }
                 }
+#pragma unroll
+                for(int s = 0; s < 8; ++s) {
+                    if(!(fill & (1u << s))) continue;
+                    const int q = ((z0 + 2 * s) << 8) | (y << 4) | x;
+                    as[q]       = res[s];
+                    ss[q]       = (uint8_t)(layer + 2);
+                }
+                open &= ~fill;
+                __syncthreads();
             }
         }
-        if(layer == 0 && !filled) ext[t] = 0;
-    }
-};
-
-// after the last layer: stamps back to valid flags
-struct k_extrap_finish {
-    static constexpr int kThreads = kBlock;
-    uint8_t* __restrict__ val;
-    size_t n;
-    __device__ void operator()(Blk blk) const {
-        const size_t t = (size_t)blk.id * blockDim.x + threadIdx.x;
-        if(t < n && val[t] > 1) val[t] = 1;
+        if(!brickXY || !inXY) return;
+#pragma unroll
+        for(int s = 0; s < 8; ++s) {
+            const int z = z0 + 2 * s, gk = bz * 8 - 4 + z, q = (z << 8) | (y << 4) | x;
+            if(z < 4 || z >= 12 || gk >= sz) continue;
+            const size_t  o  = off[s];
+            const uint8_t st = work ? ss[q] : (uint8_t)((v0[s] != 0 && v0[s] != gen) ? 1 : 0);
+            ext[o]           = st >= 2 ? 1 : 0;
+            if(st >= 2) {
+                a[o]   = as[q];
+                val[o] = gen;
+            } else if(st == 1 && v0[s] != 1) {
+                val[o] = 1;
+            }
+        }
     }
 };
 
@@ -1134,14 +1160,6 @@ struct k_boundary_sdf {
 //==================================================================================================
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
 constexpr int kTileThreads = kTileX * kTileY * kTileZ;  // 256
-
-__device__ __forceinline__ void tile_of_block(Blk blk, int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
-    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blk.id, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
-    const uint32_t t = xcd * q + min(xcd, r) + b / 8u;
-    tx = (int)(t % (uint32_t)ntx);
-    ty = (int)((t / (uint32_t)ntx) % (uint32_t)nty);
-    tz = (int)(t / (uint32_t)(ntx * nty));
-}
 
 struct k_fluid_sdf {
     static constexpr int kThreads = kTileThreads;

@@ -274,7 +274,8 @@ public:
     size_t nU = 0, nV = 0, nW = 0, nC = 0, nN = 0;
     DevBuf u, v, w, ut, vt, wt, uo, vo, wo, wu, wv, ww;
     DevBuf vu, vv, vw, eu, ew;
-    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters, extBricks;
+    DevBuf phi, bsdf, cellStart, cellCount, scanPart, counters;
+    uint8_t extGen = 2;  // the valid-flag value of faces the last extrapolation filled
     DevBuf occ;  // per 8^3 cell brick: holds a particle (binning), so P2G can skip empty neighbourhoods
     bool     weightsDirty    = true;   // boundary SDF changed since the weights were computed
     bool     weightsComputed = false;  // computeCellWeights has run at least once
@@ -402,7 +403,6 @@ public:
         for(auto* b : {&v, &vt, &vo, &wv}) b->reserve(nV * 4);
         for(auto* b : {&w, &wt, &wo, &ww}) b->reserve(nW * 4);
         // face-brick masks of the extrapolation (valid / near a valid face), per component in turn
-        extBricks.reserve(2 * (size_t)((g.nx + 8) / 8 + 1) * ((g.ny + 8) / 8 + 1) * ((g.nz + 8) / 8 + 1));
         vu.reserve(nU);
         vv.reserve(nV);
         vw.reserve(nW);
@@ -706,26 +706,20 @@ public:
         ktEnd(BNN_KT_P2G);
     }
 
-    // PIC_Solver.cpp:985-1090: 4 Jacobi layers per component, in place with layer stamps
+    // PIC_Solver.cpp:985-1090: the 4 Jacobi layers of each component in one launch over its 8^3 face
+    // bricks. Faces filled by this call are marked valid with the generation value extGen (2 or 3,
+    // alternating per call), which the same launch reads as invalid on entry.
     void extrapolateVelocity() {
         struct C {
             DevBuf *a, *val, *ext;
             int     sx, sy, sz;
         } comps[3] = {{&u, &vu, &eu, g.nx + 1, g.ny, g.nz}, {&v, &vv, &ev_, g.nx, g.ny + 1, g.nz}, {&w, &vw, &ew, g.nx, g.ny, g.nz + 1}};
+        extGen = extGen == 2 ? 3 : 2;
         for(auto& c : comps) {
-            const size_t n   = (size_t)c.sx * c.sy * c.sz;
-            const int    nbx = (c.sx + 7) / 8, nby = (c.sy + 7) / 8, nbz = (c.sz + 7) / 8, nbk = nbx * nby * nbz;
-            uint8_t*     has = extBricks.as<uint8_t>(), *near = has + nbk;
-            launch(k_extrap_brick_valid{(const uint8_t*)c.val->as<uint8_t>(), c.sx, c.sy,
-                               c.sz, has}, nbk);
-            launch(k_extrap_brick_near{(const uint8_t*)has, nbx, nby, nbz, near}, blocks(nbk));
-            for(int layer = 0; layer < 4; ++layer) {
-                ktBegin(BNN_KT_EXTRAP);
-                launch(k_extrap_stamp{c.a->as<float>(), c.val->as<uint8_t>(), c.ext->as<uint8_t>(), (const uint8_t*)near, c.sx, c.sy, c.sz, layer},
-                       (uint32_t)(((c.sx + 63) / 64) * ((c.sy + 3) / 4) * c.sz));
-                ktEnd(BNN_KT_EXTRAP);
-            }
-            launch(k_extrap_finish{c.val->as<uint8_t>(), n}, blocks(n));
+            const uint32_t nbk = (uint32_t)(((c.sx + 7) / 8) * ((c.sy + 7) / 8) * ((c.sz + 7) / 8));
+            ktBegin(BNN_KT_EXTRAP);
+            launch(k_extrap_tile{c.a->as<float>(), c.val->as<uint8_t>(), c.ext->as<uint8_t>(), c.sx, c.sy, c.sz, extGen}, nbk);
+            ktEnd(BNN_KT_EXTRAP);
         }
     }
 
@@ -1529,7 +1523,12 @@ public:
         }
         HIP_OK(hipMemcpyAsync(dst, src->p, bytes, hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
+        if(isValidField(f)) {  // any nonzero flag is valid (the extrapolation marks its faces 2 or 3)
+            uint8_t* d = static_cast<uint8_t*>(dst);
+            for(size_t i = 0; i < bytes; ++i) d[i] = d[i] != 0;
+        }
     }
+    static bool isValidField(int f) { return f == BNN_FIELD_VALID_U || f == BNN_FIELD_VALID_V || f == BNN_FIELD_VALID_W; }
 
     void setGrid(int f, const void* src) {
         if(!src) throw Error(BNN_ERR_INVALID_ARG, "null source");
@@ -1551,6 +1550,12 @@ public:
             case BNN_FIELD_PHI: dst = &phi; bytes = nC * 4; break;
             case BNN_FIELD_BOUNDARY_SDF: dst = &bsdf; bytes = nN * 4; weightsDirty = true; bsdfNonzero = true; break;
             default: throw Error(BNN_ERR_INVALID_ARG, "field cannot be set");
+        }
+        std::vector<uint8_t> flags;
+        if(isValidField(f)) {  // stored as 0 / 1
+            flags.resize(bytes);
+            for(size_t i = 0; i < bytes; ++i) flags[i] = static_cast<const uint8_t*>(src)[i] != 0;
+            src = flags.data();
         }
         HIP_OK(hipMemcpyAsync(dst->p, src, bytes, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));

@@ -62,7 +62,7 @@ KERNEL_BYTES = {
     "mg_restrict": lambda Np, F, C, V, A: 24.5 * A,         # b, diag, 3 couplings, x in; coarse b out
     "mg_prolong": lambda Np, F, C, V, A: 12.5 * A,          # x, 1/diag, coarse x in; x out
     "mg_post": lambda Np, F, C, V, A: 28 * A,               # 3 couplings, b, 1/diag, x in; x out
-    "extrap": lambda Np, F, C, V, A: 5 * F / 3,             # one layer over one component: value + flag
+    "extrap": lambda Np, F, C, V, A: 11 * F / 3,            # one component, 4 layers: value + valid in; value, valid, extrapolated out
 }
 
 

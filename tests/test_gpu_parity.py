@@ -114,14 +114,18 @@ def test_p2g(oracle, bnn):
         assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, np.abs(b).max()), c
 
 
-def test_extrapolate_bit_exact(oracle, bnn):
-    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, vel_sigma=0.5)
+@pytest.mark.parametrize("n,calls", [(32, 1), (32, 3), (27, 2)])
+def test_extrapolate_bit_exact(oracle, bnn, n, calls):
+    """Repeated calls extend the valid region by 4 more faces each (the faces one call filled are
+    valid for the next); 27 cells leaves partial face bricks at the far ends."""
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=n, sphere=True, vel_sigma=0.5)
     o.map_particles_to_grid()
     for c, f, fv in (("u", oracle.F_U, oracle.F_VALID_U), ("v", oracle.F_V, oracle.F_VALID_V), ("w", oracle.F_W, oracle.F_VALID_W)):
         g.set_grid(c, o.field(f))
         g.set_grid("valid_" + c, o.field(fv))
-    g.extrapolate_velocity()
-    o.extrapolate_velocity()
+    for _ in range(calls):
+        g.extrapolate_velocity()
+        o.extrapolate_velocity()
     for c, f, fv, fe in (("u", oracle.F_U, oracle.F_VALID_U, oracle.F_EXTRAP_U), ("v", oracle.F_V, oracle.F_VALID_V, oracle.F_EXTRAP_V),
                          ("w", oracle.F_W, oracle.F_VALID_W, oracle.F_EXTRAP_W)):
         assert np.array_equal(g.grid(c), o.field(f).astype(np.float32)), c
