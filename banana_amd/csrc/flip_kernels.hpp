@@ -705,12 +705,54 @@ __device__ __forceinline__ float stencil_weight_interior(float q, float target) 
     return s == target ? 1.0f - f : (s == target - 1.0f ? f : 0.f);
 }
 
+// A particle binned to row z, seen from column (i, j): abU / abV / abW the x-y weight products of the
+// three face grids (0 when the column is not in the particle's stencil or the grid's row is off: onU
+// etc.), fh / dh and fn / dn the z weight of the half-shifted (u, v) and node-aligned (w) stencils and
+// the window slot of their base. kInterior: no barycentric clamp can apply.
+template<bool kInterior>
+__device__ __forceinline__ void p2g_col_weights(float gx, float gy, float gz, bool onU, bool onV, bool onW, const GridGeom& g, int i,
+                                                int j, int z, float fi, float fj, float& abU, float& abV, float& abW, float& fh,
+                                                float& fn, int& dh, int& dn) {
+#pragma clang fp contract(off)
+    if(kInterior) {
+        const float axn = stencil_weight_interior(gx, fi), axh = stencil_weight_interior(gx - 0.5f, fi);
+        const float byn = stencil_weight_interior(gy, fj), byh = stencil_weight_interior(gy - 0.5f, fj);
+        const float qh = gz - 0.5f, sh = floorf(qh);
+        fh  = qh - sh;
+        dh  = sh == (float)(z - 1) ? 0 : 1;
+        fn  = gz - floorf(gz);
+        dn  = 1;  // the node-aligned base is the row itself
+        abU = onU ? axn * byh : 0.f;
+        abV = onV ? axh * byn : 0.f;
+        abW = onW ? axh * byh : 0.f;
+    } else {
+        float      axn = 0.f, axh = 0.f, byn = 0.f, byh = 0.f;
+        const bool oxn = stencil_weight(gx, g.nx + 1, i, axn);
+        const bool oxh = stencil_weight(gx - 0.5f, g.nx, i, axh);
+        const bool oyn = stencil_weight(gy, g.ny + 1, j, byn);
+        const bool oyh = stencil_weight(gy - 0.5f, g.ny, j, byh);
+        int        kh, kn;
+        bary(gz - 0.5f, g.nz, kh, fh);
+        bary(gz, g.nz + 1, kn, fn);
+        dh  = kh - (z - 1);
+        dn  = kn - (z - 1);
+        abU = (onU && oxn && oyh) ? axn * byh : 0.f;
+        abV = (onV && oxh && oyn) ? axh * byn : 0.f;
+        abW = (onW && oxh && oyh) ? axh * byh : 0.f;
+    }
+}
+
 // one column segment of P2G: nodes (i, j, k0 .. k1-1). kInterior: every cell the segment reads lies
 // in [1, n-2] on each axis, so no barycentric clamp can apply to its particles.
+// FLIP / PIC: the coordinates and velocities of kBatch particles are loaded together (one memory
+// round trip per batch instead of two or more per particle) and then added in order; a batch slot
+// past the row's end repeats its last particle with every weight 0, which adds exact zeros (a sum
+// here starts at +0 and never becomes -0), so the sums are those of the particle-by-particle loop.
 template<bool kInterior, bool kAffine>
 __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, const uint32_t* __restrict__ cell_start, const GridGeom& g,
                                            const FaceOut& out, int i, int j, int k0, int k1) {
 #pragma clang fp contract(off)
+    constexpr int kBatch = 4;
     const bool uCol = j < g.ny, vCol = i < g.nx, wCol = i < g.nx && j < g.ny;
     const int  x0 = max(i - 1, 0), x1 = min(i + 1, g.nx - 1), xu = min(i, g.nx - 1);
     const int  y0 = max(j - 1, 0), y1 = min(j + 1, g.ny - 1);
@@ -722,39 +764,42 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
             const size_t   r    = lin3(0, y, z, g.nx, g.ny);
             const uint32_t pb   = cell_start[r + x0], pe = cell_start[r + x1 + 1], pu = cell_start[r + xu + 1];
             const bool     rowU = uCol, rowV = vCol && y <= j, rowW = wCol;
-            for(uint32_t p = pb; p < pe; ++p) {
-                const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
-                float       abU, abV, abW, fh, fn;
-                int         dh, dn;
-                if(kInterior) {
-                    const float axn = stencil_weight_interior(gx, fi), axh = stencil_weight_interior(gx - 0.5f, fi);
-                    const float byn = stencil_weight_interior(gy, fj), byh = stencil_weight_interior(gy - 0.5f, fj);
-                    const float qh = gz - 0.5f, sh = floorf(qh);
-                    fh  = qh - sh;
-                    dh  = sh == (float)(z - 1) ? 0 : 1;
-                    fn  = gz - floorf(gz);
-                    dn  = 1;  // the node-aligned base is the row itself
-                    abU = (rowU && p < pu) ? axn * byh : 0.f;
-                    abV = rowV ? axh * byn : 0.f;
-                    abW = rowW ? axh * byh : 0.f;
-                } else {
-                    float      axn = 0.f, axh = 0.f, byn = 0.f, byh = 0.f;
-                    const bool oxn = stencil_weight(gx, g.nx + 1, i, axn);
-                    const bool oxh = stencil_weight(gx - 0.5f, g.nx, i, axh);
-                    const bool oyn = stencil_weight(gy, g.ny + 1, j, byn);
-                    const bool oyh = stencil_weight(gy - 0.5f, g.ny, j, byh);
-                    int        kh, kn;
-                    bary(gz - 0.5f, g.nz, kh, fh);
-                    bary(gz, g.nz + 1, kn, fn);
-                    dh  = kh - (z - 1);
-                    dn  = kn - (z - 1);
-                    abU = (rowU && p < pu && oxn && oyh) ? axn * byh : 0.f;
-                    abV = (rowV && oxh && oyn) ? axh * byn : 0.f;
-                    abW = (rowW && oxh && oyh) ? axh * byh : 0.f;
+            if constexpr(!kAffine) {
+                for(uint32_t p0 = pb; p0 < pe; p0 += kBatch) {
+                    float qx[kBatch], qy[kBatch], qz[kBatch], ux[kBatch], uy[kBatch], uz[kBatch];
+#pragma unroll
+                    for(int q = 0; q < kBatch; ++q) {
+                        const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
+                        qx[q]            = P.gx[p];
+                        qy[q]            = P.gy[p];
+                        qz[q]            = P.gz[p];
+                        ux[q]            = P.vx[p];
+                        uy[q]            = P.vy[p];
+                        uz[q]            = P.vz[p];
+                    }
+#pragma unroll
+                    for(int q = 0; q < kBatch; ++q) {
+                        const uint32_t p    = p0 + (uint32_t)q;
+                        const bool     live = p < pe;
+                        float          abU, abV, abW, fh, fn;
+                        int            dh, dn;
+                        p2g_col_weights<kInterior>(qx[q], qy[q], qz[q], live && rowU && p < pu, live && rowV, live && rowW, g, i, j, z, fi,
+                                                   fj, abU, abV, abW, fh, fn, dh, dn);
+                        // a zero weight adds exact zeros with v taken as 0
+                        const float vx = abU != 0.f ? ux[q] : 0.f, vy = abV != 0.f ? uy[q] : 0.f, vz = abW != 0.f ? uz[q] : 0.f;
+                        p2g_window_add(dh, abU, fh, vx, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
+                        p2g_window_add(dh, abV, fh, vy, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
+                        p2g_window_add(dn, abW, fn, vz, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
+                    }
                 }
-                // a zero weight adds exact zeros whatever v is, so v is read only when it counts
-                const float vx = abU != 0.f ? P.vx[p] : 0.f, vy = abV != 0.f ? P.vy[p] : 0.f, vz = abW != 0.f ? P.vz[p] : 0.f;
-                if(kAffine) {
+            } else {
+                for(uint32_t p = pb; p < pe; ++p) {
+                    float abU, abV, abW, fh, fn;
+                    int   dh, dn;
+                    p2g_col_weights<kInterior>(P.gx[p], P.gy[p], P.gz[p], rowU && p < pu, rowV, rowW, g, i, j, z, fi, fj, abU, abV, abW, fh,
+                                               fn, dh, dn);
+                    // a zero weight adds exact zeros whatever v is, so v is read only when it counts
+                    const float vx = abU != 0.f ? P.vx[p] : 0.f, vy = abV != 0.f ? P.vy[p] : 0.f, vz = abW != 0.f ? P.vz[p] : 0.f;
                     // the base node along z of the half-shifted (u, v) and node-aligned (w) stencils
                     const float zh = (float)(z - 1 + dh), zn = (float)(z - 1 + dn);
                     const float px = P.px[p], py = P.py[p], pz = P.pz[p];
@@ -774,10 +819,6 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
                     p2g_window_add(dn, abW, fn, affine_vel(vz, cw, fi, fj, zn, 0.5f, 0.5f, 0.f, g, px, py, pz),
                                    affine_vel(vz, cw, fi, fj, zn + 1.0f, 0.5f, 0.5f, 0.f, g, px, py, pz), a0.mw, a0.ww, a1.mw, a1.ww,
                                    a2.mw, a2.ww);
-                } else {
-                    p2g_window_add(dh, abU, fh, vx, vx, a0.mu, a0.wu, a1.mu, a1.wu, a2.mu, a2.wu);
-                    p2g_window_add(dh, abV, fh, vy, vy, a0.mv, a0.wv, a1.mv, a1.wv, a2.mv, a2.wv);
-                    p2g_window_add(dn, abW, fn, vz, vz, a0.mw, a0.ww, a1.mw, a1.ww, a2.mw, a2.ww);
                 }
             }
         }
