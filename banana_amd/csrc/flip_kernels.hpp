@@ -605,6 +605,17 @@ struct k_scan_down {
     }
 };
 
+// Tile (tx, ty, tz) of an ntx x nty x ntz grid of tiles for workgroup blk: blocks are dispatched
+// round-robin over the 8 XCDs, so each XCD is handed a contiguous run of tiles and shares the reads
+// of neighbouring tiles through its own L2.
+__device__ __forceinline__ void tile_of_block(Blk blk, int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
+    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blk.id, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
+    const uint32_t t = xcd * q + min(xcd, r) + b / 8u;
+    tx = (int)(t % (uint32_t)ntx);
+    ty = (int)((t / (uint32_t)ntx) % (uint32_t)nty);
+    tz = (int)(t / (uint32_t)(ntx * nty));
+}
+
 //==================================================================================================
 // mapParticles2Grid as a gather over cell-sorted particles (FLIP_Solver.cpp:57-95,
 // ArrayHelpers.cpp:50-76). Thread (i,j,k) owns u(i,j,k), v(i,j,k), w(i,j,k); it visits the particles
@@ -832,6 +843,142 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
     if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
 }
 
+// the 8 stencil nodes of one particle on face grid c (extent sx x sy x sz): weight (wx * wy) * wz and
+// momentum w * v added to the tile's sums for the nodes the tile owns
+__device__ __forceinline__ void p2g_tile_add(float* acc, int c, int ix, float fx, int iy, float fy, int iz, float fz, float v, int x0, int y0,
+                                             int z0, int sx, int sy, int sz) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for(int dz = 0; dz < 2; ++dz) {
+        const int   k  = iz + dz;
+        const float wz = dz ? fz : 1.0f - fz;
+        if(k < z0 || k >= z0 + 8 || k >= sz) continue;
+#pragma unroll
+        for(int dy = 0; dy < 2; ++dy) {
+            const int   j  = iy + dy;
+            const float wy = dy ? fy : 1.0f - fy;
+            if(j < y0 || j >= y0 + 8 || j >= sy) continue;
+#pragma unroll
+            for(int dx = 0; dx < 2; ++dx) {
+                const int   i  = ix + dx;
+                const float wx = dx ? fx : 1.0f - fx;
+                if(i < x0 || i >= x0 + 8 || i >= sx) continue;
+                const float w  = (wx * wy) * wz;
+                const int   n  = ((k - z0) * 8 + (j - y0)) * 8 + (i - x0);
+                acc[c * 1024 + n] += w * v;
+                acc[c * 1024 + 512 + n] += w;
+            }
+        }
+    }
+}
+
+//--------------------------------------------------------------------------------------------------
+// P2G (FLIP / PIC) as a deterministic scatter: one wavefront per 8^3 tile of nodes, every particle's
+// stencil weights computed once. A particle binned to cell c reaches nodes c-1 .. c+1 on each axis
+// (clamps included), so the tile's nodes [x0, x0+8) x [y0, y0+8) x [z0, z0+8) of the three face grids
+// receive only particles of the cells [x0-1, x0+8] x .. (10^3, clipped to the grid), which it
+// visits in 27 rounds, one colour class (cx mod 3, cy mod 3, cz mod 3) per round. Two cells of one
+// colour lie 3 or more cells apart on some axis and reach no common node, so in a round each lane
+// owns a cell (at most 4 x 4 x 4 per colour) and adds its particles, in their sorted order, to the
+// tile's momentum and weight sums in LDS with no other lane touching those nodes. A node's sum is
+// therefore taken in a fixed order: by colour, then by particle; each term is the reference's
+// weight (wx * wy) * wz and momentum w * v, rounded before the add (no contraction). Contributions to
+// nodes outside the tile are left to the tile that owns them.
+//--------------------------------------------------------------------------------------------------
+struct k_p2g_tile {
+    static constexpr int kThreads = 64;
+    Particles P;
+    const uint32_t* __restrict__ cell_start;
+    GridGeom g;
+    FaceOut out;
+    const uint8_t* __restrict__ occ;  // particle occupancy of the 8^3 cell bricks (binning)
+    __device__ void operator()(Blk blk) const {
+#pragma clang fp contract(off)
+        __shared__ float acc[3 * 2 * 512];  // [component][momentum, weight][node (x fastest)]
+        int tx, ty, tz;
+        tile_of_block(blk, (g.nx + 8) / 8, (g.ny + 8) / 8, (g.nz + 8) / 8, tx, ty, tz);
+        const int x0 = tx * 8, y0 = ty * 8, z0 = tz * 8, lane = threadIdx.x;
+        const int cx0 = max(x0 - 1, 0), cx1 = min(x0 + 8, g.nx - 1);  // cells read
+        const int cy0 = max(y0 - 1, 0), cy1 = min(y0 + 8, g.ny - 1);
+        const int cz0 = max(z0 - 1, 0), cz1 = min(z0 + 8, g.nz - 1);
+        for(int e = lane; e < 3 * 2 * 512; e += 64) acc[e] = 0.f;
+        bool any = false;
+        for(int bz = cz0 >> 3; bz <= (cz1 >> 3); ++bz)
+            for(int by = cy0 >> 3; by <= (cy1 >> 3); ++by)
+                for(int bx = cx0 >> 3; bx <= (cx1 >> 3); ++bx) any |= occ[(bz * g.nby + by) * g.nbx + bx] != 0;
+        __syncthreads();
+        if(any) {
+            for(int colour = 0; colour < 27; ++colour) {
+                const int kx = colour % 3, ky = (colour / 3) % 3, kz = colour / 9;
+                // the lane's cell: the (lane & 3)-th, ((lane >> 2) & 3)-th, (lane >> 4)-th cell of this colour
+                const int cx = cx0 + (kx - cx0 % 3 + 3) % 3 + 3 * (lane & 3);
+                const int cy = cy0 + (ky - cy0 % 3 + 3) % 3 + 3 * ((lane >> 2) & 3);
+                const int cz = cz0 + (kz - cz0 % 3 + 3) % 3 + 3 * (lane >> 4);
+                if(cx <= cx1 && cy <= cy1 && cz <= cz1) {
+                    const size_t   cell = lin3(cx, cy, cz, g.nx, g.ny);
+                    const uint32_t pb = cell_start[cell], pe = cell_start[cell + 1];
+                    for(uint32_t p0 = pb; p0 < pe; p0 += 4) {
+                        float qx[4], qy[4], qz[4], ux[4], uy[4], uz[4];
+#pragma unroll
+                        for(int q = 0; q < 4; ++q) {
+                            const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
+                            qx[q]            = P.gx[p];
+                            qy[q]            = P.gy[p];
+                            qz[q]            = P.gz[p];
+                            ux[q]            = P.vx[p];
+                            uy[q]            = P.vy[p];
+                            uz[q]            = P.vz[p];
+                        }
+#pragma unroll
+                        for(int q = 0; q < 4; ++q) {
+                            if(p0 + (uint32_t)q >= pe) break;
+                            // node-aligned (n) and half-shifted (h) stencils on each axis: base and fraction
+                            int   ixn, ixh, iyn, iyh, izn, izh;
+                            float fxn, fxh, fyn, fyh, fzn, fzh;
+                            bary(qx[q], g.nx + 1, ixn, fxn);
+                            bary(qx[q] - 0.5f, g.nx, ixh, fxh);
+                            bary(qy[q], g.ny + 1, iyn, fyn);
+                            bary(qy[q] - 0.5f, g.ny, iyh, fyh);
+                            bary(qz[q], g.nz + 1, izn, fzn);
+                            bary(qz[q] - 0.5f, g.nz, izh, fzh);
+                            // u: x node-aligned, y and z half-shifted; v: y node-aligned; w: z node-aligned
+                            p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, ux[q], x0, y0, z0, g.nx + 1, g.ny, g.nz);
+                            p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, uy[q], x0, y0, z0, g.nx, g.ny + 1, g.nz);
+                            p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, uz[q], x0, y0, z0, g.nx, g.ny, g.nz + 1);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // the tile's nodes: u = momentum / weight where the weight is positive, else 0 and invalid
+        for(int e = lane; e < 512; e += 64) {
+            const int i = x0 + (e & 7), j = y0 + ((e >> 3) & 7), k = z0 + (e >> 6);
+            if(i <= g.nx && j < g.ny && k < g.nz) {
+                const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
+                const float  m = acc[e], w = acc[512 + e];
+                const bool   ok = w > kFltMin;
+                out.u[id]       = ok ? m / w : 0.f;
+                out.vu[id]      = ok ? 1 : 0;
+            }
+            if(i < g.nx && j <= g.ny && k < g.nz) {
+                const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
+                const float  m = acc[1024 + e], w = acc[1536 + e];
+                const bool   ok = w > kFltMin;
+                out.v[id]       = ok ? m / w : 0.f;
+                out.vv[id]      = ok ? 1 : 0;
+            }
+            if(i < g.nx && j < g.ny && k <= g.nz) {
+                const size_t id = lin3(i, j, k, g.nx, g.ny);
+                const float  m = acc[2048 + e], w = acc[2560 + e];
+                const bool   ok = w > kFltMin;
+                out.w[id]       = ok ? m / w : 0.f;
+                out.vw[id]      = ok ? 1 : 0;
+            }
+        }
+    }
+};
+
 template<bool kAffine, int kSeg>
 struct k_p2g_col {
     static constexpr int kThreads = kBlock;
@@ -871,17 +1018,6 @@ struct k_p2g_col {
         else p2g_column<false, kAffine>(P, A, cell_start, g, out, i, j, k0, k1);
     }
 };
-
-// Tile (tx, ty, tz) of an ntx x nty x ntz grid of tiles for workgroup blk: blocks are dispatched
-// round-robin over the 8 XCDs, so each XCD is handed a contiguous run of tiles and shares the reads
-// of neighbouring tiles through its own L2.
-__device__ __forceinline__ void tile_of_block(Blk blk, int ntx, int nty, int ntz, int& tx, int& ty, int& tz) {
-    const uint32_t nt = (uint32_t)(ntx * nty * ntz), b = blk.id, xcd = b % 8u, q = nt / 8u, r = nt % 8u;
-    const uint32_t t = xcd * q + min(xcd, r) + b / 8u;
-    tx = (int)(t % (uint32_t)ntx);
-    ty = (int)((t / (uint32_t)ntx) % (uint32_t)nty);
-    tz = (int)(t / (uint32_t)(ntx * nty));
-}
 
 //==================================================================================================
 // extrapolateVelocity of one component (PIC_Solver.cpp:985-1090): 4 Jacobi layers. Layer l fills an

@@ -701,8 +701,12 @@ public:
         requireBinned();
         FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
         ktBegin(BNN_KT_P2G);
-        if(affineScheme()) p2gSegLaunch<true>(out);  // APIC_Solver.cpp:52-91, AFLIP_Solver.cpp:48-90
-        else p2gSegLaunch<false>(out);
+        if(affineScheme()) {
+            p2gSegLaunch<true>(out);  // APIC_Solver.cpp:52-91, AFLIP_Solver.cpp:48-90
+        } else {  // FLIP_Solver.cpp:57-95: one wavefront per 8^3 tile of nodes
+            const uint32_t tiles = (uint32_t)(((g.nx + 8) / 8) * ((g.ny + 8) / 8) * ((g.nz + 8) / 8));
+            launch(k_p2g_tile{cur->view(), (const uint32_t*)cellStart.as<uint32_t>(), g, out, (const uint8_t*)occ.as<uint8_t>()}, tiles);
+        }
         ktEnd(BNN_KT_P2G);
     }
 
