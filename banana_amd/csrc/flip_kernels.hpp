@@ -843,33 +843,34 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
     if(ze + 1 >= k0 && ze + 1 < k1) p2g_emit(a1, i, j, ze + 1, uCol, vCol, wCol, g, out);
 }
 
-// the 8 stencil nodes of one particle on face grid c (extent sx x sy x sz): weight (wx * wy) * wz and
-// momentum w * v added to the tile's sums for the nodes the tile owns
-__device__ __forceinline__ void p2g_tile_add(float* acc, int c, int ix, float fx, int iy, float fy, int iz, float fz, float v, int x0, int y0,
+// the 8 stencil nodes of one particle on face grid c (extent sx x sy x sz): momentum w * v and weight
+// w = (wx * wy) * wz added to the tile's sums, acc[c * 513 + node] = (momentum, weight), for the nodes
+// the tile owns; the others go to the unused slot c * 513 + 512. The 8 nodes are distinct, so all 8
+// sums are read before any is written back (one LDS round trip per component, no branches).
+__device__ __forceinline__ void p2g_tile_add(float2* acc, int c, int ix, float fx, int iy, float fy, int iz, float fz, float v, int x0, int y0,
                                              int z0, int sx, int sy, int sz) {
 #pragma clang fp contract(off)
+    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    int         n[8];
+    float       w[8];
 #pragma unroll
-    for(int dz = 0; dz < 2; ++dz) {
-        const int   k  = iz + dz;
-        const float wz = dz ? fz : 1.0f - fz;
-        if(k < z0 || k >= z0 + 8 || k >= sz) continue;
-#pragma unroll
-        for(int dy = 0; dy < 2; ++dy) {
-            const int   j  = iy + dy;
-            const float wy = dy ? fy : 1.0f - fy;
-            if(j < y0 || j >= y0 + 8 || j >= sy) continue;
-#pragma unroll
-            for(int dx = 0; dx < 2; ++dx) {
-                const int   i  = ix + dx;
-                const float wx = dx ? fx : 1.0f - fx;
-                if(i < x0 || i >= x0 + 8 || i >= sx) continue;
-                const float w  = (wx * wy) * wz;
-                const int   n  = ((k - z0) * 8 + (j - y0)) * 8 + (i - x0);
-                acc[c * 1024 + n] += w * v;
-                acc[c * 1024 + 512 + n] += w;
-            }
-        }
+    for(int e = 0; e < 8; ++e) {
+        const int  dx = e & 1, dy = (e >> 1) & 1, dz = e >> 2;
+        const int  i = ix + dx, j = iy + dy, k = iz + dz;
+        const bool own = (unsigned)(i - x0) < 8u && (unsigned)(j - y0) < 8u && (unsigned)(k - z0) < 8u && i < sx && j < sy && k < sz;
+        n[e]           = c * 513 + (own ? ((k - z0) * 8 + (j - y0)) * 8 + (i - x0) : 512);
+        w[e]           = (wx[dx] * wy[dy]) * wz[dz];
     }
+    float2 a[8];
+#pragma unroll
+    for(int e = 0; e < 8; ++e) a[e] = acc[n[e]];
+#pragma unroll
+    for(int e = 0; e < 8; ++e) {
+        a[e].x += w[e] * v;
+        a[e].y += w[e];
+    }
+#pragma unroll
+    for(int e = 0; e < 8; ++e) acc[n[e]] = a[e];
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -894,58 +895,67 @@ struct k_p2g_tile {
     const uint8_t* __restrict__ occ;  // particle occupancy of the 8^3 cell bricks (binning)
     __device__ void operator()(Blk blk) const {
 #pragma clang fp contract(off)
-        __shared__ float acc[3 * 2 * 512];  // [component][momentum, weight][node (x fastest)]
+        __shared__ float2 acc[3 * 513];  // [component][node (x fastest), + 1 unused]: (momentum, weight)
         int tx, ty, tz;
         tile_of_block(blk, (g.nx + 8) / 8, (g.ny + 8) / 8, (g.nz + 8) / 8, tx, ty, tz);
         const int x0 = tx * 8, y0 = ty * 8, z0 = tz * 8, lane = threadIdx.x;
         const int cx0 = max(x0 - 1, 0), cx1 = min(x0 + 8, g.nx - 1);  // cells read
         const int cy0 = max(y0 - 1, 0), cy1 = min(y0 + 8, g.ny - 1);
         const int cz0 = max(z0 - 1, 0), cz1 = min(z0 + 8, g.nz - 1);
-        for(int e = lane; e < 3 * 2 * 512; e += 64) acc[e] = 0.f;
+        for(int e = lane; e < 3 * 513; e += 64) acc[e] = make_float2(0.f, 0.f);
         bool any = false;
         for(int bz = cz0 >> 3; bz <= (cz1 >> 3); ++bz)
             for(int by = cy0 >> 3; by <= (cy1 >> 3); ++by)
                 for(int bx = cx0 >> 3; bx <= (cx1 >> 3); ++bx) any |= occ[(bz * g.nby + by) * g.nbx + bx] != 0;
         __syncthreads();
         if(any) {
-            for(int colour = 0; colour < 27; ++colour) {
+            // the particle range of the lane's cell in a colour round (empty when the lane has no cell)
+            auto range = [&](int colour, uint32_t& pb, uint32_t& pe) {
                 const int kx = colour % 3, ky = (colour / 3) % 3, kz = colour / 9;
-                // the lane's cell: the (lane & 3)-th, ((lane >> 2) & 3)-th, (lane >> 4)-th cell of this colour
+                // the (lane & 3)-th, ((lane >> 2) & 3)-th, (lane >> 4)-th cell of this colour on each axis
                 const int cx = cx0 + (kx - cx0 % 3 + 3) % 3 + 3 * (lane & 3);
                 const int cy = cy0 + (ky - cy0 % 3 + 3) % 3 + 3 * ((lane >> 2) & 3);
                 const int cz = cz0 + (kz - cz0 % 3 + 3) % 3 + 3 * (lane >> 4);
+                pb = pe = 0;
                 if(cx <= cx1 && cy <= cy1 && cz <= cz1) {
-                    const size_t   cell = lin3(cx, cy, cz, g.nx, g.ny);
-                    const uint32_t pb = cell_start[cell], pe = cell_start[cell + 1];
-                    for(uint32_t p0 = pb; p0 < pe; p0 += 4) {
-                        float qx[4], qy[4], qz[4], ux[4], uy[4], uz[4];
+                    const size_t cell = lin3(cx, cy, cz, g.nx, g.ny);
+                    pb                = cell_start[cell];
+                    pe                = cell_start[cell + 1];
+                }
+            };
+            uint32_t nb, ne;
+            range(0, nb, ne);
+            for(int colour = 0; colour < 27; ++colour) {
+                const uint32_t pb = nb, pe = ne;
+                if(colour + 1 < 27) range(colour + 1, nb, ne);  // the next round's range, loaded ahead
+                for(uint32_t p0 = pb; p0 < pe; p0 += 8) {
+                    float qx[8], qy[8], qz[8], ux[8], uy[8], uz[8];
 #pragma unroll
-                        for(int q = 0; q < 4; ++q) {
-                            const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
-                            qx[q]            = P.gx[p];
-                            qy[q]            = P.gy[p];
-                            qz[q]            = P.gz[p];
-                            ux[q]            = P.vx[p];
-                            uy[q]            = P.vy[p];
-                            uz[q]            = P.vz[p];
-                        }
+                    for(int q = 0; q < 8; ++q) {
+                        const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
+                        qx[q]            = P.gx[p];
+                        qy[q]            = P.gy[p];
+                        qz[q]            = P.gz[p];
+                        ux[q]            = P.vx[p];
+                        uy[q]            = P.vy[p];
+                        uz[q]            = P.vz[p];
+                    }
 #pragma unroll
-                        for(int q = 0; q < 4; ++q) {
-                            if(p0 + (uint32_t)q >= pe) break;
-                            // node-aligned (n) and half-shifted (h) stencils on each axis: base and fraction
-                            int   ixn, ixh, iyn, iyh, izn, izh;
-                            float fxn, fxh, fyn, fyh, fzn, fzh;
-                            bary(qx[q], g.nx + 1, ixn, fxn);
-                            bary(qx[q] - 0.5f, g.nx, ixh, fxh);
-                            bary(qy[q], g.ny + 1, iyn, fyn);
-                            bary(qy[q] - 0.5f, g.ny, iyh, fyh);
-                            bary(qz[q], g.nz + 1, izn, fzn);
-                            bary(qz[q] - 0.5f, g.nz, izh, fzh);
-                            // u: x node-aligned, y and z half-shifted; v: y node-aligned; w: z node-aligned
-                            p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, ux[q], x0, y0, z0, g.nx + 1, g.ny, g.nz);
-                            p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, uy[q], x0, y0, z0, g.nx, g.ny + 1, g.nz);
-                            p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, uz[q], x0, y0, z0, g.nx, g.ny, g.nz + 1);
-                        }
+                    for(int q = 0; q < 8; ++q) {
+                        if(p0 + (uint32_t)q >= pe) break;
+                        // node-aligned (n) and half-shifted (h) stencils on each axis: base and fraction
+                        int   ixn, ixh, iyn, iyh, izn, izh;
+                        float fxn, fxh, fyn, fyh, fzn, fzh;
+                        bary(qx[q], g.nx + 1, ixn, fxn);
+                        bary(qx[q] - 0.5f, g.nx, ixh, fxh);
+                        bary(qy[q], g.ny + 1, iyn, fyn);
+                        bary(qy[q] - 0.5f, g.ny, iyh, fyh);
+                        bary(qz[q], g.nz + 1, izn, fzn);
+                        bary(qz[q] - 0.5f, g.nz, izh, fzh);
+                        // u: x node-aligned, y and z half-shifted; v: y node-aligned; w: z node-aligned
+                        p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, ux[q], x0, y0, z0, g.nx + 1, g.ny, g.nz);
+                        p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, uy[q], x0, y0, z0, g.nx, g.ny + 1, g.nz);
+                        p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, uz[q], x0, y0, z0, g.nx, g.ny, g.nz + 1);
                     }
                 }
                 __syncthreads();
@@ -956,21 +966,21 @@ struct k_p2g_tile {
             const int i = x0 + (e & 7), j = y0 + ((e >> 3) & 7), k = z0 + (e >> 6);
             if(i <= g.nx && j < g.ny && k < g.nz) {
                 const size_t id = lin3(i, j, k, g.nx + 1, g.ny);
-                const float  m = acc[e], w = acc[512 + e];
+                const float  m = acc[e].x, w = acc[e].y;
                 const bool   ok = w > kFltMin;
                 out.u[id]       = ok ? m / w : 0.f;
                 out.vu[id]      = ok ? 1 : 0;
             }
             if(i < g.nx && j <= g.ny && k < g.nz) {
                 const size_t id = lin3(i, j, k, g.nx, g.ny + 1);
-                const float  m = acc[1024 + e], w = acc[1536 + e];
+                const float  m = acc[513 + e].x, w = acc[513 + e].y;
                 const bool   ok = w > kFltMin;
                 out.v[id]       = ok ? m / w : 0.f;
                 out.vv[id]      = ok ? 1 : 0;
             }
             if(i < g.nx && j < g.ny && k <= g.nz) {
                 const size_t id = lin3(i, j, k, g.nx, g.ny);
-                const float  m = acc[2048 + e], w = acc[2560 + e];
+                const float  m = acc[1026 + e].x, w = acc[1026 + e].y;
                 const bool   ok = w > kFltMin;
                 out.w[id]       = ok ? m / w : 0.f;
                 out.vw[id]      = ok ? 1 : 0;
