@@ -923,42 +923,61 @@ struct k_p2g_tile {
                     pe                = cell_start[cell + 1];
                 }
             };
-            uint32_t nb, ne;
-            range(0, nb, ne);
+            // up to 8 particles [p0, pe) of a cell: coordinates and velocities (a slot past pe repeats the last)
+            struct Batch {
+                float qx[8], qy[8], qz[8], ux[8], uy[8], uz[8];
+            };
+            auto load = [&](uint32_t p0, uint32_t pe, Batch& B) {
+#pragma unroll
+                for(int q = 0; q < 8; ++q) {
+                    const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
+                    B.qx[q]          = P.gx[p];
+                    B.qy[q]          = P.gy[p];
+                    B.qz[q]          = P.gz[p];
+                    B.ux[q]          = P.vx[p];
+                    B.uy[q]          = P.vy[p];
+                    B.uz[q]          = P.vz[p];
+                }
+            };
+            auto add = [&](const Batch& B, uint32_t p0, uint32_t pe) {
+#pragma unroll
+                for(int q = 0; q < 8; ++q) {
+                    if(p0 + (uint32_t)q >= pe) break;
+                    // node-aligned (n) and half-shifted (h) stencils on each axis: base and fraction
+                    int   ixn, ixh, iyn, iyh, izn, izh;
+                    float fxn, fxh, fyn, fyh, fzn, fzh;
+                    bary(B.qx[q], g.nx + 1, ixn, fxn);
+                    bary(B.qx[q] - 0.5f, g.nx, ixh, fxh);
+                    bary(B.qy[q], g.ny + 1, iyn, fyn);
+                    bary(B.qy[q] - 0.5f, g.ny, iyh, fyh);
+                    bary(B.qz[q], g.nz + 1, izn, fzn);
+                    bary(B.qz[q] - 0.5f, g.nz, izh, fzh);
+                    // u: x node-aligned, y and z half-shifted; v: y node-aligned; w: z node-aligned
+                    p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, B.ux[q], x0, y0, z0, g.nx + 1, g.ny, g.nz);
+                    p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, B.uy[q], x0, y0, z0, g.nx, g.ny + 1, g.nz);
+                    p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, B.uz[q], x0, y0, z0, g.nx, g.ny, g.nz + 1);
+                }
+            };
+            // software pipeline: a round's first batch is loaded during the round before, its cell
+            // range two rounds before
+            Batch    cur, nxt;
+            uint32_t cb, ce, nb, ne, nnb = 0, nne = 0;
+            range(0, cb, ce);
+            range(1, nb, ne);
+            if(cb < ce) load(cb, ce, cur);
             for(int colour = 0; colour < 27; ++colour) {
-                const uint32_t pb = nb, pe = ne;
-                if(colour + 1 < 27) range(colour + 1, nb, ne);  // the next round's range, loaded ahead
-                for(uint32_t p0 = pb; p0 < pe; p0 += 8) {
-                    float qx[8], qy[8], qz[8], ux[8], uy[8], uz[8];
-#pragma unroll
-                    for(int q = 0; q < 8; ++q) {
-                        const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
-                        qx[q]            = P.gx[p];
-                        qy[q]            = P.gy[p];
-                        qz[q]            = P.gz[p];
-                        ux[q]            = P.vx[p];
-                        uy[q]            = P.vy[p];
-                        uz[q]            = P.vz[p];
-                    }
-#pragma unroll
-                    for(int q = 0; q < 8; ++q) {
-                        if(p0 + (uint32_t)q >= pe) break;
-                        // node-aligned (n) and half-shifted (h) stencils on each axis: base and fraction
-                        int   ixn, ixh, iyn, iyh, izn, izh;
-                        float fxn, fxh, fyn, fyh, fzn, fzh;
-                        bary(qx[q], g.nx + 1, ixn, fxn);
-                        bary(qx[q] - 0.5f, g.nx, ixh, fxh);
-                        bary(qy[q], g.ny + 1, iyn, fyn);
-                        bary(qy[q] - 0.5f, g.ny, iyh, fyh);
-                        bary(qz[q], g.nz + 1, izn, fzn);
-                        bary(qz[q] - 0.5f, g.nz, izh, fzh);
-                        // u: x node-aligned, y and z half-shifted; v: y node-aligned; w: z node-aligned
-                        p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, ux[q], x0, y0, z0, g.nx + 1, g.ny, g.nz);
-                        p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, uy[q], x0, y0, z0, g.nx, g.ny + 1, g.nz);
-                        p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, uz[q], x0, y0, z0, g.nx, g.ny, g.nz + 1);
-                    }
+                if(colour + 1 < 27 && nb < ne) load(nb, ne, nxt);
+                if(colour + 2 < 27) range(colour + 2, nnb, nne);
+                for(uint32_t p0 = cb; p0 < ce; p0 += 8) {
+                    if(p0 != cb) load(p0, ce, cur);  // cells of more than 8 particles
+                    add(cur, p0, ce);
                 }
                 __syncthreads();
+                cur = nxt;
+                cb  = nb;
+                ce  = ne;
+                nb  = nnb;
+                ne  = nne;
             }
         }
         // the tile's nodes: u = momentum / weight where the weight is positive, else 0 and invalid
