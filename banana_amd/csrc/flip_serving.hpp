@@ -332,11 +332,12 @@ private:
     std::unordered_map<uint64_t, Entry> table;
 };
 
-// the batched launch of P2G with column segment seg (4, 8 or 16), FLIP or affine transfer
+// the batched launch of P2G: the affine transfer's column gather with segment seg (4, 8 or 16), or
+// the FLIP / PIC tile scatter (which has no segment)
 static inline Recorder::LaunchFn p2g_launch_fn(int seg, bool affine) {
     if(affine)
         return seg == 4 ? Recorder::fnOf<k_p2g_col<true, 4>>() : (seg == 16 ? Recorder::fnOf<k_p2g_col<true, 16>>() : Recorder::fnOf<k_p2g_col<true, 8>>());
-    return seg == 4 ? Recorder::fnOf<k_p2g_col<false, 4>>() : (seg == 16 ? Recorder::fnOf<k_p2g_col<false, 16>>() : Recorder::fnOf<k_p2g_col<false, 8>>());
+    return Recorder::fnOf<k_p2g_tile>();
 }
 
 static inline uint64_t batch_signature(const std::vector<FlipSolver*>& sc) {

@@ -685,7 +685,8 @@ public:
         if(!binned) binParticles();
     }
 
-    // FLIP_Solver.cpp:57-95; the column segment length p2gSeg (4, 8 or 16) does not change the result
+    // APIC_Solver.cpp:52-91 / AFLIP_Solver.cpp:48-90 as a column gather; the column segment length
+    // p2gSeg (4, 8 or 16) does not change the result
     template<bool kAffine, int kSeg> void p2gLaunch(const FaceOut& out) {
         const size_t n = (size_t)(g.nx + 1) * (g.ny + 1) * (((size_t)g.nz + 1 + kSeg - 1) / kSeg);  // columns x segments
         launch(k_p2g_col<kAffine, kSeg>{cur->view(), kAffine ? cur->affine() : Affine{}, (const uint32_t*)cellStart.as<uint32_t>(), g, out,

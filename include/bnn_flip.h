@@ -416,7 +416,7 @@ bnn_status  bnn_flip_batch_advance_frame(bnn_flip_batch* b, bnn_flip* const* sce
 bnn_status  bnn_flip_batch_pressure_projection(bnn_flip_batch* b, bnn_flip* const* scenes, const float* dts, uint32_t n,
                                                bnn_status* status);
 typedef struct bnn_batch_shape {
-    int32_t  p2g_segment;     /* z nodes per P2G thread: 4, 8 or 16 */
+    int32_t  p2g_segment;     /* z nodes per thread of the APIC / AFLIP P2G gather: 4, 8 or 16 */
     uint32_t pcg_chunk_first; /* PCG iterations queued before the first stop-flag poll */
     uint32_t pcg_chunk_max;   /* ... and at most per poll after it (later chunks: the iterations a geometric
                                  fit of max|r| so far says remain) */
