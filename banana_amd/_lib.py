@@ -181,6 +181,7 @@ SIGNATURES = {
     "bnn_flip_batch_pressure_projection": (_i32, [_vp, _P(_vp), _P(_f32), _u32, _P(_i32)]),
     "bnn_flip_batch_set_shape": (_i32, [_vp, _vp]),
     "bnn_flip_set_boundary_objects": (_i32, [_vp, _vp, _u32]),
+    "bnn_flip_set_finished_frame": (_i32, [_vp, _u32]),
     "bnn_boundaries_from_json": (_i32, [C.c_char_p, _vp, _u32, _P(_u32)]),
     "bnn_flip_batch_get_info": (_i32, [_vp, _vp]),
     "bnn_server_create": (_i32, [_P(ServerConfig), _P(_vp)]),

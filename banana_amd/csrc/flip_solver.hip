@@ -1861,6 +1861,14 @@ bnn_status bnn_flip_set_boundary_objects(bnn_flip* h, const bnn_boundary_object*
     });
 }
 
+bnn_status bnn_flip_set_finished_frame(bnn_flip* h, uint32_t frame) {
+    return guard(h, [&](bnn::FlipSolver& s) {
+        s.finishedFrame      = frame;
+        s.st.finished_frame  = frame;
+        return BNN_OK;
+    });
+}
+
 bnn_status bnn_flip_create_from_json(const char* json_text, uint64_t seed, bnn_flip** out) {
     if(!out) return BNN_ERR_INVALID_ARG;
     *out = nullptr;

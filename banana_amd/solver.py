@@ -234,6 +234,11 @@ class FLIPSolver:
         assert c.shape == (self.num_particles(), 3, 3)
         self._c(self.L.bnn_flip_set_affine(self.h, _ptr(c)))
 
+    def set_finished_frame(self, frame):
+        """GlobalParameters::finishedFrame after a host-side memory-state load: the frame count the
+        sort rule and the animated boundary objects follow."""
+        self._c(self.L.bnn_flip_set_finished_frame(self.h, int(frame)))
+
     def set_boundary_objects(self, objs):
         """The scene's additional boundary objects: a list of dicts (see boundary_objects) or a
         ctypes array with its count, as (array, n)."""

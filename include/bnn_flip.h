@@ -243,6 +243,10 @@ typedef struct bnn_flip bnn_flip;
 /* replaces the scene's additional boundary objects (at most BNN_MAX_BOUNDARY_OBJECTS); with
  * boundary_sdf_at_init the boundary SDF is recomputed from the domain box and the objects */
 bnn_status bnn_flip_set_boundary_objects(bnn_flip* h, const bnn_boundary_object* objs, uint32_t n);
+/* sets GlobalParameters::finishedFrame (ParticleSolverData.h:50), for a host that loaded a memory
+ * state with its own readers and handed the arrays over: the frame count the sort rule
+ * (PIC_Solver.cpp:315-328) and the animated boundary objects (PIC_Solver.cpp:70-118) follow */
+bnn_status bnn_flip_set_finished_frame(bnn_flip* h, uint32_t frame);
 /* the AdditionalBoundaryObjects of a scene file (GeometryType Box / Sphere, Translation,
  * RotationAxisAngle, Scale, Animation); *n receives their count, at most cap are written.
  * bnn_flip_create_from_json applies them. */

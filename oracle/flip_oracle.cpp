@@ -853,11 +853,16 @@ public:
         pos = out;
     }
 
-    // float/double CAS add (AtomicOperations.h:31-49), used only when threads > 1
+    // float/double CAS add (AtomicOperations.h:31-49), used only when threads > 1. ORC_PLANT_RACE
+    // (the ThreadSanitizer check's negative control, never in the library build) makes it a plain add.
     static void atomicAddR(R& dst, R v) {
+#ifdef ORC_PLANT_RACE
+        dst += v;
+#else
         std::atomic_ref<R> a(dst);
         R cur = a.load(std::memory_order_relaxed);
         while(!a.compare_exchange_weak(cur, cur + v, std::memory_order_relaxed)) {}
+#endif
     }
 
     // FLIP_Solver.cpp:57-95 (+ ArrayHelpers.cpp:50-76 getCoordinatesAndWeights)

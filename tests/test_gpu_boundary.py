@@ -105,3 +105,24 @@ def test_rotating_box_frames(oracle, bnn):
     e = np.linalg.norm(pg.astype(np.float64) - po) / np.linalg.norm(po.astype(np.float64))
     print(f"positions after 3 frames with a rotating box: rel L2 {e:.2e}")
     assert e <= 1e-5
+
+
+def test_resumed_frame_counter(oracle, bnn):
+    """A host that loads a memory state itself hands the frame count over
+    (bnn_flip_set_finished_frame): the animated box is then placed by the same frames as in the
+    oracle resumed at that frame, and differently from a device left at frame 0."""
+    fd = np.float32(1.0 / 60.0)
+    kw = dict(frame_duration=fd, min_timestep=fd / 2, max_timestep=fd / 2, pressure_mode=1)
+    params, pos, vel, g, o = _pair(oracle, bnn, 24, ROTATING, **kw)
+    g.set_finished_frame(3)
+    o.set_finished_frame(3)
+    g.advance_frame()
+    o.advance_frame()
+    assert g.stats()["finished_frame"] == 4
+    bg = g.grid("boundary_sdf")
+    assert np.array_equal(bg, o.field(oracle.F_BOUNDARY_SDF).astype(np.float32))
+    fresh = bnn.FLIPSolver(params)
+    fresh.set_boundary_objects(ROTATING)
+    fresh.set_particles(pos, vel)
+    fresh.advance_frame()  # frame 0: the animation does not run yet
+    assert np.count_nonzero(np.abs(bg - fresh.grid("boundary_sdf")) > 1e-3) > 100
