@@ -1209,9 +1209,9 @@ struct k_constrain {
         const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
         const size_t N  = (size_t)sx * sy * sz;
         if(t >= N) return;
-        const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
-        float        val = cur[t];
         if(ext[t] == 1 && weight[t] < kFltMin) {
+            const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
+            float        val = cur[t];
             int i, j, k;
         unlin3((uint32_t)t, sx, sy, i, j, k);
             float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
@@ -1231,8 +1231,23 @@ struct k_constrain {
                 vz -= dt * nz;
                 val = d == 0 ? vx : (d == 1 ? vy : vz);
             }
+            out[t] = val;
         }
-        out[t] = val;
+    }
+};
+
+// the second half of a component's constrain: the faces k_constrain computed (extrapolated, zero
+// weight) take their new values; every other face keeps its velocity
+struct k_constrain_apply {
+    static constexpr int kThreads = kBlock;
+    float* __restrict__ vel;
+    const float* __restrict__ tmp;
+    const uint8_t* __restrict__ ext;
+    const float* __restrict__ weight;
+    size_t n;
+    __device__ void operator()(Blk blk) const {
+        const size_t t = (size_t)blk.id * blockDim.x + threadIdx.x;
+        if(t < n && ext[t] == 1 && weight[t] < kFltMin) vel[t] = tmp[t];
     }
 };
 
