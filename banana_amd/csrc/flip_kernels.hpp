@@ -1204,34 +1204,47 @@ struct k_constrain {
     const float* __restrict__ weight;
     const float* __restrict__ bsdf;
     float* __restrict__ out;
+    // the new velocity of face t (extrapolated, zero weight) into out[t]
+    __device__ void face(size_t t, int sx, int sy) const {
+        const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
+        float        val = cur[t];
+        int i, j, k;
+        unlin3((uint32_t)t, sx, sy, i, j, k);
+        float gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+        gp[d]       = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
+        float vx, vy, vz, nx, ny, nz;
+        vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
+        interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
+        const float m2 = nx * nx + ny * ny + nz * nz;
+        if(m2 > kFltMin) {
+            const float s = sqrtf(m2);
+            nx /= s;
+            ny /= s;
+            nz /= s;
+            const float dt = vx * nx + vy * ny + vz * nz;
+            vx -= dt * nx;
+            vy -= dt * ny;
+            vz -= dt * nz;
+            val = d == 0 ? vx : (d == 1 ? vy : vz);
+        }
+        out[t] = val;
+    }
+    // four faces per thread: the flags and weights read as one 4-byte and one 16-byte load
     __device__ void operator()(Blk blk) const {
         const int    sx = g.nx + (d == 0), sy = g.ny + (d == 1), sz = g.nz + (d == 2);
-        const size_t t  = (size_t)blk.id * blockDim.x + threadIdx.x;
         const size_t N  = (size_t)sx * sy * sz;
-        if(t >= N) return;
-        if(ext[t] == 1 && weight[t] < kFltMin) {
-            const float* cur = d == 0 ? G.u : (d == 1 ? G.v : G.w);
-            float        val = cur[t];
-            int i, j, k;
-        unlin3((uint32_t)t, sx, sy, i, j, k);
-            float     gp[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
-            gp[d]           = d == 0 ? (float)i : (d == 1 ? (float)j : (float)k);
-            float vx, vy, vz, nx, ny, nz;
-            vel_from_grid(G, g, gp[0], gp[1], gp[2], vx, vy, vz);
-            interp_gradient(bsdf, g.nx + 1, g.ny + 1, g.nz + 1, gp[0], gp[1], gp[2], nx, ny, nz);
-            const float m2 = nx * nx + ny * ny + nz * nz;
-            if(m2 > kFltMin) {
-                const float s = sqrtf(m2);
-                nx /= s;
-                ny /= s;
-                nz /= s;
-                const float dt = vx * nx + vy * ny + vz * nz;
-                vx -= dt * nx;
-                vy -= dt * ny;
-                vz -= dt * nz;
-                val = d == 0 ? vx : (d == 1 ? vy : vz);
-            }
-            out[t] = val;
+        const size_t t0 = 4 * ((size_t)blk.id * blockDim.x + threadIdx.x);
+        if(t0 >= N) return;
+        if(t0 + 4 <= N) {
+            const uchar4 e = *reinterpret_cast<const uchar4*>(ext + t0);
+            const float4 w = *reinterpret_cast<const float4*>(weight + t0);
+            if(e.x == 1 && w.x < kFltMin) face(t0, sx, sy);
+            if(e.y == 1 && w.y < kFltMin) face(t0 + 1, sx, sy);
+            if(e.z == 1 && w.z < kFltMin) face(t0 + 2, sx, sy);
+            if(e.w == 1 && w.w < kFltMin) face(t0 + 3, sx, sy);
+        } else {
+            for(size_t t = t0; t < N; ++t)
+                if(ext[t] == 1 && weight[t] < kFltMin) face(t, sx, sy);
         }
     }
 };
@@ -1245,9 +1258,21 @@ struct k_constrain_apply {
     const uint8_t* __restrict__ ext;
     const float* __restrict__ weight;
     size_t n;
+    // four faces per thread, as in k_constrain
     __device__ void operator()(Blk blk) const {
-        const size_t t = (size_t)blk.id * blockDim.x + threadIdx.x;
-        if(t < n && ext[t] == 1 && weight[t] < kFltMin) vel[t] = tmp[t];
+        const size_t t0 = 4 * ((size_t)blk.id * blockDim.x + threadIdx.x);
+        if(t0 >= n) return;
+        if(t0 + 4 <= n) {
+            const uchar4 e = *reinterpret_cast<const uchar4*>(ext + t0);
+            const float4 w = *reinterpret_cast<const float4*>(weight + t0);
+            if(e.x == 1 && w.x < kFltMin) vel[t0] = tmp[t0];
+            if(e.y == 1 && w.y < kFltMin) vel[t0 + 1] = tmp[t0 + 1];
+            if(e.z == 1 && w.z < kFltMin) vel[t0 + 2] = tmp[t0 + 2];
+            if(e.w == 1 && w.w < kFltMin) vel[t0 + 3] = tmp[t0 + 3];
+        } else {
+            for(size_t t = t0; t < n; ++t)
+                if(ext[t] == 1 && weight[t] < kFltMin) vel[t] = tmp[t];
+        }
     }
 };
 

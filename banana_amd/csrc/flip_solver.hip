@@ -756,10 +756,10 @@ public:
             // the new values of the faces that change go to tmp first: the interpolation around one
             // face reads its neighbours, which must still hold the values before this component's pass
             launch(k_constrain{G, g, d, (const uint8_t*)ext[d]->as<uint8_t>(),
-                               (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(), tmp[d]->as<float>()}, blocks(cnt[d]));
+                               (const float*)wgt[d]->as<float>(), (const float*)bsdf.as<float>(), tmp[d]->as<float>()}, blocks((cnt[d] + 3) / 4));
             launch(k_constrain_apply{vel[d]->as<float>(), (const float*)tmp[d]->as<float>(), (const uint8_t*)ext[d]->as<uint8_t>(),
                                      (const float*)wgt[d]->as<float>(), cnt[d]},
-                   blocks(cnt[d]));
+                   blocks((cnt[d] + 3) / 4));
         }
     }
 
