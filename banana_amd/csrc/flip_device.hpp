@@ -24,9 +24,13 @@ struct GridGeom {
 
 // Particle arrays (structure of arrays). gx/gy/gz = (p - bMin) / dx (Grid.h:197), refreshed by the
 // binning permute and valid until the particles move.
+// Particle arrays, structure of arrays. gp / gv pack what the grid transfers read per particle, in
+// the cell-sorted order the binning permute writes them: gp = (grid coordinates (p - bMin) / dx, vx),
+// gv = (vy, vz); the velocity arrays stay the state (the copy in gp / gv is refreshed at each binning).
 struct Particles {
-    float *px, *py, *pz, *vx, *vy, *vz;
-    float *gx, *gy, *gz;
+    float *   px, *py, *pz, *vx, *vy, *vz;
+    float4*   gp;
+    float2*   gv;
     uint32_t* id;
 };
 

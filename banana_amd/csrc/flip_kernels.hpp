@@ -203,16 +203,29 @@ struct k_permute {
         if(i >= n) return;
         const uint32_t s = perm[i];
         const float    x = in.px[s], y = in.py[s], z = in.pz[s];
+        const float    vx = in.vx[s], vy = in.vy[s], vz = in.vz[s];
         out.px[i]        = x;
         out.py[i]        = y;
         out.pz[i]        = z;
-        out.vx[i]        = in.vx[s];
-        out.vy[i]        = in.vy[s];
-        out.vz[i]        = in.vz[s];
+        out.vx[i]        = vx;
+        out.vy[i]        = vy;
+        out.vz[i]        = vz;
         out.id[i]        = in.id[s];
-        out.gx[i]        = (x - g.bminx) / g.dx;
-        out.gy[i]        = (y - g.bminy) / g.dx;
-        out.gz[i]        = (z - g.bminz) / g.dx;
+        out.gp[i]        = make_float4((x - g.bminx) / g.dx, (y - g.bminy) / g.dx, (z - g.bminz) / g.dx, vx);
+        out.gv[i]        = make_float2(vy, vz);
+    }
+};
+
+// the velocities' copy in gp.w / gv refreshed from the velocity arrays (particles already binned)
+struct k_pack_velocity {
+    static constexpr int kThreads = kBlock;
+    Particles P;
+    uint32_t n;
+    __device__ void operator()(Blk blk) const {
+        const uint32_t i = blk.id * blockDim.x + threadIdx.x;
+        if(i >= n) return;
+        P.gp[i].w = P.vx[i];
+        P.gv[i]   = make_float2(P.vy[i], P.vz[i]);
     }
 };
 
@@ -781,12 +794,14 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
 #pragma unroll
                     for(int q = 0; q < kBatch; ++q) {
                         const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
-                        qx[q]            = P.gx[p];
-                        qy[q]            = P.gy[p];
-                        qz[q]            = P.gz[p];
-                        ux[q]            = P.vx[p];
-                        uy[q]            = P.vy[p];
-                        uz[q]            = P.vz[p];
+                        const float4   a = P.gp[p];
+                        const float2   b = P.gv[p];
+                        qx[q]            = a.x;
+                        qy[q]            = a.y;
+                        qz[q]            = a.z;
+                        ux[q]            = a.w;
+                        uy[q]            = b.x;
+                        uz[q]            = b.y;
                     }
 #pragma unroll
                     for(int q = 0; q < kBatch; ++q) {
@@ -807,8 +822,9 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
                 for(uint32_t p = pb; p < pe; ++p) {
                     float abU, abV, abW, fh, fn;
                     int   dh, dn;
-                    p2g_col_weights<kInterior>(P.gx[p], P.gy[p], P.gz[p], rowU && p < pu, rowV, rowW, g, i, j, z, fi, fj, abU, abV, abW, fh,
-                                               fn, dh, dn);
+                    const float4 gq = P.gp[p];
+                    p2g_col_weights<kInterior>(gq.x, gq.y, gq.z, rowU && p < pu, rowV, rowW, g, i, j, z, fi, fj, abU, abV, abW, fh, fn, dh,
+                                               dn);
                     // a zero weight adds exact zeros whatever v is, so v is read only when it counts
                     const float vx = abU != 0.f ? P.vx[p] : 0.f, vy = abV != 0.f ? P.vy[p] : 0.f, vz = abW != 0.f ? P.vz[p] : 0.f;
                     // the base node along z of the half-shifted (u, v) and node-aligned (w) stencils
@@ -931,12 +947,14 @@ struct k_p2g_tile {
 #pragma unroll
                 for(int q = 0; q < 8; ++q) {
                     const uint32_t p = min(p0 + (uint32_t)q, pe - 1u);
-                    B.qx[q]          = P.gx[p];
-                    B.qy[q]          = P.gy[p];
-                    B.qz[q]          = P.gz[p];
-                    B.ux[q]          = P.vx[p];
-                    B.uy[q]          = P.vy[p];
-                    B.uz[q]          = P.vz[p];
+                    const float4   a = P.gp[p];
+                    const float2   b = P.gv[p];
+                    B.qx[q]          = a.x;
+                    B.qy[q]          = a.y;
+                    B.qz[q]          = a.z;
+                    B.ux[q]          = a.w;
+                    B.uy[q]          = b.x;
+                    B.uz[q]          = b.y;
                 }
             };
             auto add = [&](const Batch& B, uint32_t p0, uint32_t pe) {
@@ -2094,7 +2112,8 @@ struct k_g2p {
     __device__ void operator()(Blk blk) const {
         const uint32_t p = blk.id * blockDim.x + threadIdx.x;
         if(p >= n) return;
-        const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+        const float4 gq = P.gp[p];
+        const float  gx = gq.x, gy = gq.y, gz = gq.z;
         float       ux, uy, uz;
         vel_from_grid(G, g, gx, gy, gz, ux, uy, uz);
         const float dx = interp3_diff(G.u, O.u, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f);
@@ -2144,7 +2163,8 @@ struct k_g2p_affine {
     #pragma clang fp contract(off)
         const uint32_t p = blk.id * blockDim.x + threadIdx.x;
         if(p >= n) return;
-        const float gx = P.gx[p], gy = P.gy[p], gz = P.gz[p];
+        const float4 gq = P.gp[p];
+        const float  gx = gq.x, gy = gq.y, gz = gq.z;
         float       v[3], gc[9];
         vel_from_grid(G, g, gx, gy, gz, v[0], v[1], v[2]);
         interp3_grad(G.u, nullptr, g.nx + 1, g.ny, g.nz, gx, gy - 0.5f, gz - 0.5f, g.dx, gc[0], gc[1], gc[2]);

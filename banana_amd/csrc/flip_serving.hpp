@@ -43,7 +43,9 @@ static size_t scene_footprint(const bnn_flip_params& prm, uint64_t n) {
             b += 12 * lnb + 28 * cap + 8 * 4 * 512 * cap;
         }
     }
-    if(n) b += 88 * (n + n / 8 + 1024);
+    // per particle: two sets of 6 float arrays, the 16 + 8-byte packed transfer records and the id
+    // (the binning writes the other set), the cell key and the permutation
+    if(n) b += 112 * (n + n / 8 + 1024);
     return b;
 }
 

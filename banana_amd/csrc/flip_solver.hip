@@ -97,10 +97,12 @@ struct DevBuf {
 };
 
 struct ParticleBufs {
-    DevBuf px, py, pz, vx, vy, vz, gx, gy, gz, id;
+    DevBuf px, py, pz, vx, vy, vz, gp, gv, id;
     DevBuf c[9];  // APIC / AFLIP affine matrices (allocated only for those schemes)
     void   reserve(size_t n, bool affine) {
-        for(DevBuf* b : {&px, &py, &pz, &vx, &vy, &vz, &gx, &gy, &gz}) b->reserve(n * 4);
+        for(DevBuf* b : {&px, &py, &pz, &vx, &vy, &vz}) b->reserve(n * 4);
+        gp.reserve(n * 16);
+        gv.reserve(n * 8);
         id.reserve(n * 4);
         if(affine)
             for(DevBuf& b : c) b.reserve(n * 4);
@@ -112,7 +114,7 @@ struct ParticleBufs {
     }
     Particles view() const {
         return Particles{px.as<float>(), py.as<float>(), pz.as<float>(), vx.as<float>(), vy.as<float>(),
-                         vz.as<float>(), gx.as<float>(), gy.as<float>(), gz.as<float>(), id.as<uint32_t>()};
+                         vz.as<float>(), gp.as<float4>(), gv.as<float2>(), id.as<uint32_t>()};
     }
 };
 
@@ -268,7 +270,8 @@ public:
     ParticleBufs *cur = &bufA, *alt = &bufB;
     DevBuf       key, perm;
     DevBuf       zsHist, zsAcc;  // z-sort digit histograms / offsets, key AND-OR
-    bool         binned = false;  // cell_start / g* valid for the current particle positions
+    bool         binned = false;     // cell_start / gp valid for the current particle positions
+    bool         packedVel = false;  // the velocities in gp.w / gv equal the velocity arrays
 
     // grid
     size_t nU = 0, nV = 0, nW = 0, nC = 0, nN = 0;
@@ -659,7 +662,8 @@ public:
         if(!np) {
             fill32(cellStart.p, 0, (nC + 1) * 4);
             fill32(occ.p, 0, occ.bytes);
-            binned = true;
+            binned    = true;
+            packedVel = true;
             return;
         }
         const ParticleBufs& P = *cur;
@@ -678,7 +682,8 @@ public:
                                (const uint32_t*)perm.as<uint32_t>(), (uint32_t)np}, blocks(np));
         ktEnd(BNN_KT_PERMUTE);
         std::swap(cur, alt);
-        binned = true;
+        binned    = true;
+        packedVel = true;
     }
 
     void requireBinned() {
@@ -700,6 +705,10 @@ public:
     }
     void mapParticles2Grid() {
         requireBinned();
+        if(!packedVel && np) {  // velocities changed since the binning (a G2P with no move after it)
+            launch(k_pack_velocity{cur->view(), (uint32_t)np}, blocks(np));
+            packedVel = true;
+        }
         FaceOut out{u.as<float>(), v.as<float>(), w.as<float>(), vu.as<uint8_t>(), vv.as<uint8_t>(), vw.as<uint8_t>()};
         ktBegin(BNN_KT_P2G);
         if(affineScheme()) {
@@ -1190,6 +1199,7 @@ public:
         else
             launch(k_g2p{cur->view(), (uint32_t)np, G, O, g, prm.pic_flip_ratio}, blocks(np));
         ktEnd(BNN_KT_G2P);
+        packedVel = false;  // new velocities; the copy in gp / gv is refreshed before the next P2G
     }
 
     Co<bool> advanceVelocity(float dt) {  // FLIP_Solver.cpp:42-53 (FLIP, AFLIP); APIC_Solver.cpp:41-48 (APIC)
@@ -1428,7 +1438,7 @@ public:
             launch(k_rs_scatter{(const uint32_t*)lo[s], (const uint32_t*)hi[s], (const uint32_t*)val[s], n, word, shift, ntiles, (const uint32_t*)offs, lo[1 - s], hi[1 - s], val[1 - s]}, ntiles);
             s = 1 - s;
         }
-        uint32_t* rank = alt->gx.as<uint32_t>();
+        uint32_t* rank = alt->gp.as<uint32_t>();
         launch(k_zsort_rank{(const uint32_t*)val[s], n, rank}, blocks(np));
         launch(k_zsort_rename{cur->id.as<uint32_t>(), n, (const uint32_t*)rank}, blocks(np));
         if(objIndex.size() == np && rankHost.size() == np && objScratch.size() == np) {
