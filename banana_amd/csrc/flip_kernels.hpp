@@ -1086,7 +1086,7 @@ struct k_p2g_col {
 // ext = 0. Any nonzero val therefore means valid.
 //==================================================================================================
 struct k_extrap_tile {
-    static constexpr int kThreads = 512;
+    static constexpr int kThreads = 256;  // one (x, y) column of the tile per thread, its 16 faces in z
     float* __restrict__ a;
     uint8_t* __restrict__ val;
     uint8_t* __restrict__ ext;
@@ -1099,57 +1099,52 @@ struct k_extrap_tile {
         __shared__ uint8_t ss[4096];
         int bx, by, bz;
         tile_of_block(blk, (sx + 7) / 8, (sy + 7) / 8, (sz + 7) / 8, bx, by, bz);
-        const int    t = threadIdx.x, x = t & 15, y = (t >> 4) & 15, z0 = t >> 8;  // faces z0, z0 + 2, .., z0 + 14
-        const int    gi = bx * 8 - 4 + x, gj = by * 8 - 4 + y;
+        const int    t = threadIdx.x, x = t & 15, y = t >> 4;  // faces (x, y, 0 .. 15)
+        const int    gi = bx * 8 - 4 + x, gj = by * 8 - 4 + y, gk0 = bz * 8 - 4;
         const bool   inXY = gi >= 0 && gi < sx && gj >= 0 && gj < sy;
         const bool   brickXY = x >= 4 && x < 12 && y >= 4 && y < 12;
-        const size_t sxy = (size_t)sx * (size_t)sy;
-        uint8_t      v0[8];  // val on entry
-        size_t       off[8];
+        const size_t sxy = (size_t)sx * (size_t)sy, col = (size_t)gj * (size_t)sx + (size_t)gi;
+        auto         off = [&](int z) { return (size_t)(gk0 + z) * sxy + col; };
+        uint8_t      v0[16];  // val on entry
         int          anyValid = 0, anyOpen = 0;
 #pragma unroll
-        for(int s = 0; s < 8; ++s) {
-            const int z = z0 + 2 * s, gk = bz * 8 - 4 + z;
-            off[s]      = (size_t)gk * sxy + (size_t)gj * (size_t)sx + (size_t)gi;
-            v0[s]       = (inXY && gk >= 0 && gk < sz) ? val[off[s]] : (uint8_t)0;
+        for(int z = 0; z < 16; ++z) {
+            const int gk = gk0 + z;
+            v0[z]        = (inXY && gk >= 0 && gk < sz) ? val[off(z)] : (uint8_t)0;
         }
 #pragma unroll
-        for(int s = 0; s < 8; ++s) {
-            const int     z = z0 + 2 * s, q = (z << 8) | (y << 4) | x;
-            const uint8_t st = (v0[s] != 0 && v0[s] != gen) ? 1 : 0;
-            const int     gk = bz * 8 - 4 + z;
-            ss[q]            = st;
+        for(int z = 0; z < 16; ++z) {
+            const uint8_t st = (v0[z] != 0 && v0[z] != gen) ? 1 : 0;
+            ss[(z << 8) | (y << 4) | x] = st;
             anyValid |= st;
-            if(!st && brickXY && z >= 4 && z < 12 && gk < sz && inXY) anyOpen = 1;
+            if(!st && brickXY && z >= 4 && z < 12 && gk0 + z < sz && inXY) anyOpen = 1;
         }
         const bool work = __syncthreads_or(anyValid) != 0 && __syncthreads_or(anyOpen) != 0;
         if(work) {
             // the values of the faces valid on entry (the others are written before they are read)
 #pragma unroll
-            for(int s = 0; s < 8; ++s) {
-                const int z = z0 + 2 * s, q = (z << 8) | (y << 4) | x;
-                if(v0[s] != 0 && v0[s] != gen) as[q] = a[off[s]];
-            }
+            for(int z = 0; z < 16; ++z)
+                if(v0[z] != 0 && v0[z] != gen) as[(z << 8) | (y << 4) | x] = a[off(z)];
             __syncthreads();
             // faces that can be filled: not on the tile's outer shell, not on the array's outer layer
             uint32_t open = 0;
 #pragma unroll
-            for(int s = 0; s < 8; ++s) {
-                const int z = z0 + 2 * s, gk = bz * 8 - 4 + z;
-                if(x >= 1 && x <= 14 && y >= 1 && y <= 14 && z >= 1 && z <= 14 && gi >= 1 && gi <= sx - 2 && gj >= 1 && gj <= sy - 2 &&
-                   gk >= 1 && gk <= sz - 2 && !(v0[s] != 0 && v0[s] != gen))
-                    open |= 1u << s;
+            for(int z = 1; z <= 14; ++z) {
+                const int gk = gk0 + z;
+                if(x >= 1 && x <= 14 && y >= 1 && y <= 14 && gi >= 1 && gi <= sx - 2 && gj >= 1 && gj <= sy - 2 && gk >= 1 && gk <= sz - 2 &&
+                   !(v0[z] != 0 && v0[z] != gen))
+                    open |= 1u << z;
             }
             for(int layer = 0; layer < 4; ++layer) {
                 const uint8_t lim = (uint8_t)(layer + 1);
-                float         res[8];
+                float         res[16];
                 uint32_t      fill = 0;
                 // all reads of the layer first (a face this layer fills is never read by it), then the writes
 #pragma unroll
-                for(int s = 0; s < 8; ++s) {
-                    res[s] = 0.f;
-                    if(!(open & (1u << s))) continue;
-                    const int q   = ((z0 + 2 * s) << 8) | (y << 4) | x;
+                for(int z = 0; z < 16; ++z) {
+                    res[z] = 0.f;
+                    if(!(open & (1u << z))) continue;
+                    const int q   = (z << 8) | (y << 4) | x;
                     float     sum = 0.f;  // starts at +0 and never becomes -0, so adding +0 for a neighbour not taken is exact
                     uint32_t  count = 0;
                     const int nb[6] = {q + 1, q - 1, q + 16, q - 16, q + 256, q - 256};
@@ -1162,15 +1157,15 @@ struct k_extrap_tile {
                         count += ok ? 1u : 0u;
                     }
                     if(count > 0) {
-                        res[s] = sum / (float)count;
-                        fill |= 1u << s;
+                        res[z] = sum / (float)count;
+                        fill |= 1u << z;
                     }
                 }
 #pragma unroll
-                for(int s = 0; s < 8; ++s) {
-                    if(!(fill & (1u << s))) continue;
-                    const int q = ((z0 + 2 * s) << 8) | (y << 4) | x;
-                    as[q]       = res[s];
+                for(int z = 0; z < 16; ++z) {
+                    if(!(fill & (1u << z))) continue;
+                    const int q = (z << 8) | (y << 4) | x;
+                    as[q]       = res[z];
                     ss[q]       = (uint8_t)(layer + 2);
                 }
                 open &= ~fill;
@@ -1179,16 +1174,16 @@ struct k_extrap_tile {
         }
         if(!brickXY || !inXY) return;
 #pragma unroll
-        for(int s = 0; s < 8; ++s) {
-            const int z = z0 + 2 * s, gk = bz * 8 - 4 + z, q = (z << 8) | (y << 4) | x;
-            if(z < 4 || z >= 12 || gk >= sz) continue;
-            const size_t  o  = off[s];
-            const uint8_t st = work ? ss[q] : (uint8_t)((v0[s] != 0 && v0[s] != gen) ? 1 : 0);
+        for(int z = 4; z < 12; ++z) {
+            if(gk0 + z >= sz) continue;
+            const int     q  = (z << 8) | (y << 4) | x;
+            const size_t  o  = off(z);
+            const uint8_t st = work ? ss[q] : (uint8_t)((v0[z] != 0 && v0[z] != gen) ? 1 : 0);
             ext[o]           = st >= 2 ? 1 : 0;
             if(st >= 2) {
                 a[o]   = as[q];
                 val[o] = gen;
-            } else if(st == 1 && v0[s] != 1) {
+            } else if(st == 1 && v0[z] != 1) {
                 val[o] = 1;
             }
         }
