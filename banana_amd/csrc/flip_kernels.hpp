@@ -179,6 +179,32 @@ struct k_segsort {
         if(c >= ncells) return;
         const uint32_t b = cell_start[c], e = cell_start[c + 1];
         if(e - b < 2) return;
+        if(e - b <= 16) {
+            // up to 16 entries in registers (padding sorts to the end): most cells arrive in order and
+            // are left as they are; the others are sorted by odd-even transposition (the entries are
+            // distinct particle indices, so the order is the one the insertion sort below gives)
+            const uint32_t n = e - b;
+            uint32_t       r[16];
+#pragma unroll
+            for(uint32_t k = 0; k < 16; ++k) r[k] = k < n ? perm[b + k] : 0xFFFFFFFFu;
+            bool sorted = true;
+#pragma unroll
+            for(uint32_t k = 1; k < 16; ++k) sorted &= r[k - 1] <= r[k];
+            if(sorted) return;
+#pragma unroll
+            for(int ph = 0; ph < 16; ++ph) {
+#pragma unroll
+                for(int k = ph & 1; k + 1 < 16; k += 2) {
+                    const uint32_t lo = min(r[k], r[k + 1]), hi = max(r[k], r[k + 1]);
+                    r[k]              = lo;
+                    r[k + 1]          = hi;
+                }
+            }
+#pragma unroll
+            for(uint32_t k = 0; k < 16; ++k)
+                if(k < n) perm[b + k] = r[k];
+            return;
+        }
         for(uint32_t i = b + 1; i < e; ++i) {
             uint32_t v = perm[i];
             uint32_t j = i;
