@@ -400,8 +400,9 @@ bnn_status  bnn_flip_footprint(const bnn_flip_params* p, uint64_t n_particles, u
  * items, all on one stream; the host reads device values (brick counts, stop flags) for all scenes
  * with one copy and one wait; launch sequences that repeat (PCG iterations) are replayed from
  * hipGraphs. A scene's blocks compute exactly what they compute alone, so each scene's results are
- * bit-identical to stepping it alone, in any batch. Launch shapes that cannot change results (P2G
- * column segment, PCG iterations per stop-flag poll, scene blocks per workgroup, graphs on or off)
+ * bit-identical to stepping it alone, in any batch. Launch shapes that cannot change results (the
+ * APIC / AFLIP P2G column segment, PCG iterations per stop-flag poll, scene blocks per workgroup,
+ * graphs on or off)
  * are chosen by an autotuner over the first calls of each batch size, or pinned with
  * bnn_flip_batch_set_shape. The scenes stay owned by their handles; a handle must not be used
  * elsewhere while a batch call runs on it. */
