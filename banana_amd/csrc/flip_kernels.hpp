@@ -2541,6 +2541,36 @@ struct k_mg_prolong {
 // kDot (finest level, post-smoother): also writes the brick's partial of z.r (z = the result,
 // r = L.b) to pz[slot], the PCG's next rho.
 template<int kProlong, int kDot>
+__device__ __forceinline__ void mg_smooth_one(const LevelDev& L, const LevelDev& C, float omega, int sweeps, double* __restrict__ pz, int s,
+                                              float* xs, double* sh) {
+    const int t = threadIdx.x, ly = (t >> 2) & 7, lz = t >> 5;
+    const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
+    const int lr = ((ly + lz) & 1) ? l0 + 1 : l0, lb = ((ly + lz) & 1) ? l0 : l0 + 1;  // red (colour 0), black
+    const int     b      = L.slotBrick[s];
+    const int     bx     = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
+    const int32_t parent = kProlong == 1 ? L.ptab[8 * s + 6] : -1;
+    const SmoothCell R = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lr, bx, by, bz, xs);
+    const SmoothCell K = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lb, bx, by, bz, xs);
+    smooth_halo<kProlong>(L, C, parent, omega, s, bx, by, bz, xs);
+    __syncthreads();
+    for(int it = 0; it < sweeps; ++it) {
+        smooth_cell_update(kProlong ? K : R, xs);
+        __syncthreads();
+        smooth_cell_update(kProlong ? R : K, xs);
+        __syncthreads();
+    }
+    float*      out             = kProlong ? L.x : L.r;
+    const float zr = xs[R.c], zb = xs[K.c];
+    out[(size_t)s * 512 + lr]   = zr;
+    out[(size_t)s * 512 + lb]   = zb;
+    if(kDot) {
+        const double d = block_sum<256>((double)zr * (double)R.bv + (double)zb * (double)K.bv, sh);
+        if(t == 0) pz[s] = d;
+    }
+    __syncthreads();  // xs is reused by the workgroup's next brick
+}
+
+template<int kProlong, int kDot>
 struct k_mg_smooth_brick {
     static constexpr int kThreads = 256;
     LevelDev L;
@@ -2553,33 +2583,46 @@ struct k_mg_smooth_brick {
         __shared__ float xs[10 * 10 * 10];  // the brick and a one-cell halo, x-fastest
         __shared__ double sh[4];
         if(st && st->done) return;
-        const int t = threadIdx.x, ly = (t >> 2) & 7, lz = t >> 5;
-        const int l0 = (lz * 8 + ly) * 8 + 2 * (t & 3);        // even x; its colour is (ly + lz) & 1
-        const int lr = ((ly + lz) & 1) ? l0 + 1 : l0, lb = ((ly + lz) & 1) ? l0 : l0 + 1;  // red (colour 0), black
+        for(uint32_t v = blk.id; v < L.nSlots; v += blk.n)
+            mg_smooth_one<kProlong, kDot>(L, C, omega, sweeps, pz, (int)xcd_slot(v, L.nSlots), xs, sh);
+    }
+};
+
+// The PCG update fused with the V-cycle's first step on the finest level (multigrid): per brick,
+// x += alpha s and r -= alpha q (PCGSolver.cpp:178-181), the brick's max|r| to pm[slot], then the
+// pre-smoother on b = r, which reads only the brick's own cells (it starts from x = 0). The finest
+// level's r array is the PCG's q buffer: this brick's q is read before the smoother overwrites it.
+// Per cell the same arithmetic as k_pcg_update followed by k_mg_smooth_brick<0, 0>.
+struct k_pcg_update_pre {
+    static constexpr int kThreads = 256;
+    const float* __restrict__ s;
+    float* __restrict__ x;
+    LevelDev L;  // the finest level: L.b = r, L.r = q
+    float omega;
+    int sweeps;
+    const PcgState* __restrict__ st;
+    double* __restrict__ pm;
+    __device__ void operator()(Blk blk) const {
+        __shared__ float  xs[10 * 10 * 10];
+        __shared__ double sh[4];
+        if(st && st->done) return;
+        const float alpha = (float)st->alpha;
+        const int   t     = threadIdx.x;
         for(uint32_t v = blk.id; v < L.nSlots; v += blk.n) {
-            const int     s      = (int)xcd_slot(v, L.nSlots);
-            const int     b      = L.slotBrick[s];
-            const int     bx     = b % L.nbx, by = (b / L.nbx) % L.nby, bz = b / (L.nbx * L.nby);
-            const int32_t parent = kProlong == 1 ? L.ptab[8 * s + 6] : -1;
-            const SmoothCell R = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lr, bx, by, bz, xs);
-            const SmoothCell K = smooth_cell_setup<kProlong>(L, C, parent, omega, s, lb, bx, by, bz, xs);
-            smooth_halo<kProlong>(L, C, parent, omega, s, bx, by, bz, xs);
-            __syncthreads();
-            for(int it = 0; it < sweeps; ++it) {
-                smooth_cell_update(kProlong ? K : R, xs);
-                __syncthreads();
-                smooth_cell_update(kProlong ? R : K, xs);
-                __syncthreads();
+            const int slot = (int)xcd_slot(v, L.nSlots);
+            double    m    = 0;
+            for(int h = 0; h < 2; ++h) {
+                const size_t o  = (size_t)slot * 512 + t + 256 * h;
+                const float  xv = x[o] + alpha * s[o];
+                const float  rv = L.b[o] + (-alpha) * L.r[o];
+                x[o]            = xv;
+                L.b[o]          = rv;
+                m               = fmax(m, (double)fabsf(rv));
             }
-            float*      out             = kProlong ? L.x : L.r;
-            const float zr = xs[R.c], zb = xs[K.c];
-            out[(size_t)s * 512 + lr]   = zr;
-            out[(size_t)s * 512 + lb]   = zb;
-            if(kDot) {
-                const double d = block_sum<256>((double)zr * (double)R.bv + (double)zb * (double)K.bv, sh);
-                if(t == 0) pz[s] = d;
-            }
-            __syncthreads();  // xs is reused by the block's next brick
+            m = block_max<256>(m, sh);
+            if(t == 0) pm[slot] = m;
+            __syncthreads();  // the brick's r, written by other threads of the workgroup, is the smoother's b
+            mg_smooth_one<0, 0>(L, L, omega, sweeps, nullptr, slot, xs, sh);
         }
     }
 };

@@ -450,7 +450,7 @@ public:
         slotBrick.reserve(c * 4);
         nbrTab.reserve((size_t)c * 6 * 4);
         pa.reserve((size_t)kMaxRedBlocks * 8);
-        pb.reserve((size_t)kMaxRedBlocks * 8);
+        pb.reserve((size_t)std::max<uint32_t>(kMaxRedBlocks, c) * 8);  // per block, or per brick (k_pcg_update_pre)
         pz.reserve((size_t)c * 8);
         if(prm.preconditioner == BNN_PRECOND_IC0)
             for(auto* b : {&icInv, &icLx, &icLy, &icLz}) b->reserve(nv * 4);
@@ -935,15 +935,18 @@ public:
     // One block per brick. On each level the pre-smoothed x lives in L.r and the final x in L.x.
     // Returns true when the fine level's post-smoother also wrote the per-brick partials of z.r to pz
     // (false when the fine level is the coarsest: the caller then forms z.r itself).
-    bool vcycle(const PcgState* stp) {
+    // finePreDone: the finest level's pre-smoother already ran (fused with the PCG update).
+    bool vcycle(const PcgState* stp, bool finePreDone = false) {
         const int   nu    = prm.mg_sweeps;
         const float omega = prm.mg_omega;
         for(int l = 0; l + 1 < mgLevels; ++l) {
             const LevelDev L = levelDev(l), C = levelDev(l + 1);
             const unsigned G = std::max<uint32_t>(L.nSlots, 1u);
-            ktBegin(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
-            launch(k_mg_smooth_brick<0, 0>{L, C, omega, nu, stp, (double*)nullptr}, G);
-            ktEnd(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
+            if(!(l == 0 && finePreDone)) {
+                ktBegin(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
+                launch(k_mg_smooth_brick<0, 0>{L, C, omega, nu, stp, (double*)nullptr}, G);
+                ktEnd(l == 0 ? BNN_KT_MG_PRE : BNN_KT_MG_COARSE);
+            }
             ktBegin(l == 0 ? BNN_KT_MG_RESTRICT : BNN_KT_MG_COARSE);
             launch(k_mg_residual_restrict{L, C, stp}, G);
             ktEnd(l == 0 ? BNN_KT_MG_RESTRICT : BNN_KT_MG_COARSE);
@@ -1024,18 +1027,21 @@ public:
         int           nrz = (int)G;
         const bool    ic0 = prm.preconditioner == BNN_PRECOND_IC0;
         if(ic0) formIC0();
-        auto          zr  = [&](const PcgState* stp) {
+        auto          zr  = [&](const PcgState* stp, bool finePreDone) {
             if(ic0) {
                 applyIC0(r.as<float>(), z.as<float>(), stp);
                 launch(k_pcg_dot_zr{(const float*)z.as<float>(), (const float*)r.as<float>(), nSlots, stp, dpa}, G);
-            } else if(vcycle(stp)) {
+            } else if(vcycle(stp, finePreDone)) {
                 rz  = pz.as<double>();
                 nrz = (int)nSlots;
             } else {
                 launch(k_pcg_dot_zr{(const float*)z.as<float>(), (const float*)r.as<float>(), nSlots, stp, dpa}, G);
             }
         };
-        if(!jac) zr(nullptr);
+        if(!jac) zr(nullptr, false);
+        // multigrid over several levels: the update is fused with the finest pre-smoother, and its
+        // max|r| partials are per brick
+        const bool fuseUpdate = !jac && !ic0 && mgLevels > 1;
         launch(k_pcg_init_fin{dst, rz, nrz, (const double*)dpb, (int)G, relTol, maxIter}, 1);
         uint32_t chunk = pcgChunkFirst, launched = 0;
         float    res0  = -1.f;  // max|r| at iteration 0 (= max|b|, x starts at 0)
@@ -1057,10 +1063,14 @@ public:
                 ktEnd(BNN_KT_PCG_SPMV);
                 launch(k_pcg_alpha{dst, (const double*)dpa, (int)G}, 1);
                 ktBegin(BNN_KT_PCG_UPDATE);
-                launch(k_pcg_update{(const float*)sB, (const float*)q.as<float>(), (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst, dpa, dpb, jac}, G);
+                if(fuseUpdate)
+                    launch(k_pcg_update_pre{(const float*)sB, x.as<float>(), fineLevel(), prm.mg_omega, prm.mg_sweeps, (const PcgState*)dst, dpb},
+                           std::max<uint32_t>(nSlots, 1u));
+                else
+                    launch(k_pcg_update{(const float*)sB, (const float*)q.as<float>(), (const float*)invd.as<float>(), x.as<float>(), r.as<float>(), z.as<float>(), nSlots, (const PcgState*)dst, dpa, dpb, jac}, G);
                 ktEnd(BNN_KT_PCG_UPDATE);
-                if(!jac) zr(dst);
-                launch(k_pcg_beta{dst, rz, nrz, (const double*)dpb, (int)G}, 1);
+                if(!jac) zr(dst, fuseUpdate);
+                launch(k_pcg_beta{dst, rz, nrz, (const double*)dpb, fuseUpdate ? (int)nSlots : (int)G}, 1);
                 std::swap(sA, sB);
             }
             launched += n;

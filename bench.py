@@ -57,7 +57,8 @@ KERNEL_BYTES = {
     "fluid_sdf": lambda Np, F, C, V, A: 12 * Np + 8 * C + 4 * V,  # coords, cell ranges, phi; boundary SDF
     "permute": lambda Np, F, C, V, A: 72 * Np,              # order + 7 words in, 7 words + coords out
     "pcg_spmv": lambda Np, F, C, V, A: 32 * A,              # z, s, 4 coefficients in; s, q out
-    "pcg_update": lambda Np, F, C, V, A: 24 * A,            # x, r, s, q in; x, r out
+    "pcg_update": lambda Np, F, C, V, A: 44 * A,            # multigrid: fused with the finest pre-smoother -- x, r, s, q,
+                                                            # 3 couplings, 1/diag in; x, r, pre-smoothed z out
     "mg_pre": lambda Np, F, C, V, A: 24 * A,                # 3 couplings, b, 1/diag in; x out
     "mg_restrict": lambda Np, F, C, V, A: 24.5 * A,         # b, diag, 3 couplings, x in; coarse b out
     "mg_prolong": lambda Np, F, C, V, A: 12.5 * A,          # x, 1/diag, coarse x in; x out
