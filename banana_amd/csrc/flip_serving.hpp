@@ -778,6 +778,34 @@ private:
                         f->t1            = tn;
                     }
                 }
+                if(fl.size() == 1 && fresh.size() == 1) {
+                    // a lone request (nothing else in flight or ready): it runs directly on its scene's
+                    // stream, without the recorder's per-sync-point flush; requests that arrive meanwhile
+                    // join the next round. The last batched round ended with a wait, and a direct run
+                    // ends with one, so the scene's work stays in order across the two modes.
+                    Flight* f = fresh[0];
+                    try {
+                        f->task.ok = run_now(f->kind == BNN_REQ_STEP ? f->sv->step(f->dt) : f->sv->projectionRequest(f->dt));
+                    } catch(const Error& e) {
+                        f->task.status = e.code;
+                        f->task.err    = e.what();
+                        (void)hipStreamSynchronize(f->sv->stream);
+                    } catch(const std::exception& e) {
+                        f->task.status = BNN_ERR_DEVICE;
+                        f->task.err    = e.what();
+                        (void)hipStreamSynchronize(f->sv->stream);
+                    }
+                    f->task.done = true;
+                    {
+                        std::unique_lock<std::mutex> lk(m);
+                        lanesSince += 1;
+                        roundsSince += 1;
+                    }
+                    std::unique_ptr<Flight> one = std::move(fl[0]);
+                    fl.clear();
+                    finish(*one);
+                    continue;
+                }
                 for(Flight* f : fresh) {
                     try {
                         R.start(f->task, f->sv, f->kind == BNN_REQ_STEP ? f->sv->step(f->dt) : f->sv->projectionRequest(f->dt));
