@@ -193,6 +193,31 @@ def test_g2p_bit_exact(oracle, bnn):
     assert np.array_equal(vg, vo), np.abs(vg - vo).max()
 
 
+def test_p2g_after_g2p_sees_new_velocities(oracle, bnn):
+    """P2G right after G2P, with no move and so no binning in between, splats the velocities G2P
+    wrote: the binning's packed copy of the velocities is refreshed first."""
+    params, pos, vel, g, o = make_pair(oracle, bnn, n=32, sphere=True, vel_sigma=0.5, shuffle=True)
+    o.map_particles_to_grid()
+    v_before = o.field(oracle.F_V).astype(np.float32)
+    o.extrapolate_velocity()
+    o.backup_grid_velocity()
+    o.add_gravity(DT)
+    for c, f, fo in (("u", oracle.F_U, oracle.F_OLD_U), ("v", oracle.F_V, oracle.F_OLD_V), ("w", oracle.F_W, oracle.F_OLD_W)):
+        g.set_grid(c, o.field(f))
+        g.set_grid("old_" + c, o.field(fo))
+    g.map_grid_to_particles()
+    o.compute_changes_grid_velocity()
+    o.map_grid_to_particles()
+    g.map_particles_to_grid()
+    o.map_particles_to_grid()
+    for c, f, fv in (("u", oracle.F_U, oracle.F_VALID_U), ("v", oracle.F_V, oracle.F_VALID_V), ("w", oracle.F_W, oracle.F_VALID_W)):
+        a, b = g.grid(c), o.field(f)
+        assert np.array_equal(g.grid("valid_" + c), o.field(fv)), c
+        assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, np.abs(b).max()), c
+    # gravity's -9.81 dt reached the particles and comes back on the v faces
+    assert np.abs(g.grid("v") - v_before).max() > 1e-3
+
+
 def test_dam_break_step1_known_answer(oracle, bnn):
     """From rest with the reference's default boundary handling the RHS is exactly zero: PCG runs
     0 iterations, particles do not move and every velocity is (0, -9.81 dt, 0)."""
