@@ -2266,22 +2266,27 @@ __device__ __forceinline__ int32_t level_cell_index(const LevelDev& L, int i, in
     return s < 0 ? -1 : s * 512 + ((k & 7) * 8 + (j & 7)) * 8 + (i & 7);
 }
 
-// flag the coarse bricks that hold children of active fine bricks
-struct k_mg_flag_parents {
+// Every coarse level's active-brick flags in one pass: a brick of level l is active when a finest
+// active brick lies below it (the parent of an active brick is active, level by level), and the
+// level-l brick of finest brick (bx, by, bz) is (bx >> l, by >> l, bz >> l). Each writes the value 1,
+// so concurrent writes to one flag agree.
+constexpr int kMaxMgLevels = 16;
+struct k_mg_flag_levels {
     static constexpr int kThreads = kBlock;
     const int32_t* __restrict__ fineSlotBrick;
     uint32_t nFine;
     int fnbx;
     int fnby;
-    int cnbx;
-    int cnby;
-    uint32_t* __restrict__ cflag;
+    int nLevels;  // coarse levels 1 .. nLevels
+    int cnbx[kMaxMgLevels];
+    int cnby[kMaxMgLevels];
+    uint32_t* flag[kMaxMgLevels];
     __device__ void operator()(Blk blk) const {
         const uint32_t s = blk.id * blockDim.x + threadIdx.x;
         if(s >= nFine) return;
         const int b = fineSlotBrick[s];
         const int bx = b % fnbx, by = (b / fnbx) % fnby, bz = b / (fnbx * fnby);
-        cflag[((bz >> 1) * cnby + (by >> 1)) * cnbx + (bx >> 1)] = 1u;
+        for(int l = 1; l <= nLevels; ++l) flag[l][((bz >> l) * cnby[l] + (by >> l)) * cnbx[l] + (bx >> l)] = 1u;
     }
 };
 

@@ -878,16 +878,23 @@ public:
                         L.x.as<float>(), L.b.as<float>(), L.r.as<float>(), L.nSlots, L.ptab.as<int32_t>()};
     }
 
+    // Coarse levels halve the grid until one brick remains (or every axis fits one brick). All levels'
+    // active bricks are flagged in one pass from the finest level's and counted together, so the
+    // host waits once for the whole hierarchy.
     Co<void> buildHierarchy() {
         mgLevels = 1;
         if(mg.empty()) mg.emplace_back(nullptr);
-        int      nx = g.nx, ny = g.ny, nz = g.nz;
-        uint32_t prevSlots = nSlots;
-        while(prevSlots > kCoarsestSlots && (nx > 8 || ny > 8 || nz > 8)) {
-            const int l = mgLevels;
+        if(nSlots <= kCoarsestSlots) co_return;
+        k_mg_flag_levels fl{};
+        fl.fineSlotBrick = slotBrick.as<int32_t>();
+        fl.nFine         = nSlots;
+        fl.fnbx          = g.nbx;
+        fl.fnby          = g.nby;
+        int nx = g.nx, ny = g.ny, nz = g.nz, nCand = 0;
+        while((nx > 8 || ny > 8 || nz > 8) && nCand + 1 < kMaxMgLevels) {
+            const int l = ++nCand;
             if((int)mg.size() <= l) mg.emplace_back(new MgLevel());
-            MgLevel&       L    = *mg[l];
-            const LevelDev prev = levelDev(l - 1);
+            MgLevel& L = *mg[l];
             nx = (nx + 1) / 2;
             ny = (ny + 1) / 2;
             nz = (nz + 1) / 2;
@@ -898,12 +905,22 @@ public:
             L.scan.reserve(L.nb * 4);
             L.brickSlot.reserve(L.nb * 4);
             fill32(L.flag.p, 0, L.nb * 4);
-            fill32(counters.p, 0, 16);
-            launch(k_mg_flag_parents{prev.slotBrick, prev.nSlots, prev.nbx, prev.nby, L.nbx, L.nby, L.flag.as<uint32_t>()}, blocks(prev.nSlots));
-            scan(L.flag.as<uint32_t>(), L.scan.as<uint32_t>(), L.nb, counters.as<uint32_t>() + 2);
-            readback(hostU32, counters.p, 16);
-            co_await syncPoint();
-            L.nSlots = hostU32[2];
+            fl.cnbx[l] = L.nbx;
+            fl.cnby[l] = L.nby;
+            fl.flag[l] = L.flag.as<uint32_t>();
+        }
+        if(nCand == 0) co_return;
+        fl.nLevels = nCand;
+        launch(fl, blocks(nSlots));
+        uint32_t* count = counters.as<uint32_t>() + 16;  // level l's active brick count at count[l]
+        for(int l = 1; l <= nCand; ++l) scan(mg[l]->flag.as<uint32_t>(), mg[l]->scan.as<uint32_t>(), mg[l]->nb, count + l);
+        readback(hostU32, count, (size_t)(nCand + 1) * 4);
+        co_await syncPoint();
+        uint32_t prevSlots = nSlots;
+        for(int l = 1; l <= nCand && prevSlots > kCoarsestSlots; ++l) {
+            MgLevel&       L    = *mg[l];
+            const LevelDev prev = levelDev(l - 1);
+            L.nSlots            = hostU32[l];
             if(L.nSlots == 0 || L.nSlots > L.nb) throw Error(BNN_ERR_DEVICE, "multigrid level has an invalid brick count");
             if(L.nSlots > L.slotCap) {
                 const uint32_t c  = L.nSlots + L.nSlots / 4 + 4;
@@ -914,7 +931,7 @@ public:
                 L.slotCap = c;
             }
             GridGeom lg = g;
-            lg.nx = nx; lg.ny = ny; lg.nz = nz;
+            lg.nx = L.nx; lg.ny = L.ny; lg.nz = L.nz;
             lg.nbx = L.nbx; lg.nby = L.nby; lg.nbz = L.nbz;
             launch(k_brick_slots{(const uint32_t*)L.flag.as<uint32_t>(), (const uint32_t*)L.scan.as<uint32_t>(), L.nb, L.brickSlot.as<int32_t>(), L.slotBrick.as<int32_t>()}, blocks(L.nb));
             launch(k_brick_nbrs{(const int32_t*)L.slotBrick.as<int32_t>(), L.nSlots, (const int32_t*)L.brickSlot.as<int32_t>(), lg, L.nbr.as<int32_t>()}, blocks(L.nSlots));
