@@ -1060,19 +1060,19 @@ public:
         // max|r| partials are per brick
         const bool fuseUpdate = !jac && !ic0 && mgLevels > 1;
         launch(k_pcg_init_fin{dst, rz, nrz, (const double*)dpb, (int)G, relTol, maxIter}, 1);
+        // the first chunk is queued without a poll (the stop flag ends it early when the system is
+        // already solved); each later poll sizes the next chunk from max|r| at iteration 0, which is
+        // max|b| since x starts at 0
         uint32_t chunk = pcgChunkFirst, launched = 0;
-        float    res0  = -1.f;  // max|r| at iteration 0 (= max|b|, x starts at 0)
-        for(;;) {
-            readback(hostSt, dst, sizeof(PcgState));
-            co_await syncPoint();
-            if(hostSt->done) break;
-            if(res0 < 0.f) {
-                res0 = hostSt->res;
-            } else {
-                chunk = pcgNextChunk(res0, hostSt->res, hostSt->tol, hostSt->iter, chunk);
+        for(bool first = true;; first = false) {
+            if(!first) {
+                readback(hostSt, dst, sizeof(PcgState));
+                co_await syncPoint();
+                if(hostSt->done) break;
+                chunk = pcgNextChunk(hostSt->bmax, hostSt->res, hostSt->tol, hostSt->iter, chunk);
             }
             // never queue more iterations than can remain
-            const uint32_t left = hostSt->maxIter > hostSt->iter ? hostSt->maxIter - hostSt->iter : 1u;
+            const uint32_t left = first ? maxIter : (hostSt->maxIter > hostSt->iter ? hostSt->maxIter - hostSt->iter : 1u);
             const uint32_t n    = std::min(chunk, left);
             for(uint32_t c = 0; c < n; ++c) {
                 ktBegin(BNN_KT_PCG_SPMV);
