@@ -204,17 +204,18 @@ typedef enum bnn_stage {
 typedef enum bnn_kernel_timer {
     BNN_KT_PCG_SPMV    = 0,  /* s = z + beta s; q = A s; s.q */
     BNN_KT_PCG_UPDATE  = 1,  /* x, r updates; max|r| (Jacobi: also z = M^-1 r and z.r; with the
-                                multigrid preconditioner z.r comes from the V-cycle) */
+                                multigrid preconditioner over several levels, the same launch also
+                                runs the finest pre-smoother, and z.r comes from the V-cycle) */
     BNN_KT_P2G         = 2,
     BNN_KT_FLUID_SDF   = 3,
     BNN_KT_G2P         = 4,
     BNN_KT_MOVE        = 5,
-    BNN_KT_MG_PRE      = 6,  /* multigrid pre-smoother, finest level */
+    BNN_KT_MG_PRE      = 6,  /* multigrid pre-smoother, finest level (the first V-cycle of a solve) */
     BNN_KT_MG_RESTRICT = 7,  /* residual + restriction, finest level */
     BNN_KT_MG_PROLONG  = 8,  /* coarse-grid correction, finest level */
     BNN_KT_MG_POST     = 9,  /* post-smoother (and z.r partials), finest level */
     BNN_KT_MG_COARSE   = 10, /* every V-cycle launch on the coarser levels */
-    BNN_KT_EXTRAP      = 11, /* extrapolation layer passes */
+    BNN_KT_EXTRAP      = 11, /* extrapolation, one launch per face component (all four layers) */
     BNN_KT_PERMUTE     = 12  /* binning payload permute */
 } bnn_kernel_timer;
 
