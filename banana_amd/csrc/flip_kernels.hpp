@@ -1453,6 +1453,7 @@ struct k_fluid_sdf {
     GridGeom g;
     const float* __restrict__ bsdf;
     float* __restrict__ phi;
+    const uint8_t* __restrict__ occ;  // particle occupancy of the 8^3 cell bricks (the same binning)
     __device__ void operator()(Blk blk) const {
         const int ntx = (g.nx + kTileX - 1) / kTileX, nty = (g.ny + kTileY - 1) / kTileY, ntz = (g.nz + kTileZ - 1) / kTileZ;
         int       tx, ty, tz;
@@ -1471,7 +1472,13 @@ struct k_fluid_sdf {
         // bound, that class and all farther ones cannot lower the minimum. min() is exact, so visiting
         // fewer cells gives the same bits.
         const float bound[4] = {0.f, 0.499f * g.dx, 0.7061068f * g.dx, 0.8650254f * g.dx};
-        for(int cls = 0; cls < 4; ++cls) {
+        // no particle in the bricks that hold the 3x3x3 cells around this one: nothing to visit, the
+        // distance stays at its initial value (the scan below would find every cell empty)
+        bool near = false;
+        for(int bz = max(k - 1, 0) >> 3; bz <= (min(k + 1, g.nz - 1) >> 3); ++bz)
+            for(int by = max(j - 1, 0) >> 3; by <= (min(j + 1, g.ny - 1) >> 3); ++by)
+                for(int bx = max(i - 1, 0) >> 3; bx <= (min(i + 1, g.nx - 1) >> 3); ++bx) near |= occ[(bz * g.nby + by) * g.nbx + bx] != 0;
+        for(int cls = 0; cls < 4 && near; ++cls) {
             if(bestLen < bound[cls]) break;
             for(int dz = -1; dz <= 1; ++dz)
                 for(int dy = -1; dy <= 1; ++dy)

@@ -800,8 +800,9 @@ public:
         requireBinned();
         ktBegin(BNN_KT_FLUID_SDF);
         const unsigned tiles = blocks(g.nx, kTileX) * blocks(g.ny, kTileY) * blocks(g.nz, kTileZ);
-        launch(k_fluid_sdf{cur->view(),
-                           (const uint32_t*)cellStart.as<uint32_t>(), g, (const float*)bsdf.as<float>(), phi.as<float>()}, tiles);
+        launch(k_fluid_sdf{cur->view(), (const uint32_t*)cellStart.as<uint32_t>(), g, (const float*)bsdf.as<float>(), phi.as<float>(),
+                           (const uint8_t*)occ.as<uint8_t>()},
+               tiles);
         ktEnd(BNN_KT_FLUID_SDF);
     }
 
