@@ -1113,6 +1113,7 @@ struct k_p2g_col {
 //==================================================================================================
 struct k_extrap_tile {
     static constexpr int kThreads = 256;  // one (x, y) column of the tile per thread, its 16 faces in z
+    static constexpr int kManyBlocks = 4;  // batched: at most 128 registers (a tighter bound spills)
     float* __restrict__ a;
     uint8_t* __restrict__ val;
     uint8_t* __restrict__ ext;
@@ -1132,6 +1133,10 @@ struct k_extrap_tile {
         const size_t sxy = (size_t)sx * (size_t)sy, col = (size_t)gj * (size_t)sx + (size_t)gi;
         auto         off = [&](int z) { return (size_t)(gk0 + z) * sxy + col; };
         uint8_t      v0[16];  // val on entry
+        // the thread's own column: mark (0 not valid, 1 valid on entry, layer + 2 filled by that layer)
+        // and value; its z neighbours are read from here, its x and y neighbours from the LDS copy
+        uint8_t      cs[16];
+        float        ca[16];
         int          anyValid = 0, anyOpen = 0;
 #pragma unroll
         for(int z = 0; z < 16; ++z) {
@@ -1142,6 +1147,8 @@ struct k_extrap_tile {
         for(int z = 0; z < 16; ++z) {
             const uint8_t st = (v0[z] != 0 && v0[z] != gen) ? 1 : 0;
             ss[(z << 8) | (y << 4) | x] = st;
+            cs[z]                       = st;
+            ca[z]                       = 0.f;
             anyValid |= st;
             if(!st && brickXY && z >= 4 && z < 12 && gk0 + z < sz && inXY) anyOpen = 1;
         }
@@ -1150,36 +1157,44 @@ struct k_extrap_tile {
             // the values of the faces valid on entry (the others are written before they are read)
 #pragma unroll
             for(int z = 0; z < 16; ++z)
-                if(v0[z] != 0 && v0[z] != gen) as[(z << 8) | (y << 4) | x] = a[off(z)];
+                if(cs[z]) {
+                    ca[z]                       = a[off(z)];
+                    as[(z << 8) | (y << 4) | x] = ca[z];
+                }
             __syncthreads();
             // faces that can be filled: not on the tile's outer shell, not on the array's outer layer
             uint32_t open = 0;
 #pragma unroll
             for(int z = 1; z <= 14; ++z) {
                 const int gk = gk0 + z;
-                if(x >= 1 && x <= 14 && y >= 1 && y <= 14 && gi >= 1 && gi <= sx - 2 && gj >= 1 && gj <= sy - 2 && gk >= 1 && gk <= sz - 2 &&
-                   !(v0[z] != 0 && v0[z] != gen))
+                if(x >= 1 && x <= 14 && y >= 1 && y <= 14 && gi >= 1 && gi <= sx - 2 && gj >= 1 && gj <= sy - 2 && gk >= 1 && gk <= sz - 2 && !cs[z])
                     open |= 1u << z;
             }
+            // Layer l (0 .. 3) updates only the faces within 3 - l cells (per axis) of the brick: a face
+            // farther out reaches the brick through fewer than the layers left, so leaving it unfilled
+            // changes no brick face, and every face a layer reads lies within the previous layer's region.
+#pragma unroll
             for(int layer = 0; layer < 4; ++layer) {
-                const uint8_t lim = (uint8_t)(layer + 1);
-                float         res[16];
-                uint32_t      fill = 0;
+                const uint8_t  lim = (uint8_t)(layer + 1);
+                const int      lo = layer + 1, hi = 14 - layer;
+                const uint32_t act = (x >= lo && x <= hi && y >= lo && y <= hi) ? open & (((2u << hi) - 1u) & ~((1u << lo) - 1u)) : 0u;
+                float          res[16];
+                uint32_t       fill = 0;
                 // all reads of the layer first (a face this layer fills is never read by it), then the writes
 #pragma unroll
-                for(int z = 0; z < 16; ++z) {
+                for(int z = 1; z <= 14; ++z) {
                     res[z] = 0.f;
-                    if(!(open & (1u << z))) continue;
+                    if(!(act & (1u << z))) continue;
                     const int q   = (z << 8) | (y << 4) | x;
                     float     sum = 0.f;  // starts at +0 and never becomes -0, so adding +0 for a neighbour not taken is exact
                     uint32_t  count = 0;
-                    const int nb[6] = {q + 1, q - 1, q + 16, q - 16, q + 256, q - 256};
+                    // neighbours in the order +x, -x, +y, -y, +z, -z
+                    const uint8_t m[6]  = {ss[q + 1], ss[q - 1], ss[q + 16], ss[q - 16], cs[z + 1], cs[z - 1]};
+                    const float   av[6] = {as[q + 1], as[q - 1], as[q + 16], as[q - 16], ca[z + 1], ca[z - 1]};
 #pragma unroll
                     for(int e = 0; e < 6; ++e) {
-                        const uint8_t m  = ss[nb[e]];
-                        const float   av = as[nb[e]];
-                        const bool    ok = m >= 1 && m <= lim;
-                        sum += ok ? av : 0.f;
+                        const bool ok = m[e] >= 1 && m[e] <= lim;
+                        sum += ok ? av[e] : 0.f;
                         count += ok ? 1u : 0u;
                     }
                     if(count > 0) {
@@ -1188,26 +1203,27 @@ struct k_extrap_tile {
                     }
                 }
 #pragma unroll
-                for(int z = 0; z < 16; ++z) {
+                for(int z = 1; z <= 14; ++z) {
                     if(!(fill & (1u << z))) continue;
                     const int q = (z << 8) | (y << 4) | x;
                     as[q]       = res[z];
                     ss[q]       = (uint8_t)(layer + 2);
+                    ca[z]       = res[z];
+                    cs[z]       = (uint8_t)(layer + 2);
                 }
                 open &= ~fill;
-                __syncthreads();
+                if(layer < 3) __syncthreads();
             }
         }
         if(!brickXY || !inXY) return;
 #pragma unroll
         for(int z = 4; z < 12; ++z) {
             if(gk0 + z >= sz) continue;
-            const int     q  = (z << 8) | (y << 4) | x;
             const size_t  o  = off(z);
-            const uint8_t st = work ? ss[q] : (uint8_t)((v0[z] != 0 && v0[z] != gen) ? 1 : 0);
+            const uint8_t st = cs[z];
             ext[o]           = st >= 2 ? 1 : 0;
             if(st >= 2) {
-                a[o]   = as[q];
+                a[o]   = ca[z];
                 val[o] = gen;
             } else if(st == 1 && v0[z] != 1) {
                 val[o] = 1;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace bnn {
 
@@ -37,12 +38,22 @@ template<class K> __global__ void __launch_bounds__(K::kThreads) k_one(K k) {
 // Scenes per batched launch at most (the scene search below is two rounds of 64 lanes)
 constexpr uint32_t kMaxLaunchScenes = 4096;
 
+// Workgroups per CU that the batched form of K is compiled to leave room for: K::kManyBlocks where the
+// functor names it (a register bound for functors whose batched form, a loop over items, otherwise
+// takes many more registers than the single-scene form), else 1 (no bound).
+template<class K, class = void> struct many_blocks {
+    static constexpr int value = 1;
+};
+template<class K> struct many_blocks<K, std::void_t<decltype(K::kManyBlocks)>> {
+    static constexpr int value = K::kManyBlocks;
+};
+
 // Batched launch. Items are the scenes' blocks laid end to end (scene s: items first[s] ..
 // first[s] + count[s] - 1, first[s] a multiple of 8); workgroup w runs the items 8 r + (w mod 8) for
 // rows r in [(w / 8) per, (w / 8 + 1) per) one after the other, so an item keeps its XCD residue and
 // one scene search serves `per` items.
 template<class K>
-__global__ void __launch_bounds__(K::kThreads) k_many(const K* __restrict__ ks, const uint32_t* __restrict__ first,
+__global__ void __launch_bounds__(K::kThreads, many_blocks<K>::value) k_many(const K* __restrict__ ks, const uint32_t* __restrict__ first,
                                                       const uint32_t* __restrict__ count, uint32_t nScenes, uint32_t rows, uint32_t per) {
     const uint32_t x = blockIdx.x & 7u, r0 = (blockIdx.x >> 3) * per;
     const uint32_t r1 = min(r0 + per, rows);
@@ -71,7 +82,10 @@ __global__ void __launch_bounds__(K::kThreads) k_many(const K* __restrict__ ks, 
         while(s + 1u < nScenes && first[s + 1u] <= i) ++s;
         s                    = __builtin_amdgcn_readfirstlane(s);
         const uint32_t local = i - first[s], n = count[s];
-        if(local < n) [[clang::always_inline]] ks[s](Blk{local, n});  // else: padding up to the next scene's multiple of 8
+        // the functor array is not written while the kernel runs: read through the constant address
+        // space, its fields are scalar loads into scalar registers, as in k_one
+        const K* kp = (const K*)((const __attribute__((address_space(4))) K*)ks + s);
+        if(local < n) [[clang::always_inline]] (*kp)(Blk{local, n});  // else: padding up to the next scene's multiple of 8
         if(r + 1u < r1) __syncthreads();      // the next item may reuse this item's LDS
     }
 }
