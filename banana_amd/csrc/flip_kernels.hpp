@@ -890,29 +890,39 @@ __device__ __forceinline__ void p2g_column(const Particles& P, const Affine& A, 
 // the tile owns; the others go to the unused slot c * 513 + 512. The 8 nodes are distinct, so all 8
 // sums are read before any is written back (one LDS round trip per component, no branches).
 __device__ __forceinline__ void p2g_tile_add(float2* acc, int c, int ix, float fx, int iy, float fy, int iz, float fz, float v, int x0, int y0,
-                                             int z0, int sx, int sy, int sz) {
+                                             int z0, int3 lim) {
 #pragma clang fp contract(off)
     const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
-    int         n[8];
+    // byte offset of a node within the component's 512 owned nodes, per axis; an axis the tile does
+    // not own (outside the tile or past the grid: lim = min(8, size - tile origin)) adds 4096, so the
+    // sum of the three, clamped to 4096, is the node's offset or the unused slot's
+    uint32_t ox[2], oy[2], oz[2];
+#pragma unroll
+    for(int d = 0; d < 2; ++d) {
+        const uint32_t i = (uint32_t)(ix + d - x0), j = (uint32_t)(iy + d - y0), k = (uint32_t)(iz + d - z0);
+        ox[d]            = i < (uint32_t)lim.x ? i * 8u : 4096u;
+        oy[d]            = j < (uint32_t)lim.y ? j * 64u : 4096u;
+        oz[d]            = k < (uint32_t)lim.z ? k * 512u : 4096u;
+    }
+    char* const base = reinterpret_cast<char*>(acc + c * 513);
+    float2*     n[8];
     float       w[8];
 #pragma unroll
     for(int e = 0; e < 8; ++e) {
-        const int  dx = e & 1, dy = (e >> 1) & 1, dz = e >> 2;
-        const int  i = ix + dx, j = iy + dy, k = iz + dz;
-        const bool own = (unsigned)(i - x0) < 8u && (unsigned)(j - y0) < 8u && (unsigned)(k - z0) < 8u && i < sx && j < sy && k < sz;
-        n[e]           = c * 513 + (own ? ((k - z0) * 8 + (j - y0)) * 8 + (i - x0) : 512);
-        w[e]           = (wx[dx] * wy[dy]) * wz[dz];
+        const int dx = e & 1, dy = (e >> 1) & 1, dz = e >> 2;
+        n[e]         = reinterpret_cast<float2*>(base + min(ox[dx] + oy[dy] + oz[dz], 4096u));
+        w[e]         = (wx[dx] * wy[dy]) * wz[dz];
     }
     float2 a[8];
 #pragma unroll
-    for(int e = 0; e < 8; ++e) a[e] = acc[n[e]];
+    for(int e = 0; e < 8; ++e) a[e] = *n[e];
 #pragma unroll
     for(int e = 0; e < 8; ++e) {
         a[e].x += w[e] * v;
         a[e].y += w[e];
     }
 #pragma unroll
-    for(int e = 0; e < 8; ++e) acc[n[e]] = a[e];
+    for(int e = 0; e < 8; ++e) *n[e] = a[e];
 }
 
 //--------------------------------------------------------------------------------------------------
@@ -944,6 +954,11 @@ struct k_p2g_tile {
         const int cx0 = max(x0 - 1, 0), cx1 = min(x0 + 8, g.nx - 1);  // cells read
         const int cy0 = max(y0 - 1, 0), cy1 = min(y0 + 8, g.ny - 1);
         const int cz0 = max(z0 - 1, 0), cz1 = min(z0 + 8, g.nz - 1);
+        // nodes owned per axis and component (face grids (nx+1, ny, nz), (nx, ny+1, nz), (nx, ny, nz+1));
+        // the tile origin never lies past a face grid's size, so each is in 0 .. 8
+        const int  mx = min(8, g.nx - x0), my = min(8, g.ny - y0), mz = min(8, g.nz - z0);
+        const int3 limU = make_int3(min(8, g.nx + 1 - x0), my, mz), limV = make_int3(mx, min(8, g.ny + 1 - y0), mz),
+                   limW = make_int3(mx, my, min(8, g.nz + 1 - z0));
         for(int e = lane; e < 3 * 513; e += 64) acc[e] = make_float2(0.f, 0.f);
         bool any = false;
         for(int bz = cz0 >> 3; bz <= (cz1 >> 3); ++bz)
@@ -997,9 +1012,9 @@ struct k_p2g_tile {
                     bary(B.qz[q], g.nz + 1, izn, fzn);
                     bary(B.qz[q] - 0.5f, g.nz, izh, fzh);
                     // u: x node-aligned, y and z half-shifted; v: y node-aligned; w: z node-aligned
-                    p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, B.ux[q], x0, y0, z0, g.nx + 1, g.ny, g.nz);
-                    p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, B.uy[q], x0, y0, z0, g.nx, g.ny + 1, g.nz);
-                    p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, B.uz[q], x0, y0, z0, g.nx, g.ny, g.nz + 1);
+                    p2g_tile_add(acc, 0, ixn, fxn, iyh, fyh, izh, fzh, B.ux[q], x0, y0, z0, limU);
+                    p2g_tile_add(acc, 1, ixh, fxh, iyn, fyn, izh, fzh, B.uy[q], x0, y0, z0, limV);
+                    p2g_tile_add(acc, 2, ixh, fxh, iyh, fyh, izn, fzn, B.uz[q], x0, y0, z0, limW);
                 }
             };
             // software pipeline: a round's first batch is loaded during the round before, its cell
